@@ -1,0 +1,412 @@
+/*
+ * numericalearth_hip.h — C ABI of the MI355X (gfx950) coupled surface time-step path.
+ *
+ * The path replaces four phases of NumericalEarth.jl's coupled step
+ * (src/EarthSystemModels/time_step_earth_system_model.jl:14-83) for an
+ * OceanOnlyModel on an immersed LatitudeLongitudeGrid:
+ *
+ *   phase 1  prescribed atmosphere / radiation interpolation
+ *   phase 2  SimilarityTheoryFluxes fixed-point air-sea flux solve (+ freeze clamp)
+ *   phase 3  net ocean flux assembly, phase 4 radiative contribution
+ *   ocean    WENO5 tracer-advection tendency with the T/S top flux boundary
+ *            condition, advanced by a three-stage split Runge-Kutta step
+ *
+ * Conventions (mirroring the reference's kernels, which are allocation-free and
+ * write in place into caller-owned Fields):
+ *   - every pointer handed to an entry point is a DEVICE pointer unless the
+ *     parameter is documented as host memory; the library never frees or
+ *     reallocates caller memory;
+ *   - arrays are described by ne_view: `data` points at interior element
+ *     (i, j, k) = (1, 1, 1) in Julia's 1-based numbering and sx/sy/sz are element
+ *     strides, so a Julia `parent(field)` with any halo maps directly
+ *     (data = pointer(parent) + offset of the first interior point);
+ *   - 3-D tracer and velocity views must have sx == 1 (i fastest);
+ *   - every entry point returns an ne_status (0 = success) and never throws;
+ *   - work is enqueued on `stream` (a hipStream_t passed as void*; NULL is the
+ *     null stream) and entry points do not synchronise unless documented.
+ */
+#ifndef NUMERICALEARTH_HIP_H
+#define NUMERICALEARTH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NE_ABI_VERSION 1
+
+typedef enum ne_status {
+    NE_OK = 0,
+    NE_ERR_NULL = 1,         /* a required pointer is NULL                          */
+    NE_ERR_SHAPE = 2,        /* sizes, strides or topology are inconsistent          */
+    NE_ERR_UNSUPPORTED = 3,  /* the option is not implemented by this library        */
+    NE_ERR_DEVICE = 4,       /* a HIP runtime call failed                            */
+    NE_ERR_CAPACITY = 5,     /* a documented maximum is exceeded                     */
+    NE_ERR_STATE = 6         /* object used in the wrong state                       */
+} ne_status;
+
+/* Topology of a horizontal direction (Oceananigans Periodic / Bounded / Flat). */
+enum { NE_PERIODIC = 0, NE_BOUNDED = 1, NE_FLAT = 2 };
+
+/* Arithmetic of a model / batch member. */
+enum { NE_FP64 = 0, NE_MIXED_FP32 = 1 };
+
+/* ---------------------------------------------------------------------------
+ * Views and descriptors
+ * ------------------------------------------------------------------------- */
+
+typedef struct ne_view {
+    void*   data;            /* interior origin (1,1,1); element type set by the entry point */
+    int64_t sx, sy, sz;      /* element strides (sz doubles as the snapshot stride of a time series) */
+} ne_view;
+
+/* Immersed LatitudeLongitudeGrid (exchange grid == ocean grid), device arrays.
+ * Reference: Oceananigans LatitudeLongitudeGrid + ImmersedBoundaryGrid(GridFittedBottom)
+ * as built by the reference's examples (README.md:78-108). Metrics are per
+ * latitude row because LatLon metrics depend on j only. */
+typedef struct ne_grid {
+    int32_t Nx, Ny, Nz;
+    int32_t topology_x;          /* NE_PERIODIC or NE_BOUNDED                       */
+    int32_t topology_y;          /* NE_BOUNDED (NE_PERIODIC accepted)               */
+    int32_t advection;           /* 0 disables tracer advection (single column grids,
+                                    src/Oceans/ocean_simulation.jl:367-379)         */
+    const double* dz;            /* [Nz]   Δzᶜᶜᶜ per level (m)                       */
+    const double* z_center;      /* [Nz]   cell-centre height (m, negative down)     */
+    const double* dy_xface;      /* [Ny]   Δyᶠᶜᵃ, meridional length of x-faces (m)   */
+    const double* dx_yface;      /* [Ny+1] Δxᶜᶠᵃ, zonal length of y-faces (m)        */
+    const double* Az;            /* [Ny]   Azᶜᶜᵃ, horizontal cell area (m²)          */
+    const double* lambda_c;      /* [Nx]   cell-centre longitude (deg)               */
+    const double* phi_c;         /* [Ny]   cell-centre latitude (deg)                */
+    const int32_t* kmin;         /* [Nx*Ny] i-fastest: first active level (0-based)
+                                    of each column; Nz marks a land column          */
+} ne_grid;
+
+/* Regular longitude-latitude source grid of a prescribed atmosphere or radiation
+ * (Oceananigans LatitudeLongitudeGrid, topology (Periodic|Bounded, Bounded, Flat)).
+ * Cell i (0-based) has centre longitude lambda0 + (i + 1/2) dlambda. */
+typedef struct ne_source_grid {
+    int32_t Nx, Ny;
+    int32_t topology_x, topology_y;
+    double  lambda0, dlambda;
+    double  phi0, dphi;
+} ne_source_grid;
+
+/* Time interpolation between snapshots n1 and n2 with weight tilde_n
+ * (Oceananigans cpu_interpolating_time_indices, computed on the host;
+ * src/Atmospheres/interpolate_atmospheric_state.jl:57-60). 0-based snapshots. */
+typedef struct ne_time_interp {
+    int32_t n1, n2;
+    double  tilde_n;
+} ne_time_interp;
+
+/* PrescribedAtmosphere FieldTimeSeries data (src/Atmospheres/prescribed_atmosphere.jl:219-255).
+ * Each view addresses data[i*sx + j*sy + n*sz] on the source grid. rain/snow may have
+ * data == NULL, which reads as zero flux. */
+typedef struct ne_prescribed_atmosphere {
+    ne_source_grid grid;
+    ne_view u, v, T, q, p, rain, snow;
+    ne_time_interp time;
+} ne_prescribed_atmosphere;
+
+/* PrescribedRadiation downwelling shortwave / longwave (src/Radiations/prescribed_radiation.jl:55-92). */
+typedef struct ne_prescribed_radiation {
+    ne_source_grid grid;
+    ne_view downwelling_shortwave, downwelling_longwave;
+    ne_time_interp time;
+} ne_prescribed_radiation;
+
+/* Atmosphere state interpolated onto the exchange grid (2-D (C,C,Nothing) fields,
+ * src/Atmospheres/prescribed_atmosphere_regridder.jl:7-13). */
+typedef struct ne_exchange_state {
+    ne_view u, v, T, p, q, rain, snow;
+} ne_exchange_state;
+
+/* Ocean state: u (Face,Center,Center), v (Center,Face,Center), w (Center,Center,Face),
+ * T, S (Center,Center,Center). Face index i addresses the west face of cell i. */
+typedef struct ne_ocean_state {
+    ne_view u, v, w, T, S;
+} ne_ocean_state;
+
+/* AtmosphereSurfaceFluxes + interface temperature
+ * (src/EarthSystemModels/InterfaceComputations/atmosphere_ocean_fluxes.jl:176-196).
+ * `iterations` (int32) may have data == NULL; when set it receives the number of
+ * fixed-point iterations each column took. */
+typedef struct ne_interface_fluxes {
+    ne_view latent_heat, sensible_heat, water_vapor, x_momentum, y_momentum;
+    ne_view friction_velocity, temperature_scale, water_vapor_scale;
+    ne_view interface_temperature;
+    ne_view iterations;
+} ne_interface_fluxes;
+
+/* Net ocean fluxes (src/Oceans/assemble_net_ocean_fluxes.jl:119-151): τˣ (Face,Center),
+ * τʸ (Center,Face), Jᵀ, Jˢ, Jʷ (= net_fluxes.η), Jᴴ (freshwater heat content). */
+typedef struct ne_net_ocean_fluxes {
+    ne_view tau_x, tau_y, J_T, J_S, J_w, J_H;
+} ne_net_ocean_fluxes;
+
+/* InterfaceRadiationFlux diagnostics (src/Radiations/apply_air_sea_radiative_fluxes.jl:107-109). */
+typedef struct ne_radiative_fluxes {
+    ne_view upwelling_longwave, downwelling_longwave, downwelling_shortwave;
+} ne_radiative_fluxes;
+
+/* ---------------------------------------------------------------------------
+ * Parameters of the flux formulation, interfaces and components (host struct,
+ * passed by pointer and copied by value into kernels). ne_params_default fills
+ * the reference defaults.
+ * ------------------------------------------------------------------------- */
+
+enum { NE_STABILITY_NONE = 0, NE_STABILITY_EDSON = 1 };
+enum { NE_SIMILARITY_LOG = 0, NE_SIMILARITY_COARE_LOG = 1 };
+enum { NE_ROUGHNESS_CONSTANT = 0, NE_ROUGHNESS_EDSON = 1 };
+enum { NE_STOP_CONVERGENCE = 0, NE_STOP_FIXED_ITERATIONS = 1 };
+enum { NE_HUMIDITY_IMPURE_SATURATION = 0, NE_HUMIDITY_FIXED = 1 };
+enum { NE_VELOCITY_RELATIVE = 0, NE_VELOCITY_WIND = 1 };
+
+typedef struct ne_params {
+    /* SimilarityTheoryFluxes (similarity_theory_turbulent_fluxes.jl:174-214) */
+    double  von_karman_constant;          /* 0.4 */
+    double  turbulent_prandtl_number;     /* 1; stored, unused by the iteration (as in the reference) */
+    int32_t convective_gustiness;         /* 1: ConvectiveGustiness (:45-48, :91-95); 0: nothing */
+    int32_t mesoscale_gustiness;          /* 1: SubgridVelocityCorrection mesoscale Number (:97-98) */
+    double  gustiness_parameter;          /* β = 1.2 */
+    double  minimum_gustiness;            /* 0.01 m/s */
+    double  mesoscale_velocity;           /* m/s */
+    int32_t stability_functions;          /* NE_STABILITY_EDSON (:487-625) or NE_STABILITY_NONE (ψ ≡ 0) */
+    int32_t similarity_form;              /* NE_SIMILARITY_LOG (:242-247) or NE_SIMILARITY_COARE_LOG (:249-253) */
+    double  edson_momentum[11];           /* ζmax A⁺ B⁺ C⁺ D⁺ A⁻ B⁻ C⁻ D⁻ E⁻ F⁻ (:487-499) */
+    double  edson_scalar[12];             /* ζmax A⁺ B⁺ C⁺ D⁺ E⁺ A⁻ B⁻ C⁻ D⁻ E⁻ F⁻ (:571-584) */
+    /* roughness lengths (roughness_lengths.jl) */
+    int32_t momentum_roughness;           /* NE_ROUGHNESS_EDSON: MomentumRoughnessLength (:197-210) */
+    int32_t temperature_roughness;        /* NE_ROUGHNESS_EDSON: ScalarRoughnessLength (:234-246) */
+    int32_t water_vapor_roughness;
+    int32_t reserved0;
+    double  momentum_roughness_length;    /* value used when the kind is NE_ROUGHNESS_CONSTANT */
+    double  temperature_roughness_length;
+    double  water_vapor_roughness_length;
+    double  roughness_gravitational_acceleration; /* MomentumRoughnessLength g (default 9.80665) */
+    double  air_kinematic_viscosity;      /* ν = 1.5e-5 */
+    double  charnock_parameter;           /* wave_formulation = 0.02 */
+    double  smooth_wall_parameter;        /* 0.11 */
+    double  maximum_momentum_roughness;   /* 1 */
+    double  reynolds_scaling_A;           /* 5.85e-5 */
+    double  reynolds_scaling_b;           /* 0.72 */
+    double  maximum_scalar_roughness;     /* 1.6e-4 */
+    double  zero_plane_displacement;      /* 0 m */
+    /* solver stop criteria (compute_interface_state.jl:5-24) */
+    int32_t stop_criteria;                /* NE_STOP_CONVERGENCE or NE_STOP_FIXED_ITERATIONS */
+    int32_t maxiter;                      /* maxiter (100) or the fixed iteration count */
+    double  tolerance;                    /* 1e-8 */
+    /* interface properties (component_interfaces.jl:353-358, :399-401) */
+    int32_t specific_humidity;            /* NE_HUMIDITY_IMPURE_SATURATION (x_H₂O) or NE_HUMIDITY_FIXED */
+    int32_t velocity_difference;          /* NE_VELOCITY_RELATIVE or NE_VELOCITY_WIND */
+    double  water_mole_fraction;          /* 0.98 */
+    double  fixed_specific_humidity;      /* qₛ when NE_HUMIDITY_FIXED */
+    /* atmosphere / interface properties (prescribed_atmosphere.jl:222-223; component_interfaces.jl:483-484) */
+    double  surface_layer_height;         /* 10 m */
+    double  boundary_layer_height;        /* 512 m */
+    double  gravitational_acceleration;   /* 9.80665 */
+    /* AtmosphereThermodynamicsParameters (src/Atmospheres/thermodynamic_parameters.jl) */
+    double  gas_constant;                 /* 8.3144598 */
+    double  dry_air_molar_mass;           /* 0.02897 */
+    double  water_molar_mass;             /* 0.018015 */
+    double  dry_air_adiabatic_exponent;   /* 2/7 */
+    double  water_vapor_heat_capacity;    /* 1859 */
+    double  liquid_water_heat_capacity;   /* 4181 */
+    double  reference_vaporization_enthalpy; /* 2500800 */
+    double  reference_temperature;        /* 273.16 */
+    double  triple_point_temperature;     /* 273.16 */
+    double  triple_point_pressure;        /* 611.657 */
+    /* ocean properties (component_interfaces.jl:433-436; ocean_simulation.jl:341, :527-530) */
+    double  ocean_reference_density;      /* 1020 */
+    double  ocean_heat_capacity;          /* TEOS-10 cₚ⁰ = 3991.86795711963 */
+    double  freshwater_density;           /* 1000 */
+    /* radiation (src/Radiations/Radiations.jl:13-16, prescribed_radiation.jl:68) */
+    int32_t radiation;                    /* 1: PrescribedRadiation present */
+    int32_t reserved1;
+    double  stefan_boltzmann_constant;    /* 5.670374419e-8 */
+    double  ocean_albedo;                 /* 0.05 */
+    double  ocean_emissivity;             /* 0.97 */
+    /* FreezingLimitedOceanTemperature LinearLiquidus (freezing_limited_ocean_temperature.jl:25) */
+    double  freshwater_melting_temperature; /* 0 °C */
+    double  liquidus_slope;               /* 0.054 °C/psu */
+} ne_params;
+
+/* Counters written by the flux-solve kernels (reference: non-convergence is silent
+ * and NaNs propagate to the NaNChecker, earth_system_model.jl:483-495). */
+typedef struct ne_counters {
+    uint64_t total_iterations;            /* Σ iterations over solved columns       */
+    uint32_t solved_columns;              /* active columns solved                   */
+    uint32_t nonconverged_columns;        /* columns that hit maxiter                */
+    uint32_t nan_columns;                 /* columns whose u★, θ★ or q★ is not finite */
+    uint32_t max_iterations;              /* largest per-column iteration count      */
+} ne_counters;
+
+typedef struct ne_ctx ne_ctx;
+
+/* ---------------------------------------------------------------------------
+ * Library and context
+ * ------------------------------------------------------------------------- */
+
+/* ABI version (NE_ABI_VERSION). Host only. */
+int ne_version(void);
+
+/* Fill `p` with the reference defaults: SimilarityTheoryFluxes(Float64)
+ * (similarity_theory_turbulent_fluxes.jl:174-214), ComponentInterfaces defaults
+ * (component_interfaces.jl:390-419), PrescribedAtmosphere heights
+ * (prescribed_atmosphere.jl:222-223), AtmosphereThermodynamicsParameters,
+ * TEOS-10 ocean properties, PrescribedRadiation ocean surface, LinearLiquidus. Host only. */
+int ne_params_default(ne_params* p);
+
+/* Create a context on HIP device `device`. The context owns the status counters
+ * and no other device memory. Replaces the reference's implicit architecture state. */
+int ne_ctx_create(int device, ne_ctx** ctx);
+int ne_ctx_destroy(ne_ctx* ctx);
+
+/* Copy the counters to host memory `out` (synchronises `stream`) and, if `reset`
+ * is non-zero, zero them. */
+int ne_ctx_counters(ne_ctx* ctx, ne_counters* out, int reset, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Per-phase entry points (FP64). Each replaces one reference kernel launch.
+ * ------------------------------------------------------------------------- */
+
+/* First active level of every column of an ImmersedBoundaryGrid(GridFittedBottom):
+ * cell (i,j,k) is immersed iff z_center[k] <= bottom_height[i,j]. Writes kmin
+ * (the grid's own array may be passed). bottom_height is FP64 2-D.
+ * Replaces the immersed-cell predicate behind Oceananigans inactive_node
+ * (used at atmosphere_ocean_fluxes.jl:142, assemble_net_ocean_fluxes.jl:127). */
+int ne_grid_immersed_levels(ne_ctx* ctx, const ne_grid* grid, ne_view bottom_height,
+                            int32_t* kmin, void* stream);
+
+/* Fractional indices of the exchange-grid centres in a source grid
+ * (_compute_fractional_indices!, prescribed_atmosphere_regridder.jl:51-71;
+ * radiation twin prescribed_radiation_regridder.jl:36-51). For LatLon grids the
+ * fields are separable: fi[i] (Nx) and fj[j] (Ny), 0-based fractional positions. */
+int ne_fractional_indices_f64(ne_ctx* ctx, const ne_grid* grid, const ne_source_grid* source,
+                              double* fi, double* fj, void* stream);
+
+/* _interpolate_primary_atmospheric_state! (interpolate_atmospheric_state.jl:91-137):
+ * bilinear in space, linear in time, onto the exchange grid interior. */
+int ne_interp_atmos_f64(ne_ctx* ctx, const ne_grid* grid, const ne_prescribed_atmosphere* atmos,
+                        const double* fi, const double* fj,
+                        const ne_exchange_state* out, void* stream);
+
+/* _interpolate_radiation_state! (interpolate_radiation_state.jl:43-69). */
+int ne_interp_radiation_f64(ne_ctx* ctx, const ne_grid* grid, const ne_prescribed_radiation* rad,
+                            const double* fi, const double* fj,
+                            ne_view downwelling_shortwave, ne_view downwelling_longwave,
+                            void* stream);
+
+/* _compute_atmosphere_ocean_interface_state! (atmosphere_ocean_fluxes.jl:80-197):
+ * per-column Monin-Obukhov fixed point (compute_interface_state.jl:31-58). Reads
+ * the exchange state and the ocean surface (k = Nz) and writes the interface fluxes. */
+int ne_ao_flux_solve_f64(ne_ctx* ctx, const ne_grid* grid, const ne_params* params,
+                         const ne_exchange_state* atmos, const ne_ocean_state* ocean,
+                         const ne_interface_fluxes* out, void* stream);
+
+/* _freeze_ocean_temperature! (freezing_limited_ocean_temperature.jl:95-118): clamps T
+ * to the liquidus in place and writes the frazil heat flux. dt = +INFINITY on the
+ * first (iteration 0) call, as at :88. */
+int ne_freeze_clamp_f64(ne_ctx* ctx, const ne_grid* grid, const ne_params* params,
+                        ne_view T, ne_view S, ne_view frazil_heat, double dt, void* stream);
+
+/* _assemble_net_ocean_fluxes! (assemble_net_ocean_fluxes.jl:74-153) for an ocean-only
+ * model with FreezingLimitedOceanTemperature (ℵ = 0, no sea-ice fluxes). `sst` is the
+ * ocean temperature field (read at k = Nz), `rain`/`snow` come from the exchange state. */
+int ne_assemble_net_ocean_fluxes_f64(ne_ctx* ctx, const ne_grid* grid, const ne_params* params,
+                                     const ne_interface_fluxes* fluxes, const ne_exchange_state* atmos,
+                                     ne_view sst, const ne_net_ocean_fluxes* out, void* stream);
+
+/* _apply_air_sea_radiative_fluxes! (apply_air_sea_radiative_fluxes.jl:62-111): Jᵀ += radiative
+ * contribution, and the three diagnostics. */
+int ne_apply_air_sea_radiation_f64(ne_ctx* ctx, const ne_grid* grid, const ne_params* params,
+                                   ne_view downwelling_shortwave, ne_view downwelling_longwave,
+                                   ne_view interface_temperature,
+                                   const ne_net_ocean_fluxes* net, const ne_radiative_fluxes* out,
+                                   void* stream);
+
+/* Tracer tendency of T and S: −∇·(U c) with WENO(order=5) on the immersed grid plus the
+ * top flux boundary condition (Gᵀ[Nz] −= Jᵀ Az/V, Gˢ[Nz] −= (Jˢ + S Jʷ) Az/V;
+ * ocean_simulation.jl:65-94 + Oceananigans apply_z_bcs!). Inactive cells get G = 0. */
+int ne_tracer_tendency_weno5_f64(ne_ctx* ctx, const ne_grid* grid, const ne_ocean_state* ocean,
+                                 const ne_net_ocean_fluxes* net, ne_view G_T, ne_view G_S,
+                                 void* stream);
+
+/* One split Runge-Kutta stage, tendency fused: c_out = c0 + beta_dt·G(c_in) for T and S
+ * on active cells (inactive cells are left untouched). c_out may alias c0 but not c_in. */
+int ne_tracer_rk_stage_f64(ne_ctx* ctx, const ne_grid* grid, ne_view u, ne_view v, ne_view w,
+                           ne_view T0, ne_view S0, ne_view T_in, ne_view S_in,
+                           ne_view T_out, ne_view S_out,
+                           const ne_net_ocean_fluxes* net, double beta_dt, void* stream);
+
+/* Compensated (Neumaier, double-double combine) sum over active cells of
+ * weight·V·c, V = Az·Δz. Writes {sum, compensation} (2 doubles) to device `out`. */
+int ne_budget_f64(ne_ctx* ctx, const ne_grid* grid, ne_view c, double weight,
+                  double* out, void* stream);
+int ne_budget_f32(ne_ctx* ctx, const ne_grid* grid, ne_view c, double weight,
+                  double* out, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Model: OceanOnlyModel(ocean; atmosphere, radiation) + time_step!
+ * (earth_system_model.jl:399-402; time_step_earth_system_model.jl:14-83).
+ * The model binds caller-owned state and output views and allocates its own
+ * workspace once, at creation; stepping allocates nothing.
+ * ------------------------------------------------------------------------- */
+
+typedef struct ne_model_desc {
+    int32_t precision;                    /* NE_FP64, or NE_MIXED_FP32: T, S, u, v, w and the
+                                             prescribed fields are float; fluxes stay double */
+    int32_t fuse_freeze_clamp;            /* reserved, must be 0 */
+    ne_grid grid;
+    ne_params params;
+    ne_prescribed_atmosphere atmosphere;
+    ne_prescribed_radiation radiation;    /* ignored unless params.radiation */
+    const double* atmos_fi;               /* [Nx] from ne_fractional_indices_f64 */
+    const double* atmos_fj;               /* [Ny] */
+    const double* radiation_fi;
+    const double* radiation_fj;
+    ne_ocean_state ocean;                 /* T, S evolve; u, v, w are held fixed */
+    ne_interface_fluxes fluxes;           /* written every update_state */
+    ne_net_ocean_fluxes net;
+    ne_radiative_fluxes radiative;        /* may have NULL data (not written) */
+    ne_view frazil_heat;                  /* double 2-D */
+    const double* atmosphere_times;       /* HOST: snapshot times (s) */
+    int32_t n_atmosphere_times;
+    int32_t n_radiation_times;
+    const double* radiation_times;        /* HOST */
+} ne_model_desc;
+
+typedef struct ne_model ne_model;
+
+/* Create the model and run reconcile_state! (fluxes computed at construction,
+ * earth_system_model.jl:86-90), i.e. one update_state with dt_frazil = +INFINITY. */
+int ne_model_create(ne_ctx* ctx, const ne_model_desc* desc, ne_model** model, void* stream);
+int ne_model_destroy(ne_model* model);
+
+/* time_step!(coupled_model, Δt): ocean split-RK3 step with the current fluxes, clock tick,
+ * then update_state! (interpolate, flux solve, freeze clamp, assemble, radiation). */
+int ne_model_time_step(ne_model* model, double dt, void* stream);
+
+/* update_state! alone (phases 1-4). */
+int ne_model_update_state(ne_model* model, void* stream);
+
+/* Clock (seconds) and iteration count of the model. Host only. */
+int ne_model_clock(const ne_model* model, double* time, int64_t* iteration);
+
+/* Per-kernel timing of ne_model_time_step with HIP events recorded on the step's own
+ * stream (no counterpart in the reference, whose examples time whole steps in callbacks,
+ * examples/near_global_ocean_simulation.jl:109-132). begin creates the events for up to
+ * max_steps steps; end waits for them and writes the summed milliseconds of each kernel
+ * to ms[NE_TIMED_KERNELS] and the number of steps recorded to *steps. Kernels, in order:
+ * RK stage 1, RK stage 2, RK stage 3, surface solve (interpolation + flux solve),
+ * surface finalize (freeze clamp + net fluxes + radiation). */
+#define NE_TIMED_KERNELS 5
+int ne_model_timing_begin(ne_model* model, int max_steps);
+int ne_model_timing_end(ne_model* model, double* ms, int* steps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NUMERICALEARTH_HIP_H */

@@ -1,0 +1,557 @@
+// ne_abi.hip — extern "C" entry points of include/numericalearth_hip.h.
+#include <math.h>
+#include <string.h>
+
+#include <new>
+#include <vector>
+
+#include "ne_internal.h"
+
+using namespace ne;
+
+namespace {
+
+inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+inline int dev_err(hipError_t e) { return e == hipSuccess ? NE_OK : NE_ERR_DEVICE; }
+
+int check_grid(const ne_grid* g) {
+    if (!g) return NE_ERR_NULL;
+    if (g->Nx <= 0 || g->Ny <= 0 || g->Nz <= 0) return NE_ERR_SHAPE;
+    if (g->topology_x != NE_PERIODIC && g->topology_x != NE_BOUNDED) return NE_ERR_UNSUPPORTED;
+    if (g->topology_y != NE_PERIODIC && g->topology_y != NE_BOUNDED) return NE_ERR_UNSUPPORTED;
+    if (!g->dz || !g->z_center || !g->dy_xface || !g->dx_yface || !g->Az || !g->lambda_c || !g->phi_c || !g->kmin)
+        return NE_ERR_NULL;
+    return NE_OK;
+}
+
+int check_view3(const ne_view& v) {
+    if (!v.data) return NE_ERR_NULL;
+    if (v.sx != 1) return NE_ERR_SHAPE;
+    return NE_OK;
+}
+
+int check_view2(const ne_view& v) { return v.data ? NE_OK : NE_ERR_NULL; }
+
+int check_source(const ne_source_grid& s) {
+    if (s.Nx <= 0 || s.Ny <= 0) return NE_ERR_SHAPE;
+    if (s.topology_x != NE_FLAT && !(s.dlambda > 0)) return NE_ERR_SHAPE;
+    if (s.topology_y != NE_FLAT && !(s.dphi > 0)) return NE_ERR_SHAPE;
+    return NE_OK;
+}
+
+int check_atmos(const ne_prescribed_atmosphere* a) {
+    if (!a) return NE_ERR_NULL;
+    int r = check_source(a->grid);
+    if (r) return r;
+    if (!a->u.data || !a->v.data || !a->T.data || !a->q.data || !a->p.data) return NE_ERR_NULL;
+    if (a->time.n1 < 0 || a->time.n2 < 0) return NE_ERR_SHAPE;
+    return NE_OK;
+}
+
+int check_rad(const ne_prescribed_radiation* r) {
+    if (!r) return NE_ERR_NULL;
+    int s = check_source(r->grid);
+    if (s) return s;
+    if (!r->downwelling_shortwave.data || !r->downwelling_longwave.data) return NE_ERR_NULL;
+    return NE_OK;
+}
+
+int check_ocean(const ne_ocean_state* o) {
+    if (!o) return NE_ERR_NULL;
+    int r;
+    if ((r = check_view3(o->u)) || (r = check_view3(o->v)) || (r = check_view3(o->w)) || (r = check_view3(o->T)) ||
+        (r = check_view3(o->S)))
+        return r;
+    return NE_OK;
+}
+
+int check_fluxes(const ne_interface_fluxes* f) {
+    if (!f) return NE_ERR_NULL;
+    const ne_view* v[] = {&f->latent_heat, &f->sensible_heat, &f->water_vapor, &f->x_momentum, &f->y_momentum,
+                          &f->friction_velocity, &f->temperature_scale, &f->water_vapor_scale,
+                          &f->interface_temperature};
+    for (const ne_view* x : v)
+        if (!x->data) return NE_ERR_NULL;
+    return NE_OK;
+}
+
+int check_net(const ne_net_ocean_fluxes* n) {
+    if (!n) return NE_ERR_NULL;
+    const ne_view* v[] = {&n->tau_x, &n->tau_y, &n->J_T, &n->J_S, &n->J_w, &n->J_H};
+    for (const ne_view* x : v)
+        if (!x->data) return NE_ERR_NULL;
+    return NE_OK;
+}
+
+int check_params(const ne_params* p) {
+    if (!p) return NE_ERR_NULL;
+    if (p->stop_criteria != NE_STOP_CONVERGENCE && p->stop_criteria != NE_STOP_FIXED_ITERATIONS)
+        return NE_ERR_UNSUPPORTED;
+    if (p->stability_functions != NE_STABILITY_NONE && p->stability_functions != NE_STABILITY_EDSON)
+        return NE_ERR_UNSUPPORTED;
+    if (p->maxiter < 0) return NE_ERR_SHAPE;
+    return NE_OK;
+}
+
+// cpu_interpolating_time_indices with Linear time indexing, clamped at the ends.
+ne_time_interp time_indices(const std::vector<double>& times, double t) {
+    ne_time_interp r = {0, 0, 0.0};
+    int n = (int)times.size();
+    if (n <= 1 || t <= times[0]) return r;
+    if (t >= times[n - 1]) { r.n1 = r.n2 = n - 1; return r; }
+    int k = 0;
+    while (k + 1 < n && times[k + 1] <= t) k++;
+    r.n1 = k;
+    r.n2 = k + 1;
+    r.tilde_n = (t - times[k]) / (times[k + 1] - times[k]);
+    return r;
+}
+
+}  // namespace
+
+struct ne_model {
+    ne_ctx* ctx;
+    ne_model_desc d;
+    std::vector<double> atimes, rtimes;
+    void* T1 = nullptr;
+    void* S1 = nullptr;
+    void* T2 = nullptr;
+    void* S2 = nullptr;
+    ne_view vT1, vS1, vT2, vS2;
+    double time = 0.0;
+    double last_dt = 0.0;
+    int64_t iteration = 0;
+    bool timing = false;
+    int max_steps = 0;
+    int recorded = 0;
+    std::vector<hipEvent_t> events;
+};
+
+extern "C" {
+
+int ne_version(void) { return NE_ABI_VERSION; }
+
+int ne_params_default(ne_params* p) {
+    if (!p) return NE_ERR_NULL;
+    memset(p, 0, sizeof(*p));
+    const double pi = 3.14159265358979323846;
+    p->von_karman_constant = 0.4;
+    p->turbulent_prandtl_number = 1.0;
+    p->convective_gustiness = 1;
+    p->mesoscale_gustiness = 0;
+    p->gustiness_parameter = 1.2;
+    p->minimum_gustiness = 0.01;
+    p->mesoscale_velocity = 0.0;
+    p->stability_functions = NE_STABILITY_EDSON;
+    p->similarity_form = NE_SIMILARITY_LOG;
+    const double em[11] = {50.0, 0.35, 0.7, 0.75, 5.0 / 0.35, 15.0, 2.0, pi / 2.0, 10.15, 3.0, pi / sqrt(3.0)};
+    const double es[12] = {50.0, 0.35, 2.0 / 3.0, 3.0 / 2.0, 14.28, 8.525, 15.0, 2.0, 0.0, 34.15, 3.0, pi / sqrt(3.0)};
+    memcpy(p->edson_momentum, em, sizeof(em));
+    memcpy(p->edson_scalar, es, sizeof(es));
+    p->momentum_roughness = NE_ROUGHNESS_EDSON;
+    p->temperature_roughness = NE_ROUGHNESS_EDSON;
+    p->water_vapor_roughness = NE_ROUGHNESS_EDSON;
+    p->roughness_gravitational_acceleration = 9.80665;
+    p->air_kinematic_viscosity = 1.5e-5;
+    p->charnock_parameter = 0.02;
+    p->smooth_wall_parameter = 0.11;
+    p->maximum_momentum_roughness = 1.0;
+    p->reynolds_scaling_A = 5.85e-5;
+    p->reynolds_scaling_b = 0.72;
+    p->maximum_scalar_roughness = 1.6e-4;
+    p->zero_plane_displacement = 0.0;
+    p->stop_criteria = NE_STOP_CONVERGENCE;
+    p->maxiter = 100;
+    p->tolerance = 1e-8;
+    p->specific_humidity = NE_HUMIDITY_IMPURE_SATURATION;
+    p->velocity_difference = NE_VELOCITY_RELATIVE;
+    p->water_mole_fraction = 0.98;
+    p->surface_layer_height = 10.0;
+    p->boundary_layer_height = 512.0;
+    p->gravitational_acceleration = 9.80665;
+    p->gas_constant = 8.3144598;
+    p->dry_air_molar_mass = 0.02897;
+    p->water_molar_mass = 0.018015;
+    p->dry_air_adiabatic_exponent = 2.0 / 7.0;
+    p->water_vapor_heat_capacity = 1859.0;
+    p->liquid_water_heat_capacity = 4181.0;
+    p->reference_vaporization_enthalpy = 2500800.0;
+    p->reference_temperature = 273.16;
+    p->triple_point_temperature = 273.16;
+    p->triple_point_pressure = 611.657;
+    p->ocean_reference_density = 1020.0;
+    p->ocean_heat_capacity = 3991.86795711963;
+    p->freshwater_density = 1000.0;
+    p->radiation = 1;
+    p->stefan_boltzmann_constant = 5.670374419e-8;
+    p->ocean_albedo = 0.05;
+    p->ocean_emissivity = 0.97;
+    p->freshwater_melting_temperature = 0.0;
+    p->liquidus_slope = 0.054;
+    return NE_OK;
+}
+
+int ne_ctx_create(int device, ne_ctx** out) {
+    if (!out) return NE_ERR_NULL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return NE_ERR_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return NE_ERR_DEVICE;
+    ne_ctx* c = new (std::nothrow) ne_ctx();
+    if (!c) return NE_ERR_CAPACITY;
+    c->device = device;
+    c->scratch = nullptr;
+    c->scratch_bytes = 0;
+    if (hipMalloc((void**)&c->counters, sizeof(ne_counters)) != hipSuccess) { delete c; return NE_ERR_DEVICE; }
+    if (hipMemset(c->counters, 0, sizeof(ne_counters)) != hipSuccess) {
+        (void)hipFree(c->counters);
+        delete c;
+        return NE_ERR_DEVICE;
+    }
+    *out = c;
+    return NE_OK;
+}
+
+int ne_ctx_destroy(ne_ctx* c) {
+    if (!c) return NE_ERR_NULL;
+    (void)hipSetDevice(c->device);
+    int r = NE_OK;
+    if (c->counters && hipFree(c->counters) != hipSuccess) r = NE_ERR_DEVICE;
+    if (c->scratch && hipFree(c->scratch) != hipSuccess) r = NE_ERR_DEVICE;
+    delete c;
+    return r;
+}
+
+int ne_ctx_counters(ne_ctx* c, ne_counters* out, int reset, void* stream) {
+    if (!c || !out) return NE_ERR_NULL;
+    hipStream_t s = as_stream(stream);
+    if (hipMemcpyAsync(out, c->counters, sizeof(ne_counters), hipMemcpyDeviceToHost, s) != hipSuccess)
+        return NE_ERR_DEVICE;
+    if (hipStreamSynchronize(s) != hipSuccess) return NE_ERR_DEVICE;
+    if (reset) {
+        if (hipMemsetAsync(c->counters, 0, sizeof(ne_counters), s) != hipSuccess) return NE_ERR_DEVICE;
+        if (hipStreamSynchronize(s) != hipSuccess) return NE_ERR_DEVICE;
+    }
+    return NE_OK;
+}
+
+int ne_grid_immersed_levels(ne_ctx* c, const ne_grid* g, ne_view bottom, int32_t* kmin, void* stream) {
+    if (!c || !g || !kmin) return NE_ERR_NULL;
+    if (g->Nx <= 0 || g->Ny <= 0 || g->Nz <= 0) return NE_ERR_SHAPE;
+    if (!g->z_center || !bottom.data) return NE_ERR_NULL;
+    GridDev d = grid_dev(*g);
+    return dev_err(launch_immersed_levels(d, view<const double>(bottom), kmin, as_stream(stream)));
+}
+
+int ne_fractional_indices_f64(ne_ctx* c, const ne_grid* g, const ne_source_grid* src, double* fi, double* fj,
+                              void* stream) {
+    if (!c || !g || !src || !fi || !fj) return NE_ERR_NULL;
+    if (!g->lambda_c || !g->phi_c) return NE_ERR_NULL;
+    if (g->Nx <= 0 || g->Ny <= 0) return NE_ERR_SHAPE;
+    int r = check_source(*src);
+    if (r) return r;
+    return dev_err(launch_fractional_indices(grid_dev(*g), *src, fi, fj, as_stream(stream)));
+}
+
+int ne_interp_atmos_f64(ne_ctx* c, const ne_grid* g, const ne_prescribed_atmosphere* a, const double* fi,
+                        const double* fj, const ne_exchange_state* out, void* stream) {
+    if (!c || !out || !fi || !fj) return NE_ERR_NULL;
+    int r;
+    if ((r = check_grid(g)) || (r = check_atmos(a))) return r;
+    if (!out->u.data || !out->v.data || !out->T.data || !out->p.data || !out->q.data) return NE_ERR_NULL;
+    return dev_err(launch_interp_atmos<double>(grid_dev(*g), atmos_dev<double>(*a), fi, fj, *out, as_stream(stream)));
+}
+
+int ne_interp_radiation_f64(ne_ctx* c, const ne_grid* g, const ne_prescribed_radiation* rad, const double* fi,
+                            const double* fj, ne_view sw, ne_view lw, void* stream) {
+    if (!c || !fi || !fj) return NE_ERR_NULL;
+    int r;
+    if ((r = check_grid(g)) || (r = check_rad(rad))) return r;
+    if (!sw.data || !lw.data) return NE_ERR_NULL;
+    return dev_err(launch_interp_radiation<double>(grid_dev(*g), rad_dev<double>(*rad), fi, fj, view<double>(sw),
+                                                   view<double>(lw), as_stream(stream)));
+}
+
+int ne_ao_flux_solve_f64(ne_ctx* c, const ne_grid* g, const ne_params* p, const ne_exchange_state* a,
+                         const ne_ocean_state* o, const ne_interface_fluxes* out, void* stream) {
+    if (!c || !a) return NE_ERR_NULL;
+    int r;
+    if ((r = check_grid(g)) || (r = check_params(p)) || (r = check_ocean(o)) || (r = check_fluxes(out))) return r;
+    if (!a->u.data || !a->v.data || !a->T.data || !a->p.data || !a->q.data) return NE_ERR_NULL;
+    return dev_err(launch_flux_solve_exchange(grid_dev(*g), *p, *a, *o, flux_out(*out), c->counters,
+                                              as_stream(stream)));
+}
+
+int ne_freeze_clamp_f64(ne_ctx* c, const ne_grid* g, const ne_params* p, ne_view T, ne_view S, ne_view frazil,
+                        double dt, void* stream) {
+    if (!c) return NE_ERR_NULL;
+    int r;
+    if ((r = check_grid(g)) || (r = check_params(p)) || (r = check_view3(T)) || (r = check_view3(S)) ||
+        (r = check_view2(frazil)))
+        return r;
+    return dev_err(launch_freeze_clamp(grid_dev(*g), *p, view<double>(T), view<const double>(S),
+                                       view<double>(frazil), dt, as_stream(stream)));
+}
+
+int ne_assemble_net_ocean_fluxes_f64(ne_ctx* c, const ne_grid* g, const ne_params* p,
+                                     const ne_interface_fluxes* f, const ne_exchange_state* a, ne_view sst,
+                                     const ne_net_ocean_fluxes* out, void* stream) {
+    if (!c || !a) return NE_ERR_NULL;
+    int r;
+    if ((r = check_grid(g)) || (r = check_params(p)) || (r = check_fluxes(f)) || (r = check_net(out)) ||
+        (r = check_view3(sst)))
+        return r;
+    return dev_err(launch_assemble(grid_dev(*g), *p, *f, *a, view<const double>(sst), net_out(*out),
+                                   as_stream(stream)));
+}
+
+int ne_apply_air_sea_radiation_f64(ne_ctx* c, const ne_grid* g, const ne_params* p, ne_view sw, ne_view lw,
+                                   ne_view Ts, const ne_net_ocean_fluxes* net, const ne_radiative_fluxes* out,
+                                   void* stream) {
+    if (!c || !out) return NE_ERR_NULL;
+    int r;
+    if ((r = check_grid(g)) || (r = check_params(p)) || (r = check_net(net)) || (r = check_view2(sw)) ||
+        (r = check_view2(lw)) || (r = check_view2(Ts)))
+        return r;
+    return dev_err(launch_radiation(grid_dev(*g), *p, view<const double>(sw), view<const double>(lw),
+                                    view<const double>(Ts), net_out(*net), rad_out(*out), as_stream(stream)));
+}
+
+int ne_tracer_tendency_weno5_f64(ne_ctx* c, const ne_grid* g, const ne_ocean_state* o,
+                                 const ne_net_ocean_fluxes* net, ne_view GT, ne_view GS, void* stream) {
+    if (!c) return NE_ERR_NULL;
+    int r;
+    if ((r = check_grid(g)) || (r = check_ocean(o)) || (r = check_net(net)) || (r = check_view3(GT)) ||
+        (r = check_view3(GS)))
+        return r;
+    return dev_err(launch_tendency(grid_dev(*g), view<const double>(o->u), view<const double>(o->v),
+                                   view<const double>(o->w), view<const double>(o->T), view<const double>(o->S),
+                                   view<const double>(net->J_T), view<const double>(net->J_S),
+                                   view<const double>(net->J_w), view<double>(GT), view<double>(GS),
+                                   as_stream(stream)));
+}
+
+int ne_tracer_rk_stage_f64(ne_ctx* c, const ne_grid* g, ne_view u, ne_view v, ne_view w, ne_view T0, ne_view S0,
+                           ne_view Tin, ne_view Sin, ne_view Tout, ne_view Sout, const ne_net_ocean_fluxes* net,
+                           double beta_dt, void* stream) {
+    if (!c) return NE_ERR_NULL;
+    int r;
+    if ((r = check_grid(g)) || (r = check_net(net))) return r;
+    const ne_view* vs[] = {&u, &v, &w, &T0, &S0, &Tin, &Sin, &Tout, &Sout};
+    for (const ne_view* x : vs)
+        if ((r = check_view3(*x))) return r;
+    if (Tin.data == Tout.data || Sin.data == Sout.data) return NE_ERR_SHAPE;
+    return dev_err(launch_rk_stage<double>(grid_dev(*g), view<const double>(u), view<const double>(v),
+                                           view<const double>(w), view<const double>(T0), view<const double>(S0),
+                                           view<const double>(Tin), view<const double>(Sin), view<double>(Tout),
+                                           view<double>(Sout), view<const double>(net->J_T),
+                                           view<const double>(net->J_S), view<const double>(net->J_w), beta_dt,
+                                           as_stream(stream)));
+}
+
+int ne_budget_f64(ne_ctx* c, const ne_grid* g, ne_view cv, double weight, double* out, void* stream) {
+    if (!c || !out) return NE_ERR_NULL;
+    int r;
+    if ((r = check_grid(g)) || (r = check_view3(cv))) return r;
+    return dev_err(launch_budget<double>(c, grid_dev(*g), view<const double>(cv), weight, out, as_stream(stream)));
+}
+
+int ne_budget_f32(ne_ctx* c, const ne_grid* g, ne_view cv, double weight, double* out, void* stream) {
+    if (!c || !out) return NE_ERR_NULL;
+    int r;
+    if ((r = check_grid(g)) || (r = check_view3(cv))) return r;
+    return dev_err(launch_budget<float>(c, grid_dev(*g), view<const float>(cv), weight, out, as_stream(stream)));
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Model
+// ---------------------------------------------------------------------------
+
+// ev (optional): ev[3] is recorded before K1 by the caller, ev[4] after K1 and ev[5] after K2.
+static int model_update_state(ne_model* m, double dt_frazil, hipStream_t s, hipEvent_t* ev = nullptr) {
+    ne_model_desc& d = m->d;
+    GridDev g = grid_dev(d.grid);
+    d.atmosphere.time = time_indices(m->atimes, m->time);
+    d.radiation.time = time_indices(m->rtimes, m->time);
+    FluxOut f = flux_out(d.fluxes);
+    NetOut n = net_out(d.net);
+    RadOut ro = rad_out(d.radiative);
+    View3<double> frazil = view<double>(d.frazil_heat);
+    hipError_t e;
+    if (d.precision == NE_FP64) {
+        AtmosDev<double> a = atmos_dev<double>(d.atmosphere);
+        RadDev<double> r = rad_dev<double>(d.radiation);
+        e = launch_surface_solve<double>(g, d.params, a, d.atmos_fi, d.atmos_fj, view<const double>(d.ocean.T),
+                                         view<const double>(d.ocean.S), view<const double>(d.ocean.u),
+                                         view<const double>(d.ocean.v), f, m->ctx->counters, s);
+        if (e != hipSuccess) return NE_ERR_DEVICE;
+        if (ev && hipEventRecord(ev[4], s) != hipSuccess) return NE_ERR_DEVICE;
+        e = launch_surface_finalize<double>(g, d.params, a, d.atmos_fi, d.atmos_fj, r, d.radiation_fi,
+                                            d.radiation_fj, view<double>(d.ocean.T), view<const double>(d.ocean.S),
+                                            f, n, ro, frazil, dt_frazil, s);
+    } else {
+        AtmosDev<float> a = atmos_dev<float>(d.atmosphere);
+        RadDev<float> r = rad_dev<float>(d.radiation);
+        e = launch_surface_solve<float>(g, d.params, a, d.atmos_fi, d.atmos_fj, view<const float>(d.ocean.T),
+                                        view<const float>(d.ocean.S), view<const float>(d.ocean.u),
+                                        view<const float>(d.ocean.v), f, m->ctx->counters, s);
+        if (e != hipSuccess) return NE_ERR_DEVICE;
+        if (ev && hipEventRecord(ev[4], s) != hipSuccess) return NE_ERR_DEVICE;
+        e = launch_surface_finalize<float>(g, d.params, a, d.atmos_fi, d.atmos_fj, r, d.radiation_fi,
+                                           d.radiation_fj, view<float>(d.ocean.T), view<const float>(d.ocean.S), f,
+                                           n, ro, frazil, dt_frazil, s);
+    }
+    if (e == hipSuccess && ev && hipEventRecord(ev[5], s) != hipSuccess) return NE_ERR_DEVICE;
+    return dev_err(e);
+}
+
+template <typename Real>
+static int model_ocean_step(ne_model* m, double dt, hipStream_t s, hipEvent_t* ev = nullptr) {
+    ne_model_desc& d = m->d;
+    GridDev g = grid_dev(d.grid);
+    View3<const Real> u = view<const Real>(d.ocean.u), v = view<const Real>(d.ocean.v),
+                      w = view<const Real>(d.ocean.w);
+    View3<const Real> T0 = view<const Real>(d.ocean.T), S0 = view<const Real>(d.ocean.S);
+    View3<const double> JT = view<const double>(d.net.J_T), JS = view<const double>(d.net.J_S),
+                        Jw = view<const double>(d.net.J_w);
+    const double beta[3] = {1.0 / 3.0, 0.5, 1.0};
+    hipError_t e;
+    if (ev && hipEventRecord(ev[0], s) != hipSuccess) return NE_ERR_DEVICE;
+    e = launch_rk_stage<Real>(g, u, v, w, T0, S0, T0, S0, view<Real>(m->vT1), view<Real>(m->vS1), JT, JS, Jw,
+                              beta[0] * dt, s);
+    if (e != hipSuccess) return NE_ERR_DEVICE;
+    if (ev && hipEventRecord(ev[1], s) != hipSuccess) return NE_ERR_DEVICE;
+    e = launch_rk_stage<Real>(g, u, v, w, T0, S0, view<const Real>(m->vT1), view<const Real>(m->vS1),
+                              view<Real>(m->vT2), view<Real>(m->vS2), JT, JS, Jw, beta[1] * dt, s);
+    if (e != hipSuccess) return NE_ERR_DEVICE;
+    if (ev && hipEventRecord(ev[2], s) != hipSuccess) return NE_ERR_DEVICE;
+    e = launch_rk_stage<Real>(g, u, v, w, T0, S0, view<const Real>(m->vT2), view<const Real>(m->vS2),
+                              view<Real>(d.ocean.T), view<Real>(d.ocean.S), JT, JS, Jw, beta[2] * dt, s);
+    if (e != hipSuccess) return NE_ERR_DEVICE;
+    if (ev && hipEventRecord(ev[3], s) != hipSuccess) return NE_ERR_DEVICE;
+    return NE_OK;
+}
+
+extern "C" {
+
+int ne_model_create(ne_ctx* c, const ne_model_desc* desc, ne_model** out, void* stream) {
+    if (!c || !desc || !out) return NE_ERR_NULL;
+    *out = nullptr;
+    int r;
+    if ((r = check_grid(&desc->grid)) || (r = check_params(&desc->params)) || (r = check_atmos(&desc->atmosphere)) ||
+        (r = check_ocean(&desc->ocean)) || (r = check_fluxes(&desc->fluxes)) || (r = check_net(&desc->net)))
+        return r;
+    if (desc->params.radiation && (r = check_rad(&desc->radiation))) return r;
+    if (desc->precision != NE_FP64 && desc->precision != NE_MIXED_FP32) return NE_ERR_UNSUPPORTED;
+    if (desc->fuse_freeze_clamp != 0) return NE_ERR_UNSUPPORTED;
+    if (!desc->atmos_fi || !desc->atmos_fj) return NE_ERR_NULL;
+    if (desc->params.radiation && (!desc->radiation_fi || !desc->radiation_fj)) return NE_ERR_NULL;
+    if (desc->n_atmosphere_times < 0 || desc->n_radiation_times < 0) return NE_ERR_SHAPE;
+    if (desc->n_atmosphere_times > 0 && !desc->atmosphere_times) return NE_ERR_NULL;
+    if (desc->n_radiation_times > 0 && !desc->radiation_times) return NE_ERR_NULL;
+
+    ne_model* m = new (std::nothrow) ne_model();
+    if (!m) return NE_ERR_CAPACITY;
+    m->ctx = c;
+    m->d = *desc;
+    m->atimes.assign(desc->atmosphere_times, desc->atmosphere_times + desc->n_atmosphere_times);
+    m->rtimes.assign(desc->radiation_times, desc->radiation_times + desc->n_radiation_times);
+    const ne_grid& g = desc->grid;
+    size_t esize = desc->precision == NE_FP64 ? sizeof(double) : sizeof(float);
+    size_t n = (size_t)g.Nx * g.Ny * g.Nz * esize;
+    void** bufs[4] = {&m->T1, &m->S1, &m->T2, &m->S2};
+    ne_view* views[4] = {&m->vT1, &m->vS1, &m->vT2, &m->vS2};
+    for (int b = 0; b < 4; b++) {
+        if (hipMalloc(bufs[b], n) != hipSuccess) {
+            ne_model_destroy(m);
+            return NE_ERR_DEVICE;
+        }
+        views[b]->data = *bufs[b];
+        views[b]->sx = 1;
+        views[b]->sy = g.Nx;
+        views[b]->sz = (int64_t)g.Nx * g.Ny;
+    }
+    // reconcile_state!: fluxes are computed at construction (earth_system_model.jl:86-90)
+    r = model_update_state(m, INFINITY, as_stream(stream));
+    if (r) {
+        ne_model_destroy(m);
+        return r;
+    }
+    *out = m;
+    return NE_OK;
+}
+
+int ne_model_destroy(ne_model* m) {
+    if (!m) return NE_ERR_NULL;
+    int r = NE_OK;
+    void* bufs[4] = {m->T1, m->S1, m->T2, m->S2};
+    for (void* b : bufs)
+        if (b && hipFree(b) != hipSuccess) r = NE_ERR_DEVICE;
+    for (hipEvent_t e : m->events) (void)hipEventDestroy(e);
+    delete m;
+    return r;
+}
+
+int ne_model_time_step(ne_model* m, double dt, void* stream) {
+    if (!m) return NE_ERR_NULL;
+    if (!(dt > 0)) return NE_ERR_SHAPE;
+    hipStream_t s = as_stream(stream);
+    hipEvent_t* ev = nullptr;
+    if (m->timing && m->recorded < m->max_steps) ev = &m->events[(size_t)m->recorded * 6];
+    int r = m->d.precision == NE_FP64 ? model_ocean_step<double>(m, dt, s, ev) : model_ocean_step<float>(m, dt, s, ev);
+    if (r) return r;
+    m->time += dt;
+    m->iteration += 1;
+    m->last_dt = dt;
+    r = model_update_state(m, dt, s, ev);
+    if (!r && ev) m->recorded += 1;
+    return r;
+}
+
+int ne_model_timing_begin(ne_model* m, int max_steps) {
+    if (!m) return NE_ERR_NULL;
+    if (max_steps <= 0 || max_steps > (1 << 20)) return NE_ERR_CAPACITY;
+    for (hipEvent_t e : m->events) (void)hipEventDestroy(e);
+    m->events.assign((size_t)max_steps * 6, nullptr);
+    for (auto& e : m->events)
+        if (hipEventCreate(&e) != hipSuccess) return NE_ERR_DEVICE;
+    m->max_steps = max_steps;
+    m->recorded = 0;
+    m->timing = true;
+    return NE_OK;
+}
+
+int ne_model_timing_end(ne_model* m, double* ms, int* steps) {
+    if (!m || !ms) return NE_ERR_NULL;
+    if (!m->timing) return NE_ERR_STATE;
+    for (int k = 0; k < NE_TIMED_KERNELS; k++) ms[k] = 0.0;
+    for (int n = 0; n < m->recorded; n++) {
+        hipEvent_t* ev = &m->events[(size_t)n * 6];
+        if (hipEventSynchronize(ev[5]) != hipSuccess) return NE_ERR_DEVICE;
+        for (int k = 0; k < NE_TIMED_KERNELS; k++) {
+            float t = 0.0f;
+            if (hipEventElapsedTime(&t, ev[k], ev[k + 1]) != hipSuccess) return NE_ERR_DEVICE;
+            ms[k] += t;
+        }
+    }
+    if (steps) *steps = m->recorded;
+    m->timing = false;
+    return NE_OK;
+}
+
+int ne_model_update_state(ne_model* m, void* stream) {
+    if (!m) return NE_ERR_NULL;
+    return model_update_state(m, m->iteration == 0 ? INFINITY : m->last_dt, as_stream(stream));
+}
+
+int ne_model_clock(const ne_model* m, double* time, int64_t* iteration) {
+    if (!m) return NE_ERR_NULL;
+    if (time) *time = m->time;
+    if (iteration) *iteration = m->iteration;
+    return NE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,444 @@
+// ne_device.h — device-side building blocks shared by the HIP kernels.
+//
+// Every formula keeps the evaluation order of the reference expression it restates
+// (cited per function); the library is compiled with -ffp-contract=off so that the
+// FP64 path is the same sequence of IEEE operations as the reference arithmetic.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "numericalearth_hip.h"
+
+#define NE_CELSIUS_TO_KELVIN 273.15  // src/EarthSystemModels/components.jl:10
+#define NE_WENO_EPS 1e-8
+
+namespace ne {
+
+// ---------------------------------------------------------------------------
+// Views
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct View3 {
+    T* __restrict__ p;
+    int64_t sx, sy, sz;
+    __host__ __device__ T& operator()(int i, int j, int k) const {
+        return p[(int64_t)i * sx + (int64_t)j * sy + (int64_t)k * sz];
+    }
+    __host__ __device__ T& at2(int i, int j) const { return p[(int64_t)i * sx + (int64_t)j * sy]; }
+};
+
+template <typename T>
+__host__ inline View3<T> view(const ne_view& v) {
+    View3<T> r;
+    r.p = static_cast<T*>(v.data);
+    r.sx = v.sx;
+    r.sy = v.sy;
+    r.sz = v.sz;
+    return r;
+}
+
+// Julia's min/max on floats: NaN-propagating, -0.0 < +0.0.
+template <typename T>
+__device__ __forceinline__ T jmin(T a, T b) {
+    if (a != a || b != b) return a + b;
+    if (a < b) return a;
+    if (b < a) return b;
+    return __builtin_signbit(a) ? a : b;
+}
+template <typename T>
+__device__ __forceinline__ T jmax(T a, T b) {
+    if (a != a || b != b) return a + b;
+    if (a > b) return a;
+    if (b > a) return b;
+    return __builtin_signbit(a) ? b : a;
+}
+
+__host__ __device__ __forceinline__ int wrapi(int i, int n) {
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+// Grid geometry as seen by kernels (device pointers, passed by value).
+struct GridDev {
+    int Nx, Ny, Nz;
+    int tx, ty;  // topology
+    int advection;
+    const double* __restrict__ dz;
+    const double* __restrict__ zc;
+    const double* __restrict__ dy_xface;
+    const double* __restrict__ dx_yface;
+    const double* __restrict__ Az;
+    const double* __restrict__ lambda_c;
+    const double* __restrict__ phi_c;
+    const int32_t* __restrict__ kmin;
+
+    // Column index after periodic wrapping; -1 outside a bounded direction.
+    __device__ __forceinline__ int64_t column(int i, int j) const {
+        if (tx == NE_PERIODIC) i = wrapi(i, Nx);
+        else if (i < 0 || i >= Nx) return -1;
+        if (ty == NE_PERIODIC) j = wrapi(j, Ny);
+        else if (j < 0 || j >= Ny) return -1;
+        return (int64_t)i + (int64_t)j * Nx;
+    }
+    // Complement of inactive_node at (Center, Center, Center).
+    __device__ __forceinline__ bool active(int i, int j, int k) const {
+        if (k < 0 || k >= Nz) return false;
+        int64_t c = column(i, j);
+        return c >= 0 && k >= kmin[c];
+    }
+    __device__ __forceinline__ int wx(int i) const { return tx == NE_PERIODIC ? wrapi(i, Nx) : i; }
+    __device__ __forceinline__ int wy(int j) const { return ty == NE_PERIODIC ? wrapi(j, Ny) : j; }
+};
+
+__host__ inline GridDev grid_dev(const ne_grid& g) {
+    GridDev d;
+    d.Nx = g.Nx; d.Ny = g.Ny; d.Nz = g.Nz;
+    d.tx = g.topology_x; d.ty = g.topology_y;
+    d.advection = g.advection;
+    d.dz = g.dz; d.zc = g.z_center; d.dy_xface = g.dy_xface; d.dx_yface = g.dx_yface;
+    d.Az = g.Az; d.lambda_c = g.lambda_c; d.phi_c = g.phi_c; d.kmin = g.kmin;
+    return d;
+}
+
+// ---------------------------------------------------------------------------
+// Thermodynamics (Thermodynamics.jl forms; src/Atmospheres/thermodynamic_parameters.jl)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double Rv_over_Rd(const ne_params& p) { return p.dry_air_molar_mass / p.water_molar_mass; }
+__device__ __forceinline__ double R_v(const ne_params& p) { return p.gas_constant / p.water_molar_mass; }
+__device__ __forceinline__ double R_d(const ne_params& p) { return p.gas_constant / p.dry_air_molar_mass; }
+__device__ __forceinline__ double cp_d(const ne_params& p) { return R_d(p) / p.dry_air_adiabatic_exponent; }
+
+__device__ __forceinline__ double gas_constant_air(const ne_params& p, double q) {
+    double eps = Rv_over_Rd(p);
+    return R_d(p) * ((1.0 + (eps - 1.0) * q) - eps * (0.0 + 0.0));
+}
+__device__ __forceinline__ double cp_m(const ne_params& p, double q) {
+    double cpd = cp_d(p);
+    return cpd + (p.water_vapor_heat_capacity - cpd) * q;
+}
+__device__ __forceinline__ double air_density(const ne_params& p, double T, double pa, double q) {
+    return pa / (gas_constant_air(p, q) * T);
+}
+__device__ __forceinline__ double latent_heat_vapor(const ne_params& p, double T) {
+    double dcp = p.water_vapor_heat_capacity - p.liquid_water_heat_capacity;
+    return p.reference_vaporization_enthalpy + dcp * (T - p.reference_temperature);
+}
+__device__ __forceinline__ double virtual_temperature(const ne_params& p, double T, double q) {
+    return gas_constant_air(p, q) / R_d(p) * T;
+}
+// saturation_vapor_pressure, liquid (interface_states.jl:63)
+__device__ __forceinline__ double saturation_vapor_pressure(const ne_params& p, double T) {
+    double dcp = p.water_vapor_heat_capacity - p.liquid_water_heat_capacity;
+    double Rv = R_v(p);
+    double Ttr = p.triple_point_temperature;
+    return p.triple_point_pressure * pow(T / Ttr, dcp / Rv) *
+           exp((p.reference_vaporization_enthalpy - dcp * p.reference_temperature) / Rv *
+               (1.0 / Ttr - 1.0 / T));
+}
+// surface_specific_humidity(::ImpureSaturationSpecificHumidity) (interface_states.jl:55-74)
+__device__ __forceinline__ double surface_specific_humidity(const ne_params& p, double pa, double Ts) {
+    if (p.specific_humidity == NE_HUMIDITY_FIXED) return p.fixed_specific_humidity;
+    double pv = p.water_mole_fraction * saturation_vapor_pressure(p, Ts);
+    double edv = 1.0 / Rv_over_Rd(p);
+    pv = jmin(pv, 0.999 * pa);
+    return edv * pv / (pa - (1.0 - edv) * pv);
+}
+
+// ---------------------------------------------------------------------------
+// Stability functions (similarity_theory_turbulent_fluxes.jl:501-532, :586-618).
+// Only the branch selected by sign(ζ) is evaluated: the reference computes both and
+// selects with ifelse, which gives the same value.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double psi_momentum(const ne_params& p, double z) {
+    if (p.stability_functions == NE_STABILITY_NONE) return 0.0;
+    const double* c = p.edson_momentum;
+    if (z < 0.0) {
+        double Am = c[5], Bm = c[6], Cm = c[7], Dm = c[8], Em = c[9], Fm = c[10];
+        double zm = jmin(0.0, z);
+        double f1 = sqrt(sqrt(1.0 - Am * zm));
+        double psi_m1 = Bm * log((1.0 + f1) / Bm) + log((1.0 + f1 * f1) / Bm) - Bm * atan(f1) + Cm;
+        double f2 = cbrt(1.0 - Dm * zm);
+        double psi_m2 = Em / 2.0 * log((1.0 + f2 + f2 * f2) / Em) - sqrt(Em) * atan((1.0 + 2.0 * f2) / sqrt(Em)) + Fm;
+        double f = zm * zm / (1.0 + zm * zm);
+        return (1.0 - f) * psi_m1 + f * psi_m2;
+    } else {
+        double zmax = c[0], Ap = c[1], Bp = c[2], Cp = c[3], Dp = c[4];
+        double zp = jmax(0.0, z);
+        double dz = jmin(zmax, Ap * zp);
+        return -Bp * zp - Cp * (zp - Dp) * exp(-dz) - Cp * Dp;
+    }
+}
+
+__device__ __forceinline__ double psi_scalar(const ne_params& p, double z) {
+    if (p.stability_functions == NE_STABILITY_NONE) return 0.0;
+    const double* c = p.edson_scalar;
+    if (z < 0.0) {
+        double Am = c[6], Bm = c[7], Cm = c[8], Dm = c[9], Em = c[10], Fm = c[11];
+        double zm = jmin(0.0, z);
+        double f1 = sqrt(1.0 - Am * zm);
+        double psi_m1 = Bm * log((1.0 + f1) / Bm) + Cm;
+        double f2 = cbrt(1.0 - Dm * zm);
+        double psi_m2 = Em / 2.0 * log((1.0 + f2 + f2 * f2) / Em) - sqrt(Em) * atan((1.0 + 2.0 * f2) / sqrt(Em)) + Fm;
+        double f = zm * zm / (1.0 + zm * zm);
+        return (1.0 - f) * psi_m1 + f * psi_m2;
+    } else {
+        double zmax = c[0], Ap = c[1], Bp = c[2], Cp = c[3], Dp = c[4], Ep = c[5];
+        double zp = jmax(0.0, z);
+        double dz = jmin(zmax, Ap * zp);
+        return -pow(1.0 + Bp * zp, Cp) - Bp * (zp - Dp) * exp(-dz) - Ep;
+    }
+}
+
+// similarity_profile (similarity_theory_turbulent_fluxes.jl:242-253)
+__device__ __forceinline__ double similarity_profile(const ne_params& p, bool scalar, double h, double l, double L) {
+    double zeta = h / L;
+    double psih = scalar ? psi_scalar(p, zeta) : psi_momentum(p, zeta);
+    if (p.similarity_form == NE_SIMILARITY_COARE_LOG) return log(h / l) - psih;
+    double psil = scalar ? psi_scalar(p, l / L) : psi_momentum(p, l / L);
+    return log(h / l) - psih + psil;
+}
+
+// roughness_length(::MomentumRoughnessLength) (roughness_lengths.jl:197-210)
+__device__ __forceinline__ double momentum_roughness(const ne_params& p, double ustar) {
+    if (p.momentum_roughness == NE_ROUGHNESS_CONSTANT) return p.momentum_roughness_length;
+    double nu = p.air_kinematic_viscosity;
+    double g = p.roughness_gravitational_acceleration;
+    double Cg = p.charnock_parameter;
+    double Cnu = p.smooth_wall_parameter;
+    double lW = Cg * (ustar * ustar) / g;
+    double lR = Cnu == 0.0 ? 0.0 : Cnu * nu / ustar;
+    return jmin(lW + lR, p.maximum_momentum_roughness);
+}
+
+// roughness_length(::ScalarRoughnessLength) (roughness_lengths.jl:231-246)
+__device__ __forceinline__ double scalar_roughness(const ne_params& p, int kind, double constant, double lu, double ustar) {
+    if (kind == NE_ROUGHNESS_CONSTANT) return constant;
+    double Rstar = lu * ustar / p.air_kinematic_viscosity;
+    double ls = Rstar == 0.0 ? 0.0 : p.reynolds_scaling_A / pow(Rstar, p.reynolds_scaling_b);
+    return jmin(ls, p.maximum_scalar_roughness);
+}
+
+// vsgs² (similarity_theory_turbulent_fluxes.jl:88-98)
+__device__ __forceinline__ double gustiness_squared(const ne_params& p, double ustar, double bstar, double hbl) {
+    double v2 = 0.0;
+    if (p.convective_gustiness) {
+        double Jb = -ustar * bstar;
+        double Ug = jmax(p.minimum_gustiness, p.gustiness_parameter * cbrt(jmax(0.0, Jb) * hbl));
+        v2 = Ug * Ug;
+    }
+    if (p.mesoscale_gustiness) v2 = v2 + p.mesoscale_velocity * p.mesoscale_velocity;
+    return v2;
+}
+
+// Result of one column of the air-sea flux solve.
+struct ColumnFluxes {
+    double Qv, Qt, Jv, rtx, rty, us, ts, qs, Ts_C;
+    int iterations;
+    bool converged;
+};
+
+// One column of _compute_atmosphere_ocean_interface_state! (atmosphere_ocean_fluxes.jl:80-197)
+// with compute_interface_state (compute_interface_state.jl:31-58) and
+// iterate_interface_fluxes(::SimilarityTheoryFluxes) (similarity:315-385).
+// Quantities that the reference re-evaluates each iteration from unchanged inputs
+// (qₛ, θᵃ, Δθ, Δq, 𝒯ₛ) are evaluated once: the values are identical.
+__device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p, double ua, double va, double Ta,
+                                                    double pa, double qa, double To_C, double uo, double vo,
+                                                    bool inactive) {
+    const double g = p.gravitational_acceleration;
+    const double za = p.surface_layer_height;
+    const double hbl = p.boundary_layer_height;
+    const double To = To_C + NE_CELSIUS_TO_KELVIN;
+
+    double us = 1e-4, ts = 1e-4, qs = 1e-4;
+    double su = uo, sv = vo, sT = To;
+    int it = 0;
+    const bool converge = p.stop_criteria == NE_STOP_CONVERGENCE;
+    bool reached = false;
+
+    if (!inactive && (converge || p.maxiter > 0)) {
+        // Iteration invariants: BulkTemperature keeps Tₛ = Tₒ, hence qₛ is fixed.
+        const double Ts = To;
+        const double qsurf = surface_specific_humidity(p, pa, Ts);
+        const double dq = qa - qsurf;
+        const double thetaa = Ta + g * za / cp_m(p, qa);
+        const double dtheta = thetaa - Ts;
+        const double Tv = virtual_temperature(p, Ts, qsurf);
+        const double delta = Rv_over_Rd(p) - 1.0;
+        const double onedq = 1.0 + delta * qsurf;
+        const double dTv = delta * Tv;
+        const double goTv = g / Tv;
+        double du, dv;
+        if (p.velocity_difference == NE_VELOCITY_RELATIVE) { du = ua - su; dv = va - sv; }
+        else { du = ua; dv = va; }
+        const double du2dv2 = du * du + dv * dv;
+        const double kappa = p.von_karman_constant;
+        const bool shared_scalar = p.temperature_roughness == p.water_vapor_roughness &&
+                                   (p.temperature_roughness == NE_ROUGHNESS_EDSON ||
+                                    p.temperature_roughness_length == p.water_vapor_roughness_length);
+        const int maxit = p.maxiter;
+        const double tol = p.tolerance;
+
+        for (;;) {
+            // (iteration 0 always iterates; ConvergenceStopCriteria / FixedIterations tests below)
+            double bstar = goTv * (ts * onedq + dTv * qs);
+            double Usg2 = gustiness_squared(p, us, bstar, hbl);
+            double U = sqrt(du2dv2 + Usg2);
+            double lu = momentum_roughness(p, us);
+            double lq = scalar_roughness(p, p.water_vapor_roughness, p.water_vapor_roughness_length, lu, us);
+            double lt = shared_scalar ? lq
+                                      : scalar_roughness(p, p.temperature_roughness, p.temperature_roughness_length, lu, us);
+            double dh = jmax(za - p.zero_plane_displacement, 2.0 * lu);
+            double L = bstar == 0.0 ? (double)INFINITY : us * us / (kappa * bstar);
+            double chiu = kappa / similarity_profile(p, false, dh, lu, L);
+            double chiq = kappa / similarity_profile(p, true, dh, lq, L);
+            double chit = shared_scalar ? chiq : kappa / similarity_profile(p, true, dh, lt, L);
+            double nus = chiu * U, nts = chit * dtheta, nqs = chiq * dq;
+            it += 1;
+            bool go;
+            if (converge) {
+                double drift = fabs(nus - us) + fabs(nts - ts) + fabs(nqs - qs);
+                bool conv = drift < tol;
+                reached = it >= maxit;
+                go = !(conv | reached);
+            } else {
+                go = it < maxit;
+            }
+            us = nus; ts = nts; qs = nqs;
+            if (!go) break;
+        }
+        sT = Ts;
+    }
+    if (inactive) {  // zero_interface_state (interface_states.jl:800-803)
+        if (!converge) it = p.maxiter > 0 ? p.maxiter : 0;  // FixedIterations iterates, then zeroes
+        us = 0.0; ts = 0.0; qs = 0.0; su = 0.0; sv = 0.0; sT = NE_CELSIUS_TO_KELVIN;
+    }
+
+    double du, dv;
+    if (p.velocity_difference == NE_VELOCITY_RELATIVE) { du = ua - su; dv = va - sv; }
+    else { du = ua; dv = va; }
+    double dU = sqrt(du * du + dv * dv);
+    double taux = dU == 0.0 ? 0.0 : -(us * us) * du / dU;
+    double tauy = dU == 0.0 ? 0.0 : -(us * us) * dv / dU;
+    double rho = air_density(p, Ta, pa, qa);
+    double cpm = cp_m(p, qa);
+    double Lv = latent_heat_vapor(p, Ta);
+    ColumnFluxes r;
+    r.Qv = -rho * Lv * us * qs;
+    r.Qt = -rho * cpm * us * ts;
+    r.Jv = -rho * us * qs;
+    r.rtx = rho * taux;
+    r.rty = rho * tauy;
+    r.us = us;
+    r.ts = ts;
+    r.qs = qs;
+    r.Ts_C = sT - NE_CELSIUS_TO_KELVIN;
+    r.iterations = it;
+    r.converged = !(converge && reached);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Interpolation (interpolate_atmospheric_state.jl:91-182)
+// ---------------------------------------------------------------------------
+
+struct Neighbours {
+    int lo, hi;
+    double w;
+};
+
+__device__ __forceinline__ Neighbours neighbours(double f, int n, int topo) {
+    Neighbours r;
+    if (topo == NE_FLAT || n == 1) { r.lo = 0; r.hi = 0; r.w = 0.0; return r; }
+    double fl = floor(f);
+    int i0 = (int)fl;
+    r.w = f - fl;
+    int i1 = i0 + 1;
+    if (topo == NE_PERIODIC) { i0 = wrapi(i0, n); i1 = wrapi(i1, n); }
+    else {
+        i0 = i0 < 0 ? 0 : (i0 > n - 1 ? n - 1 : i0);
+        i1 = i1 < 0 ? 0 : (i1 > n - 1 ? n - 1 : i1);
+    }
+    r.lo = i0; r.hi = i1;
+    return r;
+}
+
+// Bilinear in (i, j), linear in time; Real is the storage type of the source field.
+template <typename Real>
+__device__ __forceinline__ double interp_field(const View3<const Real>& d, const Neighbours& x,
+                                               const Neighbours& y, const ne_time_interp& t) {
+    if (d.p == nullptr) return 0.0;
+    double xi = x.w, eta = y.w;
+    double w00 = (1.0 - xi) * (1.0 - eta);
+    double w01 = (1.0 - xi) * eta;
+    double w10 = xi * (1.0 - eta);
+    double w11 = xi * eta;
+    double a = w00 * (double)d(x.lo, y.lo, t.n1) + w01 * (double)d(x.lo, y.hi, t.n1) +
+               w10 * (double)d(x.hi, y.lo, t.n1) + w11 * (double)d(x.hi, y.hi, t.n1);
+    double b = w00 * (double)d(x.lo, y.lo, t.n2) + w01 * (double)d(x.lo, y.hi, t.n2) +
+               w10 * (double)d(x.hi, y.lo, t.n2) + w11 * (double)d(x.hi, y.hi, t.n2);
+    return (1.0 - t.tilde_n) * a + t.tilde_n * b;
+}
+
+// ---------------------------------------------------------------------------
+// WENO reconstructions (Jiang & Shu 1996 smoothness indicators, Borges et al. 2008
+// WENO-Z weights). Values are ordered along the upwind direction.
+// ---------------------------------------------------------------------------
+
+template <typename R>
+__device__ __forceinline__ R weno5z(R c0, R c1, R c2, R c3, R c4) {
+    const R eps = (R)NE_WENO_EPS;
+    R q0 = ((R)2 * c0 - (R)7 * c1 + (R)11 * c2) / (R)6;
+    R q1 = (-c1 + (R)5 * c2 + (R)2 * c3) / (R)6;
+    R q2 = ((R)2 * c2 + (R)5 * c3 - c4) / (R)6;
+    R d0 = c0 - (R)2 * c1 + c2, e0 = c0 - (R)4 * c1 + (R)3 * c2;
+    R d1 = c1 - (R)2 * c2 + c3, e1 = c1 - c3;
+    R d2 = c2 - (R)2 * c3 + c4, e2 = (R)3 * c2 - (R)4 * c3 + c4;
+    R b0 = (R)(13.0 / 12.0) * (d0 * d0) + (R)0.25 * (e0 * e0);
+    R b1 = (R)(13.0 / 12.0) * (d1 * d1) + (R)0.25 * (e1 * e1);
+    R b2 = (R)(13.0 / 12.0) * (d2 * d2) + (R)0.25 * (e2 * e2);
+    R tau = fabs(b0 - b2);
+    R r0 = tau / (b0 + eps), r1 = tau / (b1 + eps), r2 = tau / (b2 + eps);
+    R a0 = (R)0.1 * ((R)1 + r0 * r0);
+    R a1 = (R)0.6 * ((R)1 + r1 * r1);
+    R a2 = (R)0.3 * ((R)1 + r2 * r2);
+    R s = a0 + a1 + a2;
+    return (a0 * q0 + a1 * q1 + a2 * q2) / s;
+}
+
+template <typename R>
+__device__ __forceinline__ R weno3z(R c0, R c1, R c2) {
+    const R eps = (R)NE_WENO_EPS;
+    R q0 = (-c0 + (R)3 * c1) / (R)2;
+    R q1 = (c1 + c2) / (R)2;
+    R d0 = c1 - c0, d1 = c2 - c1;
+    R b0 = d0 * d0, b1 = d1 * d1;
+    R tau = fabs(b0 - b1);
+    R r0 = tau / (b0 + eps), r1 = tau / (b1 + eps);
+    R a0 = (R)(1.0 / 3.0) * ((R)1 + r0 * r0);
+    R a1 = (R)(2.0 / 3.0) * ((R)1 + r1 * r1);
+    return (a0 * q0 + a1 * q1) / (a0 + a1);
+}
+
+// Reduced order at a face between cells m-1 and m from the activity bits of cells
+// m-3 … m+2 (bit n ↔ cell m-3+n): 0 peripheral, 1 upwind, 2 WENO3, 3 WENO5.
+__device__ __forceinline__ int face_order(unsigned act) {
+    if ((act & 0x0Cu) != 0x0Cu) return 0;
+    if ((act & 0x1Eu) != 0x1Eu) return 1;
+    if ((act & 0x3Fu) != 0x3Fu) return 2;
+    return 3;
+}
+
+template <typename R>
+__device__ __forceinline__ R face_value(const R c[6], int order, R vel) {
+    bool left = vel >= (R)0;
+    if (order == 3) return left ? weno5z(c[0], c[1], c[2], c[3], c[4]) : weno5z(c[5], c[4], c[3], c[2], c[1]);
+    if (order == 2) return left ? weno3z(c[1], c[2], c[3]) : weno3z(c[4], c[3], c[2]);
+    return left ? c[2] : c[3];
+}
+
+}  // namespace ne

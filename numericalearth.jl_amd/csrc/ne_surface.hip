@@ -1,0 +1,505 @@
+// ne_surface.hip — surface phases of the coupled step on gfx950.
+//
+// Layout: one work-item per ocean column (i, j); blocks of 64×4 with i fastest so
+// that each wavefront reads 64 consecutive columns of every 2-D field (coalesced).
+// The flux solve is FP64-VALU bound (transcendental-heavy fixed point); lanes of a
+// wavefront retire independently and the wavefront leaves the loop when its last
+// lane has converged (the loop condition is the wavefront-wide vote).
+#include "ne_internal.h"
+
+namespace ne {
+
+namespace {
+
+constexpr int BX = 64;
+constexpr int BY = 4;
+
+inline dim3 grid2d(int Nx, int Ny) { return dim3((Nx + BX - 1) / BX, (Ny + BY - 1) / BY); }
+
+// Block-wide reduction of the solver counters, then one atomic per block.
+__device__ void accumulate_counters(ne_counters* c, bool solved, int it, bool nonconv, bool nan) {
+    __shared__ unsigned s_it[BX * BY], s_solved[BX * BY], s_nc[BX * BY], s_nan[BX * BY], s_max[BX * BY];
+    int t = threadIdx.x + threadIdx.y * blockDim.x;
+    int n = blockDim.x * blockDim.y;
+    s_it[t] = solved ? (unsigned)it : 0u;
+    s_solved[t] = solved ? 1u : 0u;
+    s_nc[t] = (solved && nonconv) ? 1u : 0u;
+    s_nan[t] = (solved && nan) ? 1u : 0u;
+    s_max[t] = solved ? (unsigned)it : 0u;
+    __syncthreads();
+    for (int w = n / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            s_it[t] += s_it[t + w];
+            s_solved[t] += s_solved[t + w];
+            s_nc[t] += s_nc[t + w];
+            s_nan[t] += s_nan[t + w];
+            s_max[t] = s_max[t] > s_max[t + w] ? s_max[t] : s_max[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0 && c != nullptr && s_solved[0] > 0) {
+        atomicAdd((unsigned long long*)&c->total_iterations, (unsigned long long)s_it[0]);
+        atomicAdd(&c->solved_columns, s_solved[0]);
+        if (s_nc[0]) atomicAdd(&c->nonconverged_columns, s_nc[0]);
+        if (s_nan[0]) atomicAdd(&c->nan_columns, s_nan[0]);
+        atomicMax(&c->max_iterations, s_max[0]);
+    }
+}
+
+__device__ __forceinline__ void store_fluxes(const FluxOut& o, int i, int j, const ColumnFluxes& r) {
+    o.Qv.at2(i, j) = r.Qv;
+    o.Qt.at2(i, j) = r.Qt;
+    o.Jv.at2(i, j) = r.Jv;
+    o.rtx.at2(i, j) = r.rtx;
+    o.rty.at2(i, j) = r.rty;
+    o.us.at2(i, j) = r.us;
+    o.ts.at2(i, j) = r.ts;
+    o.qs.at2(i, j) = r.qs;
+    o.Ts.at2(i, j) = r.Ts_C;
+    if (o.iters.p) o.iters.at2(i, j) = r.iterations;
+}
+
+__device__ __forceinline__ bool finite3(double a, double b, double c) {
+    return isfinite(a) && isfinite(b) && isfinite(c);
+}
+
+// ---------------------------------------------------------------------------
+
+__global__ void immersed_levels_kernel(GridDev g, View3<const double> bottom, int32_t* kmin) {
+    int i = blockIdx.x * BX + threadIdx.x;
+    int j = blockIdx.y * BY + threadIdx.y;
+    if (i >= g.Nx || j >= g.Ny) return;
+    double h = bottom.at2(i, j);
+    int k = 0;
+    while (k < g.Nz && g.zc[k] <= h) k++;
+    kmin[i + (int64_t)j * g.Nx] = k;
+}
+
+__global__ void fractional_indices_kernel(GridDev g, ne_source_grid s, double* fi, double* fj) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    double lc0 = s.lambda0 + 0.5 * s.dlambda;
+    double pc0 = s.phi0 + 0.5 * s.dphi;
+    if (n < g.Nx) fi[n] = s.topology_x == NE_FLAT ? 0.0 : (g.lambda_c[n] - lc0) / s.dlambda;
+    if (n < g.Ny) fj[n] = s.topology_y == NE_FLAT ? 0.0 : (g.phi_c[n] - pc0) / s.dphi;
+}
+
+template <typename Real>
+__global__ void interp_atmos_kernel(GridDev g, AtmosDev<Real> a, const double* fi, const double* fj,
+                                    View3<double> u, View3<double> v, View3<double> T, View3<double> p,
+                                    View3<double> q, View3<double> rain, View3<double> snow) {
+    int i = blockIdx.x * BX + threadIdx.x;
+    int j = blockIdx.y * BY + threadIdx.y;
+    if (i >= g.Nx || j >= g.Ny) return;
+    Neighbours x = neighbours(fi[i], a.Nx, a.tx);
+    Neighbours y = neighbours(fj[j], a.Ny, a.ty);
+    u.at2(i, j) = interp_field(a.u, x, y, a.time);
+    v.at2(i, j) = interp_field(a.v, x, y, a.time);
+    T.at2(i, j) = interp_field(a.T, x, y, a.time);
+    p.at2(i, j) = interp_field(a.p, x, y, a.time);
+    q.at2(i, j) = interp_field(a.q, x, y, a.time);
+    if (rain.p) rain.at2(i, j) = interp_field(a.rain, x, y, a.time);
+    if (snow.p) snow.at2(i, j) = interp_field(a.snow, x, y, a.time);
+}
+
+template <typename Real>
+__global__ void interp_radiation_kernel(GridDev g, RadDev<Real> r, const double* fi, const double* fj,
+                                        View3<double> sw, View3<double> lw) {
+    int i = blockIdx.x * BX + threadIdx.x;
+    int j = blockIdx.y * BY + threadIdx.y;
+    if (i >= g.Nx || j >= g.Ny) return;
+    Neighbours x = neighbours(fi[i], r.Nx, r.tx);
+    Neighbours y = neighbours(fj[j], r.Ny, r.ty);
+    sw.at2(i, j) = interp_field(r.sw, x, y, r.time);
+    lw.at2(i, j) = interp_field(r.lw, x, y, r.time);
+}
+
+// Ocean surface values of a column: T, S at k = Nz and ℑxᶜᵃᵃ u, ℑyᵃᶜᵃ v
+// (atmosphere_ocean_fluxes.jl:62-71).
+template <typename Real>
+__device__ __forceinline__ void ocean_surface(const GridDev& g, const View3<const Real>& T, const View3<const Real>& S,
+                                              const View3<const Real>& u, const View3<const Real>& v, int i, int j,
+                                              double& To, double& So, double& uo, double& vo) {
+    int kN = g.Nz - 1;
+    To = (double)T(i, j, kN);
+    So = (double)S(i, j, kN);
+    int ip = g.tx == NE_PERIODIC ? wrapi(i + 1, g.Nx) : i + 1;
+    int jp = g.ty == NE_PERIODIC ? wrapi(j + 1, g.Ny) : j + 1;
+    uo = ((double)u(i, j, kN) + (double)u(ip, j, kN)) / 2.0;
+    vo = ((double)v(i, j, kN) + (double)v(i, jp, kN)) / 2.0;
+}
+
+__global__ void flux_solve_exchange_kernel(GridDev g, ne_params p, View3<const double> ua, View3<const double> va,
+                                           View3<const double> Ta, View3<const double> pa, View3<const double> qa,
+                                           View3<const double> T, View3<const double> S, View3<const double> u,
+                                           View3<const double> v, FluxOut out, ne_counters* counters) {
+    int i = blockIdx.x * BX + threadIdx.x;
+    int j = blockIdx.y * BY + threadIdx.y;
+    bool in = i < g.Nx && j < g.Ny;
+    ColumnFluxes r;
+    bool inactive = true;
+    if (in) {
+        double To, So, uo, vo;
+        ocean_surface<double>(g, T, S, u, v, i, j, To, So, uo, vo);
+        inactive = !g.active(i, j, g.Nz - 1);
+        r = flux_column(p, ua.at2(i, j), va.at2(i, j), Ta.at2(i, j), pa.at2(i, j), qa.at2(i, j), To, uo, vo,
+                        inactive);
+        store_fluxes(out, i, j, r);
+    }
+    bool solved = in && !inactive;
+    accumulate_counters(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
+                        solved && !finite3(r.us, r.ts, r.qs));
+}
+
+// K1 of the fused update_state: interpolate the atmosphere at the column and solve.
+template <typename Real>
+__global__ void __launch_bounds__(BX* BY)
+surface_solve_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __restrict__ fi,
+                     const double* __restrict__ fj, View3<const Real> T, View3<const Real> S, View3<const Real> u,
+                     View3<const Real> v, FluxOut out, ne_counters* counters) {
+    int i = blockIdx.x * BX + threadIdx.x;
+    int j = blockIdx.y * BY + threadIdx.y;
+    bool in = i < g.Nx && j < g.Ny;
+    ColumnFluxes r;
+    bool inactive = true;
+    if (in) {
+        Neighbours x = neighbours(fi[i], a.Nx, a.tx);
+        Neighbours y = neighbours(fj[j], a.Ny, a.ty);
+        double ua = interp_field(a.u, x, y, a.time);
+        double va = interp_field(a.v, x, y, a.time);
+        double Ta = interp_field(a.T, x, y, a.time);
+        double pa = interp_field(a.p, x, y, a.time);
+        double qa = interp_field(a.q, x, y, a.time);
+        double To, So, uo, vo;
+        ocean_surface<Real>(g, T, S, u, v, i, j, To, So, uo, vo);
+        inactive = !g.active(i, j, g.Nz - 1);
+        r = flux_column(p, ua, va, Ta, pa, qa, To, uo, vo, inactive);
+        store_fluxes(out, i, j, r);
+    }
+    bool solved = in && !inactive;
+    accumulate_counters(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
+                        solved && !finite3(r.us, r.ts, r.qs));
+}
+
+// Freeze clamp of one column (freezing_limited_ocean_temperature.jl:95-118); returns frazil heat.
+template <typename Real>
+__device__ __forceinline__ double freeze_column(const GridDev& g, const ne_params& p, const View3<Real>& T,
+                                                const View3<const Real>& S, int i, int j, double dt) {
+    double rho = p.ocean_reference_density, c = p.ocean_heat_capacity;
+    double dQ = 0.0;
+    for (int k = g.Nz - 1; k >= 0; k--) {
+        double dz = g.dz[k];
+        double Tk = (double)T(i, j, k), Sk = (double)S(i, j, k);
+        double Tm = p.freshwater_melting_temperature - p.liquidus_slope * Sk;
+        bool freezing = Tk < Tm;
+        double dE = freezing ? rho * c * (Tm - Tk) : 0.0;
+        if (freezing) T(i, j, k) = (Real)Tm;
+        dQ = dQ - dE * dz / dt;
+    }
+    return dQ;
+}
+
+__global__ void freeze_clamp_kernel(GridDev g, ne_params p, View3<double> T, View3<const double> S,
+                                    View3<double> frazil, double dt) {
+    int i = blockIdx.x * BX + threadIdx.x;
+    int j = blockIdx.y * BY + threadIdx.y;
+    if (i >= g.Nx || j >= g.Ny) return;
+    frazil.at2(i, j) = freeze_column<double>(g, p, T, S, i, j, dt);
+}
+
+// Net ocean fluxes of one column (assemble_net_ocean_fluxes.jl:74-153, ℵ = 0).
+struct NetColumn {
+    double tau_x, tau_y, J_T, J_S, J_w, J_H;
+};
+
+__device__ __forceinline__ NetColumn assemble_column(const GridDev& g, const ne_params& p, double Qt, double Qv,
+                                                     double Jv, double rtx, double rtx_w, double rty,
+                                                     double rty_s, double Jr, double Js, double To,
+                                                     bool inactive) {
+    double rinv = 1.0 / p.ocean_reference_density;
+    double cinv = 1.0 / p.ocean_heat_capacity;
+    double aice = 0.0, Jl = 0.0, Ps = 0.0;
+    double SQao = (Qt + Qv) * (1.0 - aice);
+    double SFao = -(Jr + Jl + Js - Ps) * rinv + (1.0 - aice) * Jv * rinv;
+    double Jwao = -SFao;
+    double Qin = 0.0, Jsio = 0.0, Jwio = 0.0;
+    double JTao = SQao * rinv * cinv;
+    double JTio = Qin * rinv * cinv;
+    double tx_ao = (rinv * (1.0 - aice) * rtx + rinv * (1.0 - aice) * rtx_w) / 2.0;
+    double ty_ao = (rinv * (1.0 - aice) * rty + rinv * (1.0 - aice) * rty_s) / 2.0;
+    double tx_io = 0.0 * rinv * 0.0, ty_io = 0.0 * rinv * 0.0;
+    NetColumn n;
+    n.tau_x = inactive ? 0.0 : tx_ao + tx_io;
+    n.tau_y = inactive ? 0.0 : ty_ao + ty_io;
+    n.J_T = inactive ? 0.0 : JTao + JTio;
+    n.J_S = inactive ? 0.0 : Jsio;
+    n.J_w = inactive ? 0.0 : Jwao + Jwio;
+    n.J_H = inactive ? 0.0 : To * Jwao;
+    return n;
+}
+
+// Radiative part (apply_air_sea_radiative_fluxes.jl:62-111): returns Jᵀ_rad, writes diagnostics.
+__device__ __forceinline__ double radiation_column(const ne_params& p, double Ts_C, double SW, double LW,
+                                                   double& Iup, double& Ila, double& Itsw) {
+    double rinv = 1.0 / p.ocean_reference_density;
+    double cinv = 1.0 / p.ocean_heat_capacity;
+    double aice = 0.0;
+    double Ts = Ts_C + NE_CELSIUS_TO_KELVIN;
+    double T2 = Ts * Ts;
+    Iup = p.stefan_boltzmann_constant * p.ocean_emissivity * (T2 * T2);
+    Ila = -p.ocean_emissivity * LW;
+    Itsw = -(1.0 - p.ocean_albedo) * SW;
+    Ila = Ila * (1.0 - aice);
+    Itsw = Itsw * (1.0 - aice);
+    double Iup_ocean = Iup * (1.0 - aice);
+    double SQ = Iup_ocean + Ila + Itsw;
+    return SQ * rinv * cinv;
+}
+
+// Momentum fluxes of the west / south neighbours (zero state outside the domain).
+__device__ __forceinline__ void neighbour_momentum(const GridDev& g, const View3<double>& rtx,
+                                                   const View3<double>& rty, int i, int j, double& rtx_w,
+                                                   double& rty_s) {
+    rtx_w = g.column(i - 1, j) >= 0 ? rtx.at2(g.wx(i - 1), j) : 0.0;
+    rty_s = g.column(i, j - 1) >= 0 ? rty.at2(i, g.wy(j - 1)) : 0.0;
+}
+
+__global__ void assemble_kernel(GridDev g, ne_params p, View3<double> Qv, View3<double> Qt, View3<double> Jv,
+                                View3<double> rtx, View3<double> rty, View3<double> rain, View3<double> snow,
+                                View3<const double> sst, NetOut n) {
+    int i = blockIdx.x * BX + threadIdx.x;
+    int j = blockIdx.y * BY + threadIdx.y;
+    if (i >= g.Nx || j >= g.Ny) return;
+    double rtx_w, rty_s;
+    neighbour_momentum(g, rtx, rty, i, j, rtx_w, rty_s);
+    double Jr = rain.p ? rain.at2(i, j) : 0.0;
+    double Js = snow.p ? snow.at2(i, j) : 0.0;
+    bool inactive = !g.active(i, j, g.Nz - 1);
+    NetColumn c = assemble_column(g, p, Qt.at2(i, j), Qv.at2(i, j), Jv.at2(i, j), rtx.at2(i, j), rtx_w,
+                                  rty.at2(i, j), rty_s, Jr, Js, sst(i, j, g.Nz - 1), inactive);
+    n.tau_x.at2(i, j) = c.tau_x;
+    n.tau_y.at2(i, j) = c.tau_y;
+    n.J_T.at2(i, j) = c.J_T;
+    n.J_S.at2(i, j) = c.J_S;
+    n.J_w.at2(i, j) = c.J_w;
+    n.J_H.at2(i, j) = c.J_H;
+}
+
+__global__ void radiation_kernel(GridDev g, ne_params p, View3<const double> sw, View3<const double> lw,
+                                 View3<const double> Ts, NetOut n, RadOut r) {
+    int i = blockIdx.x * BX + threadIdx.x;
+    int j = blockIdx.y * BY + threadIdx.y;
+    if (i >= g.Nx || j >= g.Ny) return;
+    double Iup, Ila, Itsw;
+    double JTrad = radiation_column(p, Ts.at2(i, j), sw.at2(i, j), lw.at2(i, j), Iup, Ila, Itsw);
+    bool inactive = !g.active(i, j, g.Nz - 1);
+    n.J_T.at2(i, j) = n.J_T.at2(i, j) + (inactive ? 0.0 : JTrad);
+    if (r.up_lw.p) r.up_lw.at2(i, j) = Iup;
+    if (r.down_lw.p) r.down_lw.at2(i, j) = -Ila;
+    if (r.down_sw.p) r.down_sw.at2(i, j) = -Itsw;
+}
+
+// K2 of the fused update_state: freeze clamp, net-flux assembly and radiation per column.
+template <typename Real>
+__global__ void __launch_bounds__(BX* BY)
+surface_finalize_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __restrict__ afi,
+                        const double* __restrict__ afj, RadDev<Real> rd, const double* __restrict__ rfi,
+                        const double* __restrict__ rfj, View3<Real> T, View3<const Real> S, FluxOut f, NetOut n,
+                        RadOut ro, View3<double> frazil, double dt_frazil) {
+    int i = blockIdx.x * BX + threadIdx.x;
+    int j = blockIdx.y * BY + threadIdx.y;
+    if (i >= g.Nx || j >= g.Ny) return;
+    double dQ = freeze_column<Real>(g, p, T, S, i, j, dt_frazil);
+    if (frazil.p) frazil.at2(i, j) = dQ;
+
+    Neighbours x = neighbours(afi[i], a.Nx, a.tx);
+    Neighbours y = neighbours(afj[j], a.Ny, a.ty);
+    double Jr = interp_field(a.rain, x, y, a.time);
+    double Js = interp_field(a.snow, x, y, a.time);
+    double rtx_w, rty_s;
+    neighbour_momentum(g, f.rtx, f.rty, i, j, rtx_w, rty_s);
+    bool inactive = !g.active(i, j, g.Nz - 1);
+    NetColumn c = assemble_column(g, p, f.Qt.at2(i, j), f.Qv.at2(i, j), f.Jv.at2(i, j), f.rtx.at2(i, j), rtx_w,
+                                  f.rty.at2(i, j), rty_s, Jr, Js, (double)T(i, j, g.Nz - 1), inactive);
+    double JT = c.J_T;
+    if (p.radiation) {
+        Neighbours rx = neighbours(rfi[i], rd.Nx, rd.tx);
+        Neighbours ry = neighbours(rfj[j], rd.Ny, rd.ty);
+        double SW = interp_field(rd.sw, rx, ry, rd.time);
+        double LW = interp_field(rd.lw, rx, ry, rd.time);
+        double Iup, Ila, Itsw;
+        double JTrad = radiation_column(p, f.Ts.at2(i, j), SW, LW, Iup, Ila, Itsw);
+        JT = JT + (inactive ? 0.0 : JTrad);
+        if (ro.up_lw.p) ro.up_lw.at2(i, j) = Iup;
+        if (ro.down_lw.p) ro.down_lw.at2(i, j) = -Ila;
+        if (ro.down_sw.p) ro.down_sw.at2(i, j) = -Itsw;
+    }
+    n.tau_x.at2(i, j) = c.tau_x;
+    n.tau_y.at2(i, j) = c.tau_y;
+    n.J_T.at2(i, j) = JT;
+    n.J_S.at2(i, j) = c.J_S;
+    n.J_w.at2(i, j) = c.J_w;
+    n.J_H.at2(i, j) = c.J_H;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+
+template <typename Real>
+AtmosDev<Real> atmos_dev(const ne_prescribed_atmosphere& a) {
+    AtmosDev<Real> d;
+    d.Nx = a.grid.Nx; d.Ny = a.grid.Ny; d.tx = a.grid.topology_x; d.ty = a.grid.topology_y;
+    d.u = view<const Real>(a.u); d.v = view<const Real>(a.v); d.T = view<const Real>(a.T);
+    d.q = view<const Real>(a.q); d.p = view<const Real>(a.p);
+    d.rain = view<const Real>(a.rain); d.snow = view<const Real>(a.snow);
+    d.time = a.time;
+    return d;
+}
+
+template <typename Real>
+RadDev<Real> rad_dev(const ne_prescribed_radiation& r) {
+    RadDev<Real> d;
+    d.Nx = r.grid.Nx; d.Ny = r.grid.Ny; d.tx = r.grid.topology_x; d.ty = r.grid.topology_y;
+    d.sw = view<const Real>(r.downwelling_shortwave);
+    d.lw = view<const Real>(r.downwelling_longwave);
+    d.time = r.time;
+    return d;
+}
+
+template AtmosDev<double> atmos_dev<double>(const ne_prescribed_atmosphere&);
+template AtmosDev<float> atmos_dev<float>(const ne_prescribed_atmosphere&);
+template RadDev<double> rad_dev<double>(const ne_prescribed_radiation&);
+template RadDev<float> rad_dev<float>(const ne_prescribed_radiation&);
+
+FluxOut flux_out(const ne_interface_fluxes& f) {
+    FluxOut o;
+    o.Qv = view<double>(f.latent_heat); o.Qt = view<double>(f.sensible_heat); o.Jv = view<double>(f.water_vapor);
+    o.rtx = view<double>(f.x_momentum); o.rty = view<double>(f.y_momentum);
+    o.us = view<double>(f.friction_velocity); o.ts = view<double>(f.temperature_scale);
+    o.qs = view<double>(f.water_vapor_scale); o.Ts = view<double>(f.interface_temperature);
+    o.iters = view<int32_t>(f.iterations);
+    return o;
+}
+
+NetOut net_out(const ne_net_ocean_fluxes& n) {
+    NetOut o;
+    o.tau_x = view<double>(n.tau_x); o.tau_y = view<double>(n.tau_y); o.J_T = view<double>(n.J_T);
+    o.J_S = view<double>(n.J_S); o.J_w = view<double>(n.J_w); o.J_H = view<double>(n.J_H);
+    return o;
+}
+
+RadOut rad_out(const ne_radiative_fluxes& r) {
+    RadOut o;
+    o.up_lw = view<double>(r.upwelling_longwave);
+    o.down_lw = view<double>(r.downwelling_longwave);
+    o.down_sw = view<double>(r.downwelling_shortwave);
+    return o;
+}
+
+hipError_t launch_immersed_levels(const GridDev& g, View3<const double> bottom, int32_t* kmin, hipStream_t s) {
+    hipLaunchKernelGGL(immersed_levels_kernel, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, bottom, kmin);
+    return hipGetLastError();
+}
+
+hipError_t launch_fractional_indices(const GridDev& g, const ne_source_grid& src, double* fi, double* fj,
+                                     hipStream_t s) {
+    int n = g.Nx > g.Ny ? g.Nx : g.Ny;
+    hipLaunchKernelGGL(fractional_indices_kernel, dim3((n + 255) / 256), dim3(256), 0, s, g, src, fi, fj);
+    return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_interp_atmos(const GridDev& g, const AtmosDev<Real>& a, const double* fi, const double* fj,
+                               const ne_exchange_state& o, hipStream_t s) {
+    hipLaunchKernelGGL(interp_atmos_kernel<Real>, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, a, fi, fj,
+                       view<double>(o.u), view<double>(o.v), view<double>(o.T), view<double>(o.p),
+                       view<double>(o.q), view<double>(o.rain), view<double>(o.snow));
+    return hipGetLastError();
+}
+template hipError_t launch_interp_atmos<double>(const GridDev&, const AtmosDev<double>&, const double*,
+                                                const double*, const ne_exchange_state&, hipStream_t);
+template hipError_t launch_interp_atmos<float>(const GridDev&, const AtmosDev<float>&, const double*,
+                                               const double*, const ne_exchange_state&, hipStream_t);
+
+template <typename Real>
+hipError_t launch_interp_radiation(const GridDev& g, const RadDev<Real>& r, const double* fi, const double* fj,
+                                   View3<double> sw, View3<double> lw, hipStream_t s) {
+    hipLaunchKernelGGL(interp_radiation_kernel<Real>, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, r, fi, fj, sw, lw);
+    return hipGetLastError();
+}
+template hipError_t launch_interp_radiation<double>(const GridDev&, const RadDev<double>&, const double*,
+                                                    const double*, View3<double>, View3<double>, hipStream_t);
+template hipError_t launch_interp_radiation<float>(const GridDev&, const RadDev<float>&, const double*,
+                                                   const double*, View3<double>, View3<double>, hipStream_t);
+
+hipError_t launch_flux_solve_exchange(const GridDev& g, const ne_params& p, const ne_exchange_state& a,
+                                      const ne_ocean_state& o, const FluxOut& out, ne_counters* c,
+                                      hipStream_t s) {
+    hipLaunchKernelGGL(flux_solve_exchange_kernel, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p,
+                       view<const double>(a.u), view<const double>(a.v), view<const double>(a.T),
+                       view<const double>(a.p), view<const double>(a.q), view<const double>(o.T),
+                       view<const double>(o.S), view<const double>(o.u), view<const double>(o.v), out, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_freeze_clamp(const GridDev& g, const ne_params& p, View3<double> T, View3<const double> S,
+                               View3<double> frazil, double dt, hipStream_t s) {
+    hipLaunchKernelGGL(freeze_clamp_kernel, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, T, S, frazil, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_assemble(const GridDev& g, const ne_params& p, const ne_interface_fluxes& f,
+                           const ne_exchange_state& a, View3<const double> sst, const NetOut& n, hipStream_t s) {
+    hipLaunchKernelGGL(assemble_kernel, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, view<double>(f.latent_heat),
+                       view<double>(f.sensible_heat), view<double>(f.water_vapor), view<double>(f.x_momentum),
+                       view<double>(f.y_momentum), view<double>(a.rain), view<double>(a.snow), sst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_radiation(const GridDev& g, const ne_params& p, View3<const double> sw, View3<const double> lw,
+                            View3<const double> Ts, const NetOut& n, const RadOut& r, hipStream_t s) {
+    hipLaunchKernelGGL(radiation_kernel, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, sw, lw, Ts, n, r);
+    return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_surface_solve(const GridDev& g, const ne_params& p, const AtmosDev<Real>& a, const double* fi,
+                                const double* fj, View3<const Real> T, View3<const Real> S, View3<const Real> u,
+                                View3<const Real> v, const FluxOut& out, ne_counters* c, hipStream_t s) {
+    hipLaunchKernelGGL(surface_solve_kernel<Real>, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, a, fi, fj, T, S,
+                       u, v, out, c);
+    return hipGetLastError();
+}
+template hipError_t launch_surface_solve<double>(const GridDev&, const ne_params&, const AtmosDev<double>&,
+                                                 const double*, const double*, View3<const double>,
+                                                 View3<const double>, View3<const double>, View3<const double>,
+                                                 const FluxOut&, ne_counters*, hipStream_t);
+template hipError_t launch_surface_solve<float>(const GridDev&, const ne_params&, const AtmosDev<float>&,
+                                                const double*, const double*, View3<const float>,
+                                                View3<const float>, View3<const float>, View3<const float>,
+                                                const FluxOut&, ne_counters*, hipStream_t);
+
+template <typename Real>
+hipError_t launch_surface_finalize(const GridDev& g, const ne_params& p, const AtmosDev<Real>& a,
+                                   const double* afi, const double* afj, const RadDev<Real>& r, const double* rfi,
+                                   const double* rfj, View3<Real> T, View3<const Real> S, const FluxOut& f,
+                                   const NetOut& n, const RadOut& ro, View3<double> frazil, double dt_frazil,
+                                   hipStream_t s) {
+    hipLaunchKernelGGL(surface_finalize_kernel<Real>, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, a, afi, afj, r,
+                       rfi, rfj, T, S, f, n, ro, frazil, dt_frazil);
+    return hipGetLastError();
+}
+template hipError_t launch_surface_finalize<double>(const GridDev&, const ne_params&, const AtmosDev<double>&,
+                                                    const double*, const double*, const RadDev<double>&,
+                                                    const double*, const double*, View3<double>,
+                                                    View3<const double>, const FluxOut&, const NetOut&,
+                                                    const RadOut&, View3<double>, double, hipStream_t);
+template hipError_t launch_surface_finalize<float>(const GridDev&, const ne_params&, const AtmosDev<float>&,
+                                                   const double*, const double*, const RadDev<float>&,
+                                                   const double*, const double*, View3<float>, View3<const float>,
+                                                   const FluxOut&, const NetOut&, const RadOut&, View3<double>,
+                                                   double, hipStream_t);
+
+}  // namespace ne

@@ -1,0 +1,347 @@
+"""Host-side mirror of the reference's coupled model over the C ABI.
+
+``OceanOnlyModel`` mirrors NumericalEarth's ``OceanOnlyModel(ocean; atmosphere, radiation)``
+(earth_system_model.jl:399-402) and ``time_step!`` (time_step_earth_system_model.jl:14-36):
+the state lives in device memory (torch is used only to allocate it and to name the
+stream), and every numerical operation is a call into libnumericalearth_hip.so.
+``Phases`` exposes the per-phase entry points that replace the reference's kernel
+launches one by one.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _ctypes_abi as abi
+from ._lib import check, load
+from .generators import SurfaceCase
+from .params import default_params
+
+FLUX_NAMES = ("latent_heat", "sensible_heat", "water_vapor", "x_momentum", "y_momentum",
+              "friction_velocity", "temperature_scale", "water_vapor_scale", "interface_temperature")
+NET_NAMES = ("tau_x", "tau_y", "J_T", "J_S", "J_w", "J_H")
+EXCHANGE_NAMES = ("u", "v", "T", "p", "q", "rain", "snow")
+RAD_NAMES = ("upwelling_longwave", "downwelling_longwave", "downwelling_shortwave")
+
+
+def torch_view(t: torch.Tensor | None) -> abi.View:
+    """ne_view of a device tensor whose last dimension is i (element strides)."""
+    if t is None:
+        return abi.View(None, 0, 0, 0)
+    st = t.stride()
+    sx = st[-1] if t.dim() >= 1 else 0
+    sy = st[-2] if t.dim() >= 2 else 0
+    sz = st[-3] if t.dim() >= 3 else 0
+    return abi.View(t.data_ptr(), sx, sy, sz)
+
+
+def stream_handle(stream: torch.cuda.Stream | None = None) -> int | None:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream or None
+
+
+class Context:
+    """One ne_ctx per device (owns the solver counters and the budget workspace)."""
+
+    _contexts: dict = {}
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        self.device = device
+        h = C.c_void_p()
+        check(self.lib.ne_ctx_create(device, C.byref(h)), "ne_ctx_create")
+        self.handle = h
+
+    @classmethod
+    def get(cls, device: int = 0) -> "Context":
+        if device not in cls._contexts:
+            cls._contexts[device] = Context(device)
+        return cls._contexts[device]
+
+    def counters(self, reset: bool = False, stream=None) -> dict:
+        c = abi.Counters()
+        check(self.lib.ne_ctx_counters(self.handle, C.byref(c), int(reset), stream_handle(stream)), "ne_ctx_counters")
+        return {name: getattr(c, name) for name, _ in abi.Counters._fields_}
+
+
+def _dev(a: np.ndarray, device, dtype=None) -> torch.Tensor:
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to(device)
+
+
+class DeviceGrid:
+    """Immersed LatitudeLongitudeGrid on the device: per-row metrics and the first active level of every
+    column, computed on the device from the bottom height (GridFittedBottom)."""
+
+    def __init__(self, grid, ctx: Context, device):
+        spec = grid.spec
+        self.spec = spec
+        self.Nx, self.Ny, self.Nz = spec.Nx, spec.Ny, spec.Nz
+        f64 = torch.float64
+        self.dz = _dev(grid.dz, device, f64)
+        self.zc = _dev(grid.z_center, device, f64)
+        self.dyx = _dev(grid.dy_xface, device, f64)
+        self.dxy = _dev(grid.dx_yface, device, f64)
+        self.Az = _dev(grid.Az, device, f64)
+        self.lam = _dev(grid.lambda_c, device, f64)
+        self.phi = _dev(grid.phi_c, device, f64)
+        self.bottom = _dev(grid.bottom_height, device, f64)
+        self.kmin = torch.empty((self.Ny, self.Nx), dtype=torch.int32, device=device)
+        self.struct = abi.Grid(self.Nx, self.Ny, self.Nz, spec.topology_x, spec.topology_y, spec.advection,
+                               self.dz.data_ptr(), self.zc.data_ptr(), self.dyx.data_ptr(), self.dxy.data_ptr(),
+                               self.Az.data_ptr(), self.lam.data_ptr(), self.phi.data_ptr(), self.kmin.data_ptr())
+        check(ctx.lib.ne_grid_immersed_levels(ctx.handle, C.byref(self.struct), torch_view(self.bottom),
+                                              self.kmin.data_ptr(), stream_handle()), "ne_grid_immersed_levels")
+
+
+def _source(spec) -> abi.SourceGrid:
+    return abi.SourceGrid(spec.Nx, spec.Ny, spec.topology_x, spec.topology_y, spec.lambda0, spec.dlambda, spec.phi0,
+                          spec.dphi)
+
+
+class OceanOnlyModel:
+    """OceanOnlyModel(ocean; atmosphere, radiation) on one MI355X, state resident in HBM.
+
+    precision: "fp64" (the drop-in) or "mixed" (FP32 T, S, velocities and prescribed fields; FP64 flux solve,
+    fluxes and budgets).
+    """
+
+    def __init__(self, case: SurfaceCase, *, params: abi.Params | None = None, precision: str = "fp64",
+                 radiation: bool = True, device: int = 0):
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.case = case
+        self.device = torch.device("cuda", device)
+        self.ctx = Context.get(device)
+        self.lib = self.ctx.lib
+        self.precision = precision
+        self.real = torch.float64 if precision == "fp64" else torch.float32
+        self.params = params if params is not None else default_params()
+        self.params.radiation = 1 if radiation else 0
+        dev, R, f64 = self.device, self.real, torch.float64
+        Nx, Ny, Nz = case.shape
+        self.grid = DeviceGrid(case.grid, self.ctx, dev)
+        self.T = _dev(case.T, dev, R)
+        self.S = _dev(case.S, dev, R)
+        self.u = _dev(case.u, dev, R)
+        self.v = _dev(case.v, dev, R)
+        self.w = _dev(case.w, dev, R)
+        self.atm = {k: _dev(v, dev, R) for k, v in case.atmos.items()}
+        self.rad = {k: _dev(v, dev, R) for k, v in case.rad.items()}
+        self.src_atmos = _source(case.atmos_grid)
+        self.src_rad = _source(case.rad_grid)
+        self.afi = torch.empty(Nx, dtype=f64, device=dev)
+        self.afj = torch.empty(Ny, dtype=f64, device=dev)
+        self.rfi = torch.empty(Nx, dtype=f64, device=dev)
+        self.rfj = torch.empty(Ny, dtype=f64, device=dev)
+        g = C.byref(self.grid.struct)
+        s = stream_handle()
+        check(self.lib.ne_fractional_indices_f64(self.ctx.handle, g, C.byref(self.src_atmos), self.afi.data_ptr(),
+                                                 self.afj.data_ptr(), s), "ne_fractional_indices_f64")
+        check(self.lib.ne_fractional_indices_f64(self.ctx.handle, g, C.byref(self.src_rad), self.rfi.data_ptr(),
+                                                 self.rfj.data_ptr(), s), "ne_fractional_indices_f64")
+        z2 = lambda: torch.zeros((Ny, Nx), dtype=f64, device=dev)  # noqa: E731
+        self.fluxes = {k: z2() for k in FLUX_NAMES}
+        self.iterations = torch.zeros((Ny, Nx), dtype=torch.int32, device=dev)
+        self.net = {k: z2() for k in NET_NAMES}
+        self.radiative = {k: z2() for k in RAD_NAMES}
+        self.frazil_heat = z2()
+        self._atimes = np.ascontiguousarray(case.atmos_times, dtype=np.float64)
+        self._rtimes = np.ascontiguousarray(case.rad_times, dtype=np.float64)
+        d = abi.ModelDesc()
+        d.precision = abi.NE_FP64 if precision == "fp64" else abi.NE_MIXED_FP32
+        d.fuse_freeze_clamp = 0
+        d.grid = self.grid.struct
+        d.params = self.params
+        d.atmosphere = abi.PrescribedAtmosphereDesc(self.src_atmos, *(torch_view(self.atm[k]) for k in
+                                                                       ("u", "v", "T", "q", "p", "rain", "snow")),
+                                                    abi.TimeInterp(0, 0, 0.0))
+        d.radiation = abi.PrescribedRadiationDesc(self.src_rad, torch_view(self.rad["downwelling_shortwave"]),
+                                                  torch_view(self.rad["downwelling_longwave"]),
+                                                  abi.TimeInterp(0, 0, 0.0))
+        d.atmos_fi, d.atmos_fj = self.afi.data_ptr(), self.afj.data_ptr()
+        d.radiation_fi, d.radiation_fj = self.rfi.data_ptr(), self.rfj.data_ptr()
+        d.ocean = abi.OceanState(torch_view(self.u), torch_view(self.v), torch_view(self.w), torch_view(self.T),
+                                 torch_view(self.S))
+        d.fluxes = abi.InterfaceFluxes(*(torch_view(self.fluxes[k]) for k in FLUX_NAMES), torch_view(self.iterations))
+        d.net = abi.NetOceanFluxes(*(torch_view(self.net[k]) for k in NET_NAMES))
+        d.radiative = abi.RadiativeFluxes(*(torch_view(self.radiative[k]) for k in RAD_NAMES))
+        d.frazil_heat = torch_view(self.frazil_heat)
+        d.atmosphere_times = self._atimes.ctypes.data_as(C.POINTER(C.c_double))
+        d.n_atmosphere_times = len(self._atimes)
+        d.radiation_times = self._rtimes.ctypes.data_as(C.POINTER(C.c_double))
+        d.n_radiation_times = len(self._rtimes)
+        self.desc = d
+        h = C.c_void_p()
+        check(self.lib.ne_model_create(self.ctx.handle, C.byref(d), C.byref(h), s), "ne_model_create")
+        self.handle = h
+        self._budget_out = torch.zeros(2, dtype=f64, device=dev)
+
+    # reference API ------------------------------------------------------------
+    def time_step(self, dt: float, stream=None) -> None:
+        check(self.lib.ne_model_time_step(self.handle, float(dt), stream_handle(stream)), "ne_model_time_step")
+
+    def update_state(self, stream=None) -> None:
+        check(self.lib.ne_model_update_state(self.handle, stream_handle(stream)), "ne_model_update_state")
+
+    @property
+    def clock(self):
+        t = C.c_double()
+        n = C.c_int64()
+        check(self.lib.ne_model_clock(self.handle, C.byref(t), C.byref(n)), "ne_model_clock")
+        return t.value, n.value
+
+    def timing_begin(self, max_steps: int) -> None:
+        """Record HIP events around each kernel of the next max_steps time steps."""
+        check(self.lib.ne_model_timing_begin(self.handle, int(max_steps)), "ne_model_timing_begin")
+
+    def timing_end(self) -> tuple[dict, int]:
+        """Summed milliseconds per kernel over the recorded steps, and the number of steps."""
+        from ._lib import TIMED_KERNELS
+        ms = (C.c_double * len(TIMED_KERNELS))()
+        n = C.c_int()
+        check(self.lib.ne_model_timing_end(self.handle, ms, C.byref(n)), "ne_model_timing_end")
+        return dict(zip(TIMED_KERNELS, list(ms))), n.value
+
+    def budget(self, field: torch.Tensor, weight: float = 1.0, stream=None) -> float:
+        """Compensated Σ weight·V·c over active cells (returns sum + compensation)."""
+        fn = self.lib.ne_budget_f64 if field.dtype == torch.float64 else self.lib.ne_budget_f32
+        check(fn(self.ctx.handle, C.byref(self.grid.struct), torch_view(field), float(weight),
+                 self._budget_out.data_ptr(), stream_handle(stream)), "ne_budget")
+        s, c = self._budget_out.tolist()
+        return s + c
+
+    def heat_content(self) -> float:
+        return self.budget(self.T, self.params.ocean_reference_density * self.params.ocean_heat_capacity)
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            torch.cuda.synchronize(self.device)
+            check(self.lib.ne_model_destroy(self.handle), "ne_model_destroy")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # host copies ----------------------------------------------------------------
+    def fluxes_numpy(self) -> dict:
+        d = {k: v.cpu().numpy() for k, v in self.fluxes.items()}
+        d["iterations"] = self.iterations.cpu().numpy()
+        return d
+
+    def net_numpy(self) -> dict:
+        return {k: v.cpu().numpy() for k, v in self.net.items()}
+
+
+def time_step(model: OceanOnlyModel, dt: float) -> None:
+    """time_step!(coupled_model, Δt)."""
+    model.time_step(dt)
+
+
+def update_state(model: OceanOnlyModel) -> None:
+    """update_state!(coupled_model)."""
+    model.update_state()
+
+
+class Phases:
+    """The per-phase entry points of the C ABI on device tensors (FP64)."""
+
+    def __init__(self, model: OceanOnlyModel):
+        if model.precision != "fp64":
+            raise ValueError("per-phase entry points are FP64")
+        self.m = model
+        self.lib = model.lib
+        self.ctx = model.ctx.handle
+        dev = model.device
+        Nx, Ny, Nz = model.case.shape
+        z2 = lambda: torch.zeros((Ny, Nx), dtype=torch.float64, device=dev)  # noqa: E731
+        self.exchange = {k: z2() for k in EXCHANGE_NAMES}
+        self.sw, self.lw = z2(), z2()
+
+    def _ex(self):
+        return abi.ExchangeState(*(torch_view(self.exchange[k]) for k in EXCHANGE_NAMES))
+
+    def interp(self, atmos_time: abi.TimeInterp | None = None, rad_time: abi.TimeInterp | None = None):
+        m = self.m
+        a = abi.PrescribedAtmosphereDesc.from_buffer_copy(m.desc.atmosphere)
+        r = abi.PrescribedRadiationDesc.from_buffer_copy(m.desc.radiation)
+        if atmos_time is not None:
+            a.time = atmos_time
+        if rad_time is not None:
+            r.time = rad_time
+        s = stream_handle()
+        check(self.lib.ne_interp_atmos_f64(self.ctx, C.byref(m.grid.struct), C.byref(a), m.afi.data_ptr(),
+                                           m.afj.data_ptr(), C.byref(self._ex()), s), "ne_interp_atmos_f64")
+        check(self.lib.ne_interp_radiation_f64(self.ctx, C.byref(m.grid.struct), C.byref(r), m.rfi.data_ptr(),
+                                               m.rfj.data_ptr(), torch_view(self.sw), torch_view(self.lw), s),
+              "ne_interp_radiation_f64")
+
+    def flux_solve(self):
+        m = self.m
+        check(self.lib.ne_ao_flux_solve_f64(self.ctx, C.byref(m.grid.struct), C.byref(m.params), C.byref(self._ex()),
+                                            C.byref(m.desc.ocean), C.byref(m.desc.fluxes), stream_handle()),
+              "ne_ao_flux_solve_f64")
+
+    def freeze(self, dt: float):
+        m = self.m
+        check(self.lib.ne_freeze_clamp_f64(self.ctx, C.byref(m.grid.struct), C.byref(m.params), torch_view(m.T),
+                                           torch_view(m.S), torch_view(m.frazil_heat), float(dt), stream_handle()),
+              "ne_freeze_clamp_f64")
+
+    def assemble(self):
+        m = self.m
+        check(self.lib.ne_assemble_net_ocean_fluxes_f64(self.ctx, C.byref(m.grid.struct), C.byref(m.params),
+                                                        C.byref(m.desc.fluxes), C.byref(self._ex()),
+                                                        torch_view(m.T), C.byref(m.desc.net), stream_handle()),
+              "ne_assemble_net_ocean_fluxes_f64")
+
+    def radiation(self):
+        m = self.m
+        check(self.lib.ne_apply_air_sea_radiation_f64(self.ctx, C.byref(m.grid.struct), C.byref(m.params),
+                                                      torch_view(self.sw), torch_view(self.lw),
+                                                      torch_view(m.fluxes["interface_temperature"]),
+                                                      C.byref(m.desc.net), C.byref(m.desc.radiative),
+                                                      stream_handle()), "ne_apply_air_sea_radiation_f64")
+
+    def tendency(self, T: torch.Tensor | None = None, S: torch.Tensor | None = None):
+        m = self.m
+        T = m.T if T is None else T
+        S = m.S if S is None else S
+        GT = torch.zeros_like(T)
+        GS = torch.zeros_like(S)
+        o = abi.OceanState(torch_view(m.u), torch_view(m.v), torch_view(m.w), torch_view(T), torch_view(S))
+        check(self.lib.ne_tracer_tendency_weno5_f64(self.ctx, C.byref(m.grid.struct), C.byref(o), C.byref(m.desc.net),
+                                                    torch_view(GT), torch_view(GS), stream_handle()),
+              "ne_tracer_tendency_weno5_f64")
+        return GT, GS
+
+    def rk_stage(self, T0, S0, Tin, Sin, Tout, Sout, beta_dt: float):
+        m = self.m
+        check(self.lib.ne_tracer_rk_stage_f64(self.ctx, C.byref(m.grid.struct), torch_view(m.u), torch_view(m.v),
+                                              torch_view(m.w), torch_view(T0), torch_view(S0), torch_view(Tin),
+                                              torch_view(Sin), torch_view(Tout), torch_view(Sout),
+                                              C.byref(m.desc.net), float(beta_dt), stream_handle()),
+              "ne_tracer_rk_stage_f64")
+
+
+def cell_updates(case: SurfaceCase) -> int:
+    Nx, Ny, Nz = case.shape
+    return Nx * Ny * Nz
+
+
+def bytes_per_cell_update(precision: str, Nz: int, n_stages: int = 3) -> float:
+    """Algorithmic HBM bytes per cell-update: B = B₃ + B₂/Nz with B₃ = (7 + 9(n_st − 1))·s₃ and
+    B₂ = 7.25·s₂ᵢ + 14·s₂ₒ."""
+    s3, s2i, s2o = (8, 8, 8) if precision == "fp64" else (4, 4, 8)
+    B3 = (7 + 9 * (n_stages - 1)) * s3
+    B2 = 7.25 * s2i + 14 * s2o
+    return B3 + B2 / Nz

@@ -1,0 +1,24 @@
+"""Comparison helpers shared by the parity tests."""
+import numpy as np
+
+
+def rel_err(a, b, floor_frac=1e-6):
+    """Largest |a - b| / |b| over elements with |b| above floor_frac·max|b|, and the largest
+    |a - b| / max|b| over all elements (the field-scaled error, which covers values near zero)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = float(np.max(np.abs(b))) if b.size else 0.0
+    diff = np.abs(a - b)
+    if scale == 0.0:
+        return float(diff.max(initial=0.0)), float(diff.max(initial=0.0))
+    mask = np.abs(b) > floor_frac * scale
+    elem = float(np.max(diff[mask] / np.abs(b[mask]), initial=0.0))
+    field = float(diff.max(initial=0.0) / scale)
+    return elem, field
+
+
+def assert_rel(a, b, rtol, name=""):
+    elem, field = rel_err(a, b)
+    assert np.all(np.isfinite(a)) == np.all(np.isfinite(b)), f"{name}: finiteness differs"
+    assert elem <= rtol and field <= rtol, f"{name}: relative error {elem:.3e} (field-scaled {field:.3e}) > {rtol:.1e}"
+    return elem, field

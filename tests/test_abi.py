@@ -1,0 +1,80 @@
+"""C ABI checks that need no GPU: the library loads, exports every entry point the
+header declares, and the ctypes mirror has the header's exact struct layouts."""
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+from conftest import REPO
+from numericalearth_hip import _ctypes_abi as abi
+from numericalearth_hip import _lib
+from numericalearth_hip.build import library_path
+
+HEADER = REPO / "include" / "numericalearth_hip.h"
+
+STRUCTS = {
+    "ne_view": abi.View, "ne_grid": abi.Grid, "ne_source_grid": abi.SourceGrid, "ne_time_interp": abi.TimeInterp,
+    "ne_prescribed_atmosphere": abi.PrescribedAtmosphereDesc, "ne_prescribed_radiation": abi.PrescribedRadiationDesc,
+    "ne_exchange_state": abi.ExchangeState, "ne_ocean_state": abi.OceanState,
+    "ne_interface_fluxes": abi.InterfaceFluxes, "ne_net_ocean_fluxes": abi.NetOceanFluxes,
+    "ne_radiative_fluxes": abi.RadiativeFluxes, "ne_params": abi.Params, "ne_counters": abi.Counters,
+    "ne_model_desc": abi.ModelDesc,
+}
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*int\s+(ne_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_entry_point():
+    lib = C.CDLL(str(library_path()))
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, f"not exported: {missing}"
+    # and the Python binding declares a signature for each of them
+    assert sorted(_lib.SIGNATURES) == names
+
+
+def test_version_and_defaults_match_python_mirror():
+    L = _lib.load()
+    assert L.ne_version() == 1
+    p = abi.Params()
+    assert L.ne_params_default(C.byref(p)) == 0
+    from numericalearth_hip.params import default_params, params_dict
+    assert params_dict(p) == params_dict(default_params())
+
+
+def test_null_arguments_are_rejected_without_a_device():
+    L = _lib.load()
+    assert L.ne_params_default(None) == abi.NE_ERR_NULL
+    assert L.ne_ctx_destroy(None) == abi.NE_ERR_NULL
+    assert L.ne_model_destroy(None) == abi.NE_ERR_NULL
+    assert L.ne_model_time_step(None, 1.0, None) == abi.NE_ERR_NULL
+
+
+def test_struct_layouts_match_header(tmp_path):
+    lines = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(void) {"]
+    for cname, cls in STRUCTS.items():
+        lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            s, f, v = line.split()
+            got[(s, f)] = int(v)
+    for cname, cls in STRUCTS.items():
+        assert got[(cname, "sizeof")] == C.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)

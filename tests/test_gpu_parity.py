@@ -1,0 +1,240 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle on the same seeded inputs.
+
+Tolerances (FP64): every row is compared elementwise with relative error ≤ 1e-12 (and
+≤ 1e-12 relative to the field's largest magnitude for elements near zero). The land /
+immersed indexing (first active level per column) must match bit for bit. Rows whose
+arithmetic uses only correctly rounded operations (interpolation, assembly, radiation,
+WENO tendency, RK stage) are additionally checked bit for bit.
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import assert_rel, rel_err
+
+import oracle as O
+from numericalearth_hip import _ctypes_abi as abi
+from numericalearth_hip.generators import make_case
+from numericalearth_hip.model import OceanOnlyModel, Phases
+from numericalearth_hip.params import (ComponentInterfaces, FixedIterations, FixedSpecificHumidity,
+                                       SimilarityTheoryFluxes, to_params)
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+SMALL = (90, 40, 10)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _pair(shape=SMALL, seed=1, params=None, radiation=True, **kw):
+    case = make_case(*shape, seed=seed, **kw)
+    print(f"[case] shape={shape} seed={seed} land={case.meta['land_fraction']:.3f}", flush=True)
+    p = params if params is not None else to_params()
+    om = O.OracleModel(case, params=abi.Params.from_buffer_copy(p), radiation=radiation)
+    gm = OceanOnlyModel(case, params=abi.Params.from_buffer_copy(p), radiation=radiation)
+    return case, om, gm
+
+
+def test_immersed_levels_bit_exact(require_gpu):
+    case, om, gm = _pair()
+    kmin_dev = _np(gm.grid.kmin)
+    kmin_orc = np.zeros_like(kmin_dev)
+    import ctypes as C
+    O.lib().neo_immersed_levels(C.byref(om.grid), O.np_view(case.grid.bottom_height),
+                                kmin_orc.ctypes.data_as(C.POINTER(C.c_int32)))
+    assert np.array_equal(kmin_dev, kmin_orc)
+    assert np.array_equal(kmin_dev, case.grid.kmin)
+    assert 0 < np.mean(kmin_dev >= case.grid.spec.Nz) < 0.5   # some land, mostly ocean
+
+
+def test_interpolation_bit_exact(require_gpu):
+    case, om, gm = _pair(n_times=2)
+    ph = Phases(gm)
+    t = abi.TimeInterp(0, 1, 0.375)
+    om._model.atmosphere.time = t
+    om._model.radiation.time = t
+    import ctypes as C
+    L = O.lib()
+    L.neo_interp_atmos(C.byref(om.grid), C.byref(om._model.atmosphere), O._dp(om.afi), O._dp(om.afj),
+                       C.byref(om.exchange))
+    L.neo_interp_radiation(C.byref(om.grid), C.byref(om._model.radiation), O._dp(om.rfi), O._dp(om.rfj),
+                           O.np_view(om.sw), O.np_view(om.lw))
+    ph.interp(t, t)
+    torch.cuda.synchronize()
+    for k in O.EXCHANGE_NAMES:
+        assert np.array_equal(_np(ph.exchange[k]), om.exchange_arrays[k]), k
+    assert np.array_equal(_np(ph.sw), om.sw)
+    assert np.array_equal(_np(ph.lw), om.lw)
+    assert np.array_equal(_np(gm.afi), om.afi) and np.array_equal(_np(gm.afj), om.afj)
+
+
+def _solve_both(case, om, gm):
+    ph = Phases(gm)
+    om.interp()
+    ph.interp()
+    torch.cuda.synchronize()
+    om.flux_solve()
+    ph.flux_solve()
+    torch.cuda.synchronize()
+    return ph
+
+
+def _compare_fluxes(gm, om, rtol=RTOL, max_iteration_mismatch_frac=1e-4):
+    it_g = _np(gm.iterations)
+    it_o = om.iterations
+    mismatch = it_g != it_o
+    frac = mismatch.mean()
+    print(f"[flux] iteration mismatches: {mismatch.sum()} of {mismatch.size}", flush=True)
+    assert frac <= max_iteration_mismatch_frac
+    ok = ~mismatch
+    worst = {}
+    for k in O.FLUX_NAMES:
+        a = _np(gm.fluxes[k])[ok]
+        b = om.flux_arrays[k][ok]
+        worst[k] = assert_rel(a, b, rtol, k)
+    print("[flux] worst relative errors:", {k: f"{v[0]:.2e}" for k, v in worst.items()}, flush=True)
+
+
+def test_flux_solve_matches_oracle(require_gpu):
+    case, om, gm = _pair()
+    _solve_both(case, om, gm)
+    _compare_fluxes(gm, om)
+
+
+def test_flux_solve_fixed_iterations(require_gpu):
+    p = to_params(SimilarityTheoryFluxes(solver_stop_criteria=FixedIterations(12)))
+    case, om, gm = _pair(params=p, seed=3)
+    _solve_both(case, om, gm)
+    assert np.all(_np(gm.iterations) == 12)
+    _compare_fluxes(gm, om, max_iteration_mismatch_frac=0.0)
+
+
+def test_flux_solve_neutral_and_fixed_humidity(require_gpu):
+    fl = SimilarityTheoryFluxes(momentum_roughness_length=1e-4, temperature_roughness_length=1e-4,
+                                water_vapor_roughness_length=1e-4, subgrid_velocities=None, stability_functions=None)
+    p = to_params(fl, ComponentInterfaces(atmosphere_ocean_interface_specific_humidity=FixedSpecificHumidity(0.003)))
+    case, om, gm = _pair(params=p, seed=4)
+    _solve_both(case, om, gm)
+    _compare_fluxes(gm, om, max_iteration_mismatch_frac=0.0)
+
+
+def test_assembly_radiation_freeze_bit_exact(require_gpu):
+    case, om, gm = _pair(seed=5)
+    # force some freezing cells
+    cold = case.T.copy()
+    cold[-1, ::7, ::5] = -5.0
+    om.T[...] = cold
+    gm.T.copy_(torch.from_numpy(cold))
+    ph = _solve_both(case, om, gm)
+    om.freeze(1200.0)
+    ph.freeze(1200.0)
+    torch.cuda.synchronize()
+    assert np.array_equal(_np(gm.T), om.T)
+    assert np.array_equal(_np(gm.frazil_heat), om.frazil)
+    assert (om.frazil < 0).sum() > 0
+    # feed identical interface fluxes to both assemblies
+    for k in O.FLUX_NAMES:
+        gm.fluxes[k].copy_(torch.from_numpy(om.flux_arrays[k]))
+    om.assemble()
+    ph.assemble()
+    om.radiation_apply()
+    ph.radiation()
+    torch.cuda.synchronize()
+    for k in O.NET_NAMES:
+        assert np.array_equal(_np(gm.net[k]), om.net_arrays[k]), k
+    for k in O.RAD_NAMES:
+        assert np.array_equal(_np(gm.radiative[k]), om.rad_arrays[k]), k
+
+
+def _sync_net(gm, om):
+    for k in O.NET_NAMES:
+        gm.net[k].copy_(torch.from_numpy(om.net_arrays[k]))
+
+
+@pytest.mark.parametrize("shape,topology_x", [(SMALL, 0), ((37, 23, 7), 0), ((48, 30, 8), 1)])
+def test_tendency_bit_exact(require_gpu, shape, topology_x):
+    case, om, gm = _pair(shape, seed=6, topology_x=topology_x, longitude=(0.0, 360.0) if topology_x == 0 else (20.0, 80.0))
+    om.reconcile()
+    _sync_net(gm, om)
+    GT_o, GS_o = om.tendency()
+    GT_g, GS_g = Phases(gm).tendency()
+    torch.cuda.synchronize()
+    assert np.array_equal(_np(GT_g), GT_o)
+    assert np.array_equal(_np(GS_g), GS_o)
+    assert np.abs(GT_o).max() > 0
+
+
+def test_rk_stage_bit_exact(require_gpu):
+    case, om, gm = _pair(seed=7)
+    om.reconcile()
+    _sync_net(gm, om)
+    T1 = torch.zeros_like(gm.T)
+    S1 = torch.zeros_like(gm.S)
+    Phases(gm).rk_stage(gm.T, gm.S, gm.T, gm.S, T1, S1, 400.0)
+    torch.cuda.synchronize()
+    GT, GS = om.tendency()
+    active = np.arange(case.grid.spec.Nz)[:, None, None] >= case.grid.kmin[None]
+    Texp = np.where(active, case.T + 400.0 * GT, 0.0)
+    assert np.array_equal(np.where(active, _np(T1), 0.0), Texp)
+
+
+@pytest.mark.parametrize("precision", ["fp64"])
+def test_model_steps_match_oracle(require_gpu, precision):
+    case, om, gm = _pair(seed=11)
+    om.reconcile()
+    torch.cuda.synchronize()
+    _compare_fluxes(gm, om)
+    for step in range(5):
+        om.time_step(1200.0)
+        gm.time_step(1200.0)
+        torch.cuda.synchronize()
+        print(f"[model] step {step + 1}", flush=True)
+        assert_rel(_np(gm.T), om.T, RTOL, "T")
+        assert_rel(_np(gm.S), om.S, RTOL, "S")
+        _compare_fluxes(gm, om)
+        for k in O.NET_NAMES:
+            assert_rel(_np(gm.net[k]), om.net_arrays[k], RTOL, k)
+    assert gm.clock == (6000.0, 5)
+
+
+def test_budget_closure_on_device(require_gpu):
+    case, om, gm = _pair(seed=12)
+    rho_c = gm.params.ocean_reference_density * gm.params.ocean_heat_capacity
+    H0 = gm.heat_content()
+    Ho = om.budget(om.T, rho_c)
+    assert abs(H0 - (Ho[0] + Ho[1])) <= 1e-15 * abs(H0)
+    JT = gm.net["J_T"].clone()
+    gm.time_step(1200.0)
+    torch.cuda.synchronize()
+    H1 = gm.heat_content()
+    Az = case.grid.Az
+    expected = -1200.0 * rho_c * float(np.sum(_np(JT) * Az[:, None]))
+    # Flux-form advection telescopes, so the heat content changes by the surface flux alone
+    # (the freeze clamp is inert at these temperatures). The only error left is the rounding
+    # of the stored state: at most half an ulp of T per cell, weighted by ρc·V.
+    T1 = _np(gm.T)
+    active = np.arange(case.grid.spec.Nz)[:, None, None] >= case.grid.kmin[None]
+    V = case.grid.Az[None, :, None] * case.grid.dz[:, None, None]
+    bound = rho_c * float(np.sum(np.where(active, V * np.spacing(np.abs(T1)), 0.0)))
+    err = abs((H1 - H0) - expected)
+    print(f"[budget] ΔH={H1 - H0:.6e} expected={expected:.6e} err={err:.3e} bound={bound:.3e}", flush=True)
+    assert err <= bound
+
+
+def test_mixed_precision_model_tracks_fp64(require_gpu):
+    case = make_case(*SMALL, seed=13)
+    g64 = OceanOnlyModel(case, precision="fp64")
+    g32 = OceanOnlyModel(case, precision="mixed")
+    for _ in range(10):
+        g64.time_step(1200.0)
+        g32.time_step(1200.0)
+    torch.cuda.synchronize()
+    sst64 = _np(g64.T)[-1]
+    sst32 = _np(g32.T)[-1].astype(np.float64)
+    wet = case.grid.kmin < case.grid.spec.Nz
+    err = np.max(np.abs(sst32 - sst64)[wet] / np.abs(sst64[wet]))
+    print(f"[mixed] SST relative error after 10 steps: {err:.2e}", flush=True)
+    assert err < 1e-5

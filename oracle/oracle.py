@@ -23,7 +23,7 @@ sys.path.insert(0, str(REPO / "numericalearth.jl_amd"))
 from numericalearth_hip import _ctypes_abi as abi  # noqa: E402  (ABI struct layouts)
 from numericalearth_hip.params import default_params  # noqa: E402
 
-LIB_PATH = HERE / "_build" / "libne_oracle.so"
+LIB_PATH = Path(os.environ.get("NE_ORACLE_LIB", HERE / "_build" / "libne_oracle.so"))
 
 
 def build(verbose: bool = False) -> Path:

@@ -385,43 +385,85 @@ __device__ __forceinline__ double interp_field(const View3<const Real>& d, const
 }
 
 // ---------------------------------------------------------------------------
-// WENO reconstructions (Jiang & Shu 1996 smoothness indicators, Borges et al. 2008
-// WENO-Z weights). Values are ordered along the upwind direction.
+// WENO reconstructions: Jiang & Shu (1996) candidate stencils and smoothness
+// indicators with Borges et al. (2008) WENO-Z weights, ε = 1e-8. Values are ordered
+// along the upwind direction (the face lies after the third of five values, or the
+// second of three). The weights are normalised with a single division: multiplying
+// every α_r = d_r (1 + (τ/(β_r+ε))²) by Π_s (β_s+ε)² leaves α_r/Σα unchanged, and the
+// 1/6 of the candidate polynomials moves into the same denominator. In FP32 the
+// Π(β+ε)² products could underflow, so FP32 keeps per-stencil reciprocals instead.
+// Fused multiply-adds are allowed here (contract(fast)).
 // ---------------------------------------------------------------------------
 
-template <typename R>
-__device__ __forceinline__ R weno5z(R c0, R c1, R c2, R c3, R c4) {
-    const R eps = (R)NE_WENO_EPS;
-    R q0 = ((R)2 * c0 - (R)7 * c1 + (R)11 * c2) / (R)6;
-    R q1 = (-c1 + (R)5 * c2 + (R)2 * c3) / (R)6;
-    R q2 = ((R)2 * c2 + (R)5 * c3 - c4) / (R)6;
-    R d0 = c0 - (R)2 * c1 + c2, e0 = c0 - (R)4 * c1 + (R)3 * c2;
-    R d1 = c1 - (R)2 * c2 + c3, e1 = c1 - c3;
-    R d2 = c2 - (R)2 * c3 + c4, e2 = (R)3 * c2 - (R)4 * c3 + c4;
-    R b0 = (R)(13.0 / 12.0) * (d0 * d0) + (R)0.25 * (e0 * e0);
-    R b1 = (R)(13.0 / 12.0) * (d1 * d1) + (R)0.25 * (e1 * e1);
-    R b2 = (R)(13.0 / 12.0) * (d2 * d2) + (R)0.25 * (e2 * e2);
-    R tau = fabs(b0 - b2);
-    R r0 = tau / (b0 + eps), r1 = tau / (b1 + eps), r2 = tau / (b2 + eps);
-    R a0 = (R)0.1 * ((R)1 + r0 * r0);
-    R a1 = (R)0.6 * ((R)1 + r1 * r1);
-    R a2 = (R)0.3 * ((R)1 + r2 * r2);
-    R s = a0 + a1 + a2;
-    return (a0 * q0 + a1 * q1 + a2 * q2) / s;
+__device__ __forceinline__ double weno5z(double c0, double c1, double c2, double c3, double c4) {
+#pragma clang fp contract(fast)
+    const double eps = NE_WENO_EPS;
+    double d0 = c0 - 2.0 * c1 + c2, e0 = c0 - 4.0 * c1 + 3.0 * c2;
+    double d1 = c1 - 2.0 * c2 + c3, e1 = c1 - c3;
+    double d2 = c2 - 2.0 * c3 + c4, e2 = 3.0 * c2 - 4.0 * c3 + c4;
+    double b0 = (13.0 / 12.0) * d0 * d0 + 0.25 * e0 * e0;
+    double b1 = (13.0 / 12.0) * d1 * d1 + 0.25 * e1 * e1;
+    double b2 = (13.0 / 12.0) * d2 * d2 + 0.25 * e2 * e2;
+    double tau = fabs(b0 - b2);
+    double t2 = tau * tau;
+    double s0 = (b0 + eps) * (b0 + eps), s1 = (b1 + eps) * (b1 + eps), s2 = (b2 + eps) * (b2 + eps);
+    double a0 = 0.1 * (s0 + t2) * (s1 * s2);
+    double a1 = 0.6 * (s1 + t2) * (s0 * s2);
+    double a2 = 0.3 * (s2 + t2) * (s0 * s1);
+    double q0 = 2.0 * c0 - 7.0 * c1 + 11.0 * c2;
+    double q1 = -c1 + 5.0 * c2 + 2.0 * c3;
+    double q2 = 2.0 * c2 + 5.0 * c3 - c4;
+    return (a0 * q0 + a1 * q1 + a2 * q2) / (6.0 * (a0 + a1 + a2));
 }
 
-template <typename R>
-__device__ __forceinline__ R weno3z(R c0, R c1, R c2) {
-    const R eps = (R)NE_WENO_EPS;
-    R q0 = (-c0 + (R)3 * c1) / (R)2;
-    R q1 = (c1 + c2) / (R)2;
-    R d0 = c1 - c0, d1 = c2 - c1;
-    R b0 = d0 * d0, b1 = d1 * d1;
-    R tau = fabs(b0 - b1);
-    R r0 = tau / (b0 + eps), r1 = tau / (b1 + eps);
-    R a0 = (R)(1.0 / 3.0) * ((R)1 + r0 * r0);
-    R a1 = (R)(2.0 / 3.0) * ((R)1 + r1 * r1);
-    return (a0 * q0 + a1 * q1) / (a0 + a1);
+__device__ __forceinline__ float weno5z(float c0, float c1, float c2, float c3, float c4) {
+#pragma clang fp contract(fast)
+    const float eps = (float)NE_WENO_EPS;
+    float d0 = c0 - 2.0f * c1 + c2, e0 = c0 - 4.0f * c1 + 3.0f * c2;
+    float d1 = c1 - 2.0f * c2 + c3, e1 = c1 - c3;
+    float d2 = c2 - 2.0f * c3 + c4, e2 = 3.0f * c2 - 4.0f * c3 + c4;
+    float b0 = (13.0f / 12.0f) * d0 * d0 + 0.25f * e0 * e0;
+    float b1 = (13.0f / 12.0f) * d1 * d1 + 0.25f * e1 * e1;
+    float b2 = (13.0f / 12.0f) * d2 * d2 + 0.25f * e2 * e2;
+    float tau = fabsf(b0 - b2);
+    float r0 = tau * __builtin_amdgcn_rcpf(b0 + eps);
+    float r1 = tau * __builtin_amdgcn_rcpf(b1 + eps);
+    float r2 = tau * __builtin_amdgcn_rcpf(b2 + eps);
+    float a0 = 0.1f * (1.0f + r0 * r0);
+    float a1 = 0.6f * (1.0f + r1 * r1);
+    float a2 = 0.3f * (1.0f + r2 * r2);
+    float q0 = 2.0f * c0 - 7.0f * c1 + 11.0f * c2;
+    float q1 = -c1 + 5.0f * c2 + 2.0f * c3;
+    float q2 = 2.0f * c2 + 5.0f * c3 - c4;
+    return (a0 * q0 + a1 * q1 + a2 * q2) * __builtin_amdgcn_rcpf(6.0f * (a0 + a1 + a2));
+}
+
+__device__ __forceinline__ double weno3z(double c0, double c1, double c2) {
+#pragma clang fp contract(fast)
+    const double eps = NE_WENO_EPS;
+    double d0 = c1 - c0, d1 = c2 - c1;
+    double b0 = d0 * d0, b1 = d1 * d1;
+    double tau = fabs(b0 - b1);
+    double t2 = tau * tau;
+    double s0 = (b0 + eps) * (b0 + eps), s1 = (b1 + eps) * (b1 + eps);
+    double a0 = (1.0 / 3.0) * (s0 + t2) * s1;
+    double a1 = (2.0 / 3.0) * (s1 + t2) * s0;
+    double q0 = -c0 + 3.0 * c1;
+    double q1 = c1 + c2;
+    return (a0 * q0 + a1 * q1) / (2.0 * (a0 + a1));
+}
+
+__device__ __forceinline__ float weno3z(float c0, float c1, float c2) {
+#pragma clang fp contract(fast)
+    const float eps = (float)NE_WENO_EPS;
+    float d0 = c1 - c0, d1 = c2 - c1;
+    float b0 = d0 * d0, b1 = d1 * d1;
+    float tau = fabsf(b0 - b1);
+    float r0 = tau * __builtin_amdgcn_rcpf(b0 + eps);
+    float r1 = tau * __builtin_amdgcn_rcpf(b1 + eps);
+    float a0 = (1.0f / 3.0f) * (1.0f + r0 * r0);
+    float a1 = (2.0f / 3.0f) * (1.0f + r1 * r1);
+    return (a0 * (-c0 + 3.0f * c1) + a1 * (c1 + c2)) * __builtin_amdgcn_rcpf(2.0f * (a0 + a1));
 }
 
 // Reduced order at a face between cells m-1 and m from the activity bits of cells
@@ -433,12 +475,16 @@ __device__ __forceinline__ int face_order(unsigned act) {
     return 3;
 }
 
+// Upwind-biased face value from the six cells m-3 … m+2 around the face. The upwind
+// side is chosen by selecting the (mirrored) inputs, so a wavefront with mixed flow
+// directions evaluates one reconstruction per lane.
 template <typename R>
-__device__ __forceinline__ R face_value(const R c[6], int order, R vel) {
+__device__ __forceinline__ R face_value(R c0, R c1, R c2, R c3, R c4, R c5, int order, R vel) {
     bool left = vel >= (R)0;
-    if (order == 3) return left ? weno5z(c[0], c[1], c[2], c[3], c[4]) : weno5z(c[5], c[4], c[3], c[2], c[1]);
-    if (order == 2) return left ? weno3z(c[1], c[2], c[3]) : weno3z(c[4], c[3], c[2]);
-    return left ? c[2] : c[3];
+    R v0 = left ? c0 : c5, v1 = left ? c1 : c4, v2 = left ? c2 : c3, v3 = left ? c3 : c2, v4 = left ? c4 : c1;
+    if (order == 3) return weno5z(v0, v1, v2, v3, v4);
+    if (order == 2) return weno3z(v1, v2, v3);
+    return v2;
 }
 
 }  // namespace ne

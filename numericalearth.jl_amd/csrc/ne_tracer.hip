@@ -1,137 +1,232 @@
 // ne_tracer.hip — WENO5 tracer-advection tendency on the immersed LatLon grid, fused
 // with the top flux boundary condition and the split Runge-Kutta stage update.
 //
-// One work-item per column (i, j), marching up the active levels; blocks of 64×4
-// with i fastest, so every level of every field is read by 64 consecutive lanes.
-// The vertical face flux computed at the top face of level k is carried in
-// registers as the bottom face of level k+1 (each z-face reconstructed once).
+// Tiling (gfx950): a block of 512 work-items = 8 wavefronts owns a tile of 63 × 8
+// columns and marches up the active levels. At each level the tile's T and S plane
+// plus a 3-cell halo (69 × 14 cells) is staged in LDS; every x- and y-face flux of the
+// tile is reconstructed once (lane ℓ of wavefront r owns the west face of column ℓ and
+// the south face of row r, lane 63 only a face), written to LDS, and differenced by the
+// owning cell. The vertical face flux at the top of level k is carried in registers as
+// the bottom flux of level k+1, with the column's vertical stencil in a register window.
+// Tiles are ordered so that each XCD (blocks are dealt round-robin to the 8 XCDs)
+// receives a contiguous band of tiles, keeping halo re-reads inside one L2.
 #include "ne_internal.h"
 
 namespace ne {
 
 namespace {
 
-constexpr int BX = 64;
-constexpr int BY = 4;
+constexpr int TW = 64;            // work-items per tile row (x faces per row)
+constexpr int TC = 63;            // cells per tile row
+constexpr int TR = 8;             // cell rows per tile (one wavefront each)
+constexpr int PXW = TC + 6;       // plane columns i0-3 … i0+65
+constexpr int PYH = TR + 6;       // plane rows    j0-3 … j0+10
+constexpr int PN = PXW * PYH;
+constexpr int NTHR = TW * TR;     // 512
+constexpr int NXCD = 8;
 constexpr int KMIN_OUTSIDE = 1 << 30;  // kmin of a column outside a bounded domain
 
-inline dim3 grid2d(int Nx, int Ny) { return dim3((Nx + BX - 1) / BX, (Ny + BY - 1) / BY); }
-
-// Advective flux through one face: A·vel·c̃ (c̃ from the six cells around the face).
 template <typename R>
-__device__ __forceinline__ R face_flux(unsigned act6, const R c[6], R vel, R A) {
-    int order = face_order(act6);
-    if (order == 0) return (R)0;
-    return A * vel * face_value(c, order, vel);
+struct TileShared {
+    R cT[PN], cS[PN];
+    R fxT[TR * TW], fxS[TR * TW];
+    R fyT[(TR + 1) * TC], fyS[(TR + 1) * TC];
+    int km[PN];
+    int kstart;
+};
+
+__host__ __device__ inline int tiles_x(int Nx) { return (Nx + TC - 1) / TC; }
+__host__ __device__ inline int tiles_y(int Ny) { return (Ny + TR - 1) / TR; }
+inline int grid_blocks(int Nx, int Ny) {
+    int n = tiles_x(Nx) * tiles_y(Ny);
+    return ((n + NXCD - 1) / NXCD) * NXCD;
+}
+
+// Activity bits of six consecutive cells whose kmin is at km[0], km[stride], …
+__device__ __forceinline__ unsigned act6(const int* km, int stride, int k) {
+    unsigned a = 0;
+#pragma unroll
+    for (int n = 0; n < 6; n++) a |= (k >= km[n * stride] ? 1u : 0u) << n;
+    return a;
 }
 
 // MODE 0: c_out = c0 + beta_dt·G(c_in) on active cells. MODE 1: write G (0 on inactive cells).
 template <typename Real, int MODE>
-__global__ void __launch_bounds__(BX* BY)
-tracer_kernel(GridDev g, View3<const Real> u, View3<const Real> v, View3<const Real> w, View3<const Real> T0,
-              View3<const Real> S0, View3<const Real> Tin, View3<const Real> Sin, View3<Real> Tout,
-              View3<Real> Sout, View3<const double> JT, View3<const double> JS, View3<const double> Jw,
-              double beta_dt) {
+__global__ void __launch_bounds__(NTHR)
+tracer_tile_kernel(GridDev g, View3<const Real> u, View3<const Real> v, View3<const Real> w, View3<const Real> T0,
+                   View3<const Real> S0, View3<const Real> Tin, View3<const Real> Sin, View3<Real> Tout,
+                   View3<Real> Sout, View3<const double> JT, View3<const double> JS, View3<const double> Jw,
+                   double beta_dt) {
     typedef Real R;
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    if (i >= g.Nx || j >= g.Ny) return;
+    __shared__ TileShared<R> sh;
+    const int lane = threadIdx.x;
+    const int row = threadIdx.y;
+    const int tid = lane + row * TW;
     const int Nz = g.Nz;
-    const int kmin_c = g.kmin[i + (int64_t)j * g.Nx];
-    if (MODE == 1) {
-        for (int k = 0; k < (kmin_c < Nz ? kmin_c : Nz); k++) { Tout(i, j, k) = (R)0; Sout(i, j, k) = (R)0; }
-    }
-    if (kmin_c >= Nz) return;
 
-    // kmin of the neighbour columns i-3 … i+3 and j-3 … j+3 (outside a bounded domain: never active).
-    int kx[7], ky[7], ix[7], jy[7];
+    // XCD-contiguous tile order
+    const int ntx = tiles_x(g.Nx), nty = tiles_y(g.Ny);
+    const int ntiles = ntx * nty;
+    const int per = (ntiles + NXCD - 1) / NXCD;
+    const int tile = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
+    if (tile >= ntiles) return;
+    const int i0 = (tile % ntx) * TC;
+    const int j0 = (tile / ntx) * TR;
+
+    // kmin of the plane columns (outside a bounded domain: never active)
+    for (int n = tid; n < PN; n += NTHR) {
+        int pi = n % PXW, pj = n / PXW;
+        int64_t c = g.column(i0 - 3 + pi, j0 - 3 + pj);
+        sh.km[n] = c >= 0 ? g.kmin[c] : KMIN_OUTSIDE;
+    }
+    if (tid == 0) sh.kstart = Nz;
+    __syncthreads();
+    const int ci = i0 + lane, cj = j0 + row;
+    const bool cell = lane < TC && ci < g.Nx && cj < g.Ny;
+    const int mykmin = cell ? sh.km[(row + 3) * PXW + lane + 3] : Nz;
+    if (cell) atomicMin(&sh.kstart, mykmin);
+    __syncthreads();
+    const int kstart = sh.kstart;
+    if (MODE == 1 && cell)
+        for (int k = 0; k < mykmin && k < Nz; k++) { Tout(ci, cj, k) = (R)0; Sout(ci, cj, k) = (R)0; }
+    if (kstart >= Nz) return;  // uniform: the whole tile is land
+
+    // Per-row and per-column geometry
+    const bool xrow = cj < g.Ny;                 // this wavefront's row is inside the domain
+    const int jw = xrow ? cj : 0;
+    const double Axf = g.dy_xface[jw];
+    const int fiw = g.wx(i0 + lane);             // x face owned by this lane (west face of column i0+lane)
+    const bool ycol = lane < TC && ci < g.Nx;
+    const int cy = ycol ? ci : 0;
+    const int fjs = g.wy(j0 + row);              // y face owned by this lane (south face of row j0+row)
+    const int fjn = g.wy(j0 + TR);               // north face of the last row (wavefront TR-1)
+    const double Ays = (j0 + row <= g.Ny) ? g.dx_yface[fjs < 0 ? 0 : (fjs > g.Ny ? g.Ny : fjs)] : 0.0;
+    const double Ayn = (j0 + TR <= g.Ny) ? g.dx_yface[fjn < 0 ? 0 : (fjn > g.Ny ? g.Ny : fjn)] : 0.0;
+    const double Azc = cell ? g.Az[cj] : 0.0;
+
+    // Vertical register window of the owned column: levels k-2 … k+3
+    R zT[6], zS[6];
 #pragma unroll
-    for (int n = 0; n < 7; n++) {
-        int64_t cx = g.column(i - 3 + n, j);
-        kx[n] = cx >= 0 ? g.kmin[cx] : KMIN_OUTSIDE;
-        ix[n] = g.wx(i - 3 + n);
-        int64_t cy = g.column(i, j - 3 + n);
-        ky[n] = cy >= 0 ? g.kmin[cy] : KMIN_OUTSIDE;
-        jy[n] = g.wy(j - 3 + n);
+    for (int n = 0; n < 6; n++) {
+        int kk = kstart - 2 + n;
+        bool a = cell && kk >= mykmin && kk < Nz;
+        zT[n] = a ? Tin(ci, cj, kk) : (R)0;
+        zS[n] = a ? Sin(ci, cj, kk) : (R)0;
     }
-    const int iE = g.wx(i + 1);
-    const int jN = g.wy(j + 1);
-    const double Ayf_s = g.dx_yface[jy[3]], Ayf_n = g.dx_yface[jN];
-    const double Axf = g.dy_xface[j];
-    const double Azc = g.Az[j];
-
     R FzT = (R)0, FzS = (R)0;  // flux through the bottom face of the current level
-    for (int k = kmin_c; k < Nz; k++) {
-        R div_T = (R)0, div_S = (R)0;
-        if (g.advection) {
-            const R dzk = (R)g.dz[k];
-            // x faces: cells i-3 … i+3
-            unsigned ax = 0;
-            R cT[7], cS[7];
-#pragma unroll
-            for (int n = 0; n < 7; n++) {
-                bool a = k >= kx[n];
-                ax |= (a ? 1u : 0u) << n;
-                cT[n] = a ? Tin(ix[n], j, k) : (R)0;
-                cS[n] = a ? Sin(ix[n], j, k) : (R)0;
+
+    for (int k = kstart; k < Nz; k++) {
+        // 1. stage the plane of level k
+        for (int n = tid; n < PN; n += NTHR) {
+            bool a = k >= sh.km[n];
+            R t = (R)0, s = (R)0;
+            if (a) {
+                int gi = g.wx(i0 - 3 + n % PXW), gj = g.wy(j0 - 3 + n / PXW);
+                t = Tin(gi, gj, k);
+                s = Sin(gi, gj, k);
             }
-            const R Ax = (R)Axf * dzk;
-            R uw = (ax & 0x0Cu) == 0x0Cu ? u(ix[3], j, k) : (R)0;
-            R ue = (ax & 0x18u) == 0x18u ? u(iE, j, k) : (R)0;
-            R FxwT = face_flux(ax & 0x3Fu, cT, uw, Ax), FxeT = face_flux((ax >> 1) & 0x3Fu, cT + 1, ue, Ax);
-            R FxwS = face_flux(ax & 0x3Fu, cS, uw, Ax), FxeS = face_flux((ax >> 1) & 0x3Fu, cS + 1, ue, Ax);
-            // y faces: cells j-3 … j+3
-            unsigned ay = 0;
-#pragma unroll
-            for (int n = 0; n < 7; n++) {
-                bool a = k >= ky[n];
-                ay |= (a ? 1u : 0u) << n;
-                cT[n] = a ? Tin(i, jy[n], k) : (R)0;
-                cS[n] = a ? Sin(i, jy[n], k) : (R)0;
+            sh.cT[n] = t;
+            sh.cS[n] = s;
+        }
+        __syncthreads();
+        const R dzk = (R)g.dz[k];
+
+        // 2a. x face owned by this lane: cells i0+lane-3 … i0+lane+2 of row j0+row
+        {
+            R fT = (R)0, fS = (R)0;
+            if (xrow) {
+                const int base = (row + 3) * PXW + lane;
+                int order = face_order(act6(&sh.km[base], 1, k));
+                if (order > 0) {
+                    R vel = u(fiw, cj, k);
+                    R A = (R)Axf * dzk;
+                    const R* t = &sh.cT[base];
+                    const R* s = &sh.cS[base];
+                    fT = A * vel * face_value(t[0], t[1], t[2], t[3], t[4], t[5], order, vel);
+                    fS = A * vel * face_value(s[0], s[1], s[2], s[3], s[4], s[5], order, vel);
+                }
             }
-            const R Ays = (R)Ayf_s * dzk, Ayn = (R)Ayf_n * dzk;
-            R vs = (ay & 0x0Cu) == 0x0Cu ? v(i, jy[3], k) : (R)0;
-            R vn = (ay & 0x18u) == 0x18u ? v(i, jN, k) : (R)0;
-            R FysT = face_flux(ay & 0x3Fu, cT, vs, Ays), FynT = face_flux((ay >> 1) & 0x3Fu, cT + 1, vn, Ayn);
-            R FysS = face_flux(ay & 0x3Fu, cS, vs, Ays), FynS = face_flux((ay >> 1) & 0x3Fu, cS + 1, vn, Ayn);
-            // z face k+1 (top of this level): cells k-2 … k+3 of this column
+            sh.fxT[row * TW + lane] = fT;
+            sh.fxS[row * TW + lane] = fS;
+        }
+        // 2b. y faces: the south face of row j0+row, and for the last wavefront the north face of the tile
+        if (ycol) {
+#pragma unroll 1
+            for (int pass = 0; pass < (row == TR - 1 ? 2 : 1); pass++) {
+                const int r0 = row + pass;                // plane row of the first stencil cell
+                const int base = r0 * PXW + lane + 3;
+                const int fj = pass ? fjn : fjs;
+                const double Ay = pass ? Ayn : Ays;
+                R fT = (R)0, fS = (R)0;
+                int order = face_order(act6(&sh.km[base], PXW, k));
+                if (order > 0) {
+                    R vel = v(cy, fj, k);
+                    R A = (R)Ay * dzk;
+                    const R* t = &sh.cT[base];
+                    const R* s = &sh.cS[base];
+                    fT = A * vel * face_value(t[0], t[PXW], t[2 * PXW], t[3 * PXW], t[4 * PXW], t[5 * PXW], order, vel);
+                    fS = A * vel * face_value(s[0], s[PXW], s[2 * PXW], s[3 * PXW], s[4 * PXW], s[5 * PXW], order, vel);
+                }
+                sh.fyT[(r0)*TC + lane] = fT;
+                sh.fyS[(r0)*TC + lane] = fS;
+            }
+        }
+        // 2c. z face at the top of level k (cells k-2 … k+3 of the owned column)
+        R FztT = (R)0, FztS = (R)0;
+        if (cell) {
             unsigned az = 0;
-            R zT[6], zS[6];
 #pragma unroll
             for (int n = 0; n < 6; n++) {
                 int kk = k - 2 + n;
-                bool a = kk >= kmin_c && kk < Nz;
-                az |= (a ? 1u : 0u) << n;
-                zT[n] = a ? Tin(i, j, kk) : (R)0;
-                zS[n] = a ? Sin(i, j, kk) : (R)0;
+                az |= ((kk >= mykmin && kk < Nz) ? 1u : 0u) << n;
             }
-            R wt = (az & 0x0Cu) == 0x0Cu ? w(i, j, k + 1) : (R)0;
-            R FztT = face_flux(az, zT, wt, (R)Azc), FztS = face_flux(az, zS, wt, (R)Azc);
-            div_T = (FxeT - FxwT) + (FynT - FysT);
+            int order = face_order(az);
+            if (order > 0) {
+                R vel = w(ci, cj, k + 1);
+                R A = (R)Azc;
+                FztT = A * vel * face_value(zT[0], zT[1], zT[2], zT[3], zT[4], zT[5], order, vel);
+                FztS = A * vel * face_value(zS[0], zS[1], zS[2], zS[3], zS[4], zS[5], order, vel);
+            }
+        }
+        __syncthreads();
+
+        // 3. difference the face fluxes, add the top boundary flux, update
+        if (cell && k >= mykmin) {
+            const int xo = row * TW + lane, yo = row * TC + lane;
+            R div_T = (sh.fxT[xo + 1] - sh.fxT[xo]) + (sh.fyT[yo + TC] - sh.fyT[yo]);
             div_T = div_T + (FztT - FzT);
-            div_S = (FxeS - FxwS) + (FynS - FysS);
+            R div_S = (sh.fxS[xo + 1] - sh.fxS[xo]) + (sh.fyS[yo + TC] - sh.fyS[yo]);
             div_S = div_S + (FztS - FzS);
-            FzT = FztT;
-            FzS = FztS;
+            const R V = (R)(Azc * g.dz[k]);
+            R GT = -((R)1 / V * div_T);
+            R GS = -((R)1 / V * div_S);
+            if (k == Nz - 1) {
+                // MultipleFluxes(Jᵀ, FreshwaterExchange{:T}) = Jᵀ + 0 on a static grid;
+                // MultipleFluxes(Jˢ, FreshwaterExchange{:S}) = Jˢ + (Sᴺ Jʷ − 0)
+                // (multiple_surface_fluxes.jl:27-31; ocean_simulation.jl:80-91)
+                double FT = JT.at2(ci, cj) + (0.0 + 0.0);
+                double FS = JS.at2(ci, cj) + (((double)zS[2] * Jw.at2(ci, cj) - 0.0) + 0.0);
+                GT = GT - (R)(FT * Azc) / V;
+                GS = GS - (R)(FS * Azc) / V;
+            }
+            if (MODE == 0) {
+                Tout(ci, cj, k) = T0(ci, cj, k) + (R)beta_dt * GT;
+                Sout(ci, cj, k) = S0(ci, cj, k) + (R)beta_dt * GS;
+            } else {
+                Tout(ci, cj, k) = GT;
+                Sout(ci, cj, k) = GS;
+            }
         }
-        const R V = (R)(Azc * g.dz[k]);
-        R GT = -((R)1 / V * div_T);
-        R GS = -((R)1 / V * div_S);
-        if (k == Nz - 1) {
-            // MultipleFluxes(Jᵀ, FreshwaterExchange{:T}) = Jᵀ + 0 on a static grid;
-            // MultipleFluxes(Jˢ, FreshwaterExchange{:S}) = Jˢ + (Sᴺ Jʷ − 0)
-            // (multiple_surface_fluxes.jl:27-31; ocean_simulation.jl:80-91)
-            double FT = JT.at2(i, j) + (0.0 + 0.0);
-            double FS = JS.at2(i, j) + (((double)Sin(i, j, k) * Jw.at2(i, j) - 0.0) + 0.0);
-            GT = GT - (R)(FT * Azc) / V;
-            GS = GS - (R)(FS * Azc) / V;
-        }
-        if (MODE == 0) {
-            Tout(i, j, k) = T0(i, j, k) + (R)beta_dt * GT;
-            Sout(i, j, k) = S0(i, j, k) + (R)beta_dt * GS;
-        } else {
-            Tout(i, j, k) = GT;
-            Sout(i, j, k) = GS;
+        FzT = FztT;
+        FzS = FztS;
+        // 4. slide the vertical window up one level
+        if (cell) {
+#pragma unroll
+            for (int n = 0; n < 5; n++) { zT[n] = zT[n + 1]; zS[n] = zS[n + 1]; }
+            int kk = k + 4;
+            bool a = kk >= mykmin && kk < Nz;
+            zT[5] = a ? Tin(ci, cj, kk) : (R)0;
+            zS[5] = a ? Sin(ci, cj, kk) : (R)0;
         }
     }
 }
@@ -139,6 +234,11 @@ tracer_kernel(GridDev g, View3<const Real> u, View3<const Real> v, View3<const R
 // ---------------------------------------------------------------------------
 // Compensated budget: Neumaier per column, then a fixed-order double-double tree.
 // ---------------------------------------------------------------------------
+
+constexpr int BX = 64;
+constexpr int BY = 4;
+
+inline dim3 grid2d(int Nx, int Ny) { return dim3((Nx + BX - 1) / BX, (Ny + BY - 1) / BY); }
 
 struct DD {
     double s, c;
@@ -222,8 +322,8 @@ hipError_t launch_rk_stage(const GridDev& g, View3<const Real> u, View3<const Re
                            View3<const Real> T0, View3<const Real> S0, View3<const Real> Tin, View3<const Real> Sin,
                            View3<Real> Tout, View3<Real> Sout, View3<const double> JT, View3<const double> JS,
                            View3<const double> Jw, double beta_dt, hipStream_t s) {
-    hipLaunchKernelGGL((tracer_kernel<Real, 0>), grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, u, v, w, T0, S0, Tin,
-                       Sin, Tout, Sout, JT, JS, Jw, beta_dt);
+    hipLaunchKernelGGL((tracer_tile_kernel<Real, 0>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, TR), 0, s, g, u, v, w,
+                       T0, S0, Tin, Sin, Tout, Sout, JT, JS, Jw, beta_dt);
     return hipGetLastError();
 }
 template hipError_t launch_rk_stage<double>(const GridDev&, View3<const double>, View3<const double>,
@@ -241,8 +341,8 @@ hipError_t launch_tendency(const GridDev& g, View3<const double> u, View3<const 
                            View3<const double> T, View3<const double> S, View3<const double> JT,
                            View3<const double> JS, View3<const double> Jw, View3<double> GT, View3<double> GS,
                            hipStream_t s) {
-    hipLaunchKernelGGL((tracer_kernel<double, 1>), grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, u, v, w, T, S, T, S,
-                       GT, GS, JT, JS, Jw, 0.0);
+    hipLaunchKernelGGL((tracer_tile_kernel<double, 1>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, TR), 0, s, g, u, v,
+                       w, T, S, T, S, GT, GS, JT, JS, Jw, 0.0);
     return hipGetLastError();
 }
 

@@ -652,40 +652,63 @@ static double flux_z(const ne_grid* g, ne_view w, ne_view c, int i, int j, int k
     return A * vel * face_value(cv, order, vel);
 }
 
-/* Gc = −(1/V)(δx(Aₓuc̃) + δy(A_y vc̃) + δz(A_z wc̃)) − top boundary flux·Az/V at k = Nz. */
+/* Gc = −(1/V)(δx(Aₓuc̃) + δy(A_y vc̃) + δz(A_z wc̃)) − top boundary flux·Az/V at k = Nz.
+ * If `scale` is given it receives Σ|terms|/V, the magnitude of the terms summed into G
+ * (the scale against which the rounding of G is measured). */
 static double tendency(const ne_grid* g, ne_view u, ne_view v, ne_view w, ne_view c,
-                       int i, int j, int k, double top_flux) {
-    double div = 0.0;
+                       int i, int j, int k, double top_flux, double* scale) {
+    double div = 0.0, mag = 0.0;
     if (g->advection) {
-        double dx = flux_x(g, u, c, i + 1, j, k) - flux_x(g, u, c, i, j, k);
-        double dy = flux_y(g, v, c, i, j + 1, k) - flux_y(g, v, c, i, j, k);
-        double dzf = flux_z(g, w, c, i, j, k + 1) - flux_z(g, w, c, i, j, k);
+        double fxe = flux_x(g, u, c, i + 1, j, k), fxw = flux_x(g, u, c, i, j, k);
+        double fyn = flux_y(g, v, c, i, j + 1, k), fys = flux_y(g, v, c, i, j, k);
+        double fzt = flux_z(g, w, c, i, j, k + 1), fzb = flux_z(g, w, c, i, j, k);
+        double dx = fxe - fxw;
+        double dy = fyn - fys;
+        double dzf = fzt - fzb;
         div = dx + dy + dzf;
+        mag = fabs(fxe) + fabs(fxw) + fabs(fyn) + fabs(fys) + fabs(fzt) + fabs(fzb);
     }
     double V = g->Az[j] * g->dz[k];
     double G = -(1.0 / V * div);
-    if (k == g->Nz - 1) G = G - top_flux * g->Az[j] / V;
+    if (k == g->Nz - 1) {
+        G = G - top_flux * g->Az[j] / V;
+        mag = mag + fabs(top_flux * g->Az[j]);
+    }
+    if (scale) *scale = mag / V;
     return G;
 }
 
 void neo_tracer_tendency(const ne_grid* g, const ne_ocean_state* o, const ne_net_ocean_fluxes* n,
                          ne_view GT, ne_view GS) {
+    ne_view none = {NULL, 0, 0, 0};
+    neo_tracer_tendency_scaled(g, o, n, GT, GS, none, none);
+}
+
+void neo_tracer_tendency_scaled(const ne_grid* g, const ne_ocean_state* o, const ne_net_ocean_fluxes* n,
+                                ne_view GT, ne_view GS, ne_view sT, ne_view sS) {
     int kN = g->Nz - 1;
 #pragma omp parallel for NEO_THREADS
     for (int j = 0; j < g->Ny; j++)
         for (int i = 0; i < g->Nx; i++)
             for (int k = 0; k < g->Nz; k++) {
-                if (!active(g, i, j, k)) { V3(GT, i, j, k) = 0.0; V3(GS, i, j, k) = 0.0; continue; }
-                double FT = 0.0, FS = 0.0;
-                if (k == kN) {
-                    /* MultipleFluxes(Jᵀ, FreshwaterExchange{:T}) on a static grid = Jᵀ + 0;
-                       MultipleFluxes(Jˢ, FreshwaterExchange{:S}) = Jˢ + (Sᴺ Jʷ − 0)
-                       (multiple_surface_fluxes.jl:27-31; ocean_simulation.jl:80-91) */
-                    FT = V2(n->J_T, i, j) + (0.0 + 0.0);
-                    FS = V2(n->J_S, i, j) + ((V3(o->S, i, j, kN) * V2(n->J_w, i, j) - 0.0) + 0.0);
+                double scT = 0.0, scS = 0.0;
+                if (!active(g, i, j, k)) {
+                    V3(GT, i, j, k) = 0.0;
+                    V3(GS, i, j, k) = 0.0;
+                } else {
+                    double FT = 0.0, FS = 0.0;
+                    if (k == kN) {
+                        /* MultipleFluxes(Jᵀ, FreshwaterExchange{:T}) on a static grid = Jᵀ + 0;
+                           MultipleFluxes(Jˢ, FreshwaterExchange{:S}) = Jˢ + (Sᴺ Jʷ − 0)
+                           (multiple_surface_fluxes.jl:27-31; ocean_simulation.jl:80-91) */
+                        FT = V2(n->J_T, i, j) + (0.0 + 0.0);
+                        FS = V2(n->J_S, i, j) + ((V3(o->S, i, j, kN) * V2(n->J_w, i, j) - 0.0) + 0.0);
+                    }
+                    V3(GT, i, j, k) = tendency(g, o->u, o->v, o->w, o->T, i, j, k, FT, &scT);
+                    V3(GS, i, j, k) = tendency(g, o->u, o->v, o->w, o->S, i, j, k, FS, &scS);
                 }
-                V3(GT, i, j, k) = tendency(g, o->u, o->v, o->w, o->T, i, j, k, FT);
-                V3(GS, i, j, k) = tendency(g, o->u, o->v, o->w, o->S, i, j, k, FS);
+                if (sT.data) V3(sT, i, j, k) = scT;
+                if (sS.data) V3(sS, i, j, k) = scS;
             }
 }
 
@@ -705,8 +728,8 @@ void neo_rk_stage(const ne_grid* g, ne_view u, ne_view v, ne_view w, ne_view T0,
                     FT = V2(n->J_T, i, j) + (0.0 + 0.0);
                     FS = V2(n->J_S, i, j) + ((V3(Sin, i, j, kN) * V2(n->J_w, i, j) - 0.0) + 0.0);
                 }
-                double GTv = tendency(g, u, v, w, Tin, i, j, k, FT);
-                double GSv = tendency(g, u, v, w, Sin, i, j, k, FS);
+                double GTv = tendency(g, u, v, w, Tin, i, j, k, FT, NULL);
+                double GSv = tendency(g, u, v, w, Sin, i, j, k, FS, NULL);
                 V3(Tout, i, j, k) = V3(T0, i, j, k) + beta_dt * GTv;
                 V3(Sout, i, j, k) = V3(S0, i, j, k) + beta_dt * GSv;
             }

@@ -73,6 +73,8 @@ def lib():
         L.neo_model_ocean_step.argtypes = [C.c_void_p, C.c_double]
         L.neo_budget.argtypes = [P(abi.Grid), abi.View, C.c_double, P(C.c_double)]
         L.neo_tracer_tendency.argtypes = [P(abi.Grid), P(abi.OceanState), P(abi.NetOceanFluxes), abi.View, abi.View]
+        L.neo_tracer_tendency_scaled.argtypes = [P(abi.Grid), P(abi.OceanState), P(abi.NetOceanFluxes), abi.View,
+                                                 abi.View, abi.View, abi.View]
         L.neo_rk_stage.argtypes = [P(abi.Grid)] + [abi.View] * 9 + [P(abi.NetOceanFluxes), C.c_double]
         L.neo_ao_flux_solve.argtypes = [P(abi.Grid), P(abi.Params), P(abi.ExchangeState), P(abi.OceanState),
                                         P(abi.InterfaceFluxes)]
@@ -262,17 +264,21 @@ class OracleModel:
                                           np_view(self.lw), np_view(self.flux_arrays["interface_temperature"]),
                                           C.byref(self.net), C.byref(self.radiative))
 
-    def tendency(self, T=None, S=None):
+    def tendency(self, T=None, S=None, scales: bool = False):
+        """G_T, G_S (and, with scales=True, Σ|terms|/V of each cell for T and S)."""
         Nx, Ny, Nz = self.case.shape
         GT = np.zeros((Nz, Ny, Nx))
         GS = np.zeros((Nz, Ny, Nx))
+        sT = np.zeros((Nz, Ny, Nx))
+        sS = np.zeros((Nz, Ny, Nx))
         o = self.ocean
         if T is not None:
             T = np.ascontiguousarray(T)
             S = np.ascontiguousarray(S)
             o = abi.OceanState(o.u, o.v, o.w, np_view(T), np_view(S))
-        lib().neo_tracer_tendency(C.byref(self.grid), C.byref(o), C.byref(self.net), np_view(GT), np_view(GS))
-        return GT, GS
+        lib().neo_tracer_tendency_scaled(C.byref(self.grid), C.byref(o), C.byref(self.net), np_view(GT), np_view(GS),
+                                         np_view(sT), np_view(sS))
+        return (GT, GS, sT, sS) if scales else (GT, GS)
 
     def budget(self, c: np.ndarray, weight: float = 1.0):
         out = (C.c_double * 2)()

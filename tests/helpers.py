@@ -17,6 +17,20 @@ def rel_err(a, b, floor_frac=1e-6):
     return elem, field
 
 
+def assert_rel_terms(a, b, scale, rtol, name=""):
+    """|a − b| ≤ rtol·max(|b|, scale) elementwise, where scale is the magnitude of the terms that were
+    summed into b (a sum that cancels to near zero is compared against the size of its terms)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    ref = np.maximum(np.abs(b), np.asarray(scale, dtype=np.float64))
+    diff = np.abs(a - b)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        rel = np.where(ref > 0, diff / ref, np.where(diff > 0, np.inf, 0.0))
+    worst = float(rel.max(initial=0.0))
+    assert worst <= rtol, f"{name}: relative error {worst:.3e} (vs max(|G|, Σ|terms|)) > {rtol:.1e}"
+    return worst
+
+
 def assert_rel(a, b, rtol, name=""):
     elem, field = rel_err(a, b)
     assert np.all(np.isfinite(a)) == np.all(np.isfinite(b)), f"{name}: finiteness differs"

@@ -1,16 +1,18 @@
 """Parity of the HIP path (through the C ABI) with the CPU oracle on the same seeded inputs.
 
 Tolerances (FP64): every row is compared elementwise with relative error ≤ 1e-12 (and
-≤ 1e-12 relative to the field's largest magnitude for elements near zero). The land /
-immersed indexing (first active level per column) must match bit for bit. Rows whose
-arithmetic uses only correctly rounded operations (interpolation, assembly, radiation,
-WENO tendency, RK stage) are additionally checked bit for bit.
+≤ 1e-12 relative to the field's largest magnitude for elements near zero). The WENO5
+tendency, a sum of face fluxes that may cancel, is compared per cell relative to
+max(|G|, Σ|flux terms|/V). The land / immersed indexing (first active level per column)
+must match bit for bit, and so must the rows whose GPU arithmetic is the oracle's
+sequence of correctly rounded operations (interpolation, freeze clamp, net-flux
+assembly, radiation).
 """
 import numpy as np
 import pytest
 import torch
 
-from helpers import assert_rel, rel_err
+from helpers import assert_rel, assert_rel_terms, rel_err
 
 import oracle as O
 from numericalearth_hip import _ctypes_abi as abi
@@ -154,20 +156,30 @@ def _sync_net(gm, om):
         gm.net[k].copy_(torch.from_numpy(om.net_arrays[k]))
 
 
-@pytest.mark.parametrize("shape,topology_x", [(SMALL, 0), ((37, 23, 7), 0), ((48, 30, 8), 1)])
-def test_tendency_bit_exact(require_gpu, shape, topology_x):
+TENDENCY_SHAPES = [(SMALL, 0), ((37, 23, 7), 0), ((48, 30, 8), 1), ((130, 17, 12), 0), ((5, 4, 3), 0)]
+
+
+@pytest.mark.parametrize("shape,topology_x", TENDENCY_SHAPES)
+def test_tendency_matches_oracle(require_gpu, shape, topology_x):
+    """The WENO5 tendency (order reduction at immersed / bounded faces, top flux BC). The GPU kernel
+    uses fused multiply-adds and a one-division WENO-Z normalisation, so parity is to 1e-12 relative
+    to max(|G|, Σ|flux terms|/V) per cell: G is a sum of face fluxes that can cancel."""
     case, om, gm = _pair(shape, seed=6, topology_x=topology_x, longitude=(0.0, 360.0) if topology_x == 0 else (20.0, 80.0))
     om.reconcile()
     _sync_net(gm, om)
-    GT_o, GS_o = om.tendency()
+    GT_o, GS_o, sT, sS = om.tendency(scales=True)
     GT_g, GS_g = Phases(gm).tendency()
     torch.cuda.synchronize()
-    assert np.array_equal(_np(GT_g), GT_o)
-    assert np.array_equal(_np(GS_g), GS_o)
+    eT = assert_rel_terms(_np(GT_g), GT_o, sT, RTOL, "G_T")
+    eS = assert_rel_terms(_np(GS_g), GS_o, sS, RTOL, "G_S")
+    fT = rel_err(_np(GT_g), GT_o)[1]
+    print(f"[tendency] {shape}: G_T {eT:.2e} G_S {eS:.2e} (field-scaled G_T {fT:.2e})", flush=True)
+    active = np.arange(case.grid.spec.Nz)[:, None, None] >= case.grid.kmin[None]
+    assert np.all(_np(GT_g)[~active] == 0) and np.all(_np(GS_g)[~active] == 0)
     assert np.abs(GT_o).max() > 0
 
 
-def test_rk_stage_bit_exact(require_gpu):
+def test_rk_stage_matches_oracle(require_gpu):
     case, om, gm = _pair(seed=7)
     om.reconcile()
     _sync_net(gm, om)
@@ -178,7 +190,11 @@ def test_rk_stage_bit_exact(require_gpu):
     GT, GS = om.tendency()
     active = np.arange(case.grid.spec.Nz)[:, None, None] >= case.grid.kmin[None]
     Texp = np.where(active, case.T + 400.0 * GT, 0.0)
-    assert np.array_equal(np.where(active, _np(T1), 0.0), Texp)
+    Sexp = np.where(active, case.S + 400.0 * GS, 0.0)
+    assert_rel(np.where(active, _np(T1), 0.0), Texp, RTOL, "T1")
+    assert_rel(np.where(active, _np(S1), 0.0), Sexp, RTOL, "S1")
+    # inactive cells of the output are not written
+    assert np.all(_np(T1)[~active] == 0)
 
 
 @pytest.mark.parametrize("precision", ["fp64"])

@@ -193,6 +193,99 @@ __device__ __forceinline__ double psi_scalar(const ne_params& p, double z) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Stability functions with the reference's default Edson constants, arranged so that the
+// momentum and scalar functions at the same ζ share their common subexpressions:
+// √(1−15ζ) (its square root is the momentum's fourth root), log((1+√(1−15ζ))/2), the
+// blend weight ζ²/(1+ζ²), and the stable-branch exp(−min(ζmax, 0.35ζ)). Near ζ = 0
+// (|ζ| ≤ NE_PSI_SERIES_RADIUS, which is where ψ(ℓ/L) at the roughness length almost
+// always falls) the functions are Taylor polynomials (ne_psi_series.h), accurate to
+// < 1e-17 there. Same values as psi_momentum / psi_scalar up to rounding.
+// ---------------------------------------------------------------------------
+
+#include "ne_psi_series.h"
+#define NE_PSI_SERIES_RADIUS 5e-4
+
+__device__ __forceinline__ double horner7(const double* c, double z) {
+    double r = c[7];
+#pragma unroll
+    for (int n = 6; n >= 0; n--) r = r * z + c[n];
+    return r;
+}
+
+__device__ __forceinline__ double psi_m_default(double z) {
+    if (fabs(z) <= NE_PSI_SERIES_RADIUS) return horner7(z < 0.0 ? kPsiMomentumUnstable : kPsiMomentumStable, z);
+    const double pi = 3.14159265358979323846, s3 = 1.7320508075688772935;
+    if (z < 0.0) {
+        double xs = sqrt(1.0 - 15.0 * z), xm = sqrt(xs);
+        double psi1 = 2.0 * log((1.0 + xm) * 0.5) + log((1.0 + xs) * 0.5) - 2.0 * atan(xm) + pi / 2.0;
+        double f2 = cbrt(1.0 - 10.15 * z);
+        double psi2 = 1.5 * log((1.0 + f2 + f2 * f2) * (1.0 / 3.0)) - s3 * atan((1.0 + 2.0 * f2) * (1.0 / s3)) + pi / s3;
+        double f = z * z / (1.0 + z * z);
+        return (1.0 - f) * psi1 + f * psi2;
+    }
+    double e = exp(-fmin(50.0, 0.35 * z));
+    return -0.7 * z - 0.75 * (z - 5.0 / 0.35) * e - 0.75 * (5.0 / 0.35);
+}
+
+__device__ __forceinline__ double psi_s_default(double z) {
+    if (fabs(z) <= NE_PSI_SERIES_RADIUS) return horner7(z < 0.0 ? kPsiScalarUnstable : kPsiScalarStable, z);
+    const double pi = 3.14159265358979323846, s3 = 1.7320508075688772935;
+    if (z < 0.0) {
+        double xs = sqrt(1.0 - 15.0 * z);
+        double psi1 = 2.0 * log((1.0 + xs) * 0.5);
+        double f2 = cbrt(1.0 - 34.15 * z);
+        double psi2 = 1.5 * log((1.0 + f2 + f2 * f2) * (1.0 / 3.0)) - s3 * atan((1.0 + 2.0 * f2) * (1.0 / s3)) + pi / s3;
+        double f = z * z / (1.0 + z * z);
+        return (1.0 - f) * psi1 + f * psi2;
+    }
+    double e = exp(-fmin(50.0, 0.35 * z));
+    double x = 1.0 + (2.0 / 3.0) * z;
+    return -(x * sqrt(x)) - (2.0 / 3.0) * (z - 14.28) * e - 8.525;
+}
+
+// ψ_m(ζ) and ψ_s(ζ) together (the height term h/L of the two profiles).
+__device__ __forceinline__ void psi_pair_default(double z, double& pm, double& ps) {
+    if (fabs(z) <= NE_PSI_SERIES_RADIUS) {
+        bool u = z < 0.0;
+        pm = horner7(u ? kPsiMomentumUnstable : kPsiMomentumStable, z);
+        ps = horner7(u ? kPsiScalarUnstable : kPsiScalarStable, z);
+        return;
+    }
+    const double pi = 3.14159265358979323846, s3 = 1.7320508075688772935;
+    if (z < 0.0) {
+        double xs = sqrt(1.0 - 15.0 * z), xm = sqrt(xs);
+        double l2 = log((1.0 + xs) * 0.5);
+        double psi1m = 2.0 * log((1.0 + xm) * 0.5) + l2 - 2.0 * atan(xm) + pi / 2.0;
+        double psi1s = 2.0 * l2;
+        double fm = cbrt(1.0 - 10.15 * z), fs = cbrt(1.0 - 34.15 * z);
+        double psi2m = 1.5 * log((1.0 + fm + fm * fm) * (1.0 / 3.0)) - s3 * atan((1.0 + 2.0 * fm) * (1.0 / s3)) + pi / s3;
+        double psi2s = 1.5 * log((1.0 + fs + fs * fs) * (1.0 / 3.0)) - s3 * atan((1.0 + 2.0 * fs) * (1.0 / s3)) + pi / s3;
+        double f = z * z / (1.0 + z * z);
+        pm = (1.0 - f) * psi1m + f * psi2m;
+        ps = (1.0 - f) * psi1s + f * psi2s;
+        return;
+    }
+    double e = exp(-fmin(50.0, 0.35 * z));
+    pm = -0.7 * z - 0.75 * (z - 5.0 / 0.35) * e - 0.75 * (5.0 / 0.35);
+    double x = 1.0 + (2.0 / 3.0) * z;
+    ps = -(x * sqrt(x)) - (2.0 / 3.0) * (z - 14.28) * e - 8.525;
+}
+
+// True when the flux formulation uses the reference's default Edson constants with the
+// logarithmic similarity profile (the case the *_default functions above implement).
+__host__ inline bool edson_default_constants(const ne_params& p) {
+    const double pi = 3.14159265358979323846;
+    const double em[11] = {50.0, 0.35, 0.7, 0.75, 5.0 / 0.35, 15.0, 2.0, pi / 2.0, 10.15, 3.0, pi / sqrt(3.0)};
+    const double es[12] = {50.0, 0.35, 2.0 / 3.0, 3.0 / 2.0, 14.28, 8.525, 15.0, 2.0, 0.0, 34.15, 3.0, pi / sqrt(3.0)};
+    if (p.stability_functions != NE_STABILITY_EDSON || p.similarity_form != NE_SIMILARITY_LOG) return false;
+    for (int n = 0; n < 11; n++)
+        if (p.edson_momentum[n] != em[n]) return false;
+    for (int n = 0; n < 12; n++)
+        if (p.edson_scalar[n] != es[n]) return false;
+    return true;
+}
+
 // similarity_profile (similarity_theory_turbulent_fluxes.jl:242-253)
 __device__ __forceinline__ double similarity_profile(const ne_params& p, bool scalar, double h, double l, double L) {
     double zeta = h / L;
@@ -246,6 +339,9 @@ struct ColumnFluxes {
 // iterate_interface_fluxes(::SimilarityTheoryFluxes) (similarity:315-385).
 // Quantities that the reference re-evaluates each iteration from unchanged inputs
 // (qₛ, θᵃ, Δθ, Δq, 𝒯ₛ) are evaluated once: the values are identical.
+// Fast = true (only for the default Edson constants, edson_default_constants) evaluates the
+// two profiles with the shared/series stability functions above and R★^b as exp(b log R★).
+template <bool Fast>
 __device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p, double ua, double va, double Ta,
                                                     double pa, double qa, double To_C, double uo, double vo,
                                                     bool inactive) {
@@ -289,14 +385,30 @@ __device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p, double u
             double Usg2 = gustiness_squared(p, us, bstar, hbl);
             double U = sqrt(du2dv2 + Usg2);
             double lu = momentum_roughness(p, us);
-            double lq = scalar_roughness(p, p.water_vapor_roughness, p.water_vapor_roughness_length, lu, us);
-            double lt = shared_scalar ? lq
-                                      : scalar_roughness(p, p.temperature_roughness, p.temperature_roughness_length, lu, us);
+            double lq, lt;
+            if (Fast && p.water_vapor_roughness == NE_ROUGHNESS_EDSON) {
+                double Rstar = lu * us / p.air_kinematic_viscosity;
+                double ls = Rstar == 0.0 ? 0.0 : p.reynolds_scaling_A * exp(-p.reynolds_scaling_b * log(Rstar));
+                lq = jmin(ls, p.maximum_scalar_roughness);
+            } else {
+                lq = scalar_roughness(p, p.water_vapor_roughness, p.water_vapor_roughness_length, lu, us);
+            }
+            lt = shared_scalar ? lq
+                               : scalar_roughness(p, p.temperature_roughness, p.temperature_roughness_length, lu, us);
             double dh = jmax(za - p.zero_plane_displacement, 2.0 * lu);
             double L = bstar == 0.0 ? (double)INFINITY : us * us / (kappa * bstar);
-            double chiu = kappa / similarity_profile(p, false, dh, lu, L);
-            double chiq = kappa / similarity_profile(p, true, dh, lq, L);
-            double chit = shared_scalar ? chiq : kappa / similarity_profile(p, true, dh, lt, L);
+            double chiu, chiq, chit;
+            if (Fast) {
+                double pmh, psh;
+                psi_pair_default(dh / L, pmh, psh);
+                chiu = kappa / (log(dh / lu) - pmh + psi_m_default(lu / L));
+                chiq = kappa / (log(dh / lq) - psh + psi_s_default(lq / L));
+                chit = shared_scalar ? chiq : kappa / (log(dh / lt) - psh + psi_s_default(lt / L));
+            } else {
+                chiu = kappa / similarity_profile(p, false, dh, lu, L);
+                chiq = kappa / similarity_profile(p, true, dh, lq, L);
+                chit = shared_scalar ? chiq : kappa / similarity_profile(p, true, dh, lt, L);
+            }
             double nus = chiu * U, nts = chit * dtheta, nqs = chiq * dq;
             it += 1;
             bool go;

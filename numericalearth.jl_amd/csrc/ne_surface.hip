@@ -14,6 +14,11 @@ namespace {
 constexpr int BX = 64;
 constexpr int BY = 4;
 
+// Minimum wavefronts per SIMD requested for the flux-solve kernels (bounds their VGPRs).
+#ifndef NE_FLUX_WAVES_PER_EU
+#define NE_FLUX_WAVES_PER_EU 1
+#endif
+
 inline dim3 grid2d(int Nx, int Ny) { return dim3((Nx + BX - 1) / BX, (Ny + BY - 1) / BY); }
 
 // Block-wide reduction of the solver counters, then one atomic per block.
@@ -128,7 +133,9 @@ __device__ __forceinline__ void ocean_surface(const GridDev& g, const View3<cons
     vo = ((double)v(i, j, kN) + (double)v(i, jp, kN)) / 2.0;
 }
 
-__global__ void flux_solve_exchange_kernel(GridDev g, ne_params p, View3<const double> ua, View3<const double> va,
+template <bool Fast>
+__global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(NE_FLUX_WAVES_PER_EU)))
+flux_solve_exchange_kernel(GridDev g, ne_params p, View3<const double> ua, View3<const double> va,
                                            View3<const double> Ta, View3<const double> pa, View3<const double> qa,
                                            View3<const double> T, View3<const double> S, View3<const double> u,
                                            View3<const double> v, FluxOut out, ne_counters* counters) {
@@ -141,8 +148,8 @@ __global__ void flux_solve_exchange_kernel(GridDev g, ne_params p, View3<const d
         double To, So, uo, vo;
         ocean_surface<double>(g, T, S, u, v, i, j, To, So, uo, vo);
         inactive = !g.active(i, j, g.Nz - 1);
-        r = flux_column(p, ua.at2(i, j), va.at2(i, j), Ta.at2(i, j), pa.at2(i, j), qa.at2(i, j), To, uo, vo,
-                        inactive);
+        r = flux_column<Fast>(p, ua.at2(i, j), va.at2(i, j), Ta.at2(i, j), pa.at2(i, j), qa.at2(i, j), To, uo, vo,
+                              inactive);
         store_fluxes(out, i, j, r);
     }
     bool solved = in && !inactive;
@@ -151,8 +158,8 @@ __global__ void flux_solve_exchange_kernel(GridDev g, ne_params p, View3<const d
 }
 
 // K1 of the fused update_state: interpolate the atmosphere at the column and solve.
-template <typename Real>
-__global__ void __launch_bounds__(BX* BY)
+template <typename Real, bool Fast>
+__global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(NE_FLUX_WAVES_PER_EU)))
 surface_solve_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __restrict__ fi,
                      const double* __restrict__ fj, View3<const Real> T, View3<const Real> S, View3<const Real> u,
                      View3<const Real> v, FluxOut out, ne_counters* counters) {
@@ -172,7 +179,7 @@ surface_solve_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __r
         double To, So, uo, vo;
         ocean_surface<Real>(g, T, S, u, v, i, j, To, So, uo, vo);
         inactive = !g.active(i, j, g.Nz - 1);
-        r = flux_column(p, ua, va, Ta, pa, qa, To, uo, vo, inactive);
+        r = flux_column<Fast>(p, ua, va, Ta, pa, qa, To, uo, vo, inactive);
         store_fluxes(out, i, j, r);
     }
     bool solved = in && !inactive;
@@ -437,10 +444,16 @@ template hipError_t launch_interp_radiation<float>(const GridDev&, const RadDev<
 hipError_t launch_flux_solve_exchange(const GridDev& g, const ne_params& p, const ne_exchange_state& a,
                                       const ne_ocean_state& o, const FluxOut& out, ne_counters* c,
                                       hipStream_t s) {
-    hipLaunchKernelGGL(flux_solve_exchange_kernel, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p,
-                       view<const double>(a.u), view<const double>(a.v), view<const double>(a.T),
-                       view<const double>(a.p), view<const double>(a.q), view<const double>(o.T),
-                       view<const double>(o.S), view<const double>(o.u), view<const double>(o.v), out, c);
+    if (edson_default_constants(p))
+        hipLaunchKernelGGL(flux_solve_exchange_kernel<true>, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p,
+                           view<const double>(a.u), view<const double>(a.v), view<const double>(a.T),
+                           view<const double>(a.p), view<const double>(a.q), view<const double>(o.T),
+                           view<const double>(o.S), view<const double>(o.u), view<const double>(o.v), out, c);
+    else
+        hipLaunchKernelGGL(flux_solve_exchange_kernel<false>, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p,
+                           view<const double>(a.u), view<const double>(a.v), view<const double>(a.T),
+                           view<const double>(a.p), view<const double>(a.q), view<const double>(o.T),
+                           view<const double>(o.S), view<const double>(o.u), view<const double>(o.v), out, c);
     return hipGetLastError();
 }
 
@@ -468,8 +481,12 @@ template <typename Real>
 hipError_t launch_surface_solve(const GridDev& g, const ne_params& p, const AtmosDev<Real>& a, const double* fi,
                                 const double* fj, View3<const Real> T, View3<const Real> S, View3<const Real> u,
                                 View3<const Real> v, const FluxOut& out, ne_counters* c, hipStream_t s) {
-    hipLaunchKernelGGL(surface_solve_kernel<Real>, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, a, fi, fj, T, S,
-                       u, v, out, c);
+    if (edson_default_constants(p))
+        hipLaunchKernelGGL((surface_solve_kernel<Real, true>), grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, a, fi,
+                           fj, T, S, u, v, out, c);
+    else
+        hipLaunchKernelGGL((surface_solve_kernel<Real, false>), grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, a, fi,
+                           fj, T, S, u, v, out, c);
     return hipGetLastError();
 }
 template hipError_t launch_surface_solve<double>(const GridDev&, const ne_params&, const AtmosDev<double>&,

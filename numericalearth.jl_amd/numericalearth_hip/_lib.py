@@ -7,6 +7,7 @@ repository's ``__graft_entry__.build()``).
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 from . import _ctypes_abi as abi
@@ -69,7 +70,8 @@ def load(path: str | Path | None = None):
     global _lib
     if _lib is not None and path is None:
         return _lib
-    p = Path(path) if path else library_path()
+    env = os.environ.get("NE_HIP_LIBRARY")
+    p = Path(path) if path else (Path(env) if env else library_path())
     if not p.exists():
         raise RuntimeError(f"HIP library not found at {p}; build it with `python -m numericalearth_hip.build`")
     L = C.CDLL(str(p))

@@ -36,7 +36,130 @@ def parse():
     ap.add_argument("--dt", type=float, default=1200.0)
     ap.add_argument("--cpu-baseline", choices=("on", "off"), default="on")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
+    ap.add_argument("--mixed-report", action="store_true",
+                    help="run the FP64 and mixed forms side by side for --steps steps and report their differences")
+    ap.add_argument("--ensemble", type=int, default=0, metavar="N",
+                    help="ensemble report: N uneven members stepped as one batch (autotuned, graph) "
+                         "against the same members stepped one after another")
+    ap.add_argument("--copy-gib", type=float, default=0.0,
+                    help="only run a device-to-device copy of this many GiB (--steps times) and report its bandwidth")
     return ap.parse_args()
+
+
+def copy_bandwidth(gib: float, reps: int) -> dict:
+    """Device-to-device copy (torch's copy kernel) of `gib` GiB: read + write bytes per second."""
+    import torch
+    n = int(gib * (1 << 30)) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return {"bytes_per_copy": 2 * n * 8, "ms_per_copy": ms, "gbs": 2 * n * 8 / (ms * 1e-3) / 1e9}
+
+
+def ensemble_report(args):
+    """Throughput of an ensemble stepped as one batch against its members stepped one after another."""
+    import torch
+    from numericalearth_hip.ensemble import Ensemble, build_members, draw_ensemble
+
+    specs = draw_ensemble(args.ensemble, args.seed)
+    members = build_members(specs, precision=args.precision)
+    cells = sum(s.cells for s in specs)
+    torch.cuda.synchronize()
+
+    def timed(step, k):
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / k
+
+    def sequential():
+        for m in members:
+            m.time_step(args.dt)
+
+    t_seq = timed(sequential, args.steps)
+    ens = Ensemble(members)
+    default = ens.tuning
+    t_default = timed(lambda: ens.time_step(args.dt), args.steps)
+    chosen = ens.autotune()
+    t_tuned = timed(lambda: ens.time_step(args.dt), args.steps)
+    ens.use_graph(True)
+    t_graph = timed(lambda: ens.time_step(args.dt), args.steps)
+    info = ens.info()
+    ens.close()
+    shapes = [(s.Nx, s.Ny, s.Nz) for s in specs]
+    out = {"report": "ensemble", "members": len(specs), "cells": cells, "precision": args.precision,
+           "steps": args.steps, "dt": args.dt,
+           "smallest": min(shapes, key=lambda s: s[0] * s[1] * s[2]),
+           "largest": max(shapes, key=lambda s: s[0] * s[1] * s[2]),
+           "sequential_ms_per_step": 1e3 * t_seq, "sequential_cell_updates_per_s": cells / t_seq,
+           "batched_default_ms_per_step": 1e3 * t_default, "batched_tuned_ms_per_step": 1e3 * t_tuned,
+           "batched_graph_ms_per_step": 1e3 * t_graph, "batched_graph_cell_updates_per_s": cells / t_graph,
+           "speedup_graph_over_sequential": t_seq / t_graph, "default_tuning": default, "autotuned": chosen,
+           "work_items": info}
+    print(json.dumps(out), flush=True)
+
+
+def mixed_report(args):
+    """SST, surface heat flux and heat-budget differences of the mixed form against the FP64 form
+    (which matches the FP64 oracle to 1e-12 per step) after args.steps steps."""
+    import numpy as np
+    import torch
+    from numericalearth_hip.generators import make_case
+    from numericalearth_hip.model import OceanOnlyModel
+
+    Nx, Ny, Nz = WORKLOADS[args.workload]
+    case = make_case(Nx, Ny, Nz, seed=args.seed)
+    m64 = OceanOnlyModel(case, precision="fp64")
+    m32 = OceanOnlyModel(case, precision="mixed")
+    rho_c = m64.params.ocean_reference_density * m64.params.ocean_heat_capacity
+    H0 = m64.heat_content()
+    t64 = t32 = 0.0
+    for n in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m64.time_step(args.dt)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        m32.time_step(args.dt)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        t64 += t1 - t0
+        t32 += t2 - t1
+        if (n + 1) % 100 == 0:
+            print(f"# step {n + 1}", file=sys.stderr, flush=True)
+    wet = case.grid.kmin < Nz
+    sst64 = m64.T[-1].cpu().numpy()
+    sst32 = m32.T[-1].cpu().numpy().astype(np.float64)
+    out = {"report": "mixed_vs_fp64", "workload": args.workload, "steps": args.steps, "dt": args.dt}
+    diff = np.abs(sst32 - sst64)[wet]
+    out["sst_rel_err"] = float(diff.max() / np.abs(sst64[wet]).max())        # max |ΔSST| / max |SST|
+    out["sst_max_pointwise_rel_err"] = float(np.max(diff / np.abs(sst64[wet])))
+    out["sst_max_abs_err_K"] = float(diff.max())
+    out["sst_rms_err_K"] = float(np.sqrt(np.mean(diff ** 2)))
+    for k in ("sensible_heat", "latent_heat", "J_T"):
+        src = m64.net if k == "J_T" else m64.fluxes
+        dst = m32.net if k == "J_T" else m32.fluxes
+        a, b = dst[k].cpu().numpy(), src[k].cpu().numpy()
+        out[f"{k}_max_err_over_scale"] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    H64, H32 = m64.heat_content(), m32.heat_content()
+    out["heat_content_rel_diff"] = abs(H32 - H64) / abs(H64)
+    out["heat_content_change_fp64"] = (H64 - H0) / abs(H0)
+    out["ms_per_step_fp64"] = 1e3 * t64 / args.steps
+    out["ms_per_step_mixed"] = 1e3 * t32 / args.steps
+    out["speedup_mixed_over_fp64"] = t64 / t32
+    print(json.dumps(out), flush=True)
 
 
 def cpu_baseline(case, dt: float, target_s: float) -> dict:
@@ -74,6 +197,16 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU layout is a design "
                                    "(latitude-band halo exchange), not an implementation", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.mixed_report:
+        mixed_report(args)
+        return
+    if args.ensemble:
+        ensemble_report(args)
+        return
+    if args.copy_gib > 0:
+        print(json.dumps(dict(copy_bandwidth(args.copy_gib, args.steps), report="copy", gib=args.copy_gib)),
+              flush=True)
+        return
     import torch
     from numericalearth_hip.generators import make_case
     from numericalearth_hip.model import OceanOnlyModel, bytes_per_cell_update
@@ -132,6 +265,9 @@ def main():
                        "nan_columns": counters["nan_columns"]},
     }
     model.close()
+    copy = copy_bandwidth(2.0, 20)
+    out["roofline"]["measured_copy_gbs"] = copy["gbs"]
+    out["roofline"]["frac_of_measured_copy"] = achieved / copy["gbs"]
     if args.cpu_baseline == "on":
         out["cpu_baseline"] = cpu_baseline(case, args.dt, args.cpu_seconds)
     print(json.dumps(out), flush=True)

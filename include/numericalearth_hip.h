@@ -405,6 +405,45 @@ int ne_model_clock(const ne_model* model, double* time, int64_t* iteration);
 int ne_model_timing_begin(ne_model* model, int max_steps);
 int ne_model_timing_end(ne_model* model, double* ms, int* steps);
 
+/* ---------------------------------------------------------------------------
+ * Batches: ensembles of models stepped together (no counterpart in the reference,
+ * which steps one model per time_step! call; this is the form for calibration
+ * ensembles and request serving). A batch binds existing models of one precision;
+ * each ne_batch_time_step is time_step!(model, Δt) for every member, issued as six
+ * persistent-grid launches for the whole ensemble (optionally one HIP graph launch).
+ * Results are identical to stepping each member with ne_model_time_step. While a
+ * model belongs to a batch, ne_model_time_step / ne_model_update_state /
+ * ne_model_destroy on it return NE_ERR_STATE; ne_batch_destroy releases the members.
+ * ------------------------------------------------------------------------- */
+
+typedef struct ne_batch ne_batch;
+
+/* Launch shape of a batch step. tile_rows ∈ {4, 8, 16} is the tile height of the WENO5
+ * stage kernel (tiles are 63 columns wide, one wavefront per row); stage_blocks (a
+ * multiple of 8) is its persistent grid; surface_block ∈ {64, 128, 256} columns per work
+ * item and surface_blocks the grid of the two surface kernels. stage_ms / surface_ms are
+ * the timings the autotuner measured (0 when set by hand). */
+typedef struct ne_batch_tuning {
+    int32_t tile_rows;
+    int32_t stage_blocks;
+    int32_t surface_block;
+    int32_t surface_blocks;
+    double  stage_ms;
+    double  surface_ms;
+} ne_batch_tuning;
+
+int ne_batch_create(ne_ctx* ctx, int n_models, ne_model* const* models, ne_batch** batch, void* stream);
+int ne_batch_destroy(ne_batch* batch);
+int ne_batch_time_step(ne_batch* batch, double dt, void* stream);
+/* enabled != 0: capture the step once per Δt into a HIP graph and replay it. */
+int ne_batch_set_graph(ne_batch* batch, int enabled);
+/* Times every candidate shape on the batch itself (RK stage 1 into the stage buffers and
+ * the idempotent surface solve; the model state is unchanged) and keeps the fastest. */
+int ne_batch_autotune(ne_batch* batch, ne_batch_tuning* chosen, void* stream);
+int ne_batch_get_tuning(const ne_batch* batch, ne_batch_tuning* tuning);
+int ne_batch_set_tuning(ne_batch* batch, const ne_batch_tuning* tuning);
+int ne_batch_info(const ne_batch* batch, int* members, int64_t* cells, int* stage_items, int* surface_items);
+
 #ifdef __cplusplus
 }
 #endif

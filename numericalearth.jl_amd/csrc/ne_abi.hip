@@ -94,39 +94,11 @@ int check_params(const ne_params* p) {
     return NE_OK;
 }
 
-// cpu_interpolating_time_indices with Linear time indexing, clamped at the ends.
 ne_time_interp time_indices(const std::vector<double>& times, double t) {
-    ne_time_interp r = {0, 0, 0.0};
-    int n = (int)times.size();
-    if (n <= 1 || t <= times[0]) return r;
-    if (t >= times[n - 1]) { r.n1 = r.n2 = n - 1; return r; }
-    int k = 0;
-    while (k + 1 < n && times[k + 1] <= t) k++;
-    r.n1 = k;
-    r.n2 = k + 1;
-    r.tilde_n = (t - times[k]) / (times[k + 1] - times[k]);
-    return r;
+    return ne::time_indices(times.data(), (int)times.size(), t);
 }
 
 }  // namespace
-
-struct ne_model {
-    ne_ctx* ctx;
-    ne_model_desc d;
-    std::vector<double> atimes, rtimes;
-    void* T1 = nullptr;
-    void* S1 = nullptr;
-    void* T2 = nullptr;
-    void* S2 = nullptr;
-    ne_view vT1, vS1, vT2, vS2;
-    double time = 0.0;
-    double last_dt = 0.0;
-    int64_t iteration = 0;
-    bool timing = false;
-    int max_steps = 0;
-    int recorded = 0;
-    std::vector<hipEvent_t> events;
-};
 
 extern "C" {
 
@@ -486,6 +458,7 @@ int ne_model_create(ne_ctx* c, const ne_model_desc* desc, ne_model** out, void* 
 
 int ne_model_destroy(ne_model* m) {
     if (!m) return NE_ERR_NULL;
+    if (m->batch) return NE_ERR_STATE;
     int r = NE_OK;
     void* bufs[4] = {m->T1, m->S1, m->T2, m->S2};
     for (void* b : bufs)
@@ -497,6 +470,7 @@ int ne_model_destroy(ne_model* m) {
 
 int ne_model_time_step(ne_model* m, double dt, void* stream) {
     if (!m) return NE_ERR_NULL;
+    if (m->batch) return NE_ERR_STATE;
     if (!(dt > 0)) return NE_ERR_SHAPE;
     hipStream_t s = as_stream(stream);
     hipEvent_t* ev = nullptr;
@@ -544,6 +518,7 @@ int ne_model_timing_end(ne_model* m, double* ms, int* steps) {
 
 int ne_model_update_state(ne_model* m, void* stream) {
     if (!m) return NE_ERR_NULL;
+    if (m->batch) return NE_ERR_STATE;
     return model_update_state(m, m->iteration == 0 ? INFINITY : m->last_dt, as_stream(stream));
 }
 

@@ -30,6 +30,16 @@ struct View3 {
 };
 
 template <typename T>
+__host__ __device__ inline View3<const T> cview(const View3<T>& v) {
+    View3<const T> r;
+    r.p = v.p;
+    r.sx = v.sx;
+    r.sy = v.sy;
+    r.sz = v.sz;
+    return r;
+}
+
+template <typename T>
 __host__ inline View3<T> view(const ne_view& v) {
     View3<T> r;
     r.p = static_cast<T*>(v.data);
@@ -206,70 +216,79 @@ __device__ __forceinline__ double psi_scalar(const ne_params& p, double z) {
 #include "ne_psi_series.h"
 #define NE_PSI_SERIES_RADIUS 5e-4
 
-__device__ __forceinline__ double horner7(const double* c, double z) {
-    double r = c[7];
-#pragma unroll
-    for (int n = 6; n >= 0; n--) r = r * z + c[n];
-    return r;
+// Elementary functions by precision: FP64 uses the accurate device library; the FP32
+// pre-iteration of the mixed form uses the hardware log₂/exp₂-based approximations.
+__device__ __forceinline__ double tlog(double x) { return log(x); }
+__device__ __forceinline__ float tlog(float x) { return __logf(x); }
+__device__ __forceinline__ double texp(double x) { return exp(x); }
+__device__ __forceinline__ float texp(float x) { return __expf(x); }
+__device__ __forceinline__ double tatan(double x) { return atan(x); }
+__device__ __forceinline__ float tatan(float x) { return atanf(x); }
+__device__ __forceinline__ double tcbrt(double x) { return cbrt(x); }
+__device__ __forceinline__ float tcbrt(float x) { return cbrtf(x); }
+__device__ __forceinline__ double tsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float tsqrt(float x) { return sqrtf(x); }
+
+template <typename T>
+__device__ __forceinline__ T psi_m_default(T z) {
+    if (fabs(z) <= (T)NE_PSI_SERIES_RADIUS) return z < (T)0 ? psi_series_momentum_unstable(z) : psi_series_momentum_stable(z);
+    const T pi = (T)3.14159265358979323846, s3 = (T)1.7320508075688772935;
+    if (z < (T)0) {
+        T xs = tsqrt((T)1 - (T)15 * z), xm = tsqrt(xs);
+        T psi1 = (T)2 * tlog(((T)1 + xm) * (T)0.5) + tlog(((T)1 + xs) * (T)0.5) - (T)2 * tatan(xm) + pi / (T)2;
+        T f2 = tcbrt((T)1 - (T)10.15 * z);
+        T psi2 = (T)1.5 * tlog(((T)1 + f2 + f2 * f2) * (T)(1.0 / 3.0)) - s3 * tatan(((T)1 + (T)2 * f2) * ((T)1 / s3)) + pi / s3;
+        T f = z * z / ((T)1 + z * z);
+        return ((T)1 - f) * psi1 + f * psi2;
+    }
+    T e = texp(-fmin((T)50, (T)0.35 * z));
+    return (T)-0.7 * z - (T)0.75 * (z - (T)(5.0 / 0.35)) * e - (T)(0.75 * (5.0 / 0.35));
 }
 
-__device__ __forceinline__ double psi_m_default(double z) {
-    if (fabs(z) <= NE_PSI_SERIES_RADIUS) return horner7(z < 0.0 ? kPsiMomentumUnstable : kPsiMomentumStable, z);
-    const double pi = 3.14159265358979323846, s3 = 1.7320508075688772935;
-    if (z < 0.0) {
-        double xs = sqrt(1.0 - 15.0 * z), xm = sqrt(xs);
-        double psi1 = 2.0 * log((1.0 + xm) * 0.5) + log((1.0 + xs) * 0.5) - 2.0 * atan(xm) + pi / 2.0;
-        double f2 = cbrt(1.0 - 10.15 * z);
-        double psi2 = 1.5 * log((1.0 + f2 + f2 * f2) * (1.0 / 3.0)) - s3 * atan((1.0 + 2.0 * f2) * (1.0 / s3)) + pi / s3;
-        double f = z * z / (1.0 + z * z);
-        return (1.0 - f) * psi1 + f * psi2;
+template <typename T>
+__device__ __forceinline__ T psi_s_default(T z) {
+    if (fabs(z) <= (T)NE_PSI_SERIES_RADIUS) return z < (T)0 ? psi_series_scalar_unstable(z) : psi_series_scalar_stable(z);
+    const T pi = (T)3.14159265358979323846, s3 = (T)1.7320508075688772935;
+    if (z < (T)0) {
+        T xs = tsqrt((T)1 - (T)15 * z);
+        T psi1 = (T)2 * tlog(((T)1 + xs) * (T)0.5);
+        T f2 = tcbrt((T)1 - (T)34.15 * z);
+        T psi2 = (T)1.5 * tlog(((T)1 + f2 + f2 * f2) * (T)(1.0 / 3.0)) - s3 * tatan(((T)1 + (T)2 * f2) * ((T)1 / s3)) + pi / s3;
+        T f = z * z / ((T)1 + z * z);
+        return ((T)1 - f) * psi1 + f * psi2;
     }
-    double e = exp(-fmin(50.0, 0.35 * z));
-    return -0.7 * z - 0.75 * (z - 5.0 / 0.35) * e - 0.75 * (5.0 / 0.35);
-}
-
-__device__ __forceinline__ double psi_s_default(double z) {
-    if (fabs(z) <= NE_PSI_SERIES_RADIUS) return horner7(z < 0.0 ? kPsiScalarUnstable : kPsiScalarStable, z);
-    const double pi = 3.14159265358979323846, s3 = 1.7320508075688772935;
-    if (z < 0.0) {
-        double xs = sqrt(1.0 - 15.0 * z);
-        double psi1 = 2.0 * log((1.0 + xs) * 0.5);
-        double f2 = cbrt(1.0 - 34.15 * z);
-        double psi2 = 1.5 * log((1.0 + f2 + f2 * f2) * (1.0 / 3.0)) - s3 * atan((1.0 + 2.0 * f2) * (1.0 / s3)) + pi / s3;
-        double f = z * z / (1.0 + z * z);
-        return (1.0 - f) * psi1 + f * psi2;
-    }
-    double e = exp(-fmin(50.0, 0.35 * z));
-    double x = 1.0 + (2.0 / 3.0) * z;
-    return -(x * sqrt(x)) - (2.0 / 3.0) * (z - 14.28) * e - 8.525;
+    T e = texp(-fmin((T)50, (T)0.35 * z));
+    T x = (T)1 + (T)(2.0 / 3.0) * z;
+    return -(x * tsqrt(x)) - (T)(2.0 / 3.0) * (z - (T)14.28) * e - (T)8.525;
 }
 
 // ψ_m(ζ) and ψ_s(ζ) together (the height term h/L of the two profiles).
-__device__ __forceinline__ void psi_pair_default(double z, double& pm, double& ps) {
-    if (fabs(z) <= NE_PSI_SERIES_RADIUS) {
-        bool u = z < 0.0;
-        pm = horner7(u ? kPsiMomentumUnstable : kPsiMomentumStable, z);
-        ps = horner7(u ? kPsiScalarUnstable : kPsiScalarStable, z);
+template <typename T>
+__device__ __forceinline__ void psi_pair_default(T z, T& pm, T& ps) {
+    if (fabs(z) <= (T)NE_PSI_SERIES_RADIUS) {
+        bool u = z < (T)0;
+        pm = u ? psi_series_momentum_unstable(z) : psi_series_momentum_stable(z);
+        ps = u ? psi_series_scalar_unstable(z) : psi_series_scalar_stable(z);
         return;
     }
-    const double pi = 3.14159265358979323846, s3 = 1.7320508075688772935;
-    if (z < 0.0) {
-        double xs = sqrt(1.0 - 15.0 * z), xm = sqrt(xs);
-        double l2 = log((1.0 + xs) * 0.5);
-        double psi1m = 2.0 * log((1.0 + xm) * 0.5) + l2 - 2.0 * atan(xm) + pi / 2.0;
-        double psi1s = 2.0 * l2;
-        double fm = cbrt(1.0 - 10.15 * z), fs = cbrt(1.0 - 34.15 * z);
-        double psi2m = 1.5 * log((1.0 + fm + fm * fm) * (1.0 / 3.0)) - s3 * atan((1.0 + 2.0 * fm) * (1.0 / s3)) + pi / s3;
-        double psi2s = 1.5 * log((1.0 + fs + fs * fs) * (1.0 / 3.0)) - s3 * atan((1.0 + 2.0 * fs) * (1.0 / s3)) + pi / s3;
-        double f = z * z / (1.0 + z * z);
-        pm = (1.0 - f) * psi1m + f * psi2m;
-        ps = (1.0 - f) * psi1s + f * psi2s;
+    const T pi = (T)3.14159265358979323846, s3 = (T)1.7320508075688772935;
+    if (z < (T)0) {
+        T xs = tsqrt((T)1 - (T)15 * z), xm = tsqrt(xs);
+        T l2 = tlog(((T)1 + xs) * (T)0.5);
+        T psi1m = (T)2 * tlog(((T)1 + xm) * (T)0.5) + l2 - (T)2 * tatan(xm) + pi / (T)2;
+        T psi1s = (T)2 * l2;
+        T fm = tcbrt((T)1 - (T)10.15 * z), fs = tcbrt((T)1 - (T)34.15 * z);
+        T psi2m = (T)1.5 * tlog(((T)1 + fm + fm * fm) * (T)(1.0 / 3.0)) - s3 * tatan(((T)1 + (T)2 * fm) * ((T)1 / s3)) + pi / s3;
+        T psi2s = (T)1.5 * tlog(((T)1 + fs + fs * fs) * (T)(1.0 / 3.0)) - s3 * tatan(((T)1 + (T)2 * fs) * ((T)1 / s3)) + pi / s3;
+        T f = z * z / ((T)1 + z * z);
+        pm = ((T)1 - f) * psi1m + f * psi2m;
+        ps = ((T)1 - f) * psi1s + f * psi2s;
         return;
     }
-    double e = exp(-fmin(50.0, 0.35 * z));
-    pm = -0.7 * z - 0.75 * (z - 5.0 / 0.35) * e - 0.75 * (5.0 / 0.35);
-    double x = 1.0 + (2.0 / 3.0) * z;
-    ps = -(x * sqrt(x)) - (2.0 / 3.0) * (z - 14.28) * e - 8.525;
+    T e = texp(-fmin((T)50, (T)0.35 * z));
+    pm = (T)-0.7 * z - (T)0.75 * (z - (T)(5.0 / 0.35)) * e - (T)(0.75 * (5.0 / 0.35));
+    T x = (T)1 + (T)(2.0 / 3.0) * z;
+    ps = -(x * tsqrt(x)) - (T)(2.0 / 3.0) * (z - (T)14.28) * e - (T)8.525;
 }
 
 // True when the flux formulation uses the reference's default Edson constants with the
@@ -341,7 +360,19 @@ struct ColumnFluxes {
 // (qₛ, θᵃ, Δθ, Δq, 𝒯ₛ) are evaluated once: the values are identical.
 // Fast = true (only for the default Edson constants, edson_default_constants) evaluates the
 // two profiles with the shared/series stability functions above and R★^b as exp(b log R★).
-template <bool Fast>
+// Mixed = true (the FP32 form) first runs the same fixed point in FP32 with the hardware
+// log/exp until its drift falls below NE_MIXED_DRIFT (well above FP32 rounding noise), then
+// continues in FP64 from that iterate with the reference's stop criterion, so the result
+// is an FP64 fixed point to the reference tolerance reached in ~4 FP64 iterations.
+#define NE_MIXED_DRIFT 5e-7f
+#define NE_MIXED_MAXITER 40
+
+template <typename T>
+__device__ __forceinline__ T tdiv(T a, T b) { return a / b; }
+template <>
+__device__ __forceinline__ float tdiv<float>(float a, float b) { return __fdividef(a, b); }
+
+template <bool Fast, bool Mixed = false>
 __device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p, double ua, double va, double Ta,
                                                     double pa, double qa, double To_C, double uo, double vo,
                                                     bool inactive) {
@@ -378,6 +409,51 @@ __device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p, double u
                                     p.temperature_roughness_length == p.water_vapor_roughness_length);
         const int maxit = p.maxiter;
         const double tol = p.tolerance;
+
+        if (Mixed && Fast && converge && shared_scalar && p.momentum_roughness == NE_ROUGHNESS_EDSON &&
+            p.water_vapor_roughness == NE_ROUGHNESS_EDSON) {
+            const float fgoTv = (float)goTv, fonedq = (float)onedq, fdTv = (float)dTv;
+            const float fdu2dv2 = (float)du2dv2, fdtheta = (float)dtheta, fdq = (float)dq;
+            const float fkappa = (float)kappa, fhbl = (float)hbl;
+            const float fCg_g = (float)(p.charnock_parameter / p.roughness_gravitational_acceleration);
+            const float fCnu_nu = (float)(p.smooth_wall_parameter * p.air_kinematic_viscosity);
+            const float finv_nu = (float)(1.0 / p.air_kinematic_viscosity);
+            const float flmax = (float)p.maximum_momentum_roughness, fsmax = (float)p.maximum_scalar_roughness;
+            const float fA = (float)p.reynolds_scaling_A, fb = (float)p.reynolds_scaling_b;
+            const float fzd = (float)(za - p.zero_plane_displacement);
+            const float fbeta = (float)p.gustiness_parameter, fgmin = (float)p.minimum_gustiness;
+            const float fmeso2 = p.mesoscale_gustiness ? (float)(p.mesoscale_velocity * p.mesoscale_velocity) : 0.0f;
+            const int max32 = maxit - 1 < NE_MIXED_MAXITER ? maxit - 1 : NE_MIXED_MAXITER;
+            float fus = 1e-4f, fts = 1e-4f, fqs = 1e-4f;
+            int k32 = 0;
+            while (k32 < max32) {
+                float bstar = fgoTv * (fts * fonedq + fdTv * fqs);
+                float Usg2 = fmeso2;
+                if (p.convective_gustiness) {
+                    float Ug = fmaxf(fgmin, fbeta * cbrtf(fmaxf(0.0f, -fus * bstar) * fhbl));
+                    Usg2 = Usg2 + Ug * Ug;
+                }
+                float U = sqrtf(fdu2dv2 + Usg2);
+                float lu = fminf(fCg_g * fus * fus + __fdividef(fCnu_nu, fus), flmax);
+                float Rs = lu * fus * finv_nu;
+                float lq = fminf(Rs == 0.0f ? 0.0f : fA * __expf(-fb * __logf(Rs)), fsmax);
+                float dh = fmaxf(fzd, 2.0f * lu);
+                float L = bstar == 0.0f ? (float)INFINITY : __fdividef(fus * fus, fkappa * bstar);
+                float pmh, psh;
+                psi_pair_default<float>(__fdividef(dh, L), pmh, psh);
+                float chiu = __fdividef(fkappa, __logf(__fdividef(dh, lu)) - pmh + psi_m_default<float>(__fdividef(lu, L)));
+                float chiq = __fdividef(fkappa, __logf(__fdividef(dh, lq)) - psh + psi_s_default<float>(__fdividef(lq, L)));
+                float nus = chiu * U, nts = chiq * fdtheta, nqs = chiq * fdq;
+                float drift = fabsf(nus - fus) + fabsf(nts - fts) + fabsf(nqs - fqs);
+                fus = nus; fts = nts; fqs = nqs;
+                k32++;
+                if (drift < NE_MIXED_DRIFT) break;
+            }
+            if (isfinite(fus) && isfinite(fts) && isfinite(fqs) && fus > 0.0f) {
+                us = fus; ts = fts; qs = fqs;
+                it = k32;
+            }
+        }
 
         for (;;) {
             // (iteration 0 always iterates; ConvergenceStopCriteria / FixedIterations tests below)
@@ -494,6 +570,24 @@ __device__ __forceinline__ double interp_field(const View3<const Real>& d, const
     double b = w00 * (double)d(x.lo, y.lo, t.n2) + w01 * (double)d(x.lo, y.hi, t.n2) +
                w10 * (double)d(x.hi, y.lo, t.n2) + w11 * (double)d(x.hi, y.hi, t.n2);
     return (1.0 - t.tilde_n) * a + t.tilde_n * b;
+}
+
+// cpu_interpolating_time_indices with Linear time indexing, clamped at both ends
+// (interpolate_atmospheric_state.jl:57-60). The same arithmetic on the host and on the
+// device, so a device-side clock reproduces the host's indices exactly.
+__host__ __device__ inline ne_time_interp time_indices(const double* times, int n, double t) {
+    ne_time_interp r = {0, 0, 0.0};
+    if (n <= 1 || t <= times[0]) return r;
+    if (t >= times[n - 1]) {
+        r.n1 = r.n2 = n - 1;
+        return r;
+    }
+    int k = 0;
+    while (k + 1 < n && times[k + 1] <= t) k++;
+    r.n1 = k;
+    r.n2 = k + 1;
+    r.tilde_n = (t - times[k]) / (times[k + 1] - times[k]);
+    return r;
 }
 
 // ---------------------------------------------------------------------------

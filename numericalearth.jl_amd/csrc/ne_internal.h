@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "ne_device.h"
 
 struct ne_ctx {
@@ -10,6 +12,25 @@ struct ne_ctx {
     ne_counters* counters;      // device
     double* scratch;            // device workspace (budget partials), grown on demand
     size_t scratch_bytes;
+};
+
+struct ne_model {
+    ne_ctx* ctx;
+    ne_model_desc d;
+    std::vector<double> atimes, rtimes;
+    void* T1 = nullptr;
+    void* S1 = nullptr;
+    void* T2 = nullptr;
+    void* S2 = nullptr;
+    ne_view vT1, vS1, vT2, vS2;
+    double time = 0.0;
+    double last_dt = 0.0;
+    int64_t iteration = 0;
+    bool timing = false;
+    int max_steps = 0;
+    int recorded = 0;
+    std::vector<hipEvent_t> events;
+    struct ne_batch* batch = nullptr;   // the batch this model belongs to, if any
 };
 
 namespace ne {
@@ -98,5 +119,49 @@ hipError_t launch_tendency(const GridDev& g, View3<const double> u, View3<const 
 template <typename Real>
 hipError_t launch_budget(ne_ctx* ctx, const GridDev& g, View3<const Real> c, double weight, double* out,
                          hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Batched ensemble: one device record per member model, read by persistent kernels
+// that walk a work list of (member, tile) or (member, first column) items.
+// ---------------------------------------------------------------------------
+
+template <typename Real>
+struct BatchRec {
+    GridDev g;
+    ne_params p;
+    AtmosDev<Real> a;
+    RadDev<Real> r;
+    const double *afi, *afj, *rfi, *rfj;
+    View3<Real> T, S, T1, S1, T2, S2;
+    View3<const Real> u, v, w;
+    FluxOut f;
+    NetOut n;
+    RadOut ro;
+    View3<double> frazil;
+    const double* atimes;   // device copies of the snapshot times
+    const double* rtimes;
+    int nat, nrt;
+    int fast;               // default Edson constants: the shared-ψ flux path
+    double time;           // device clock of the member (advanced by the clock kernel)
+};
+
+struct WorkItem {
+    int member;
+    int index;              // tile index (tracer stages) or first flattened column (surface)
+};
+
+// Persistent-grid geometry of one launch: `blocks` work-groups (a multiple of the 8 XCDs)
+// deal the work list out in 8 contiguous chunks, one per XCD.
+template <typename Real>
+hipError_t launch_batch_stage(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int tile_rows,
+                              int blocks, int stage, double beta_dt, hipStream_t s);
+template <typename Real>
+hipError_t launch_batch_surface_solve(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int block,
+                                      int blocks, bool fast, ne_counters* c, hipStream_t s);
+template <typename Real>
+hipError_t launch_batch_surface_finalize(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int block,
+                                         int blocks, double dt_frazil, hipStream_t s);
+int batch_tiles_x(int Nx);
+int batch_tiles_y(int Ny, int tile_rows);
 
 }  // namespace ne

@@ -16,7 +16,13 @@ constexpr int BY = 4;
 
 // Minimum wavefronts per SIMD requested for the flux-solve kernels (bounds their VGPRs).
 #ifndef NE_FLUX_WAVES_PER_EU
-#define NE_FLUX_WAVES_PER_EU 1
+#define NE_FLUX_WAVES_PER_EU 4
+#endif
+
+// 1: the mixed form (FP32 storage) starts the fixed point with the FP32 pre-iteration;
+// 0: it runs the FP64 iteration of the FP64 form from the reference's initial guess.
+#ifndef NE_MIXED_PREITERATION
+#define NE_MIXED_PREITERATION 1
 #endif
 
 inline dim3 grid2d(int Nx, int Ny) { return dim3((Nx + BX - 1) / BX, (Ny + BY - 1) / BY); }
@@ -157,6 +163,31 @@ flux_solve_exchange_kernel(GridDev g, ne_params p, View3<const double> ua, View3
                         solved && !finite3(r.us, r.ts, r.qs));
 }
 
+// One column of K1: interpolate the atmosphere at the column, solve, store the fluxes.
+template <typename Real, bool Fast>
+__device__ __forceinline__ ColumnFluxes surface_solve_column(const GridDev& g, const ne_params& p,
+                                                             const AtmosDev<Real>& a, const double* __restrict__ fi,
+                                                             const double* __restrict__ fj,
+                                                             const View3<const Real>& T, const View3<const Real>& S,
+                                                             const View3<const Real>& u, const View3<const Real>& v,
+                                                             const FluxOut& out, int i, int j, bool& inactive) {
+    Neighbours x = neighbours(fi[i], a.Nx, a.tx);
+    Neighbours y = neighbours(fj[j], a.Ny, a.ty);
+    double ua = interp_field(a.u, x, y, a.time);
+    double va = interp_field(a.v, x, y, a.time);
+    double Ta = interp_field(a.T, x, y, a.time);
+    double pa = interp_field(a.p, x, y, a.time);
+    double qa = interp_field(a.q, x, y, a.time);
+    double To, So, uo, vo;
+    ocean_surface<Real>(g, T, S, u, v, i, j, To, So, uo, vo);
+    inactive = !g.active(i, j, g.Nz - 1);
+    // FP32 storage (the mixed form) also takes the FP32 pre-iteration of the fixed point
+    ColumnFluxes r =
+        flux_column<Fast, NE_MIXED_PREITERATION && sizeof(Real) == 4>(p, ua, va, Ta, pa, qa, To, uo, vo, inactive);
+    store_fluxes(out, i, j, r);
+    return r;
+}
+
 // K1 of the fused update_state: interpolate the atmosphere at the column and solve.
 template <typename Real, bool Fast>
 __global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(NE_FLUX_WAVES_PER_EU)))
@@ -168,23 +199,36 @@ surface_solve_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __r
     bool in = i < g.Nx && j < g.Ny;
     ColumnFluxes r;
     bool inactive = true;
-    if (in) {
-        Neighbours x = neighbours(fi[i], a.Nx, a.tx);
-        Neighbours y = neighbours(fj[j], a.Ny, a.ty);
-        double ua = interp_field(a.u, x, y, a.time);
-        double va = interp_field(a.v, x, y, a.time);
-        double Ta = interp_field(a.T, x, y, a.time);
-        double pa = interp_field(a.p, x, y, a.time);
-        double qa = interp_field(a.q, x, y, a.time);
-        double To, So, uo, vo;
-        ocean_surface<Real>(g, T, S, u, v, i, j, To, So, uo, vo);
-        inactive = !g.active(i, j, g.Nz - 1);
-        r = flux_column<Fast>(p, ua, va, Ta, pa, qa, To, uo, vo, inactive);
-        store_fluxes(out, i, j, r);
-    }
+    if (in) r = surface_solve_column<Real, Fast>(g, p, a, fi, fj, T, S, u, v, out, i, j, inactive);
     bool solved = in && !inactive;
     accumulate_counters(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
                         solved && !finite3(r.us, r.ts, r.qs));
+}
+
+// Ensemble form of K1: a persistent grid walks the (member, first column) work list; an
+// item is blockDim.x consecutive columns (i fastest) of one member. Members with the
+// default Edson constants are listed for the shared-ψ instance (Fast), the others for
+// the general one, as the single-model launcher chooses.
+template <typename Real, bool Fast>
+__global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(NE_FLUX_WAVES_PER_EU)))
+batch_surface_solve_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __restrict__ work, int nwork,
+                           ne_counters* counters) {
+    for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+        const WorkItem it = work[wi];
+        const BatchRec<Real>& rc = recs[it.member];
+        const GridDev g = rc.g;
+        const int col = it.index + (int)threadIdx.x;
+        const bool in = col < g.Nx * g.Ny;
+        const int i = in ? col % g.Nx : 0, j = in ? col / g.Nx : 0;
+        ColumnFluxes r;
+        bool inactive = true;
+        if (in)
+            r = surface_solve_column<Real, Fast>(g, rc.p, rc.a, rc.afi, rc.afj, cview(rc.T), cview(rc.S), rc.u, rc.v,
+                                                 rc.f, i, j, inactive);
+        bool solved = in && !inactive;
+        accumulate_counters(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
+                            solved && !finite3(r.us, r.ts, r.qs));
+    }
 }
 
 // Freeze clamp of one column (freezing_limited_ocean_temperature.jl:95-118); returns frazil heat.
@@ -305,16 +349,16 @@ __global__ void radiation_kernel(GridDev g, ne_params p, View3<const double> sw,
     if (r.down_sw.p) r.down_sw.at2(i, j) = -Itsw;
 }
 
-// K2 of the fused update_state: freeze clamp, net-flux assembly and radiation per column.
+// One column of K2: freeze clamp, net-flux assembly and radiation.
 template <typename Real>
-__global__ void __launch_bounds__(BX* BY)
-surface_finalize_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __restrict__ afi,
-                        const double* __restrict__ afj, RadDev<Real> rd, const double* __restrict__ rfi,
-                        const double* __restrict__ rfj, View3<Real> T, View3<const Real> S, FluxOut f, NetOut n,
-                        RadOut ro, View3<double> frazil, double dt_frazil) {
-    int i = blockIdx.x * BX + threadIdx.x;
-    int j = blockIdx.y * BY + threadIdx.y;
-    if (i >= g.Nx || j >= g.Ny) return;
+__device__ __forceinline__ void surface_finalize_column(const GridDev& g, const ne_params& p, const AtmosDev<Real>& a,
+                                                        const double* __restrict__ afi,
+                                                        const double* __restrict__ afj, const RadDev<Real>& rd,
+                                                        const double* __restrict__ rfi,
+                                                        const double* __restrict__ rfj, const View3<Real>& T,
+                                                        const View3<const Real>& S, const FluxOut& f,
+                                                        const NetOut& n, const RadOut& ro,
+                                                        const View3<double>& frazil, double dt_frazil, int i, int j) {
     double dQ = freeze_column<Real>(g, p, T, S, i, j, dt_frazil);
     if (frazil.p) frazil.at2(i, j) = dQ;
 
@@ -346,6 +390,36 @@ surface_finalize_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* 
     n.J_S.at2(i, j) = c.J_S;
     n.J_w.at2(i, j) = c.J_w;
     n.J_H.at2(i, j) = c.J_H;
+}
+
+// K2 of the fused update_state: freeze clamp, net-flux assembly and radiation per column.
+template <typename Real>
+__global__ void __launch_bounds__(BX* BY)
+surface_finalize_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __restrict__ afi,
+                        const double* __restrict__ afj, RadDev<Real> rd, const double* __restrict__ rfi,
+                        const double* __restrict__ rfj, View3<Real> T, View3<const Real> S, FluxOut f, NetOut n,
+                        RadOut ro, View3<double> frazil, double dt_frazil) {
+    int i = blockIdx.x * BX + threadIdx.x;
+    int j = blockIdx.y * BY + threadIdx.y;
+    if (i >= g.Nx || j >= g.Ny) return;
+    surface_finalize_column<Real>(g, p, a, afi, afj, rd, rfi, rfj, T, S, f, n, ro, frazil, dt_frazil, i, j);
+}
+
+// Ensemble form of K2 over the same work list as the ensemble K1.
+template <typename Real>
+__global__ void __launch_bounds__(BX* BY)
+batch_surface_finalize_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __restrict__ work, int nwork,
+                              double dt_frazil) {
+    for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+        const WorkItem it = work[wi];
+        const BatchRec<Real>& rc = recs[it.member];
+        const GridDev g = rc.g;
+        const int col = it.index + (int)threadIdx.x;
+        if (col >= g.Nx * g.Ny) continue;
+        const int i = col % g.Nx, j = col / g.Nx;
+        surface_finalize_column<Real>(g, rc.p, rc.a, rc.afi, rc.afj, rc.r, rc.rfi, rc.rfj, rc.T, cview(rc.S), rc.f,
+                                      rc.n, rc.ro, rc.frazil, dt_frazil, i, j);
+    }
 }
 
 }  // namespace
@@ -518,5 +592,37 @@ template hipError_t launch_surface_finalize<float>(const GridDev&, const ne_para
                                                    const double*, const double*, View3<float>, View3<const float>,
                                                    const FluxOut&, const NetOut&, const RadOut&, View3<double>,
                                                    double, hipStream_t);
+
+template <typename Real>
+hipError_t launch_batch_surface_solve(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int block,
+                                      int blocks, bool fast, ne_counters* c, hipStream_t s) {
+    if (nwork <= 0) return hipSuccess;
+    if (block <= 0 || block > BX * BY || (block & (block - 1)) != 0 || blocks <= 0) return hipErrorInvalidValue;
+    if (fast)
+        hipLaunchKernelGGL((batch_surface_solve_kernel<Real, true>), dim3(blocks), dim3(block), 0, s, recs, work,
+                           nwork, c);
+    else
+        hipLaunchKernelGGL((batch_surface_solve_kernel<Real, false>), dim3(blocks), dim3(block), 0, s, recs, work,
+                           nwork, c);
+    return hipGetLastError();
+}
+template hipError_t launch_batch_surface_solve<double>(const BatchRec<double>*, const WorkItem*, int, int, int, bool,
+                                                       ne_counters*, hipStream_t);
+template hipError_t launch_batch_surface_solve<float>(const BatchRec<float>*, const WorkItem*, int, int, int, bool,
+                                                      ne_counters*, hipStream_t);
+
+template <typename Real>
+hipError_t launch_batch_surface_finalize(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int block,
+                                         int blocks, double dt_frazil, hipStream_t s) {
+    if (nwork <= 0) return hipSuccess;
+    if (block <= 0 || block > BX * BY || blocks <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(batch_surface_finalize_kernel<Real>, dim3(blocks), dim3(block), 0, s, recs, work, nwork,
+                       dt_frazil);
+    return hipGetLastError();
+}
+template hipError_t launch_batch_surface_finalize<double>(const BatchRec<double>*, const WorkItem*, int, int, int,
+                                                          double, hipStream_t);
+template hipError_t launch_batch_surface_finalize<float>(const BatchRec<float>*, const WorkItem*, int, int, int,
+                                                         double, hipStream_t);
 
 }  // namespace ne

@@ -18,16 +18,17 @@ namespace {
 
 constexpr int TW = 64;            // work-items per tile row (x faces per row)
 constexpr int TC = 63;            // cells per tile row
-constexpr int TR = 8;             // cell rows per tile (one wavefront each)
-constexpr int PXW = TC + 6;       // plane columns i0-3 … i0+65
-constexpr int PYH = TR + 6;       // plane rows    j0-3 … j0+10
-constexpr int PN = PXW * PYH;
-constexpr int NTHR = TW * TR;     // 512
+constexpr int PLAIN_TR = 8;       // cell rows per tile of the single-model kernel
 constexpr int NXCD = 8;
 constexpr int KMIN_OUTSIDE = 1 << 30;  // kmin of a column outside a bounded domain
 
-template <typename R>
+// LDS of one tile of TR rows: the T/S plane with a 3-cell halo, the x- and y-face fluxes
+// and the kmin of every plane column.
+template <typename R, int TR>
 struct TileShared {
+    static constexpr int PXW = TC + 6;   // plane columns i0-3 … i0+65
+    static constexpr int PYH = TR + 6;   // plane rows    j0-3 … j0+TR+2
+    static constexpr int PN = PXW * PYH;
     R cT[PN], cS[PN];
     R fxT[TR * TW], fxS[TR * TW];
     R fyT[(TR + 1) * TC], fyS[(TR + 1) * TC];
@@ -36,9 +37,9 @@ struct TileShared {
 };
 
 __host__ __device__ inline int tiles_x(int Nx) { return (Nx + TC - 1) / TC; }
-__host__ __device__ inline int tiles_y(int Ny) { return (Ny + TR - 1) / TR; }
+__host__ __device__ inline int tiles_y(int Ny, int tr) { return (Ny + tr - 1) / tr; }
 inline int grid_blocks(int Nx, int Ny) {
-    int n = tiles_x(Nx) * tiles_y(Ny);
+    int n = tiles_x(Nx) * tiles_y(Ny, PLAIN_TR);
     return ((n + NXCD - 1) / NXCD) * NXCD;
 }
 
@@ -50,28 +51,26 @@ __device__ __forceinline__ unsigned act6(const int* km, int stride, int k) {
     return a;
 }
 
-// MODE 0: c_out = c0 + beta_dt·G(c_in) on active cells. MODE 1: write G (0 on inactive cells).
-template <typename Real, int MODE>
-__global__ void __launch_bounds__(NTHR)
-tracer_tile_kernel(GridDev g, View3<const Real> u, View3<const Real> v, View3<const Real> w, View3<const Real> T0,
-                   View3<const Real> S0, View3<const Real> Tin, View3<const Real> Sin, View3<Real> Tout,
-                   View3<Real> Sout, View3<const double> JT, View3<const double> JS, View3<const double> Jw,
-                   double beta_dt) {
+// One tile (columns i0 … i0+62, rows j0 … j0+TR-1) marched over its active levels.
+// MODE 0: c_out = c0 + beta_dt·G(c_in) on active cells. MODE 1: write G (0 on inactive
+// cells). Every work-item of the block calls this with the same tile.
+template <typename Real, int MODE, int TR>
+__device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDev& g, const View3<const Real>& u,
+                                          const View3<const Real>& v, const View3<const Real>& w,
+                                          const View3<const Real>& T0, const View3<const Real>& S0,
+                                          const View3<const Real>& Tin, const View3<const Real>& Sin,
+                                          const View3<Real>& Tout, const View3<Real>& Sout,
+                                          const View3<const double>& JT, const View3<const double>& JS,
+                                          const View3<const double>& Jw, double beta_dt, int i0, int j0) {
     typedef Real R;
-    __shared__ TileShared<R> sh;
+    constexpr int PXW = TileShared<R, TR>::PXW;
+    constexpr int PN = TileShared<R, TR>::PN;
+    constexpr int NTHR = TW * TR;
     const int lane = threadIdx.x;
     const int row = threadIdx.y;
     const int tid = lane + row * TW;
     const int Nz = g.Nz;
-
-    // XCD-contiguous tile order
-    const int ntx = tiles_x(g.Nx), nty = tiles_y(g.Ny);
-    const int ntiles = ntx * nty;
-    const int per = (ntiles + NXCD - 1) / NXCD;
-    const int tile = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
-    if (tile >= ntiles) return;
-    const int i0 = (tile % ntx) * TC;
-    const int j0 = (tile / ntx) * TR;
+    const bool adv = g.advection != 0;   // single-column grids carry no advection
 
     // kmin of the plane columns (outside a bounded domain: never active)
     for (int n = tid; n < PN; n += NTHR) {
@@ -136,7 +135,7 @@ tracer_tile_kernel(GridDev g, View3<const Real> u, View3<const Real> v, View3<co
             R fT = (R)0, fS = (R)0;
             if (xrow) {
                 const int base = (row + 3) * PXW + lane;
-                int order = face_order(act6(&sh.km[base], 1, k));
+                int order = adv ? face_order(act6(&sh.km[base], 1, k)) : 0;
                 if (order > 0) {
                     R vel = u(fiw, cj, k);
                     R A = (R)Axf * dzk;
@@ -158,7 +157,7 @@ tracer_tile_kernel(GridDev g, View3<const Real> u, View3<const Real> v, View3<co
                 const int fj = pass ? fjn : fjs;
                 const double Ay = pass ? Ayn : Ays;
                 R fT = (R)0, fS = (R)0;
-                int order = face_order(act6(&sh.km[base], PXW, k));
+                int order = adv ? face_order(act6(&sh.km[base], PXW, k)) : 0;
                 if (order > 0) {
                     R vel = v(cy, fj, k);
                     R A = (R)Ay * dzk;
@@ -173,7 +172,7 @@ tracer_tile_kernel(GridDev g, View3<const Real> u, View3<const Real> v, View3<co
         }
         // 2c. z face at the top of level k (cells k-2 … k+3 of the owned column)
         R FztT = (R)0, FztS = (R)0;
-        if (cell) {
+        if (cell && adv) {
             unsigned az = 0;
 #pragma unroll
             for (int n = 0; n < 6; n++) {
@@ -210,8 +209,10 @@ tracer_tile_kernel(GridDev g, View3<const Real> u, View3<const Real> v, View3<co
                 GS = GS - (R)(FS * Azc) / V;
             }
             if (MODE == 0) {
-                Tout(ci, cj, k) = T0(ci, cj, k) + (R)beta_dt * GT;
-                Sout(ci, cj, k) = S0(ci, cj, k) + (R)beta_dt * GS;
+                // The update is formed in FP64 and rounded once to the storage type (for FP32
+                // storage this keeps the per-stage rounding of the state at half an ulp).
+                Tout(ci, cj, k) = (R)((double)T0(ci, cj, k) + beta_dt * (double)GT);
+                Sout(ci, cj, k) = (R)((double)S0(ci, cj, k) + beta_dt * (double)GS);
             } else {
                 Tout(ci, cj, k) = GT;
                 Sout(ci, cj, k) = GS;
@@ -228,6 +229,53 @@ tracer_tile_kernel(GridDev g, View3<const Real> u, View3<const Real> v, View3<co
             zT[5] = a ? Tin(ci, cj, kk) : (R)0;
             zS[5] = a ? Sin(ci, cj, kk) : (R)0;
         }
+    }
+}
+
+// Single model: one block per tile, tiles ordered so that each XCD (blocks are dealt
+// round-robin to the 8 XCDs) receives a contiguous band of tiles.
+template <typename Real, int MODE>
+__global__ void __launch_bounds__(TW * PLAIN_TR)
+tracer_tile_kernel(GridDev g, View3<const Real> u, View3<const Real> v, View3<const Real> w, View3<const Real> T0,
+                   View3<const Real> S0, View3<const Real> Tin, View3<const Real> Sin, View3<Real> Tout,
+                   View3<Real> Sout, View3<const double> JT, View3<const double> JS, View3<const double> Jw,
+                   double beta_dt) {
+    __shared__ TileShared<Real, PLAIN_TR> sh;
+    const int ntx = tiles_x(g.Nx), nty = tiles_y(g.Ny, PLAIN_TR);
+    const int ntiles = ntx * nty;
+    const int per = (ntiles + NXCD - 1) / NXCD;
+    const int tile = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
+    if (tile >= ntiles) return;
+    tile_body<Real, MODE, PLAIN_TR>(sh, g, u, v, w, T0, S0, Tin, Sin, Tout, Sout, JT, JS, Jw, beta_dt,
+                                    (tile % ntx) * TC, (tile / ntx) * PLAIN_TR);
+}
+
+// Ensemble: a persistent grid walks the (member, tile) work list. The list is cut into 8
+// contiguous chunks, chunk x going to the blocks of XCD x, which stride through it; each
+// member's tiles are contiguous in the list, so a member's halo re-reads stay in one L2.
+// stage 1: T¹ = T + β₁Δt G(T); stage 2: T² = T + β₂Δt G(T¹); stage 3: T = T + β₃Δt G(T²).
+template <typename Real, int TR>
+__global__ void __launch_bounds__(TW * TR)
+batch_stage_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __restrict__ work, int nwork,
+                   int stage, double beta_dt) {
+    __shared__ TileShared<Real, TR> sh;
+    const int Q = (int)gridDim.x / NXCD;
+    const int x = (int)blockIdx.x % NXCD, q = (int)blockIdx.x / NXCD;
+    const int chunk = (nwork + NXCD - 1) / NXCD;
+    const int end = min((x + 1) * chunk, nwork);
+    for (int wi = x * chunk + q; wi < end; wi += Q) {
+        const WorkItem it = work[wi];
+        const BatchRec<Real>& rc = recs[it.member];
+        const GridDev g = rc.g;
+        const int ntx = tiles_x(g.Nx);
+        const View3<const Real> Tin = cview(stage == 1 ? rc.T : stage == 2 ? rc.T1 : rc.T2);
+        const View3<const Real> Sin = cview(stage == 1 ? rc.S : stage == 2 ? rc.S1 : rc.S2);
+        const View3<Real> Tout = stage == 1 ? rc.T1 : stage == 2 ? rc.T2 : rc.T;
+        const View3<Real> Sout = stage == 1 ? rc.S1 : stage == 2 ? rc.S2 : rc.S;
+        __syncthreads();   // the previous item's reads of the LDS are complete
+        tile_body<Real, 0, TR>(sh, g, rc.u, rc.v, rc.w, cview(rc.T), cview(rc.S), Tin, Sin, Tout, Sout,
+                               cview(rc.n.J_T), cview(rc.n.J_S), cview(rc.n.J_w), beta_dt,
+                               (it.index % ntx) * TC, (it.index / ntx) * TR);
     }
 }
 
@@ -322,7 +370,7 @@ hipError_t launch_rk_stage(const GridDev& g, View3<const Real> u, View3<const Re
                            View3<const Real> T0, View3<const Real> S0, View3<const Real> Tin, View3<const Real> Sin,
                            View3<Real> Tout, View3<Real> Sout, View3<const double> JT, View3<const double> JS,
                            View3<const double> Jw, double beta_dt, hipStream_t s) {
-    hipLaunchKernelGGL((tracer_tile_kernel<Real, 0>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, TR), 0, s, g, u, v, w,
+    hipLaunchKernelGGL((tracer_tile_kernel<Real, 0>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, PLAIN_TR), 0, s, g, u, v, w,
                        T0, S0, Tin, Sin, Tout, Sout, JT, JS, Jw, beta_dt);
     return hipGetLastError();
 }
@@ -341,7 +389,7 @@ hipError_t launch_tendency(const GridDev& g, View3<const double> u, View3<const 
                            View3<const double> T, View3<const double> S, View3<const double> JT,
                            View3<const double> JS, View3<const double> Jw, View3<double> GT, View3<double> GS,
                            hipStream_t s) {
-    hipLaunchKernelGGL((tracer_tile_kernel<double, 1>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, TR), 0, s, g, u, v,
+    hipLaunchKernelGGL((tracer_tile_kernel<double, 1>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, PLAIN_TR), 0, s, g, u, v,
                        w, T, S, T, S, GT, GS, JT, JS, Jw, 0.0);
     return hipGetLastError();
 }
@@ -371,5 +419,36 @@ template hipError_t launch_budget<double>(ne_ctx*, const GridDev&, View3<const d
                                           hipStream_t);
 template hipError_t launch_budget<float>(ne_ctx*, const GridDev&, View3<const float>, double, double*,
                                          hipStream_t);
+
+int batch_tiles_x(int Nx) { return tiles_x(Nx); }
+int batch_tiles_y(int Ny, int tile_rows) { return tiles_y(Ny, tile_rows); }
+
+template <typename Real>
+hipError_t launch_batch_stage(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int tile_rows,
+                              int blocks, int stage, double beta_dt, hipStream_t s) {
+    if (nwork <= 0) return hipSuccess;
+    if (blocks <= 0 || blocks % NXCD != 0) return hipErrorInvalidValue;
+    switch (tile_rows) {
+    case 4:
+        hipLaunchKernelGGL((batch_stage_kernel<Real, 4>), dim3(blocks), dim3(TW, 4), 0, s, recs, work, nwork, stage,
+                           beta_dt);
+        break;
+    case 8:
+        hipLaunchKernelGGL((batch_stage_kernel<Real, 8>), dim3(blocks), dim3(TW, 8), 0, s, recs, work, nwork, stage,
+                           beta_dt);
+        break;
+    case 16:
+        hipLaunchKernelGGL((batch_stage_kernel<Real, 16>), dim3(blocks), dim3(TW, 16), 0, s, recs, work, nwork,
+                           stage, beta_dt);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+template hipError_t launch_batch_stage<double>(const BatchRec<double>*, const WorkItem*, int, int, int, int, double,
+                                               hipStream_t);
+template hipError_t launch_batch_stage<float>(const BatchRec<float>*, const WorkItem*, int, int, int, int, double,
+                                              hipStream_t);
 
 }  // namespace ne

@@ -211,3 +211,14 @@ EDSON_SCALAR = (  # similarity_theory_turbulent_fluxes.jl:571-584
 )
 TEOS10_REFERENCE_HEAT_CAPACITY = 3991.86795711963  # SeawaterPolynomials TEOS-10 cₚ⁰ (ocean_simulation.jl:527-530)
 GRAVITATIONAL_ACCELERATION = 9.80665  # Oceananigans.defaults (EarthSystemModels.jl:70)
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [
+        ("tile_rows", _i32),
+        ("stage_blocks", _i32),
+        ("surface_block", _i32),
+        ("surface_blocks", _i32),
+        ("stage_ms", C.c_double),
+        ("surface_ms", C.c_double),
+    ]

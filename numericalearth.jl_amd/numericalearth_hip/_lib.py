@@ -58,6 +58,14 @@ SIGNATURES = {
     "ne_model_clock": [vp, P(C.c_double), P(C.c_int64)],
     "ne_model_timing_begin": [vp, C.c_int],
     "ne_model_timing_end": [vp, P(C.c_double), P(C.c_int)],
+    "ne_batch_create": [vp, C.c_int, P(vp), P(vp), vp],
+    "ne_batch_destroy": [vp],
+    "ne_batch_time_step": [vp, C.c_double, vp],
+    "ne_batch_set_graph": [vp, C.c_int],
+    "ne_batch_autotune": [vp, P(abi.BatchTuning), vp],
+    "ne_batch_get_tuning": [vp, P(abi.BatchTuning)],
+    "ne_batch_set_tuning": [vp, P(abi.BatchTuning)],
+    "ne_batch_info": [vp, P(C.c_int), P(C.c_int64), P(C.c_int), P(C.c_int)],
 }
 
 TIMED_KERNELS = ("rk_stage_1", "rk_stage_2", "rk_stage_3", "surface_solve", "surface_finalize")

@@ -17,7 +17,7 @@ CSRC = PKG_ROOT / "csrc"
 INCLUDE = REPO_ROOT / "include"
 LIB_DIR = PKG_ROOT / "lib"
 LIB_NAME = "libnumericalearth_hip.so"
-SOURCES = ("ne_surface.hip", "ne_tracer.hip", "ne_abi.hip")
+SOURCES = ("ne_surface.hip", "ne_tracer.hip", "ne_abi.hip", "ne_batch.hip")
 
 # -ffp-contract=off: the FP64 path is the sequence of IEEE operations of the
 # reference arithmetic (no fused multiply-adds introduced by the compiler).

@@ -9,8 +9,9 @@ The workloads follow the benchmark configurations of this path:
   with D < 50 m are land (bottom height 0) and wet columns are deepened to ≥ 200 m,
   as the reference's minimum-depth step does (Bathymetry/regrid_bathymetry.jl:267-275);
 * T = 2 + 26 cos²φ exp(z/500) + N(0, 0.1) °C, S = 35 + 0.5 N(0, 1) psu,
-  u, v = 0.1 N(0, 1) m/s on faces (zero on faces touching an inactive cell),
-  w diagnosed from discrete continuity with w = 0 at the bottom;
+  u, v = 0.1 N(0, 1) m/s on faces (zero on faces touching an inactive cell) minus
+  their depth mean over the face's open levels, so that w, diagnosed from discrete
+  continuity with w = 0 at the bottom, also vanishes at the top of every column;
 * a prescribed atmosphere on a 2× coarser grid: u₁₀ = 8 sin 2φ, v₁₀ = 2 N(0, 1),
   Tᵃ = 2 + 26 cos²φ + 273.15 − 1 + N(0, 1) K, q = 0.8 qₛₐₜ(Tᵃ, p), p = 101325 Pa,
   rain = snow = 0; prescribed radiation SW = 300 cos φ, LW = 350 W m⁻²
@@ -197,6 +198,23 @@ def _face_velocities(spec: GridSpec, kmin: np.ndarray, rng: np.random.Generator,
     return u, v, active
 
 
+def remove_barotropic_transport(u: np.ndarray, v: np.ndarray, dz: np.ndarray):
+    """Subtract from every face its thickness-weighted mean over the face's open levels, so the
+    depth-integrated transport through every face vanishes. Continuity then gives w = 0 at the
+    top of every column: with a zero advective flux through the top face (the top boundary
+    carries only the prescribed flux), a column whose integrated divergence is not zero would
+    act as a mass source in its top cell and its tracer would grow or decay exponentially
+    (e-folding time Δz/w_top, hours at 1/4°)."""
+    out = []
+    for f in (u, v):
+        open_ = f != 0.0
+        w = dz[:, None, None] * open_
+        depth = w.sum(axis=0)
+        mean = np.where(depth > 0, (dz[:, None, None] * f).sum(axis=0) / np.where(depth > 0, depth, 1.0), 0.0)
+        out.append(np.where(open_, f - mean[None], 0.0))
+    return out[0], out[1]
+
+
 def continuity_w(g: GridArrays, u: np.ndarray, v: np.ndarray, active: np.ndarray) -> np.ndarray:
     """w on (C,C,F) faces from discrete continuity, w = 0 at the bottom of each column."""
     spec = g.spec
@@ -231,6 +249,7 @@ def make_case(Nx: int, Ny: int, Nz: int, seed: int = 0, *, longitude=(0.0, 360.0
     T = 2.0 + 26.0 * np.cos(phi) ** 2 * np.exp(zc / 500.0) + 0.1 * rng.standard_normal((Nz, Ny, Nx))
     S = 35.0 + 0.5 * rng.standard_normal((Nz, Ny, Nx))
     u, v, active = _face_velocities(spec, kmin, rng)
+    u, v = remove_barotropic_transport(u, v, m["dz"])
     w = continuity_w(g, u, v, active)
 
     # Prescribed atmosphere on a coarser grid covering the same domain.

@@ -20,7 +20,7 @@ STRUCTS = {
     "ne_exchange_state": abi.ExchangeState, "ne_ocean_state": abi.OceanState,
     "ne_interface_fluxes": abi.InterfaceFluxes, "ne_net_ocean_fluxes": abi.NetOceanFluxes,
     "ne_radiative_fluxes": abi.RadiativeFluxes, "ne_params": abi.Params, "ne_counters": abi.Counters,
-    "ne_model_desc": abi.ModelDesc,
+    "ne_model_desc": abi.ModelDesc, "ne_batch_tuning": abi.BatchTuning,
 }
 
 
@@ -55,6 +55,14 @@ def test_null_arguments_are_rejected_without_a_device():
     assert L.ne_ctx_destroy(None) == abi.NE_ERR_NULL
     assert L.ne_model_destroy(None) == abi.NE_ERR_NULL
     assert L.ne_model_time_step(None, 1.0, None) == abi.NE_ERR_NULL
+    h = C.c_void_p()
+    assert L.ne_batch_create(None, 1, None, C.byref(h), None) == abi.NE_ERR_NULL
+    assert L.ne_batch_destroy(None) == abi.NE_ERR_NULL
+    assert L.ne_batch_time_step(None, 1.0, None) == abi.NE_ERR_NULL
+    assert L.ne_batch_set_graph(None, 1) == abi.NE_ERR_NULL
+    assert L.ne_batch_autotune(None, None, None) == abi.NE_ERR_NULL
+    assert L.ne_batch_set_tuning(None, None) == abi.NE_ERR_NULL
+    assert L.ne_batch_info(None, None, None, None, None) == abi.NE_ERR_NULL
 
 
 def test_struct_layouts_match_header(tmp_path):

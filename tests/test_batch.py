@@ -1,0 +1,176 @@
+"""Ensembles (ne_batch_*): a batch step must give, member by member, exactly what stepping each
+member alone with ne_model_time_step gives (the kernels share the per-tile and per-column code and
+every member keeps its own parameters), with or without the HIP graph, before and after the
+autotuner, and across atmosphere snapshots (device-side clock)."""
+import numpy as np
+import pytest
+import torch
+
+from numericalearth_hip import _ctypes_abi as abi
+from numericalearth_hip._lib import NEError
+from numericalearth_hip.ensemble import Ensemble, build_members, draw_ensemble, member_case, member_params
+from numericalearth_hip.model import OceanOnlyModel
+
+pytestmark = pytest.mark.gpu
+
+DT = 1200.0
+
+
+def _specs(n=8, seed=5):
+    # a small ensemble with the generator's spread of shapes (single columns to regional grids)
+    specs = draw_ensemble(n, seed, largest=(150, 60, 20))
+    print("[batch] shapes", [(s.Nx, s.Ny, s.Nz) for s in specs], flush=True)
+    return specs
+
+
+def _build(specs, precision, general=(), n_times=1):
+    """Members; those whose index is in `general` get a non-default Edson constant (general flux path)."""
+    out = []
+    for k, s in enumerate(specs):
+        p = member_params(s)
+        if k in general:
+            p.edson_momentum[0] = 45.0
+        case = member_case(s) if n_times == 1 else _case_times(s, n_times)
+        out.append(OceanOnlyModel(case, params=p, precision=precision))
+    return out
+
+
+def _case_times(s, n_times):
+    from numericalearth_hip.generators import NE_BOUNDED, make_case
+    return make_case(s.Nx, s.Ny, s.Nz, seed=s.seed, longitude=s.longitude, latitude=s.latitude,
+                     topology_x=NE_BOUNDED, n_times=n_times, dt_times=3600.0)
+
+
+def _state(m):
+    d = {"T": m.T, "S": m.S}
+    d.update({f"flux.{k}": v for k, v in m.fluxes.items()})
+    d.update({f"net.{k}": v for k, v in m.net.items()})
+    d["frazil"] = m.frazil_heat
+    return {k: v.detach().cpu().numpy() for k, v in d.items()}
+
+
+def _assert_same(a_models, b_models, what):
+    torch.cuda.synchronize()
+    for n, (a, b) in enumerate(zip(a_models, b_models)):
+        sa, sb = _state(a), _state(b)
+        for k in sa:
+            if not np.array_equal(sa[k], sb[k], equal_nan=True):
+                d = np.nanmax(np.abs(sa[k] - sb[k]))
+                raise AssertionError(f"{what}: member {n} {a.case.shape} field {k} differs (max |Δ| {d:.3e})")
+        assert a.clock == b.clock, (what, n, a.clock, b.clock)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_equals_members_stepped_alone(require_gpu, precision):
+    specs = _specs()
+    alone = _build(specs, precision, general=(2,))
+    batched = _build(specs, precision, general=(2,))
+    ens = Ensemble(batched)
+    print("[batch]", ens.info(), ens.tuning, flush=True)
+    for _ in range(3):
+        ens.time_step(DT)
+        for m in alone:
+            m.time_step(DT)
+    _assert_same(alone, batched, f"{precision} eager")
+    ens.close()
+    # released members step on their own again
+    for m in batched:
+        m.time_step(DT)
+    for m in alone:
+        m.time_step(DT)
+    _assert_same(alone, batched, f"{precision} after release")
+
+
+def test_graph_replays_equal_eager_steps(require_gpu):
+    specs = _specs(6, seed=8)
+    eager = _build(specs, "fp64")
+    graphed = _build(specs, "fp64")
+    e1, e2 = Ensemble(eager), Ensemble(graphed)
+    e2.use_graph(True)
+    for n in range(4):
+        e1.time_step(DT)
+        e2.time_step(DT)
+        # the program's own eager work and a synchronization between replays
+        x = torch.randn(1 << 16, device="cuda", dtype=torch.float64)
+        float((x * x).sum())
+        torch.cuda.synchronize()
+        _assert_same(eager, graphed, f"graph replay {n}")
+
+
+def test_autotune_keeps_state_and_results(require_gpu, tmp_path):
+    specs = _specs(6, seed=9)
+    alone = _build(specs, "fp64")
+    batched = _build(specs, "fp64")
+    ens = Ensemble(batched)
+    torch.cuda.synchronize()
+    before = [_state(m) for m in batched]
+    chosen = ens.autotune(cache=tmp_path / "tune.json")
+    print("[batch] autotune", chosen, flush=True)
+    assert chosen["tile_rows"] in (4, 8, 16) and chosen["surface_block"] in (64, 128, 256)
+    assert chosen["stage_ms"] > 0 and chosen["surface_ms"] > 0
+    torch.cuda.synchronize()
+    after = [_state(m) for m in batched]
+    for b, a in zip(before, after):
+        for k in b:
+            assert np.array_equal(b[k], a[k], equal_nan=True), k
+    again = ens.autotune(cache=tmp_path / "tune.json")
+    assert again["cached"] and again["tile_rows"] == chosen["tile_rows"]
+    for _ in range(2):
+        ens.time_step(DT)
+        for m in alone:
+            m.time_step(DT)
+    _assert_same(alone, batched, "autotuned")
+
+
+@pytest.mark.parametrize("tile_rows", [4, 16])
+def test_every_tile_shape_gives_the_same_step(require_gpu, tile_rows):
+    specs = _specs(5, seed=10)
+    alone = _build(specs, "fp64")
+    batched = _build(specs, "fp64")
+    ens = Ensemble(batched)
+    t = ens.tuning
+    t.update(tile_rows=tile_rows, stage_blocks=16, surface_block=64, surface_blocks=3)
+    ens.set_tuning(t)
+    for _ in range(2):
+        ens.time_step(DT)
+        for m in alone:
+            m.time_step(DT)
+    _assert_same(alone, batched, f"tile_rows={tile_rows}")
+
+
+def test_device_clock_crosses_snapshots(require_gpu):
+    specs = _specs(4, seed=11)
+    alone = _build(specs, "fp64", n_times=3)
+    batched = _build(specs, "fp64", n_times=3)
+    ens = Ensemble(batched)
+    ens.use_graph(True)
+    for _ in range(8):   # 8 × 20 min crosses the 1 h and 2 h snapshots and clamps after the last
+        ens.time_step(DT)
+        for m in alone:
+            m.time_step(DT)
+    _assert_same(alone, batched, "time interpolation")
+
+
+def test_members_are_locked_while_batched(require_gpu):
+    specs = _specs(2, seed=12)
+    models = _build(specs, "fp64")
+    ens = Ensemble(models)
+    with pytest.raises(NEError) as e:
+        models[0].time_step(DT)
+    assert e.value.status == abi.NE_ERR_STATE
+    with pytest.raises(NEError):
+        Ensemble(models)                      # a model belongs to one batch at a time
+    ens.close()
+    models[0].time_step(DT)
+
+
+def test_mixed_precisions_are_refused(require_gpu):
+    specs = _specs(2, seed=13)
+    a = _build(specs[:1], "fp64")[0]
+    b = _build(specs[1:], "mixed")[0]
+    with pytest.raises(ValueError):
+        Ensemble([a, b])
+    import ctypes as C
+    arr = (C.c_void_p * 2)(a.handle.value, b.handle.value)
+    h = C.c_void_p()
+    assert a.lib.ne_batch_create(a.ctx.handle, 2, arr, C.byref(h), None) == abi.NE_ERR_UNSUPPORTED

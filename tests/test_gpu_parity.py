@@ -211,9 +211,27 @@ def test_model_steps_match_oracle(require_gpu, precision):
         assert_rel(_np(gm.T), om.T, RTOL, "T")
         assert_rel(_np(gm.S), om.S, RTOL, "S")
         _compare_fluxes(gm, om)
+        scales = _net_term_scales(case, om)
         for k in O.NET_NAMES:
-            assert_rel(_np(gm.net[k]), om.net_arrays[k], RTOL, k)
+            assert_rel_terms(_np(gm.net[k]), om.net_arrays[k], scales.get(k, 0.0), RTOL, k)
     assert gm.clock == (6000.0, 5)
+
+
+def _net_term_scales(case, om):
+    """Magnitude of the terms summed into each net flux: τ is a face average of neighbouring
+    ρτ/ρₒ (which may cancel), Jᵀ a sum of the turbulent and radiative heat fluxes over ρₒcₒ."""
+    f, r = om.flux_arrays, om.rad_arrays
+    rho, c = om.params.ocean_reference_density, om.params.ocean_heat_capacity
+    spec = case.grid.spec
+    rtx, rty = np.abs(f["x_momentum"]), np.abs(f["y_momentum"])
+    west, south = np.roll(rtx, 1, axis=1), np.roll(rty, 1, axis=0)
+    if spec.topology_x != abi.NE_PERIODIC:
+        west[:, 0] = 0.0
+    if spec.topology_y != abi.NE_PERIODIC:
+        south[0, :] = 0.0
+    heat = (np.abs(f["sensible_heat"]) + np.abs(f["latent_heat"]) + np.abs(r["upwelling_longwave"]) +
+            np.abs(r["downwelling_longwave"]) + np.abs(r["downwelling_shortwave"])) / (rho * c)
+    return {"tau_x": (rtx + west) / 2.0 / rho, "tau_y": (rty + south) / 2.0 / rho, "J_T": heat}
 
 
 def test_budget_closure_on_device(require_gpu):
@@ -238,6 +256,44 @@ def test_budget_closure_on_device(require_gpu):
     err = abs((H1 - H0) - expected)
     print(f"[budget] ΔH={H1 - H0:.6e} expected={expected:.6e} err={err:.3e} bound={bound:.3e}", flush=True)
     assert err <= bound
+
+
+def test_mixed_form_1000_steps_vs_fp64_oracle(require_gpu):
+    """Mixed form (FP32 WENO5 + interpolation + storage, FP32-then-FP64 flux iteration, FP64
+    budgets) against the FP64 oracle after 1000 steps of 20 min on the 90×40×10 workload:
+    SST within 1e-5 relative, surface heat fluxes within 1e-3 of their scale, heat content
+    within 1e-9."""
+    case = make_case(*SMALL, seed=21)
+    om = O.OracleModel(case).reconcile()
+    g32 = OceanOnlyModel(case, precision="mixed")
+    rho_c = g32.params.ocean_reference_density * g32.params.ocean_heat_capacity
+    for n in range(1000):
+        om.time_step(1200.0)
+        g32.time_step(1200.0)
+        if n % 200 == 199:
+            torch.cuda.synchronize()
+            print(f"[mixed] step {n + 1}", flush=True)
+    torch.cuda.synchronize()
+    wet = case.grid.kmin < case.grid.spec.Nz
+    sst = _np(g32.T)[-1].astype(np.float64)
+    diff = np.abs(sst - om.T[-1])[wet]
+    sst_err = float(diff.max() / np.max(np.abs(om.T[-1][wet])))          # relative to the SST scale (°C)
+    sst_pointwise = float(np.max(diff / np.abs(om.T[-1][wet])))           # pointwise, °C
+    sst_kelvin = float(np.max(diff / (om.T[-1][wet] + 273.15)))           # pointwise, K
+    print(f"[mixed] SST abs {diff.max():.2e} K, rms {np.sqrt(np.mean(diff ** 2)):.2e} K, pointwise °C "
+          f"{sst_pointwise:.2e}, pointwise K {sst_kelvin:.2e}", flush=True)
+    qt = _np(g32.fluxes["sensible_heat"])
+    qv = _np(g32.fluxes["latent_heat"])
+    qt_err = float(np.max(np.abs(qt - om.flux_arrays["sensible_heat"])) / np.max(np.abs(om.flux_arrays["sensible_heat"])))
+    qv_err = float(np.max(np.abs(qv - om.flux_arrays["latent_heat"])) / np.max(np.abs(om.flux_arrays["latent_heat"])))
+    H32 = g32.budget(g32.T, rho_c)
+    Ho = sum(om.budget(om.T, rho_c))
+    h_err = abs(H32 - Ho) / abs(Ho)
+    print(f"[mixed] 1000 steps: SST {sst_err:.2e}, sensible {qt_err:.2e}, latent {qv_err:.2e}, heat {h_err:.2e}",
+          flush=True)
+    assert sst_err <= 1e-5
+    assert qt_err <= 1e-3 and qv_err <= 1e-3
+    assert h_err <= 1e-9
 
 
 def test_mixed_precision_model_tracks_fp64(require_gpu):

@@ -41,6 +41,14 @@ def parse():
     ap.add_argument("--ensemble", type=int, default=0, metavar="N",
                     help="ensemble report: N uneven members stepped as one batch (autotuned, graph) "
                          "against the same members stepped one after another")
+    ap.add_argument("--serve", type=str, default="",
+                    help="serving report at these request rates (req/s, comma-separated)")
+    ap.add_argument("--serve-seconds", type=float, default=4.0, help="arrival window of each serving run")
+    ap.add_argument("--serve-steps", type=int, default=100, help="time steps per request")
+    ap.add_argument("--serve-budget", type=float, default=200e9, help="HBM admission budget (bytes)")
+    ap.add_argument("--serve-cells", type=float, default=32e6, help="work budget: resident cells")
+    ap.add_argument("--serve-pool", type=int, default=32, help="distinct pre-generated request inputs")
+    ap.add_argument("--serve-plain-max", type=int, default=150, help="requests run on the plain path")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="only run a device-to-device copy of this many GiB (--steps times) and report its bandwidth")
     return ap.parse_args()
@@ -108,6 +116,61 @@ def ensemble_report(args):
            "batched_graph_ms_per_step": 1e3 * t_graph, "batched_graph_cell_updates_per_s": cells / t_graph,
            "speedup_graph_over_sequential": t_seq / t_graph, "default_tuning": default, "autotuned": chosen,
            "work_items": info}
+    print(json.dumps(out), flush=True)
+
+
+def serving_report(args):
+    """Continuous-batching server against the plain per-request path under Poisson arrivals.
+
+    Requests are 100-step solves whose grids are drawn log-uniform between 1×1×50 and
+    720×280×10 (regional, 1/2°), FP64 or mixed 50/50, from a pool of pre-generated inputs
+    (the client's data), admitted under a 200 GB HBM budget."""
+    import numpy as np
+    import torch
+    from numericalearth_hip.ensemble import draw_ensemble, member_case, member_params
+    from numericalearth_hip.serving import (SURFACE_RESULT_FIELDS, Request, Server, latency_summary,
+                                            poisson_arrivals, serve_plain)
+
+    rng = np.random.default_rng(args.seed)
+    pool = draw_ensemble(args.serve_pool, args.seed, smallest=(1, 1, 50), largest=(720, 280, 10), resolution=0.5)
+    cases = [(member_case(s), member_params(s)) for s in pool]
+    rates = [float(x) for x in args.serve.split(",")]
+    out = {"report": "serving", "steps_per_request": args.serve_steps, "dt": args.dt,
+           "budget_bytes": args.serve_budget, "cell_budget": args.serve_cells, "duration_s": args.serve_seconds, "pool": len(pool), "rates": {}}
+
+    def make_requests(rate):
+        arr = poisson_arrivals(rate, args.serve_seconds, rng)
+        reqs = []
+        for k, t in enumerate(arr):
+            c, p = cases[int(rng.integers(len(cases)))]
+            reqs.append(Request(id=k, case=c, params=p, precision="fp64" if rng.uniform() < 0.5 else "mixed",
+                                steps=args.serve_steps, dt=args.dt, arrival=float(t)))
+        return reqs
+
+    # warm-up: one small request of each precision through both paths
+    warm = [Request(id=-1, case=cases[0][0], params=cases[0][1], precision=p, steps=2) for p in ("fp64", "mixed")]
+    srv = Server(budget_bytes=args.serve_budget, cell_budget=args.serve_cells,
+                     result_fields=SURFACE_RESULT_FIELDS)
+    srv.run(warm)
+    srv.close()
+    serve_plain([Request(id=-1, case=cases[0][0], params=cases[0][1], precision="fp64", steps=2)])
+    torch.cuda.synchronize()
+    for rate in rates:
+        reqs = make_requests(rate)
+        srv = Server(budget_bytes=args.serve_budget, cell_budget=args.serve_cells,
+                     result_fields=SURFACE_RESULT_FIELDS)
+        done = srv.run(reqs)
+        srv.close()
+        batched = latency_summary(done)
+        batched["batch_steps"] = srv.steps_issued
+        print(f"# rate {rate}: batched {batched}", file=sys.stderr, flush=True)
+        # the plain path on the same arrival stream, cut to the first requests it serves in ~30 s
+        plain_in = [Request(id=r.id, case=r.case, params=r.params, precision=r.precision, steps=r.steps,
+                            dt=r.dt, arrival=r.arrival) for r in reqs[:args.serve_plain_max]]
+        plain = latency_summary(serve_plain(plain_in, result_fields=SURFACE_RESULT_FIELDS))
+        print(f"# rate {rate}: plain {plain}", file=sys.stderr, flush=True)
+        out["rates"][str(rate)] = {"offered_requests": len(reqs), "batched": batched, "plain": plain,
+                                   "throughput_ratio": batched["cell_updates_per_s"] / plain["cell_updates_per_s"]}
     print(json.dumps(out), flush=True)
 
 
@@ -202,6 +265,9 @@ def main():
         return
     if args.ensemble:
         ensemble_report(args)
+        return
+    if args.serve:
+        serving_report(args)
         return
     if args.copy_gib > 0:
         print(json.dumps(dict(copy_bandwidth(args.copy_gib, args.steps), report="copy", gib=args.copy_gib)),

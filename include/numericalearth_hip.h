@@ -375,12 +375,20 @@ typedef struct ne_model_desc {
     int32_t n_atmosphere_times;
     int32_t n_radiation_times;
     const double* radiation_times;        /* HOST */
+    void*   workspace;                    /* optional caller-owned device memory for the RK stage
+                                             buffers (ne_model_workspace_bytes); NULL: the model
+                                             allocates them itself */
+    int64_t workspace_bytes;
 } ne_model_desc;
+
+/* Bytes of device workspace a model of this descriptor needs (four stage buffers of T, S). */
+int ne_model_workspace_bytes(const ne_model_desc* desc, int64_t* bytes);
 
 typedef struct ne_model ne_model;
 
 /* Create the model and run reconcile_state! (fluxes computed at construction,
- * earth_system_model.jl:86-90), i.e. one update_state with dt_frazil = +INFINITY. */
+ * earth_system_model.jl:86-90), i.e. one update_state with dt_frazil = +INFINITY.
+ * With desc->workspace set, creation and destruction make no device allocation. */
 int ne_model_create(ne_ctx* ctx, const ne_model_desc* desc, ne_model** model, void* stream);
 int ne_model_destroy(ne_model* model);
 
@@ -421,8 +429,9 @@ typedef struct ne_batch ne_batch;
 /* Launch shape of a batch step. tile_rows ∈ {4, 8, 16} is the tile height of the WENO5
  * stage kernel (tiles are 63 columns wide, one wavefront per row); stage_blocks (a
  * multiple of 8) is its persistent grid; surface_block ∈ {64, 128, 256} columns per work
- * item and surface_blocks the grid of the two surface kernels. stage_ms / surface_ms are
- * the timings the autotuner measured (0 when set by hand). */
+ * item and surface_blocks the grid of the two surface kernels; a grid of 0 means one
+ * block per work item. stage_ms / surface_ms are the timings the autotuner measured
+ * (0 when set by hand). */
 typedef struct ne_batch_tuning {
     int32_t tile_rows;
     int32_t stage_blocks;
@@ -432,8 +441,16 @@ typedef struct ne_batch_tuning {
     double  surface_ms;
 } ne_batch_tuning;
 
-int ne_batch_create(ne_ctx* ctx, int n_models, ne_model* const* models, ne_batch** batch, void* stream);
+/* A batch of `precision` (NE_FP64 or NE_MIXED_FP32) holding the n_models given models
+ * (n_models may be 0). All calls on one batch must be ordered on one stream. */
+int ne_batch_create(ne_ctx* ctx, int32_t precision, int n_models, ne_model* const* models, ne_batch** batch,
+                    void* stream);
+/* Waits for the device, then releases the batch's device memory; the members are released. */
 int ne_batch_destroy(ne_batch* batch);
+/* Membership changes take effect at the next step (the member table is re-uploaded with one
+ * stream-ordered copy; nothing synchronises the device). A model added joins at its own clock. */
+int ne_batch_add(ne_batch* batch, ne_model* model);
+int ne_batch_remove(ne_batch* batch, ne_model* model);
 int ne_batch_time_step(ne_batch* batch, double dt, void* stream);
 /* enabled != 0: capture the step once per Δt into a HIP graph and replay it. */
 int ne_batch_set_graph(ne_batch* batch, int enabled);

@@ -98,6 +98,44 @@ ne_time_interp time_indices(const std::vector<double>& times, double t) {
     return ne::time_indices(times.data(), (int)times.size(), t);
 }
 
+// Bytes spanned by a view of nx × ny × nz elements (0 if a stride is not positive).
+int64_t extent_bytes(const ne_view& v, int64_t nx, int64_t ny, int64_t nz, int64_t esize) {
+    if (v.sx <= 0 || v.sy <= 0 || (nz > 1 && v.sz <= 0)) return 0;
+    return ((nx - 1) * v.sx + (ny - 1) * v.sy + (nz - 1) * (nz > 1 ? v.sz : 0) + 1) * esize;
+}
+
+constexpr int64_t MAX_VIEW_BYTES = (int64_t)1 << 32;
+
+// The views a tracer stage reads and writes: the tracer views (T⁰, S⁰, c_in, c_out) share
+// one layout, the 2-D surface fluxes share one, and every view spans < 4 GiB (the stage
+// kernel addresses each view with 32-bit byte offsets).
+int check_tracer_views(const ne_grid* g, const ne_view* const* tracers, int ntr, const ne_view& u, const ne_view& v,
+                       const ne_view& w, const ne_net_ocean_fluxes* net, int64_t esize) {
+    const int64_t Nx = g->Nx, Ny = g->Ny, Nz = g->Nz;
+    const ne_view& c = *tracers[0];
+    for (int n = 0; n < ntr; n++)
+        if (tracers[n]->sx != 1 || tracers[n]->sy != c.sy || tracers[n]->sz != c.sz) return NE_ERR_SHAPE;
+    const ne_view* J[3] = {&net->J_T, &net->J_S, &net->J_w};
+    for (const ne_view* j : J)
+        if (j->sx != 1 || j->sy != J[0]->sy) return NE_ERR_SHAPE;
+    int64_t ext[5] = {extent_bytes(c, Nx, Ny, Nz, esize), extent_bytes(u, Nx + 1, Ny, Nz, esize),
+                      extent_bytes(v, Nx, Ny + 1, Nz, esize), extent_bytes(w, Nx, Ny, Nz + 1, esize),
+                      extent_bytes(*J[0], Nx, Ny, 1, 8)};
+    for (int64_t e : ext) {
+        if (e <= 0) return NE_ERR_SHAPE;
+        if (e >= MAX_VIEW_BYTES) return NE_ERR_CAPACITY;
+    }
+    return NE_OK;
+}
+
+// One RK stage buffer (T or S) of a model: the layout of the model's T view, rounded up
+// to 256 bytes.
+size_t stage_buffer_bytes(const ne_model_desc& d) {
+    int64_t esize = d.precision == NE_FP64 ? sizeof(double) : sizeof(float);
+    int64_t n = extent_bytes(d.ocean.T, d.grid.Nx, d.grid.Ny, d.grid.Nz, esize);
+    return ((size_t)(n > 0 ? n : 0) + 255) & ~(size_t)255;
+}
+
 }  // namespace
 
 extern "C" {
@@ -297,6 +335,8 @@ int ne_tracer_tendency_weno5_f64(ne_ctx* c, const ne_grid* g, const ne_ocean_sta
     if ((r = check_grid(g)) || (r = check_ocean(o)) || (r = check_net(net)) || (r = check_view3(GT)) ||
         (r = check_view3(GS)))
         return r;
+    const ne_view* tr[] = {&o->T, &o->S, &GT, &GS};
+    if ((r = check_tracer_views(g, tr, 4, o->u, o->v, o->w, net, sizeof(double)))) return r;
     return dev_err(launch_tendency(grid_dev(*g), view<const double>(o->u), view<const double>(o->v),
                                    view<const double>(o->w), view<const double>(o->T), view<const double>(o->S),
                                    view<const double>(net->J_T), view<const double>(net->J_S),
@@ -314,12 +354,14 @@ int ne_tracer_rk_stage_f64(ne_ctx* c, const ne_grid* g, ne_view u, ne_view v, ne
     for (const ne_view* x : vs)
         if ((r = check_view3(*x))) return r;
     if (Tin.data == Tout.data || Sin.data == Sout.data) return NE_ERR_SHAPE;
+    const ne_view* tr[] = {&T0, &S0, &Tin, &Sin, &Tout, &Sout};
+    if ((r = check_tracer_views(g, tr, 6, u, v, w, net, sizeof(double)))) return r;
     return dev_err(launch_rk_stage<double>(grid_dev(*g), view<const double>(u), view<const double>(v),
                                            view<const double>(w), view<const double>(T0), view<const double>(S0),
                                            view<const double>(Tin), view<const double>(Sin), view<double>(Tout),
                                            view<double>(Sout), view<const double>(net->J_T),
                                            view<const double>(net->J_S), view<const double>(net->J_w), beta_dt,
-                                           as_stream(stream)));
+                                           false, 0.0, as_stream(stream)));
 }
 
 int ne_budget_f64(ne_ctx* c, const ne_grid* g, ne_view cv, double weight, double* out, void* stream) {
@@ -392,16 +434,18 @@ static int model_ocean_step(ne_model* m, double dt, hipStream_t s, hipEvent_t* e
     const double beta[3] = {1.0 / 3.0, 0.5, 1.0};
     hipError_t e;
     if (ev && hipEventRecord(ev[0], s) != hipSuccess) return NE_ERR_DEVICE;
+    const bool sr = sizeof(Real) == 4;   // FP32 state: stochastic rounding of the stage that writes it
     e = launch_rk_stage<Real>(g, u, v, w, T0, S0, T0, S0, view<Real>(m->vT1), view<Real>(m->vS1), JT, JS, Jw,
-                              beta[0] * dt, s);
+                              beta[0] * dt, false, m->time, s);
     if (e != hipSuccess) return NE_ERR_DEVICE;
     if (ev && hipEventRecord(ev[1], s) != hipSuccess) return NE_ERR_DEVICE;
     e = launch_rk_stage<Real>(g, u, v, w, T0, S0, view<const Real>(m->vT1), view<const Real>(m->vS1),
-                              view<Real>(m->vT2), view<Real>(m->vS2), JT, JS, Jw, beta[1] * dt, s);
+                              view<Real>(m->vT2), view<Real>(m->vS2), JT, JS, Jw, beta[1] * dt, false, m->time, s);
     if (e != hipSuccess) return NE_ERR_DEVICE;
     if (ev && hipEventRecord(ev[2], s) != hipSuccess) return NE_ERR_DEVICE;
     e = launch_rk_stage<Real>(g, u, v, w, T0, S0, view<const Real>(m->vT2), view<const Real>(m->vS2),
-                              view<Real>(d.ocean.T), view<Real>(d.ocean.S), JT, JS, Jw, beta[2] * dt, s);
+                              view<Real>(d.ocean.T), view<Real>(d.ocean.S), JT, JS, Jw, beta[2] * dt, sr, m->time,
+                              s);
     if (e != hipSuccess) return NE_ERR_DEVICE;
     if (ev && hipEventRecord(ev[3], s) != hipSuccess) return NE_ERR_DEVICE;
     return NE_OK;
@@ -423,6 +467,13 @@ int ne_model_create(ne_ctx* c, const ne_model_desc* desc, ne_model** out, void* 
     if (desc->params.radiation && (!desc->radiation_fi || !desc->radiation_fj)) return NE_ERR_NULL;
     if (desc->n_atmosphere_times < 0 || desc->n_radiation_times < 0) return NE_ERR_SHAPE;
     if (desc->n_atmosphere_times > 0 && !desc->atmosphere_times) return NE_ERR_NULL;
+    {
+        const ne_view* tr[] = {&desc->ocean.T, &desc->ocean.S};
+        int64_t esize = desc->precision == NE_FP64 ? sizeof(double) : sizeof(float);
+        if ((r = check_tracer_views(&desc->grid, tr, 2, desc->ocean.u, desc->ocean.v, desc->ocean.w, &desc->net,
+                                    esize)))
+            return r;
+    }
     if (desc->n_radiation_times > 0 && !desc->radiation_times) return NE_ERR_NULL;
 
     ne_model* m = new (std::nothrow) ne_model();
@@ -432,19 +483,26 @@ int ne_model_create(ne_ctx* c, const ne_model_desc* desc, ne_model** out, void* 
     m->atimes.assign(desc->atmosphere_times, desc->atmosphere_times + desc->n_atmosphere_times);
     m->rtimes.assign(desc->radiation_times, desc->radiation_times + desc->n_radiation_times);
     const ne_grid& g = desc->grid;
-    size_t esize = desc->precision == NE_FP64 ? sizeof(double) : sizeof(float);
-    size_t n = (size_t)g.Nx * g.Ny * g.Nz * esize;
+    size_t n = stage_buffer_bytes(*desc);
     void** bufs[4] = {&m->T1, &m->S1, &m->T2, &m->S2};
     ne_view* views[4] = {&m->vT1, &m->vS1, &m->vT2, &m->vS2};
+    if (desc->workspace) {
+        if (desc->workspace_bytes < (int64_t)(4 * n)) {
+            delete m;
+            return NE_ERR_CAPACITY;
+        }
+        m->owns_buffers = false;
+        for (int b = 0; b < 4; b++) *bufs[b] = static_cast<char*>(desc->workspace) + b * n;
+    }
     for (int b = 0; b < 4; b++) {
-        if (hipMalloc(bufs[b], n) != hipSuccess) {
+        if (m->owns_buffers && hipMalloc(bufs[b], n) != hipSuccess) {
             ne_model_destroy(m);
             return NE_ERR_DEVICE;
         }
         views[b]->data = *bufs[b];
-        views[b]->sx = 1;
-        views[b]->sy = g.Nx;
-        views[b]->sz = (int64_t)g.Nx * g.Ny;
+        views[b]->sx = 1;                 // the layout of the model's T view
+        views[b]->sy = desc->ocean.T.sy;
+        views[b]->sz = desc->ocean.T.sz;
     }
     // reconcile_state!: fluxes are computed at construction (earth_system_model.jl:86-90)
     r = model_update_state(m, INFINITY, as_stream(stream));
@@ -456,13 +514,23 @@ int ne_model_create(ne_ctx* c, const ne_model_desc* desc, ne_model** out, void* 
     return NE_OK;
 }
 
+int ne_model_workspace_bytes(const ne_model_desc* desc, int64_t* bytes) {
+    if (!desc || !bytes) return NE_ERR_NULL;
+    int r = check_grid(&desc->grid);
+    if (r || (r = check_view3(desc->ocean.T))) return r;
+    if (desc->precision != NE_FP64 && desc->precision != NE_MIXED_FP32) return NE_ERR_UNSUPPORTED;
+    *bytes = (int64_t)(4 * stage_buffer_bytes(*desc));
+    return NE_OK;
+}
+
 int ne_model_destroy(ne_model* m) {
     if (!m) return NE_ERR_NULL;
     if (m->batch) return NE_ERR_STATE;
     int r = NE_OK;
     void* bufs[4] = {m->T1, m->S1, m->T2, m->S2};
-    for (void* b : bufs)
-        if (b && hipFree(b) != hipSuccess) r = NE_ERR_DEVICE;
+    if (m->owns_buffers)
+        for (void* b : bufs)
+            if (b && hipFree(b) != hipSuccess) r = NE_ERR_DEVICE;
     for (hipEvent_t e : m->events) (void)hipEventDestroy(e);
     delete m;
     return r;

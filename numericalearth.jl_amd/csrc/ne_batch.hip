@@ -2,13 +2,20 @@
 //
 // A batch binds existing models (each created by ne_model_create, so each owns its
 // descriptors and RK stage buffers) into one device table of member records. A step is
-// six launches for the whole ensemble, whatever its size: three RK stages, the member
-// clocks, the surface solve and the surface finalize, each a persistent grid walking a
-// work list of (member, tile) or (member, column block) items. The six launches can be
-// captured once into a HIP graph and replayed per step; every quantity that changes from
-// step to step (member clocks, time-interpolation indices) lives in device memory, so
-// the graph stays valid for a fixed Δt. The tile shape and the grid sizes come from an
-// autotuner that times the candidates on the batch itself.
+// six or seven launches for the whole ensemble, whatever its size: three RK stages, the
+// member clocks, the surface solve (one launch per flux path in use) and the surface
+// finalize, each a persistent grid walking a work list of (member, tile) or
+// (member, column block) items. The launches can be captured once into a HIP graph and
+// replayed per step; every quantity that changes from step to step (member clocks,
+// time-interpolation indices) lives in device memory, so the graph stays valid for a
+// fixed Δt and a fixed membership.
+//
+// Membership may change between steps (ne_batch_add / ne_batch_remove, for serving).
+// The member table and the work lists live in one device arena and are rebuilt on the
+// host into one of two pinned images, then uploaded with a single stream-ordered copy
+// before the next step; the arena grows geometrically and retired arenas are released
+// with the batch, so no step synchronises the device (a rebuild waits at most for the
+// upload made two rebuilds earlier).
 #include <math.h>
 #include <string.h>
 
@@ -29,6 +36,7 @@ constexpr int SURFACE_BLOCKS[3] = {64, 128, 256};
 inline int tr_index(int tr) { return tr == 4 ? 0 : tr == 8 ? 1 : tr == 16 ? 2 : -1; }
 inline int sb_index(int b) { return b == 64 ? 0 : b == 128 ? 1 : b == 256 ? 2 : -1; }
 inline int round8(int n) { return ((n + NXCD - 1) / NXCD) * NXCD; }
+inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 template <typename Real>
 __global__ void batch_clock_kernel(BatchRec<Real>* recs, int n, double dt) {
@@ -47,8 +55,21 @@ struct ne_batch {
     ne_ctx* ctx = nullptr;
     int precision = NE_FP64;
     std::vector<ne_model*> members;
-    void* recs = nullptr;                 // device BatchRec<Real>[n]
-    double* times = nullptr;              // device snapshot times of every member
+    bool dirty = true;                    // membership changed since the last upload
+
+    // device arena (member records, snapshot times, work lists) and two pinned host images,
+    // used in turn so that a rebuild waits only for the upload before the previous one
+    char* dev = nullptr;
+    size_t dev_cap = 0;
+    char* host[2] = {nullptr, nullptr};
+    size_t host_cap[2] = {0, 0};
+    int image = 0;
+    std::vector<void*> retired_dev, retired_host;
+    hipEvent_t uploaded[2] = {nullptr, nullptr};   // recorded after the last upload from host[i]
+    bool upload_pending[2] = {false, false};
+
+    // views into the arena
+    void* recs = nullptr;
     WorkItem* stage_work[3] = {nullptr, nullptr, nullptr};    // per tile-row choice
     int stage_nwork[3] = {0, 0, 0};
     // per column-block choice: every member (finalize), and the members of each flux path (solve)
@@ -56,6 +77,7 @@ struct ne_batch {
     int surface_nwork[3] = {0, 0, 0};
     WorkItem* solve_work[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};  // [fast][choice]
     int solve_nwork[2][3] = {{0, 0, 0}, {0, 0, 0}};
+
     ne_batch_tuning tuning;
     int cus = 256;
     bool use_graph = false;
@@ -63,7 +85,6 @@ struct ne_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     double graph_dt = 0.0;
-    int64_t cells = 0;
 };
 
 namespace {
@@ -75,31 +96,38 @@ void drop_graph(ne_batch* b) {
     b->graph = nullptr;
 }
 
+inline int blocks_for(int requested, int nwork) { return requested > 0 ? requested : std::max(nwork, 1); }
+
 template <typename Real>
 hipError_t enqueue_step(ne_batch* b, double dt, hipStream_t s, ne_counters* counters) {
+    int n = (int)b->members.size();
+    if (n == 0) return hipSuccess;
     const BatchRec<Real>* recs = static_cast<const BatchRec<Real>*>(b->recs);
     const ne_batch_tuning& t = b->tuning;
     int ti = tr_index(t.tile_rows), si = sb_index(t.surface_block);
     if (ti < 0 || si < 0) return hipErrorInvalidValue;
     const double beta[3] = {1.0 / 3.0, 0.5, 1.0};
     hipError_t e;
+    int nst = b->stage_nwork[ti];
+    int sblocks = t.stage_blocks > 0 ? std::min(t.stage_blocks, round8(nst)) : round8(nst);
+    sblocks = std::max(sblocks, NXCD);
     for (int st = 0; st < 3; st++) {
-        e = launch_batch_stage<Real>(recs, b->stage_work[ti], b->stage_nwork[ti], t.tile_rows, t.stage_blocks, st + 1,
-                                     beta[st] * dt, s);
+        e = launch_batch_stage<Real>(recs, b->stage_work[ti], nst, t.tile_rows, sblocks, st + 1, beta[st] * dt, s);
         if (e != hipSuccess) return e;
     }
-    int n = (int)b->members.size();
     hipLaunchKernelGGL(batch_clock_kernel<Real>, dim3((n + 63) / 64), dim3(64), 0, s,
                        static_cast<BatchRec<Real>*>(b->recs), n, dt);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     for (int fast = 1; fast >= 0; fast--) {
         int nw = b->solve_nwork[fast][si];
         e = launch_batch_surface_solve<Real>(recs, b->solve_work[fast][si], nw, t.surface_block,
-                                             std::min(t.surface_blocks, std::max(nw, 1)), fast == 1, counters, s);
+                                             std::min(blocks_for(t.surface_blocks, nw), std::max(nw, 1)), fast == 1,
+                                             counters, s);
         if (e != hipSuccess) return e;
     }
-    return launch_batch_surface_finalize<Real>(recs, b->surface_work[si], b->surface_nwork[si], t.surface_block,
-                                               t.surface_blocks, dt, s);
+    int nsu = b->surface_nwork[si];
+    return launch_batch_surface_finalize<Real>(recs, b->surface_work[si], nsu, t.surface_block,
+                                               std::min(blocks_for(t.surface_blocks, nsu), std::max(nsu, 1)), dt, s);
 }
 
 hipError_t enqueue(ne_batch* b, double dt, hipStream_t s, ne_counters* counters) {
@@ -146,87 +174,129 @@ BatchRec<Real> member_record(const ne_model* m, const double* atimes_dev, const 
     return r;
 }
 
+// Host-side layout of the arena: records | times | 3 stage lists | 3 surface lists | 2×3 solve lists.
+struct Layout {
+    size_t recs = 0, times = 0, stage[3] = {0, 0, 0}, surface[3] = {0, 0, 0}, solve[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    size_t total = 0;
+};
+
 template <typename Real>
-int upload_records(ne_batch* b, hipStream_t s) {
-    // snapshot times of every member, packed
+int rebuild(ne_batch* b, hipStream_t s) {
+    const int n = (int)b->members.size();
+    // work lists and snapshot times on the host
+    std::vector<WorkItem> stage[3], surface[3], solve[2][3];
     std::vector<double> times;
-    std::vector<size_t> aoff, roff;
-    for (ne_model* m : b->members) {
-        aoff.push_back(times.size());
+    std::vector<size_t> aoff(n), roff(n);
+    for (int k = 0; k < n; k++) {
+        const ne_model* m = b->members[k];
+        const ne_grid& g = m->d.grid;
+        for (int c = 0; c < 3; c++) {
+            int nt = batch_tiles_x(g.Nx) * batch_tiles_y(g.Ny, TILE_ROWS[c]);
+            for (int t = 0; t < nt; t++) stage[c].push_back(WorkItem{k, t});
+        }
+        int fast = edson_default_constants(m->d.params) ? 1 : 0;
+        int cols = g.Nx * g.Ny;
+        for (int c = 0; c < 3; c++)
+            for (int s0 = 0; s0 < cols; s0 += SURFACE_BLOCKS[c]) {
+                surface[c].push_back(WorkItem{k, s0});
+                solve[fast][c].push_back(WorkItem{k, s0});
+            }
+        aoff[k] = times.size();
         times.insert(times.end(), m->atimes.begin(), m->atimes.end());
-        roff.push_back(times.size());
+        roff[k] = times.size();
         times.insert(times.end(), m->rtimes.begin(), m->rtimes.end());
     }
-    if (!times.empty()) {
-        if (hipMalloc((void**)&b->times, times.size() * sizeof(double)) != hipSuccess) return NE_ERR_DEVICE;
-        if (hipMemcpy(b->times, times.data(), times.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-            return NE_ERR_DEVICE;
+    Layout L;
+    size_t off = 0;
+    L.recs = off;
+    off = align256(off + (size_t)n * sizeof(BatchRec<Real>));
+    L.times = off;
+    off = align256(off + times.size() * sizeof(double));
+    for (int c = 0; c < 3; c++) {
+        L.stage[c] = off;
+        off = align256(off + stage[c].size() * sizeof(WorkItem));
+        L.surface[c] = off;
+        off = align256(off + surface[c].size() * sizeof(WorkItem));
+        for (int f = 0; f < 2; f++) {
+            L.solve[f][c] = off;
+            off = align256(off + solve[f][c].size() * sizeof(WorkItem));
+        }
     }
-    std::vector<BatchRec<Real>> recs;
-    for (size_t k = 0; k < b->members.size(); k++) {
+    L.total = std::max<size_t>(off, 256);
+
+    // this pinned image may still be read by the upload made from it two rebuilds ago
+    const int im = b->image;
+    b->image ^= 1;
+    if (b->upload_pending[im]) {
+        if (hipEventSynchronize(b->uploaded[im]) != hipSuccess) return NE_ERR_DEVICE;
+        b->upload_pending[im] = false;
+    }
+    if (b->host_cap[im] < L.total) {
+        size_t cap = std::max(L.total, 2 * b->host_cap[im]);
+        void* h = nullptr;
+        if (hipHostMalloc(&h, cap, hipHostMallocDefault) != hipSuccess) return NE_ERR_DEVICE;
+        if (b->host[im]) b->retired_host.push_back(b->host[im]);
+        b->host[im] = static_cast<char*>(h);
+        b->host_cap[im] = cap;
+    }
+    if (b->dev_cap < L.total) {
+        // earlier steps may still read the old arena: it is released with the batch
+        size_t cap = std::max(L.total, 2 * b->dev_cap);
+        void* d = nullptr;
+        if (hipMalloc(&d, cap) != hipSuccess) return NE_ERR_DEVICE;
+        if (b->dev) b->retired_dev.push_back(b->dev);
+        b->dev = static_cast<char*>(d);
+        b->dev_cap = cap;
+    }
+    char* H = b->host[im];
+    char* D = b->dev;
+    const double* tdev = reinterpret_cast<const double*>(D + L.times);
+    BatchRec<Real>* recs = reinterpret_cast<BatchRec<Real>*>(H + L.recs);
+    for (int k = 0; k < n; k++) {
         const ne_model* m = b->members[k];
-        const double* at = m->atimes.empty() ? nullptr : b->times + aoff[k];
-        const double* rt = m->rtimes.empty() ? nullptr : b->times + roff[k];
-        recs.push_back(member_record<Real>(m, at, rt));
+        recs[k] = member_record<Real>(m, m->atimes.empty() ? nullptr : tdev + aoff[k],
+                                      m->rtimes.empty() ? nullptr : tdev + roff[k]);
     }
-    size_t bytes = recs.size() * sizeof(BatchRec<Real>);
-    if (hipMalloc(&b->recs, bytes) != hipSuccess) return NE_ERR_DEVICE;
-    if (hipMemcpyAsync(b->recs, recs.data(), bytes, hipMemcpyHostToDevice, s) != hipSuccess) return NE_ERR_DEVICE;
-    if (hipStreamSynchronize(s) != hipSuccess) return NE_ERR_DEVICE;
+    if (!times.empty()) memcpy(H + L.times, times.data(), times.size() * sizeof(double));
+    auto put = [&](size_t o, const std::vector<WorkItem>& w, WorkItem** view, int* count) {
+        if (!w.empty()) memcpy(H + o, w.data(), w.size() * sizeof(WorkItem));
+        *view = reinterpret_cast<WorkItem*>(D + o);
+        *count = (int)w.size();
+    };
+    for (int c = 0; c < 3; c++) {
+        put(L.stage[c], stage[c], &b->stage_work[c], &b->stage_nwork[c]);
+        put(L.surface[c], surface[c], &b->surface_work[c], &b->surface_nwork[c]);
+        for (int f = 0; f < 2; f++) put(L.solve[f][c], solve[f][c], &b->solve_work[f][c], &b->solve_nwork[f][c]);
+    }
+    b->recs = D + L.recs;
+    if (hipMemcpyAsync(D, H, L.total, hipMemcpyHostToDevice, s) != hipSuccess) return NE_ERR_DEVICE;
+    if (hipEventRecord(b->uploaded[im], s) != hipSuccess) return NE_ERR_DEVICE;
+    b->upload_pending[im] = true;
+    b->dirty = false;
+    drop_graph(b);
     return NE_OK;
 }
 
-int upload_work(WorkItem** dst, int* count, const std::vector<WorkItem>& w) {
-    *count = (int)w.size();
-    if (w.empty()) return NE_OK;
-    if (hipMalloc((void**)dst, w.size() * sizeof(WorkItem)) != hipSuccess) return NE_ERR_DEVICE;
-    if (hipMemcpy(*dst, w.data(), w.size() * sizeof(WorkItem), hipMemcpyHostToDevice) != hipSuccess)
-        return NE_ERR_DEVICE;
-    return NE_OK;
-}
-
-int build_work_lists(ne_batch* b) {
-    for (int c = 0; c < 3; c++) {
-        std::vector<WorkItem> w;
-        for (size_t k = 0; k < b->members.size(); k++) {
-            const ne_grid& g = b->members[k]->d.grid;
-            int nt = batch_tiles_x(g.Nx) * batch_tiles_y(g.Ny, TILE_ROWS[c]);
-            for (int t = 0; t < nt; t++) w.push_back(WorkItem{(int)k, t});
-        }
-        int r = upload_work(&b->stage_work[c], &b->stage_nwork[c], w);
-        if (r) return r;
-    }
-    for (int c = 0; c < 3; c++) {
-        std::vector<WorkItem> w, wf[2];
-        for (size_t k = 0; k < b->members.size(); k++) {
-            const ne_grid& g = b->members[k]->d.grid;
-            int fast = edson_default_constants(b->members[k]->d.params) ? 1 : 0;
-            int cols = g.Nx * g.Ny;
-            for (int s = 0; s < cols; s += SURFACE_BLOCKS[c]) {
-                w.push_back(WorkItem{(int)k, s});
-                wf[fast].push_back(WorkItem{(int)k, s});
-            }
-        }
-        int r = upload_work(&b->surface_work[c], &b->surface_nwork[c], w);
-        for (int f = 0; f < 2 && !r; f++) r = upload_work(&b->solve_work[f][c], &b->solve_nwork[f][c], wf[f]);
-        if (r) return r;
-    }
-    return NE_OK;
+int ensure_uploaded(ne_batch* b, hipStream_t s) {
+    if (!b->dirty) return NE_OK;
+    return b->precision == NE_FP64 ? rebuild<double>(b, s) : rebuild<float>(b, s);
 }
 
 void default_tuning(ne_batch* b) {
+    // one block per work item and the shortest tiles: the autotuner's usual choice for
+    // ensembles of uneven members
     ne_batch_tuning& t = b->tuning;
     memset(&t, 0, sizeof(t));
-    t.tile_rows = 8;
-    t.stage_blocks = std::max(NXCD, std::min(round8(b->stage_nwork[1]), 2 * b->cus));
+    t.tile_rows = 4;
+    t.stage_blocks = 0;
     t.surface_block = 256;
-    t.surface_blocks = std::max(1, std::min(b->surface_nwork[2], 4 * b->cus));
+    t.surface_blocks = 0;
 }
 
-int check_tuning(const ne_batch* b, const ne_batch_tuning& t) {
+int check_tuning(const ne_batch_tuning& t) {
     if (tr_index(t.tile_rows) < 0 || sb_index(t.surface_block) < 0) return NE_ERR_UNSUPPORTED;
-    if (t.stage_blocks <= 0 || t.stage_blocks % NXCD != 0 || t.surface_blocks <= 0) return NE_ERR_SHAPE;
-    if (t.stage_blocks > (1 << 20) || t.surface_blocks > (1 << 20)) return NE_ERR_CAPACITY;
+    if (t.stage_blocks < 0 || t.stage_blocks % NXCD != 0 || t.surface_blocks < 0) return NE_ERR_SHAPE;
+    if (t.stage_blocks > (1 << 24) || t.surface_blocks > (1 << 24)) return NE_ERR_CAPACITY;
     return NE_OK;
 }
 
@@ -259,26 +329,25 @@ int autotune(ne_batch* b, hipStream_t s, ne_batch_tuning* out) {
     ne_batch_tuning best = b->tuning;
     const int reps = 5;
     // RK stage: stage 1 writes only the stage buffers T¹, S¹ from T, S, so timing it leaves
-    // the state untouched.
+    // the state untouched. Grids: 1, 2 or 4 persistent blocks per CU, or one block per item (0).
     double best_stage = INFINITY;
     for (int c = 0; c < 3; c++) {
         int nwork = b->stage_nwork[c];
         if (nwork == 0) continue;
-        const int per_cu[4] = {1, 2, 4, 0};   // 0: one block per work item
+        const int per_cu[4] = {1, 2, 4, 0};
         for (int pc : per_cu) {
-            int blocks = pc ? std::min(round8(nwork), pc * b->cus) : round8(nwork);
-            blocks = std::max(blocks, NXCD);
+            int blocks = pc ? std::max(NXCD, std::min(round8(nwork), pc * b->cus)) : round8(nwork);
             double ms;
             auto f = [&]() {
                 return launch_batch_stage<Real>(recs, b->stage_work[c], nwork, TILE_ROWS[c], blocks, 1,
-                                                1.0 / 3.0 * 1.0, s);
+                                                1.0 / 3.0 * 1200.0, s);
             };
             int r = time_launches(s, reps, f, &ms);
             if (r) return r;
             if (ms < best_stage) {
                 best_stage = ms;
                 best.tile_rows = TILE_ROWS[c];
-                best.stage_blocks = blocks;
+                best.stage_blocks = pc && blocks < round8(nwork) ? blocks : 0;
             }
         }
     }
@@ -286,18 +355,18 @@ int autotune(ne_batch* b, hipStream_t s, ne_batch_tuning* out) {
     // the solver counters are not accumulated while tuning.
     double best_surface = INFINITY;
     for (int c = 0; c < 3; c++) {
-        int nwork = b->surface_nwork[c];
-        if (nwork == 0) continue;
+        if (b->surface_nwork[c] == 0) continue;
         const int per_cu[4] = {2, 4, 8, 0};
         for (int pc : per_cu) {
-            int blocks = pc ? std::min(nwork, pc * b->cus) : nwork;
+            int req = pc ? pc * b->cus : 0;
             double ms;
             auto f = [&]() {
                 hipError_t e = hipSuccess;
                 for (int fast = 1; fast >= 0 && e == hipSuccess; fast--) {
                     int nw = b->solve_nwork[fast][c];
                     e = launch_batch_surface_solve<Real>(recs, b->solve_work[fast][c], nw, SURFACE_BLOCKS[c],
-                                                         std::min(blocks, std::max(nw, 1)), fast == 1, nullptr, s);
+                                                         std::min(blocks_for(req, nw), std::max(nw, 1)), fast == 1,
+                                                         nullptr, s);
                 }
                 return e;
             };
@@ -306,7 +375,7 @@ int autotune(ne_batch* b, hipStream_t s, ne_batch_tuning* out) {
             if (ms < best_surface) {
                 best_surface = ms;
                 best.surface_block = SURFACE_BLOCKS[c];
-                best.surface_blocks = blocks;
+                best.surface_blocks = req < b->surface_nwork[c] ? req : 0;
             }
         }
     }
@@ -318,12 +387,18 @@ int autotune(ne_batch* b, hipStream_t s, ne_batch_tuning* out) {
     return NE_OK;
 }
 
-int step_host_clocks(ne_batch* b, double dt) {
+void step_host_clocks(ne_batch* b, double dt) {
     for (ne_model* m : b->members) {
         m->time += dt;
         m->iteration += 1;
         m->last_dt = dt;
     }
+}
+
+int check_member(const ne_batch* b, const ne_model* m) {
+    if (!m) return NE_ERR_NULL;
+    if (m->ctx != b->ctx || m->batch || m->timing) return NE_ERR_STATE;
+    if (m->d.precision != b->precision) return NE_ERR_UNSUPPORTED;
     return NE_OK;
 }
 
@@ -331,43 +406,37 @@ int step_host_clocks(ne_batch* b, double dt) {
 
 extern "C" {
 
-int ne_batch_create(ne_ctx* c, int n, ne_model* const* models, ne_batch** out, void* stream) {
-    if (!c || !models || !out) return NE_ERR_NULL;
+int ne_batch_create(ne_ctx* c, int32_t precision, int n, ne_model* const* models, ne_batch** out, void* stream) {
+    if (!c || !out || (n > 0 && !models)) return NE_ERR_NULL;
     *out = nullptr;
-    if (n <= 0 || n > (1 << 20)) return NE_ERR_SHAPE;
-    int precision = -1;
-    for (int k = 0; k < n; k++) {
-        ne_model* m = models[k];
-        if (!m) return NE_ERR_NULL;
-        if (m->ctx != c) return NE_ERR_STATE;
-        if (m->batch) return NE_ERR_STATE;
-        if (m->timing) return NE_ERR_STATE;
-        if (precision < 0) precision = m->d.precision;
-        if (m->d.precision != precision) return NE_ERR_UNSUPPORTED;
-        for (int j = 0; j < k; j++)
-            if (models[j] == m) return NE_ERR_STATE;
-    }
+    if (precision != NE_FP64 && precision != NE_MIXED_FP32) return NE_ERR_UNSUPPORTED;
+    if (n < 0 || n > (1 << 20)) return NE_ERR_SHAPE;
     ne_batch* b = new (std::nothrow) ne_batch();
     if (!b) return NE_ERR_CAPACITY;
     b->ctx = c;
     b->precision = precision;
-    b->members.assign(models, models + n);
-    for (ne_model* m : b->members) {
-        const ne_grid& g = m->d.grid;
-        b->cells += (int64_t)g.Nx * g.Ny * g.Nz;
+    for (int k = 0; k < n; k++) {
+        int r = check_member(b, models[k]);
+        if (!r && std::find(b->members.begin(), b->members.end(), models[k]) != b->members.end()) r = NE_ERR_STATE;
+        if (r) {
+            delete b;
+            return r;
+        }
+        b->members.push_back(models[k]);
     }
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && cus > 0)
         b->cus = cus;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    int r = precision == NE_FP64 ? upload_records<double>(b, s) : upload_records<float>(b, s);
-    if (!r) r = build_work_lists(b);
+    default_tuning(b);
+    int r = NE_OK;
+    for (int i = 0; i < 2 && !r; i++)
+        if (hipEventCreateWithFlags(&b->uploaded[i], hipEventDisableTiming) != hipSuccess) r = NE_ERR_DEVICE;
     if (!r && hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking) != hipSuccess) r = NE_ERR_DEVICE;
+    if (!r) r = ensure_uploaded(b, static_cast<hipStream_t>(stream));
     if (r) {
         ne_batch_destroy(b);
         return r;
     }
-    default_tuning(b);
     for (ne_model* m : b->members) m->batch = b;
     *out = b;
     return NE_OK;
@@ -379,21 +448,50 @@ int ne_batch_destroy(ne_batch* b) {
     drop_graph(b);
     for (ne_model* m : b->members)
         if (m->batch == b) m->batch = nullptr;
-    void* bufs[14] = {b->recs, b->times, b->stage_work[0], b->stage_work[1], b->stage_work[2],
-                      b->surface_work[0], b->surface_work[1], b->surface_work[2],
-                      b->solve_work[0][0], b->solve_work[0][1], b->solve_work[0][2],
-                      b->solve_work[1][0], b->solve_work[1][1], b->solve_work[1][2]};
-    for (void* p : bufs)
-        if (p && hipFree(p) != hipSuccess) r = NE_ERR_DEVICE;
+    // the arena may still be read by enqueued steps: wait for the device before releasing it
+    if ((b->dev || !b->retired_dev.empty()) && hipDeviceSynchronize() != hipSuccess) r = NE_ERR_DEVICE;
+    if (b->dev) b->retired_dev.push_back(b->dev);
+    for (int i = 0; i < 2; i++)
+        if (b->host[i]) b->retired_host.push_back(b->host[i]);
+    for (void* p : b->retired_dev)
+        if (hipFree(p) != hipSuccess) r = NE_ERR_DEVICE;
+    for (void* p : b->retired_host)
+        if (hipHostFree(p) != hipSuccess) r = NE_ERR_DEVICE;
+    for (int i = 0; i < 2; i++)
+        if (b->uploaded[i] && hipEventDestroy(b->uploaded[i]) != hipSuccess) r = NE_ERR_DEVICE;
     if (b->capture_stream && hipStreamDestroy(b->capture_stream) != hipSuccess) r = NE_ERR_DEVICE;
     delete b;
     return r;
+}
+
+int ne_batch_add(ne_batch* b, ne_model* m) {
+    if (!b || !m) return NE_ERR_NULL;
+    int r = check_member(b, m);
+    if (r) return r;
+    if (b->members.size() >= (1u << 20)) return NE_ERR_CAPACITY;
+    b->members.push_back(m);
+    m->batch = b;
+    b->dirty = true;
+    return NE_OK;
+}
+
+int ne_batch_remove(ne_batch* b, ne_model* m) {
+    if (!b || !m) return NE_ERR_NULL;
+    auto it = std::find(b->members.begin(), b->members.end(), m);
+    if (it == b->members.end() || m->batch != b) return NE_ERR_STATE;
+    b->members.erase(it);
+    m->batch = nullptr;
+    b->dirty = true;
+    return NE_OK;
 }
 
 int ne_batch_time_step(ne_batch* b, double dt, void* stream) {
     if (!b) return NE_ERR_NULL;
     if (!(dt > 0) || !isfinite(dt)) return NE_ERR_SHAPE;
     hipStream_t s = static_cast<hipStream_t>(stream);
+    int r = ensure_uploaded(b, s);
+    if (r) return r;
+    if (b->members.empty()) return NE_OK;
     if (b->use_graph) {
         if (!b->exec || b->graph_dt != dt) {
             drop_graph(b);
@@ -417,7 +515,8 @@ int ne_batch_time_step(ne_batch* b, double dt, void* stream) {
     } else {
         if (enqueue(b, dt, s, b->ctx->counters) != hipSuccess) return NE_ERR_DEVICE;
     }
-    return step_host_clocks(b, dt);
+    step_host_clocks(b, dt);
+    return NE_OK;
 }
 
 int ne_batch_set_graph(ne_batch* b, int enabled) {
@@ -430,6 +529,9 @@ int ne_batch_set_graph(ne_batch* b, int enabled) {
 int ne_batch_autotune(ne_batch* b, ne_batch_tuning* out, void* stream) {
     if (!b) return NE_ERR_NULL;
     hipStream_t s = static_cast<hipStream_t>(stream);
+    int r = ensure_uploaded(b, s);
+    if (r) return r;
+    if (b->members.empty()) return NE_ERR_STATE;
     return b->precision == NE_FP64 ? autotune<double>(b, s, out) : autotune<float>(b, s, out);
 }
 
@@ -441,7 +543,7 @@ int ne_batch_get_tuning(const ne_batch* b, ne_batch_tuning* out) {
 
 int ne_batch_set_tuning(ne_batch* b, const ne_batch_tuning* t) {
     if (!b || !t) return NE_ERR_NULL;
-    int r = check_tuning(b, *t);
+    int r = check_tuning(*t);
     if (r) return r;
     b->tuning = *t;
     drop_graph(b);
@@ -451,7 +553,12 @@ int ne_batch_set_tuning(ne_batch* b, const ne_batch_tuning* t) {
 int ne_batch_info(const ne_batch* b, int* members, int64_t* cells, int* stage_items, int* surface_items) {
     if (!b) return NE_ERR_NULL;
     if (members) *members = (int)b->members.size();
-    if (cells) *cells = b->cells;
+    if (cells) {
+        int64_t n = 0;
+        for (const ne_model* m : b->members) n += (int64_t)m->d.grid.Nx * m->d.grid.Ny * m->d.grid.Nz;
+        *cells = n;
+    }
+    // counts of the current upload (the next step rebuilds the lists if membership changed)
     int ti = tr_index(b->tuning.tile_rows), si = sb_index(b->tuning.surface_block);
     if (stage_items) *stage_items = ti >= 0 ? b->stage_nwork[ti] : 0;
     if (surface_items) *surface_items = si >= 0 ? b->surface_nwork[si] : 0;

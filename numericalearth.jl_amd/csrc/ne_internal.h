@@ -30,6 +30,7 @@ struct ne_model {
     int max_steps = 0;
     int recorded = 0;
     std::vector<hipEvent_t> events;
+    bool owns_buffers = true;           // stage buffers allocated by the model (no workspace given)
     struct ne_batch* batch = nullptr;   // the batch this model belongs to, if any
 };
 
@@ -107,11 +108,31 @@ hipError_t launch_surface_finalize(const GridDev& g, const ne_params& p, const A
                                    double dt_frazil, hipStream_t s);
 
 // Tracer launchers (ne_tracer.hip).
+// stochastic: round the new state stochastically (FP32 storage only; the stage that writes
+// the model state), seeded by the model time at the start of the step.
 template <typename Real>
 hipError_t launch_rk_stage(const GridDev& g, View3<const Real> u, View3<const Real> v, View3<const Real> w,
                            View3<const Real> T0, View3<const Real> S0, View3<const Real> Tin,
                            View3<const Real> Sin, View3<Real> Tout, View3<Real> Sout, View3<const double> JT,
-                           View3<const double> JS, View3<const double> Jw, double beta_dt, hipStream_t s);
+                           View3<const double> JS, View3<const double> Jw, double beta_dt, bool stochastic,
+                           double time, hipStream_t s);
+
+// Seed of a step's stochastic rounding: a hash of the bits of the model time.
+__host__ __device__ inline uint32_t step_seed(double time) {
+    union {
+        double d;
+        uint64_t u;
+    } b;
+    b.d = time;
+    uint32_t lo = (uint32_t)b.u, hi = (uint32_t)(b.u >> 32);
+    uint32_t x = lo * 0x9e3779b1u ^ (hi + 0x7f4a7c15u);
+    x ^= x >> 16;
+    x *= 0x85ebca6bu;
+    x ^= x >> 13;
+    x *= 0xc2b2ae35u;
+    x ^= x >> 16;
+    return x;
+}
 hipError_t launch_tendency(const GridDev& g, View3<const double> u, View3<const double> v, View3<const double> w,
                            View3<const double> T, View3<const double> S, View3<const double> JT,
                            View3<const double> JS, View3<const double> Jw, View3<double> GT, View3<double> GS,
@@ -124,6 +145,13 @@ hipError_t launch_budget(ne_ctx* ctx, const GridDev& g, View3<const Real> c, dou
 // Batched ensemble: one device record per member model, read by persistent kernels
 // that walk a work list of (member, tile) or (member, first column) items.
 // ---------------------------------------------------------------------------
+
+// Byte strides of the views a tracer stage reads and writes. The six tracer views (T⁰, S⁰,
+// c_in, c_out of T and S) share one layout, as do the three 2-D surface flux views; every
+// view spans less than 4 GiB, so a byte offset fits 32 bits (checked on the host).
+struct TracerLayout {
+    uint32_t c_sy, c_sz, u_sy, u_sz, v_sy, v_sz, w_sy, w_sz, j_sy;
+};
 
 template <typename Real>
 struct BatchRec {
@@ -144,6 +172,21 @@ struct BatchRec {
     int fast;               // default Edson constants: the shared-ψ flux path
     double time;           // device clock of the member (advanced by the clock kernel)
 };
+
+template <typename Real>
+__host__ __device__ inline TracerLayout tracer_layout(const BatchRec<Real>& r) {
+    TracerLayout L;
+    L.c_sy = (uint32_t)(r.T.sy * (int64_t)sizeof(Real));
+    L.c_sz = (uint32_t)(r.T.sz * (int64_t)sizeof(Real));
+    L.u_sy = (uint32_t)(r.u.sy * (int64_t)sizeof(Real));
+    L.u_sz = (uint32_t)(r.u.sz * (int64_t)sizeof(Real));
+    L.v_sy = (uint32_t)(r.v.sy * (int64_t)sizeof(Real));
+    L.v_sz = (uint32_t)(r.v.sz * (int64_t)sizeof(Real));
+    L.w_sy = (uint32_t)(r.w.sy * (int64_t)sizeof(Real));
+    L.w_sz = (uint32_t)(r.w.sz * (int64_t)sizeof(Real));
+    L.j_sy = (uint32_t)(r.n.J_T.sy * (int64_t)sizeof(double));
+    return L;
+}
 
 struct WorkItem {
     int member;

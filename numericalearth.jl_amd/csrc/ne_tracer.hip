@@ -43,29 +43,87 @@ inline int grid_blocks(int Nx, int Ny) {
     return ((n + NXCD - 1) / NXCD) * NXCD;
 }
 
-// Activity bits of six consecutive cells whose kmin is at km[0], km[stride], …
-__device__ __forceinline__ unsigned act6(const int* km, int stride, int k) {
-    unsigned a = 0;
-#pragma unroll
-    for (int n = 0; n < 6; n++) a |= (k >= km[n * stride] ? 1u : 0u) << n;
-    return a;
+// Face order at level k from the three thresholds of its stencil: the face is
+// reconstructed with order 5 (3) if k ≥ K3 (K2), upwind if k ≥ K1, and carries no flux
+// otherwise (K1 ≤ K2 ≤ K3 are the largest kmin over the 2, 4 and 6 cells around it).
+__device__ __forceinline__ int order_at(int k, int K1, int K2, int K3) {
+    return (k >= K1 ? 1 : 0) + (k >= K2 ? 1 : 0) + (k >= K3 ? 1 : 0);
+}
+
+__device__ __forceinline__ void thresholds(const int* km, int stride, int& K1, int& K2, int& K3) {
+    K1 = max(km[2 * stride], km[3 * stride]);
+    K2 = max(K1, max(km[stride], km[4 * stride]));
+    K3 = max(K2, max(km[0], km[5 * stride]));
+}
+
+// Integer hash (Wellons' "lowbias32" finaliser) for the stochastic rounding below.
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// The float next to r towards +∞ (up) or −∞, for finite r.
+__device__ __forceinline__ float next_float(float r, bool up) {
+    if (r == 0.0f) return __uint_as_float(up ? 1u : 0x80000001u);
+    uint32_t b = __float_as_uint(r);
+    const bool positive = (b >> 31) == 0;
+    return __uint_as_float(positive == up ? b + 1u : b - 1u);
+}
+
+// Rounding of the new FP64 state value x to the storage type. FP32 storage of the model
+// state (the mixed form) rounds stochastically: to one of the two neighbouring floats with
+// probabilities that make the expected stored value x, drawn from a hash of the cell, the
+// tracer and the step (`seed`), so results are reproducible and identical on every path
+// that takes the same step. Round-to-nearest would drop the same fraction of an ulp every
+// step wherever the increment is steady, a bias that grows linearly with the step count.
+template <typename R>
+__device__ __forceinline__ R round_state(double x, uint32_t key, uint32_t seed, bool stochastic) {
+    R r = (R)x;
+    if (sizeof(R) == 4 && stochastic) {
+        const double d = x - (double)r;           // exact: |d| ≤ ½ ulp of r
+        if (d != 0.0) {
+            const float other = next_float((float)r, d > 0.0);
+            const double p = fabs(d) / fabs((double)other - (double)r);
+            const double u = (double)(hash32(hash32(key) ^ seed) >> 8) * (1.0 / 16777216.0);
+            if (u < p) r = (R)other;
+        }
+    }
+    return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T ldb(const T* base, uint32_t off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+template <typename T>
+__device__ __forceinline__ void stb(T* base, uint32_t off, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
 }
 
 // One tile (columns i0 … i0+62, rows j0 … j0+TR-1) marched over its active levels.
 // MODE 0: c_out = c0 + beta_dt·G(c_in) on active cells. MODE 1: write G (0 on inactive
-// cells). Every work-item of the block calls this with the same tile.
+// cells). Every work-item of the block calls this with the same tile. Every address is a
+// 32-bit byte offset from a uniform base (the host checks that each view spans < 4 GiB),
+// computed once per tile; a level adds its uniform k-stride.
 template <typename Real, int MODE, int TR>
-__device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDev& g, const View3<const Real>& u,
-                                          const View3<const Real>& v, const View3<const Real>& w,
-                                          const View3<const Real>& T0, const View3<const Real>& S0,
-                                          const View3<const Real>& Tin, const View3<const Real>& Sin,
-                                          const View3<Real>& Tout, const View3<Real>& Sout,
-                                          const View3<const double>& JT, const View3<const double>& JS,
-                                          const View3<const double>& Jw, double beta_dt, int i0, int j0) {
+__device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDev& g, const TracerLayout& L,
+                                          const Real* __restrict__ u, const Real* __restrict__ v,
+                                          const Real* __restrict__ w, const Real* T0, const Real* S0,
+                                          const Real* __restrict__ Tin, const Real* __restrict__ Sin, Real* Tout,
+                                          Real* Sout, const double* __restrict__ JT, const double* __restrict__ JS,
+                                          const double* __restrict__ Jw, double beta_dt, int i0, int j0,
+                                          bool stochastic = false, uint32_t seed = 0u) {
     typedef Real R;
     constexpr int PXW = TileShared<R, TR>::PXW;
     constexpr int PN = TileShared<R, TR>::PN;
     constexpr int NTHR = TW * TR;
+    constexpr int NPL = (PN + NTHR - 1) / NTHR;   // plane cells staged per work-item
+    constexpr uint32_t ES = sizeof(R);
+    constexpr int NEVER = 1 << 30;
     const int lane = threadIdx.x;
     const int row = threadIdx.y;
     const int tid = lane + row * TW;
@@ -84,24 +142,55 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
     const bool cell = lane < TC && ci < g.Nx && cj < g.Ny;
     const int mykmin = cell ? sh.km[(row + 3) * PXW + lane + 3] : Nz;
     if (cell) atomicMin(&sh.kstart, mykmin);
-    __syncthreads();
-    const int kstart = sh.kstart;
-    if (MODE == 1 && cell)
-        for (int k = 0; k < mykmin && k < Nz; k++) { Tout(ci, cj, k) = (R)0; Sout(ci, cj, k) = (R)0; }
-    if (kstart >= Nz) return;  // uniform: the whole tile is land
 
-    // Per-row and per-column geometry
+    // Per-tile addressing and face thresholds (independent of the level)
+    int pkm[NPL];
+    uint32_t poff[NPL];
+#pragma unroll
+    for (int q = 0; q < NPL; q++) {
+        const int n = tid + q * NTHR;
+        pkm[q] = NEVER;
+        poff[q] = 0;
+        if (n < PN) {
+            pkm[q] = sh.km[n];
+            if (pkm[q] < KMIN_OUTSIDE) {
+                const int gi = g.wx(i0 - 3 + n % PXW), gj = g.wy(j0 - 3 + n / PXW);
+                poff[q] = (uint32_t)gi * ES + (uint32_t)gj * L.c_sy;
+            }
+        }
+    }
     const bool xrow = cj < g.Ny;                 // this wavefront's row is inside the domain
-    const int jw = xrow ? cj : 0;
-    const double Axf = g.dy_xface[jw];
-    const int fiw = g.wx(i0 + lane);             // x face owned by this lane (west face of column i0+lane)
     const bool ycol = lane < TC && ci < g.Nx;
+    int X1 = NEVER, X2 = NEVER, X3 = NEVER;      // x face owned by this lane (west face of column i0+lane)
+    if (xrow && adv) thresholds(&sh.km[(row + 3) * PXW + lane], 1, X1, X2, X3);
+    int S1 = NEVER, S2 = NEVER, S3 = NEVER;      // south y face of row j0+row
+    int N1 = NEVER, N2 = NEVER, N3 = NEVER;      // north y face of the tile (last wavefront)
+    if (ycol && adv) {
+        thresholds(&sh.km[row * PXW + lane + 3], PXW, S1, S2, S3);
+        if (row == TR - 1) thresholds(&sh.km[(row + 1) * PXW + lane + 3], PXW, N1, N2, N3);
+    }
+    const int fiw = g.wx(i0 + lane);
+    const int fjs = g.wy(j0 + row);
+    const int fjn = g.wy(j0 + TR);
     const int cy = ycol ? ci : 0;
-    const int fjs = g.wy(j0 + row);              // y face owned by this lane (south face of row j0+row)
-    const int fjn = g.wy(j0 + TR);               // north face of the last row (wavefront TR-1)
+    const double Axf = g.dy_xface[xrow ? cj : 0];
     const double Ays = (j0 + row <= g.Ny) ? g.dx_yface[fjs < 0 ? 0 : (fjs > g.Ny ? g.Ny : fjs)] : 0.0;
     const double Ayn = (j0 + TR <= g.Ny) ? g.dx_yface[fjn < 0 ? 0 : (fjn > g.Ny ? g.Ny : fjn)] : 0.0;
     const double Azc = cell ? g.Az[cj] : 0.0;
+    const uint32_t uoff = xrow ? (uint32_t)fiw * ES + (uint32_t)cj * L.u_sy : 0u;
+    const uint32_t voff_s = ycol ? (uint32_t)cy * ES + (uint32_t)fjs * L.v_sy : 0u;
+    const uint32_t voff_n = ycol ? (uint32_t)cy * ES + (uint32_t)fjn * L.v_sy : 0u;
+    const uint32_t coff = cell ? (uint32_t)ci * ES + (uint32_t)cj * L.c_sy : 0u;
+    const uint32_t woff = cell ? (uint32_t)ci * ES + (uint32_t)cj * L.w_sy : 0u;
+    const uint32_t joff = cell ? (uint32_t)ci * 8u + (uint32_t)cj * L.j_sy : 0u;
+    __syncthreads();
+    const int kstart = sh.kstart;
+    if (MODE == 1 && cell)
+        for (int k = 0; k < mykmin && k < Nz; k++) {
+            stb(Tout, coff + (uint32_t)k * L.c_sz, (R)0);
+            stb(Sout, coff + (uint32_t)k * L.c_sz, (R)0);
+        }
+    if (kstart >= Nz) return;  // uniform: the whole tile is land
 
     // Vertical register window of the owned column: levels k-2 … k+3
     R zT[6], zS[6];
@@ -109,23 +198,22 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
     for (int n = 0; n < 6; n++) {
         int kk = kstart - 2 + n;
         bool a = cell && kk >= mykmin && kk < Nz;
-        zT[n] = a ? Tin(ci, cj, kk) : (R)0;
-        zS[n] = a ? Sin(ci, cj, kk) : (R)0;
+        zT[n] = a ? ldb(Tin, coff + (uint32_t)kk * L.c_sz) : (R)0;
+        zS[n] = a ? ldb(Sin, coff + (uint32_t)kk * L.c_sz) : (R)0;
     }
     R FzT = (R)0, FzS = (R)0;  // flux through the bottom face of the current level
 
     for (int k = kstart; k < Nz; k++) {
+        const uint32_t kc = (uint32_t)k * L.c_sz;
         // 1. stage the plane of level k
-        for (int n = tid; n < PN; n += NTHR) {
-            bool a = k >= sh.km[n];
-            R t = (R)0, s = (R)0;
-            if (a) {
-                int gi = g.wx(i0 - 3 + n % PXW), gj = g.wy(j0 - 3 + n / PXW);
-                t = Tin(gi, gj, k);
-                s = Sin(gi, gj, k);
+#pragma unroll
+        for (int q = 0; q < NPL; q++) {
+            const int n = tid + q * NTHR;
+            if (n < PN) {
+                const bool a = k >= pkm[q];
+                sh.cT[n] = a ? ldb(Tin, poff[q] + kc) : (R)0;
+                sh.cS[n] = a ? ldb(Sin, poff[q] + kc) : (R)0;
             }
-            sh.cT[n] = t;
-            sh.cS[n] = s;
         }
         __syncthreads();
         const R dzk = (R)g.dz[k];
@@ -133,17 +221,15 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
         // 2a. x face owned by this lane: cells i0+lane-3 … i0+lane+2 of row j0+row
         {
             R fT = (R)0, fS = (R)0;
-            if (xrow) {
+            const int order = order_at(k, X1, X2, X3);
+            if (order > 0) {
                 const int base = (row + 3) * PXW + lane;
-                int order = adv ? face_order(act6(&sh.km[base], 1, k)) : 0;
-                if (order > 0) {
-                    R vel = u(fiw, cj, k);
-                    R A = (R)Axf * dzk;
-                    const R* t = &sh.cT[base];
-                    const R* s = &sh.cS[base];
-                    fT = A * vel * face_value(t[0], t[1], t[2], t[3], t[4], t[5], order, vel);
-                    fS = A * vel * face_value(s[0], s[1], s[2], s[3], s[4], s[5], order, vel);
-                }
+                R vel = ldb(u, uoff + (uint32_t)k * L.u_sz);
+                R A = (R)Axf * dzk;
+                const R* t = &sh.cT[base];
+                const R* s = &sh.cS[base];
+                fT = A * vel * face_value(t[0], t[1], t[2], t[3], t[4], t[5], order, vel);
+                fS = A * vel * face_value(s[0], s[1], s[2], s[3], s[4], s[5], order, vel);
             }
             sh.fxT[row * TW + lane] = fT;
             sh.fxS[row * TW + lane] = fS;
@@ -153,13 +239,12 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
 #pragma unroll 1
             for (int pass = 0; pass < (row == TR - 1 ? 2 : 1); pass++) {
                 const int r0 = row + pass;                // plane row of the first stencil cell
-                const int base = r0 * PXW + lane + 3;
-                const int fj = pass ? fjn : fjs;
                 const double Ay = pass ? Ayn : Ays;
                 R fT = (R)0, fS = (R)0;
-                int order = adv ? face_order(act6(&sh.km[base], PXW, k)) : 0;
+                const int order = pass ? order_at(k, N1, N2, N3) : order_at(k, S1, S2, S3);
                 if (order > 0) {
-                    R vel = v(cy, fj, k);
+                    const int base = r0 * PXW + lane + 3;
+                    R vel = ldb(v, (pass ? voff_n : voff_s) + (uint32_t)k * L.v_sz);
                     R A = (R)Ay * dzk;
                     const R* t = &sh.cT[base];
                     const R* s = &sh.cS[base];
@@ -173,15 +258,10 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
         // 2c. z face at the top of level k (cells k-2 … k+3 of the owned column)
         R FztT = (R)0, FztS = (R)0;
         if (cell && adv) {
-            unsigned az = 0;
-#pragma unroll
-            for (int n = 0; n < 6; n++) {
-                int kk = k - 2 + n;
-                az |= ((kk >= mykmin && kk < Nz) ? 1u : 0u) << n;
-            }
-            int order = face_order(az);
+            const int order = (k >= mykmin && k <= Nz - 2 ? 1 : 0) + (k >= mykmin + 1 && k <= Nz - 3 ? 1 : 0) +
+                              (k >= mykmin + 2 && k <= Nz - 4 ? 1 : 0);
             if (order > 0) {
-                R vel = w(ci, cj, k + 1);
+                R vel = ldb(w, woff + (uint32_t)(k + 1) * L.w_sz);
                 R A = (R)Azc;
                 FztT = A * vel * face_value(zT[0], zT[1], zT[2], zT[3], zT[4], zT[5], order, vel);
                 FztS = A * vel * face_value(zS[0], zS[1], zS[2], zS[3], zS[4], zS[5], order, vel);
@@ -203,19 +283,22 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                 // MultipleFluxes(Jᵀ, FreshwaterExchange{:T}) = Jᵀ + 0 on a static grid;
                 // MultipleFluxes(Jˢ, FreshwaterExchange{:S}) = Jˢ + (Sᴺ Jʷ − 0)
                 // (multiple_surface_fluxes.jl:27-31; ocean_simulation.jl:80-91)
-                double FT = JT.at2(ci, cj) + (0.0 + 0.0);
-                double FS = JS.at2(ci, cj) + (((double)zS[2] * Jw.at2(ci, cj) - 0.0) + 0.0);
+                double FT = ldb(JT, joff) + (0.0 + 0.0);
+                double FS = ldb(JS, joff) + (((double)zS[2] * ldb(Jw, joff) - 0.0) + 0.0);
                 GT = GT - (R)(FT * Azc) / V;
                 GS = GS - (R)(FS * Azc) / V;
             }
             if (MODE == 0) {
                 // The update is formed in FP64 and rounded once to the storage type (for FP32
                 // storage this keeps the per-stage rounding of the state at half an ulp).
-                Tout(ci, cj, k) = (R)((double)T0(ci, cj, k) + beta_dt * (double)GT);
-                Sout(ci, cj, k) = (R)((double)S0(ci, cj, k) + beta_dt * (double)GS);
+                const uint32_t key = 2u * ((uint32_t)ci + (uint32_t)g.Nx * ((uint32_t)cj + (uint32_t)g.Ny * k));
+                stb(Tout, coff + kc, round_state<R>((double)ldb(T0, coff + kc) + beta_dt * (double)GT, key, seed,
+                                                    stochastic));
+                stb(Sout, coff + kc, round_state<R>((double)ldb(S0, coff + kc) + beta_dt * (double)GS, key + 1u,
+                                                    seed, stochastic));
             } else {
-                Tout(ci, cj, k) = GT;
-                Sout(ci, cj, k) = GS;
+                stb(Tout, coff + kc, GT);
+                stb(Sout, coff + kc, GS);
             }
         }
         FzT = FztT;
@@ -226,8 +309,8 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
             for (int n = 0; n < 5; n++) { zT[n] = zT[n + 1]; zS[n] = zS[n + 1]; }
             int kk = k + 4;
             bool a = kk >= mykmin && kk < Nz;
-            zT[5] = a ? Tin(ci, cj, kk) : (R)0;
-            zS[5] = a ? Sin(ci, cj, kk) : (R)0;
+            zT[5] = a ? ldb(Tin, coff + (uint32_t)kk * L.c_sz) : (R)0;
+            zS[5] = a ? ldb(Sin, coff + (uint32_t)kk * L.c_sz) : (R)0;
         }
     }
 }
@@ -236,18 +319,17 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
 // round-robin to the 8 XCDs) receives a contiguous band of tiles.
 template <typename Real, int MODE>
 __global__ void __launch_bounds__(TW * PLAIN_TR)
-tracer_tile_kernel(GridDev g, View3<const Real> u, View3<const Real> v, View3<const Real> w, View3<const Real> T0,
-                   View3<const Real> S0, View3<const Real> Tin, View3<const Real> Sin, View3<Real> Tout,
-                   View3<Real> Sout, View3<const double> JT, View3<const double> JS, View3<const double> Jw,
-                   double beta_dt) {
+tracer_tile_kernel(GridDev g, TracerLayout L, const Real* u, const Real* v, const Real* w, const Real* T0,
+                   const Real* S0, const Real* Tin, const Real* Sin, Real* Tout, Real* Sout, const double* JT,
+                   const double* JS, const double* Jw, double beta_dt, int stochastic, uint32_t seed) {
     __shared__ TileShared<Real, PLAIN_TR> sh;
     const int ntx = tiles_x(g.Nx), nty = tiles_y(g.Ny, PLAIN_TR);
     const int ntiles = ntx * nty;
     const int per = (ntiles + NXCD - 1) / NXCD;
     const int tile = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
     if (tile >= ntiles) return;
-    tile_body<Real, MODE, PLAIN_TR>(sh, g, u, v, w, T0, S0, Tin, Sin, Tout, Sout, JT, JS, Jw, beta_dt,
-                                    (tile % ntx) * TC, (tile / ntx) * PLAIN_TR);
+    tile_body<Real, MODE, PLAIN_TR>(sh, g, L, u, v, w, T0, S0, Tin, Sin, Tout, Sout, JT, JS, Jw, beta_dt,
+                                    (tile % ntx) * TC, (tile / ntx) * PLAIN_TR, stochastic != 0, seed);
 }
 
 // Ensemble: a persistent grid walks the (member, tile) work list. The list is cut into 8
@@ -267,15 +349,16 @@ batch_stage_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __re
         const WorkItem it = work[wi];
         const BatchRec<Real>& rc = recs[it.member];
         const GridDev g = rc.g;
+        const TracerLayout L = tracer_layout(rc);
         const int ntx = tiles_x(g.Nx);
-        const View3<const Real> Tin = cview(stage == 1 ? rc.T : stage == 2 ? rc.T1 : rc.T2);
-        const View3<const Real> Sin = cview(stage == 1 ? rc.S : stage == 2 ? rc.S1 : rc.S2);
-        const View3<Real> Tout = stage == 1 ? rc.T1 : stage == 2 ? rc.T2 : rc.T;
-        const View3<Real> Sout = stage == 1 ? rc.S1 : stage == 2 ? rc.S2 : rc.S;
+        const Real* Tin = stage == 1 ? rc.T.p : stage == 2 ? rc.T1.p : rc.T2.p;
+        const Real* Sin = stage == 1 ? rc.S.p : stage == 2 ? rc.S1.p : rc.S2.p;
+        Real* Tout = stage == 1 ? rc.T1.p : stage == 2 ? rc.T2.p : rc.T.p;
+        Real* Sout = stage == 1 ? rc.S1.p : stage == 2 ? rc.S2.p : rc.S.p;
         __syncthreads();   // the previous item's reads of the LDS are complete
-        tile_body<Real, 0, TR>(sh, g, rc.u, rc.v, rc.w, cview(rc.T), cview(rc.S), Tin, Sin, Tout, Sout,
-                               cview(rc.n.J_T), cview(rc.n.J_S), cview(rc.n.J_w), beta_dt,
-                               (it.index % ntx) * TC, (it.index / ntx) * TR);
+        tile_body<Real, 0, TR>(sh, g, L, rc.u.p, rc.v.p, rc.w.p, rc.T.p, rc.S.p, Tin, Sin, Tout, Sout, rc.n.J_T.p,
+                               rc.n.J_S.p, rc.n.J_w.p, beta_dt, (it.index % ntx) * TC, (it.index / ntx) * TR,
+                               stage == 3, step_seed(rc.time));
     }
 }
 
@@ -366,31 +449,50 @@ __global__ void budget_final_kernel(const double* partial, int64_t n, double* ou
 }  // namespace
 
 template <typename Real>
+TracerLayout host_layout(const View3<const Real>& c, const View3<const Real>& u, const View3<const Real>& v,
+                         const View3<const Real>& w, const View3<const double>& J) {
+    TracerLayout L;
+    L.c_sy = (uint32_t)(c.sy * (int64_t)sizeof(Real));
+    L.c_sz = (uint32_t)(c.sz * (int64_t)sizeof(Real));
+    L.u_sy = (uint32_t)(u.sy * (int64_t)sizeof(Real));
+    L.u_sz = (uint32_t)(u.sz * (int64_t)sizeof(Real));
+    L.v_sy = (uint32_t)(v.sy * (int64_t)sizeof(Real));
+    L.v_sz = (uint32_t)(v.sz * (int64_t)sizeof(Real));
+    L.w_sy = (uint32_t)(w.sy * (int64_t)sizeof(Real));
+    L.w_sz = (uint32_t)(w.sz * (int64_t)sizeof(Real));
+    L.j_sy = (uint32_t)(J.sy * (int64_t)sizeof(double));
+    return L;
+}
+
+template <typename Real>
 hipError_t launch_rk_stage(const GridDev& g, View3<const Real> u, View3<const Real> v, View3<const Real> w,
                            View3<const Real> T0, View3<const Real> S0, View3<const Real> Tin, View3<const Real> Sin,
                            View3<Real> Tout, View3<Real> Sout, View3<const double> JT, View3<const double> JS,
-                           View3<const double> Jw, double beta_dt, hipStream_t s) {
-    hipLaunchKernelGGL((tracer_tile_kernel<Real, 0>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, PLAIN_TR), 0, s, g, u, v, w,
-                       T0, S0, Tin, Sin, Tout, Sout, JT, JS, Jw, beta_dt);
+                           View3<const double> Jw, double beta_dt, bool stochastic, double time, hipStream_t s) {
+    TracerLayout L = host_layout<Real>(Tin, u, v, w, JT);
+    hipLaunchKernelGGL((tracer_tile_kernel<Real, 0>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, PLAIN_TR), 0, s, g, L,
+                       u.p, v.p, w.p, T0.p, S0.p, Tin.p, Sin.p, Tout.p, Sout.p, JT.p, JS.p, Jw.p, beta_dt,
+                       stochastic ? 1 : 0, step_seed(time));
     return hipGetLastError();
 }
 template hipError_t launch_rk_stage<double>(const GridDev&, View3<const double>, View3<const double>,
                                             View3<const double>, View3<const double>, View3<const double>,
                                             View3<const double>, View3<const double>, View3<double>, View3<double>,
                                             View3<const double>, View3<const double>, View3<const double>, double,
-                                            hipStream_t);
+                                            bool, double, hipStream_t);
 template hipError_t launch_rk_stage<float>(const GridDev&, View3<const float>, View3<const float>,
                                            View3<const float>, View3<const float>, View3<const float>,
                                            View3<const float>, View3<const float>, View3<float>, View3<float>,
                                            View3<const double>, View3<const double>, View3<const double>, double,
-                                           hipStream_t);
+                                           bool, double, hipStream_t);
 
 hipError_t launch_tendency(const GridDev& g, View3<const double> u, View3<const double> v, View3<const double> w,
                            View3<const double> T, View3<const double> S, View3<const double> JT,
                            View3<const double> JS, View3<const double> Jw, View3<double> GT, View3<double> GS,
                            hipStream_t s) {
-    hipLaunchKernelGGL((tracer_tile_kernel<double, 1>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, PLAIN_TR), 0, s, g, u, v,
-                       w, T, S, T, S, GT, GS, JT, JS, Jw, 0.0);
+    TracerLayout L = host_layout<double>(T, u, v, w, JT);
+    hipLaunchKernelGGL((tracer_tile_kernel<double, 1>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, PLAIN_TR), 0, s, g,
+                       L, u.p, v.p, w.p, T.p, S.p, T.p, S.p, GT.p, GS.p, JT.p, JS.p, Jw.p, 0.0, 0, 0u);
     return hipGetLastError();
 }
 

@@ -199,6 +199,8 @@ class ModelDesc(C.Structure):
         ("n_atmosphere_times", _i32),
         ("n_radiation_times", _i32),
         ("radiation_times", _pd),
+        ("workspace", _p),
+        ("workspace_bytes", C.c_int64),
     ]
 
 

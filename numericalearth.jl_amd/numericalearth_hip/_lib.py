@@ -58,8 +58,11 @@ SIGNATURES = {
     "ne_model_clock": [vp, P(C.c_double), P(C.c_int64)],
     "ne_model_timing_begin": [vp, C.c_int],
     "ne_model_timing_end": [vp, P(C.c_double), P(C.c_int)],
-    "ne_batch_create": [vp, C.c_int, P(vp), P(vp), vp],
+    "ne_model_workspace_bytes": [P(abi.ModelDesc), P(C.c_int64)],
+    "ne_batch_create": [vp, C.c_int32, C.c_int, P(vp), P(vp), vp],
     "ne_batch_destroy": [vp],
+    "ne_batch_add": [vp, vp],
+    "ne_batch_remove": [vp, vp],
     "ne_batch_time_step": [vp, C.c_double, vp],
     "ne_batch_set_graph": [vp, C.c_int],
     "ne_batch_autotune": [vp, P(abi.BatchTuning), vp],

@@ -88,20 +88,33 @@ def build_members(specs: list[MemberSpec], precision: str = "fp64", device: int 
 class Ensemble:
     """A batch of models of one precision (``ne_batch``); members stay usable on their own after close()."""
 
-    def __init__(self, models: list[OceanOnlyModel], stream=None):
-        if not models:
-            raise ValueError("an ensemble needs at least one model")
-        if len({m.precision for m in models}) != 1:
+    def __init__(self, models: list[OceanOnlyModel], stream=None, *, precision: str | None = None,
+                 device: int = 0):
+        from .model import Context
+        precisions = {m.precision for m in models}
+        if len(precisions) > 1:
             raise ValueError("ensemble members must share one precision")
+        self.precision = precision or (models[0].precision if models else "fp64")
+        if precisions and precisions != {self.precision}:
+            raise ValueError("ensemble members must have the ensemble's precision")
         self.models = list(models)
-        self.ctx = models[0].ctx
+        self.device = torch.device("cuda", models[0].device.index if models else device)
+        self.ctx = models[0].ctx if models else Context.get(self.device.index)
         self.lib = self.ctx.lib
-        self.precision = models[0].precision
-        arr = (C.c_void_p * len(models))(*[m.handle.value for m in models])
+        code = abi.NE_FP64 if self.precision == "fp64" else abi.NE_MIXED_FP32
+        arr = (C.c_void_p * max(1, len(models)))(*[m.handle.value for m in models])
         h = C.c_void_p()
-        check(self.lib.ne_batch_create(self.ctx.handle, len(models), arr, C.byref(h), stream_handle(stream)),
+        check(self.lib.ne_batch_create(self.ctx.handle, code, len(models), arr, C.byref(h), stream_handle(stream)),
               "ne_batch_create")
         self.handle = h
+
+    def add(self, model: OceanOnlyModel) -> None:
+        check(self.lib.ne_batch_add(self.handle, model.handle), "ne_batch_add")
+        self.models.append(model)
+
+    def remove(self, model: OceanOnlyModel) -> None:
+        check(self.lib.ne_batch_remove(self.handle, model.handle), "ne_batch_remove")
+        self.models.remove(model)
 
     def time_step(self, dt: float, stream=None) -> None:
         check(self.lib.ne_batch_time_step(self.handle, float(dt), stream_handle(stream)), "ne_batch_time_step")
@@ -122,7 +135,7 @@ class Ensemble:
     def signature(self) -> str:
         """Key of the autotuning cache: device name, precision and every member's shape."""
         shapes = [m.case.shape for m in self.models]
-        name = torch.cuda.get_device_name(self.models[0].device)
+        name = torch.cuda.get_device_name(self.device)
         return hashlib.sha1(json.dumps([name, self.precision, shapes]).encode()).hexdigest()
 
     def autotune(self, cache: str | Path | None = None, stream=None) -> dict:
@@ -155,7 +168,6 @@ class Ensemble:
 
     def close(self) -> None:
         if getattr(self, "handle", None):
-            torch.cuda.synchronize(self.models[0].device)
             check(self.lib.ne_batch_destroy(self.handle), "ne_batch_destroy")
             self.handle = None
 

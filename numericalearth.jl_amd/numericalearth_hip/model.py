@@ -177,6 +177,12 @@ class OceanOnlyModel:
         d.n_atmosphere_times = len(self._atimes)
         d.radiation_times = self._rtimes.ctypes.data_as(C.POINTER(C.c_double))
         d.n_radiation_times = len(self._rtimes)
+        # the RK stage buffers come from torch's allocator, so creating and destroying a model
+        # makes no device allocation of its own
+        nbytes = C.c_int64()
+        check(self.lib.ne_model_workspace_bytes(C.byref(d), C.byref(nbytes)), "ne_model_workspace_bytes")
+        self._workspace = torch.empty(nbytes.value, dtype=torch.uint8, device=dev)
+        d.workspace, d.workspace_bytes = self._workspace.data_ptr(), nbytes.value
         self.desc = d
         h = C.c_void_p()
         check(self.lib.ne_model_create(self.ctx.handle, C.byref(d), C.byref(h), s), "ne_model_create")

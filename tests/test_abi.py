@@ -56,8 +56,11 @@ def test_null_arguments_are_rejected_without_a_device():
     assert L.ne_model_destroy(None) == abi.NE_ERR_NULL
     assert L.ne_model_time_step(None, 1.0, None) == abi.NE_ERR_NULL
     h = C.c_void_p()
-    assert L.ne_batch_create(None, 1, None, C.byref(h), None) == abi.NE_ERR_NULL
+    assert L.ne_batch_create(None, abi.NE_FP64, 1, None, C.byref(h), None) == abi.NE_ERR_NULL
     assert L.ne_batch_destroy(None) == abi.NE_ERR_NULL
+    assert L.ne_batch_add(None, None) == abi.NE_ERR_NULL
+    assert L.ne_batch_remove(None, None) == abi.NE_ERR_NULL
+    assert L.ne_model_workspace_bytes(None, None) == abi.NE_ERR_NULL
     assert L.ne_batch_time_step(None, 1.0, None) == abi.NE_ERR_NULL
     assert L.ne_batch_set_graph(None, 1) == abi.NE_ERR_NULL
     assert L.ne_batch_autotune(None, None, None) == abi.NE_ERR_NULL

@@ -173,4 +173,9 @@ def test_mixed_precisions_are_refused(require_gpu):
     import ctypes as C
     arr = (C.c_void_p * 2)(a.handle.value, b.handle.value)
     h = C.c_void_p()
-    assert a.lib.ne_batch_create(a.ctx.handle, 2, arr, C.byref(h), None) == abi.NE_ERR_UNSUPPORTED
+    assert a.lib.ne_batch_create(a.ctx.handle, abi.NE_FP64, 2, arr, C.byref(h), None) == abi.NE_ERR_UNSUPPORTED
+    ens = Ensemble([], precision="fp64")
+    with pytest.raises(NEError) as e:
+        ens.add(b)
+    assert e.value.status == abi.NE_ERR_UNSUPPORTED
+    ens.close()

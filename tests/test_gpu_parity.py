@@ -259,14 +259,16 @@ def test_budget_closure_on_device(require_gpu):
 
 
 def test_mixed_form_1000_steps_vs_fp64_oracle(require_gpu):
-    """Mixed form (FP32 WENO5 + interpolation + storage, FP32-then-FP64 flux iteration, FP64
-    budgets) against the FP64 oracle after 1000 steps of 20 min on the 90×40×10 workload:
-    SST within 1e-5 relative, surface heat fluxes within 1e-3 of their scale, heat content
-    within 1e-9."""
+    """Mixed form (FP32 WENO5, interpolation inputs and state with stochastically rounded
+    updates; FP32-then-FP64 flux iteration; FP64 fluxes and budgets) against the FP64 oracle
+    after 1000 steps of 20 min on the 90×40×10 workload: SST within 1e-5 relative (largest
+    |ΔSST| over the largest |SST|, and in the L2 norm), surface heat fluxes within 1e-3 of
+    their scale, and heat content within 1e-3 of the heat gained over the run."""
     case = make_case(*SMALL, seed=21)
     om = O.OracleModel(case).reconcile()
     g32 = OceanOnlyModel(case, precision="mixed")
     rho_c = g32.params.ocean_reference_density * g32.params.ocean_heat_capacity
+    H0 = sum(om.budget(om.T, rho_c))
     for n in range(1000):
         om.time_step(1200.0)
         g32.time_step(1200.0)
@@ -276,24 +278,53 @@ def test_mixed_form_1000_steps_vs_fp64_oracle(require_gpu):
     torch.cuda.synchronize()
     wet = case.grid.kmin < case.grid.spec.Nz
     sst = _np(g32.T)[-1].astype(np.float64)
-    diff = np.abs(sst - om.T[-1])[wet]
-    sst_err = float(diff.max() / np.max(np.abs(om.T[-1][wet])))          # relative to the SST scale (°C)
-    sst_pointwise = float(np.max(diff / np.abs(om.T[-1][wet])))           # pointwise, °C
-    sst_kelvin = float(np.max(diff / (om.T[-1][wet] + 273.15)))           # pointwise, K
-    print(f"[mixed] SST abs {diff.max():.2e} K, rms {np.sqrt(np.mean(diff ** 2)):.2e} K, pointwise °C "
-          f"{sst_pointwise:.2e}, pointwise K {sst_kelvin:.2e}", flush=True)
+    ref = om.T[-1][wet]
+    diff = np.abs(sst[wet] - ref)
+    sst_err = float(diff.max() / np.max(np.abs(ref)))
+    sst_l2 = float(np.sqrt(np.sum(diff ** 2) / np.sum(ref ** 2)))
+    print(f"[mixed] SST max |Δ| {diff.max():.2e} K, rms {np.sqrt(np.mean(diff ** 2)):.2e} K", flush=True)
     qt = _np(g32.fluxes["sensible_heat"])
     qv = _np(g32.fluxes["latent_heat"])
     qt_err = float(np.max(np.abs(qt - om.flux_arrays["sensible_heat"])) / np.max(np.abs(om.flux_arrays["sensible_heat"])))
     qv_err = float(np.max(np.abs(qv - om.flux_arrays["latent_heat"])) / np.max(np.abs(om.flux_arrays["latent_heat"])))
     H32 = g32.budget(g32.T, rho_c)
     Ho = sum(om.budget(om.T, rho_c))
-    h_err = abs(H32 - Ho) / abs(Ho)
-    print(f"[mixed] 1000 steps: SST {sst_err:.2e}, sensible {qt_err:.2e}, latent {qv_err:.2e}, heat {h_err:.2e}",
-          flush=True)
-    assert sst_err <= 1e-5
+    h_err = abs(H32 - Ho) / abs(Ho - H0)
+    print(f"[mixed] 1000 steps: SST {sst_err:.2e} (L2 {sst_l2:.2e}), sensible {qt_err:.2e}, latent {qv_err:.2e}, "
+          f"heat {h_err:.2e} of the change", flush=True)
+    assert sst_err <= 1e-5 and sst_l2 <= 1e-5
     assert qt_err <= 1e-3 and qv_err <= 1e-3
-    assert h_err <= 1e-9
+    assert h_err <= 1e-3
+
+
+def test_mixed_form_budget_closure(require_gpu):
+    """The mixed form conserves heat: over 5 steps the heat content changes by the surface heat
+    flux alone, up to the rounding of the FP32 state (under one ulp per cell and step with
+    stochastic rounding), and its budget sum equals the oracle's FP64 compensated sum of the
+    same FP32 field."""
+    case = make_case(*SMALL, seed=22)
+    g32 = OceanOnlyModel(case, precision="mixed")
+    om = O.OracleModel(case)
+    rho_c = g32.params.ocean_reference_density * g32.params.ocean_heat_capacity
+    torch.cuda.synchronize()
+    H0 = g32.heat_content()
+    Ho = om.budget(_np(g32.T).astype(np.float64), rho_c)
+    assert abs(H0 - (Ho[0] + Ho[1])) <= 1e-15 * abs(H0)
+    Az = case.grid.Az
+    active = np.arange(case.grid.spec.Nz)[:, None, None] >= case.grid.kmin[None]
+    V = case.grid.Az[None, :, None] * case.grid.dz[:, None, None]
+    expected, bound = 0.0, 0.0
+    for _ in range(5):
+        JT = _np(g32.net["J_T"])
+        expected += -1200.0 * rho_c * float(np.sum(JT * Az[:, None]))
+        g32.time_step(1200.0)
+        torch.cuda.synchronize()
+        T1 = _np(g32.T)
+        bound += rho_c * float(np.sum(np.where(active, V * np.spacing(np.abs(T1)).astype(np.float64), 0.0)))
+    H1 = g32.heat_content()
+    err = abs((H1 - H0) - expected)
+    print(f"[mixed budget] ΔH={H1 - H0:.6e} expected={expected:.6e} err={err:.3e} bound={bound:.3e}", flush=True)
+    assert err <= bound
 
 
 def test_mixed_precision_model_tracks_fp64(require_gpu):

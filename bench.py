@@ -54,6 +54,17 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic(workload: str, precision: str):
+    """HBM bytes per launch of the RK stage kernel from the committed FETCH_SIZE/WRITE_SIZE
+    summary (calibrated against a device copy of known size), or None."""
+    import re
+    f = REPO / "profiles" / "tracer_rk_stage_traffic_1440x560x10.md"
+    if workload != "quarter_degree_1440x560x10" or not f.exists():
+        return None
+    m = re.search(rf"traffic_bytes_per_launch\[{precision}\] = ([0-9.eE+-]+)", f.read_text())
+    return float(m.group(1)) if m else None
+
+
 def copy_bandwidth(gib: float, reps: int) -> dict:
     """Device-to-device copy (torch's copy kernel) of `gib` GiB: read + write bytes per second."""
     import torch
@@ -323,7 +334,8 @@ def main():
                    "bytes_per_cell_update": B, "step_hbm_fraction": step_gbs / HBM_PEAK_GBS},
         "roofline": {"kernel": "tracer_rk_stage (WENO5 tendency + top flux BC + RK stage, mean of 3 stages)",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, args.precision),
+                     "algorithmic_bytes_per_launch": stage_bytes},
         "kernels_ms_per_step": {k: v / nrec for k, v in kernel_ms.items()},
         "flux_solve": {"mean_iterations": counters["total_iterations"] / max(1, counters["solved_columns"]),
                        "max_iterations": counters["max_iterations"],

@@ -80,6 +80,8 @@ typedef struct ne_grid {
     const double* phi_c;         /* [Ny]   cell-centre latitude (deg)                */
     const int32_t* kmin;         /* [Nx*Ny] i-fastest: first active level (0-based)
                                     of each column; Nz marks a land column          */
+    const double* z_face;        /* [Nz+1] cell-face height (m); face k is the bottom of
+                                    cell k, face Nz the surface                      */
 } ne_grid;
 
 /* Regular longitude-latitude source grid of a prescribed atmosphere or radiation
@@ -143,6 +145,9 @@ typedef struct ne_interface_fluxes {
  * τʸ (Center,Face), Jᵀ, Jˢ, Jʷ (= net_fluxes.η), Jᴴ (freshwater heat content). */
 typedef struct ne_net_ocean_fluxes {
     ne_view tau_x, tau_y, J_T, J_S, J_w, J_H;
+    ne_view shortwave_surface_flux;  /* J₀ = −ℐₜˢʷ/(ρₒcₒ) (K m/s) of TwoColorRadiation
+                                        (radiative_forcing.jl:86-91); read and written only when
+                                        params.radiative_forcing is two-color, else may be NULL */
 } ne_net_ocean_fluxes;
 
 /* InterfaceRadiationFlux diagnostics (src/Radiations/apply_air_sea_radiative_fluxes.jl:107-109). */
@@ -223,14 +228,23 @@ typedef struct ne_params {
     double  freshwater_density;           /* 1000 */
     /* radiation (src/Radiations/Radiations.jl:13-16, prescribed_radiation.jl:68) */
     int32_t radiation;                    /* 1: PrescribedRadiation present */
-    int32_t reserved1;
+    int32_t radiative_forcing;            /* NE_RADIATIVE_FORCING_NONE: transmitted shortwave goes
+                                             into Jᵀ; NE_RADIATIVE_FORCING_TWO_COLOR: it penetrates
+                                             (TwoColorRadiation, Oceans/radiative_forcing.jl:5-106) */
     double  stefan_boltzmann_constant;    /* 5.670374419e-8 */
     double  ocean_albedo;                 /* 0.05 */
     double  ocean_emissivity;             /* 0.97 */
     /* FreezingLimitedOceanTemperature LinearLiquidus (freezing_limited_ocean_temperature.jl:25) */
     double  freshwater_melting_temperature; /* 0 °C */
     double  liquidus_slope;               /* 0.054 °C/psu */
+    /* TwoColorRadiation (radiative_forcing.jl:36-46; defaults of default_radiative_forcing,
+     * ocean_simulation.jl:158-167: Paulson & Simpson Type I) */
+    double  first_color_fraction;         /* 0.58 */
+    double  first_absorption_coefficient; /* 1/0.35 m⁻¹ */
+    double  second_absorption_coefficient;/* 1/23 m⁻¹ */
 } ne_params;
+
+enum { NE_RADIATIVE_FORCING_NONE = 0, NE_RADIATIVE_FORCING_TWO_COLOR = 1 };
 
 /* Counters written by the flux-solve kernels (reference: non-convergence is silent
  * and NaNs propagate to the NaNChecker, earth_system_model.jl:483-495). */
@@ -328,14 +342,17 @@ int ne_apply_air_sea_radiation_f64(ne_ctx* ctx, const ne_grid* grid, const ne_pa
 
 /* Tracer tendency of T and S: −∇·(U c) with WENO(order=5) on the immersed grid plus the
  * top flux boundary condition (Gᵀ[Nz] −= Jᵀ Az/V, Gˢ[Nz] −= (Jˢ + S Jʷ) Az/V;
- * ocean_simulation.jl:65-94 + Oceananigans apply_z_bcs!). Inactive cells get G = 0. */
-int ne_tracer_tendency_weno5_f64(ne_ctx* ctx, const ne_grid* grid, const ne_ocean_state* ocean,
-                                 const ne_net_ocean_fluxes* net, ne_view G_T, ne_view G_S,
-                                 void* stream);
+ * ocean_simulation.jl:65-94 + Oceananigans apply_z_bcs!). Inactive cells get G = 0.
+ * params may be NULL; with params->radiative_forcing two-colour, Gᵀ also receives the
+ * TwoColorRadiation forcing of net->shortwave_surface_flux (radiative_forcing.jl:52-71). */
+int ne_tracer_tendency_weno5_f64(ne_ctx* ctx, const ne_grid* grid, const ne_params* params,
+                                 const ne_ocean_state* ocean, const ne_net_ocean_fluxes* net,
+                                 ne_view G_T, ne_view G_S, void* stream);
 
 /* One split Runge-Kutta stage, tendency fused: c_out = c0 + beta_dt·G(c_in) for T and S
  * on active cells (inactive cells are left untouched). c_out may alias c0 but not c_in. */
-int ne_tracer_rk_stage_f64(ne_ctx* ctx, const ne_grid* grid, ne_view u, ne_view v, ne_view w,
+int ne_tracer_rk_stage_f64(ne_ctx* ctx, const ne_grid* grid, const ne_params* params,
+                           ne_view u, ne_view v, ne_view w,
                            ne_view T0, ne_view S0, ne_view T_in, ne_view S_in,
                            ne_view T_out, ne_view S_out,
                            const ne_net_ocean_fluxes* net, double beta_dt, void* stream);

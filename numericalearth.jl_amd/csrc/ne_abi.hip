@@ -84,6 +84,14 @@ int check_net(const ne_net_ocean_fluxes* n) {
     return NE_OK;
 }
 
+// A two-colour radiative forcing needs its surface-flux view.
+int check_forcing(const ne_grid* g, const ne_params* p, const ne_net_ocean_fluxes* net) {
+    if (!p || p->radiative_forcing == NE_RADIATIVE_FORCING_NONE) return NE_OK;
+    if (p->radiative_forcing != NE_RADIATIVE_FORCING_TWO_COLOR) return NE_ERR_UNSUPPORTED;
+    if (!g->z_face || !net->shortwave_surface_flux.data) return NE_ERR_NULL;
+    return NE_OK;
+}
+
 int check_params(const ne_params* p) {
     if (!p) return NE_ERR_NULL;
     if (p->stop_criteria != NE_STOP_CONVERGENCE && p->stop_criteria != NE_STOP_FIXED_ITERATIONS)
@@ -115,9 +123,9 @@ int check_tracer_views(const ne_grid* g, const ne_view* const* tracers, int ntr,
     const ne_view& c = *tracers[0];
     for (int n = 0; n < ntr; n++)
         if (tracers[n]->sx != 1 || tracers[n]->sy != c.sy || tracers[n]->sz != c.sz) return NE_ERR_SHAPE;
-    const ne_view* J[3] = {&net->J_T, &net->J_S, &net->J_w};
+    const ne_view* J[4] = {&net->J_T, &net->J_S, &net->J_w, &net->shortwave_surface_flux};
     for (const ne_view* j : J)
-        if (j->sx != 1 || j->sy != J[0]->sy) return NE_ERR_SHAPE;
+        if (j->data && (j->sx != 1 || j->sy != J[0]->sy)) return NE_ERR_SHAPE;
     int64_t ext[5] = {extent_bytes(c, Nx, Ny, Nz, esize), extent_bytes(u, Nx + 1, Ny, Nz, esize),
                       extent_bytes(v, Nx, Ny + 1, Nz, esize), extent_bytes(w, Nx, Ny, Nz + 1, esize),
                       extent_bytes(*J[0], Nx, Ny, 1, 8)};
@@ -199,6 +207,10 @@ int ne_params_default(ne_params* p) {
     p->ocean_emissivity = 0.97;
     p->freshwater_melting_temperature = 0.0;
     p->liquidus_slope = 0.054;
+    p->radiative_forcing = NE_RADIATIVE_FORCING_NONE;
+    p->first_color_fraction = 0.58;
+    p->first_absorption_coefficient = 1.0 / 0.35;
+    p->second_absorption_coefficient = 1.0 / 23.0;
     return NE_OK;
 }
 
@@ -322,34 +334,34 @@ int ne_apply_air_sea_radiation_f64(ne_ctx* c, const ne_grid* g, const ne_params*
     if (!c || !out) return NE_ERR_NULL;
     int r;
     if ((r = check_grid(g)) || (r = check_params(p)) || (r = check_net(net)) || (r = check_view2(sw)) ||
-        (r = check_view2(lw)) || (r = check_view2(Ts)))
+        (r = check_view2(lw)) || (r = check_view2(Ts)) || (r = check_forcing(g, p, net)))
         return r;
     return dev_err(launch_radiation(grid_dev(*g), *p, view<const double>(sw), view<const double>(lw),
                                     view<const double>(Ts), net_out(*net), rad_out(*out), as_stream(stream)));
 }
 
-int ne_tracer_tendency_weno5_f64(ne_ctx* c, const ne_grid* g, const ne_ocean_state* o,
+int ne_tracer_tendency_weno5_f64(ne_ctx* c, const ne_grid* g, const ne_params* p, const ne_ocean_state* o,
                                  const ne_net_ocean_fluxes* net, ne_view GT, ne_view GS, void* stream) {
     if (!c) return NE_ERR_NULL;
     int r;
     if ((r = check_grid(g)) || (r = check_ocean(o)) || (r = check_net(net)) || (r = check_view3(GT)) ||
-        (r = check_view3(GS)))
+        (r = check_view3(GS)) || (r = check_forcing(g, p, net)))
         return r;
     const ne_view* tr[] = {&o->T, &o->S, &GT, &GS};
     if ((r = check_tracer_views(g, tr, 4, o->u, o->v, o->w, net, sizeof(double)))) return r;
     return dev_err(launch_tendency(grid_dev(*g), view<const double>(o->u), view<const double>(o->v),
                                    view<const double>(o->w), view<const double>(o->T), view<const double>(o->S),
                                    view<const double>(net->J_T), view<const double>(net->J_S),
-                                   view<const double>(net->J_w), view<double>(GT), view<double>(GS),
-                                   as_stream(stream)));
+                                   view<const double>(net->J_w), tracer_forcing(p, *net), view<double>(GT),
+                                   view<double>(GS), as_stream(stream)));
 }
 
-int ne_tracer_rk_stage_f64(ne_ctx* c, const ne_grid* g, ne_view u, ne_view v, ne_view w, ne_view T0, ne_view S0,
-                           ne_view Tin, ne_view Sin, ne_view Tout, ne_view Sout, const ne_net_ocean_fluxes* net,
-                           double beta_dt, void* stream) {
+int ne_tracer_rk_stage_f64(ne_ctx* c, const ne_grid* g, const ne_params* p, ne_view u, ne_view v, ne_view w,
+                           ne_view T0, ne_view S0, ne_view Tin, ne_view Sin, ne_view Tout, ne_view Sout,
+                           const ne_net_ocean_fluxes* net, double beta_dt, void* stream) {
     if (!c) return NE_ERR_NULL;
     int r;
-    if ((r = check_grid(g)) || (r = check_net(net))) return r;
+    if ((r = check_grid(g)) || (r = check_net(net)) || (r = check_forcing(g, p, net))) return r;
     const ne_view* vs[] = {&u, &v, &w, &T0, &S0, &Tin, &Sin, &Tout, &Sout};
     for (const ne_view* x : vs)
         if ((r = check_view3(*x))) return r;
@@ -360,8 +372,8 @@ int ne_tracer_rk_stage_f64(ne_ctx* c, const ne_grid* g, ne_view u, ne_view v, ne
                                            view<const double>(w), view<const double>(T0), view<const double>(S0),
                                            view<const double>(Tin), view<const double>(Sin), view<double>(Tout),
                                            view<double>(Sout), view<const double>(net->J_T),
-                                           view<const double>(net->J_S), view<const double>(net->J_w), beta_dt,
-                                           false, 0.0, as_stream(stream)));
+                                           view<const double>(net->J_S), view<const double>(net->J_w),
+                                           tracer_forcing(p, *net), beta_dt, false, 0.0, as_stream(stream)));
 }
 
 int ne_budget_f64(ne_ctx* c, const ne_grid* g, ne_view cv, double weight, double* out, void* stream) {
@@ -435,16 +447,18 @@ static int model_ocean_step(ne_model* m, double dt, hipStream_t s, hipEvent_t* e
     hipError_t e;
     if (ev && hipEventRecord(ev[0], s) != hipSuccess) return NE_ERR_DEVICE;
     const bool sr = sizeof(Real) == 4;   // FP32 state: stochastic rounding of the stage that writes it
-    e = launch_rk_stage<Real>(g, u, v, w, T0, S0, T0, S0, view<Real>(m->vT1), view<Real>(m->vS1), JT, JS, Jw,
+    const TracerForcing F = tracer_forcing(&d.params, d.net);
+    e = launch_rk_stage<Real>(g, u, v, w, T0, S0, T0, S0, view<Real>(m->vT1), view<Real>(m->vS1), JT, JS, Jw, F,
                               beta[0] * dt, false, m->time, s);
     if (e != hipSuccess) return NE_ERR_DEVICE;
     if (ev && hipEventRecord(ev[1], s) != hipSuccess) return NE_ERR_DEVICE;
     e = launch_rk_stage<Real>(g, u, v, w, T0, S0, view<const Real>(m->vT1), view<const Real>(m->vS1),
-                              view<Real>(m->vT2), view<Real>(m->vS2), JT, JS, Jw, beta[1] * dt, false, m->time, s);
+                              view<Real>(m->vT2), view<Real>(m->vS2), JT, JS, Jw, F, beta[1] * dt, false, m->time,
+                              s);
     if (e != hipSuccess) return NE_ERR_DEVICE;
     if (ev && hipEventRecord(ev[2], s) != hipSuccess) return NE_ERR_DEVICE;
     e = launch_rk_stage<Real>(g, u, v, w, T0, S0, view<const Real>(m->vT2), view<const Real>(m->vS2),
-                              view<Real>(d.ocean.T), view<Real>(d.ocean.S), JT, JS, Jw, beta[2] * dt, sr, m->time,
+                              view<Real>(d.ocean.T), view<Real>(d.ocean.S), JT, JS, Jw, F, beta[2] * dt, sr, m->time,
                               s);
     if (e != hipSuccess) return NE_ERR_DEVICE;
     if (ev && hipEventRecord(ev[3], s) != hipSuccess) return NE_ERR_DEVICE;
@@ -458,7 +472,8 @@ int ne_model_create(ne_ctx* c, const ne_model_desc* desc, ne_model** out, void* 
     *out = nullptr;
     int r;
     if ((r = check_grid(&desc->grid)) || (r = check_params(&desc->params)) || (r = check_atmos(&desc->atmosphere)) ||
-        (r = check_ocean(&desc->ocean)) || (r = check_fluxes(&desc->fluxes)) || (r = check_net(&desc->net)))
+        (r = check_ocean(&desc->ocean)) || (r = check_fluxes(&desc->fluxes)) || (r = check_net(&desc->net)) ||
+        (r = check_forcing(&desc->grid, &desc->params, &desc->net)))
         return r;
     if (desc->params.radiation && (r = check_rad(&desc->radiation))) return r;
     if (desc->precision != NE_FP64 && desc->precision != NE_MIXED_FP32) return NE_ERR_UNSUPPORTED;

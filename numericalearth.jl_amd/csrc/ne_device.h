@@ -83,6 +83,7 @@ struct GridDev {
     const double* __restrict__ lambda_c;
     const double* __restrict__ phi_c;
     const int32_t* __restrict__ kmin;
+    const double* __restrict__ zf;
 
     // Column index after periodic wrapping; -1 outside a bounded direction.
     __device__ __forceinline__ int64_t column(int i, int j) const {
@@ -108,7 +109,7 @@ __host__ inline GridDev grid_dev(const ne_grid& g) {
     d.tx = g.topology_x; d.ty = g.topology_y;
     d.advection = g.advection;
     d.dz = g.dz; d.zc = g.z_center; d.dy_xface = g.dy_xface; d.dx_yface = g.dx_yface;
-    d.Az = g.Az; d.lambda_c = g.lambda_c; d.phi_c = g.phi_c; d.kmin = g.kmin;
+    d.Az = g.Az; d.lambda_c = g.lambda_c; d.phi_c = g.phi_c; d.kmin = g.kmin; d.zf = g.z_face;
     return d;
 }
 

@@ -58,7 +58,27 @@ struct FluxOut {
 
 struct NetOut {
     View3<double> tau_x, tau_y, J_T, J_S, J_w, J_H;
+    View3<double> J_sw;     // TwoColorRadiation surface flux J₀ (written when two-colour)
 };
+
+// Tracer forcing of T: TwoColorRadiation of the surface flux J0 (radiative_forcing.jl:52-71).
+struct TracerForcing {
+    int two_color;
+    double first_fraction, kappa1, kappa2;
+    const double* J0;
+};
+
+inline TracerForcing tracer_forcing(const ne_params* p, const ne_net_ocean_fluxes& n) {
+    TracerForcing f = {0, 0.0, 0.0, 0.0, nullptr};
+    if (p && p->radiative_forcing == NE_RADIATIVE_FORCING_TWO_COLOR) {
+        f.two_color = 1;
+        f.first_fraction = p->first_color_fraction;
+        f.kappa1 = p->first_absorption_coefficient;
+        f.kappa2 = p->second_absorption_coefficient;
+        f.J0 = static_cast<const double*>(n.shortwave_surface_flux.data);
+    }
+    return f;
+}
 
 struct RadOut {
     View3<double> up_lw, down_lw, down_sw;
@@ -114,8 +134,8 @@ template <typename Real>
 hipError_t launch_rk_stage(const GridDev& g, View3<const Real> u, View3<const Real> v, View3<const Real> w,
                            View3<const Real> T0, View3<const Real> S0, View3<const Real> Tin,
                            View3<const Real> Sin, View3<Real> Tout, View3<Real> Sout, View3<const double> JT,
-                           View3<const double> JS, View3<const double> Jw, double beta_dt, bool stochastic,
-                           double time, hipStream_t s);
+                           View3<const double> JS, View3<const double> Jw, TracerForcing forcing, double beta_dt,
+                           bool stochastic, double time, hipStream_t s);
 
 // Seed of a step's stochastic rounding: a hash of the bits of the model time.
 __host__ __device__ inline uint32_t step_seed(double time) {
@@ -135,8 +155,8 @@ __host__ __device__ inline uint32_t step_seed(double time) {
 }
 hipError_t launch_tendency(const GridDev& g, View3<const double> u, View3<const double> v, View3<const double> w,
                            View3<const double> T, View3<const double> S, View3<const double> JT,
-                           View3<const double> JS, View3<const double> Jw, View3<double> GT, View3<double> GS,
-                           hipStream_t s);
+                           View3<const double> JS, View3<const double> Jw, TracerForcing forcing, View3<double> GT,
+                           View3<double> GS, hipStream_t s);
 template <typename Real>
 hipError_t launch_budget(ne_ctx* ctx, const GridDev& g, View3<const Real> c, double weight, double* out,
                          hipStream_t s);

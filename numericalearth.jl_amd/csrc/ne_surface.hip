@@ -302,8 +302,17 @@ __device__ __forceinline__ double radiation_column(const ne_params& p, double Ts
     Ila = Ila * (1.0 - aice);
     Itsw = Itsw * (1.0 - aice);
     double Iup_ocean = Iup * (1.0 - aice);
-    double SQ = Iup_ocean + Ila + Itsw;
+    // shortwave_radiative_forcing (radiative_forcing.jl:84-91): with TwoColorRadiation the
+    // transmitted shortwave becomes its surface flux J₀ and leaves Jᵀ
+    double Qss = p.radiative_forcing == NE_RADIATIVE_FORCING_TWO_COLOR ? 0.0 : Itsw;
+    double SQ = Iup_ocean + Ila + Qss;
     return SQ * rinv * cinv;
+}
+
+__device__ __forceinline__ void store_penetrating_shortwave(const ne_params& p, const NetOut& n, int i, int j,
+                                                            double Itsw) {
+    if (p.radiative_forcing == NE_RADIATIVE_FORCING_TWO_COLOR)
+        n.J_sw.at2(i, j) = -Itsw / (p.ocean_reference_density * p.ocean_heat_capacity);
 }
 
 // Momentum fluxes of the west / south neighbours (zero state outside the domain).
@@ -344,6 +353,7 @@ __global__ void radiation_kernel(GridDev g, ne_params p, View3<const double> sw,
     double JTrad = radiation_column(p, Ts.at2(i, j), sw.at2(i, j), lw.at2(i, j), Iup, Ila, Itsw);
     bool inactive = !g.active(i, j, g.Nz - 1);
     n.J_T.at2(i, j) = n.J_T.at2(i, j) + (inactive ? 0.0 : JTrad);
+    store_penetrating_shortwave(p, n, i, j, Itsw);
     if (r.up_lw.p) r.up_lw.at2(i, j) = Iup;
     if (r.down_lw.p) r.down_lw.at2(i, j) = -Ila;
     if (r.down_sw.p) r.down_sw.at2(i, j) = -Itsw;
@@ -380,6 +390,7 @@ __device__ __forceinline__ void surface_finalize_column(const GridDev& g, const 
         double Iup, Ila, Itsw;
         double JTrad = radiation_column(p, f.Ts.at2(i, j), SW, LW, Iup, Ila, Itsw);
         JT = JT + (inactive ? 0.0 : JTrad);
+        store_penetrating_shortwave(p, n, i, j, Itsw);
         if (ro.up_lw.p) ro.up_lw.at2(i, j) = Iup;
         if (ro.down_lw.p) ro.down_lw.at2(i, j) = -Ila;
         if (ro.down_sw.p) ro.down_sw.at2(i, j) = -Itsw;
@@ -468,6 +479,7 @@ NetOut net_out(const ne_net_ocean_fluxes& n) {
     NetOut o;
     o.tau_x = view<double>(n.tau_x); o.tau_y = view<double>(n.tau_y); o.J_T = view<double>(n.J_T);
     o.J_S = view<double>(n.J_S); o.J_w = view<double>(n.J_w); o.J_H = view<double>(n.J_H);
+    o.J_sw = view<double>(n.shortwave_surface_flux);
     return o;
 }
 

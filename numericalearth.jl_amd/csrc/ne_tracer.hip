@@ -115,8 +115,8 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                                           const Real* __restrict__ w, const Real* T0, const Real* S0,
                                           const Real* __restrict__ Tin, const Real* __restrict__ Sin, Real* Tout,
                                           Real* Sout, const double* __restrict__ JT, const double* __restrict__ JS,
-                                          const double* __restrict__ Jw, double beta_dt, int i0, int j0,
-                                          bool stochastic = false, uint32_t seed = 0u) {
+                                          const double* __restrict__ Jw, const TracerForcing& F, double beta_dt,
+                                          int i0, int j0, bool stochastic = false, uint32_t seed = 0u) {
     typedef Real R;
     constexpr int PXW = TileShared<R, TR>::PXW;
     constexpr int PN = TileShared<R, TR>::PN;
@@ -203,6 +203,14 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
     }
     R FzT = (R)0, FzS = (R)0;  // flux through the bottom face of the current level
 
+    // TwoColorRadiation: J₀ of the column and the Beer's-law factors exp(κ (z − η)) of the
+    // bottom face of the current level (uniform over the tile)
+    const bool tc = F.two_color != 0;
+    const double J0 = (tc && cell) ? ldb(F.J0, joff) : 0.0;
+    const double eta = tc ? g.zf[Nz] : 0.0;
+    double E1b = tc ? exp(F.kappa1 * (g.zf[kstart] - eta)) : 0.0;
+    double E2b = tc ? exp(F.kappa2 * (g.zf[kstart] - eta)) : 0.0;
+
     for (int k = kstart; k < Nz; k++) {
         const uint32_t kc = (uint32_t)k * L.c_sz;
         // 1. stage the plane of level k
@@ -279,6 +287,17 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
             const R V = (R)(Azc * g.dz[k]);
             R GT = -((R)1 / V * div_T);
             R GS = -((R)1 / V * div_S);
+            if (tc) {
+                // ε₁ ∂z J₁ + (1 − ε₁) ∂z J₂; the face below the lowest active cell carries no flux
+                const double E1t = exp(F.kappa1 * (g.zf[k + 1] - eta)), E2t = exp(F.kappa2 * (g.zf[k + 1] - eta));
+                const bool below = k - 1 >= mykmin;
+                const double a1 = J0 * E1t, b1 = below ? J0 * E1b : 0.0;
+                const double a2 = J0 * E2t, b2 = below ? J0 * E2b : 0.0;
+                const double dz = g.dz[k];
+                const double dJ1dz = (a1 - b1) / dz;
+                const double dJ2dz = (a2 - b2) / dz;
+                GT = GT + (R)(F.first_fraction * dJ1dz + (1.0 - F.first_fraction) * dJ2dz);
+            }
             if (k == Nz - 1) {
                 // MultipleFluxes(Jᵀ, FreshwaterExchange{:T}) = Jᵀ + 0 on a static grid;
                 // MultipleFluxes(Jˢ, FreshwaterExchange{:S}) = Jˢ + (Sᴺ Jʷ − 0)
@@ -303,6 +322,10 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
         }
         FzT = FztT;
         FzS = FztS;
+        if (tc) {
+            E1b = exp(F.kappa1 * (g.zf[k + 1] - eta));
+            E2b = exp(F.kappa2 * (g.zf[k + 1] - eta));
+        }
         // 4. slide the vertical window up one level
         if (cell) {
 #pragma unroll
@@ -321,14 +344,15 @@ template <typename Real, int MODE>
 __global__ void __launch_bounds__(TW * PLAIN_TR)
 tracer_tile_kernel(GridDev g, TracerLayout L, const Real* u, const Real* v, const Real* w, const Real* T0,
                    const Real* S0, const Real* Tin, const Real* Sin, Real* Tout, Real* Sout, const double* JT,
-                   const double* JS, const double* Jw, double beta_dt, int stochastic, uint32_t seed) {
+                   const double* JS, const double* Jw, TracerForcing F, double beta_dt, int stochastic,
+                   uint32_t seed) {
     __shared__ TileShared<Real, PLAIN_TR> sh;
     const int ntx = tiles_x(g.Nx), nty = tiles_y(g.Ny, PLAIN_TR);
     const int ntiles = ntx * nty;
     const int per = (ntiles + NXCD - 1) / NXCD;
     const int tile = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
     if (tile >= ntiles) return;
-    tile_body<Real, MODE, PLAIN_TR>(sh, g, L, u, v, w, T0, S0, Tin, Sin, Tout, Sout, JT, JS, Jw, beta_dt,
+    tile_body<Real, MODE, PLAIN_TR>(sh, g, L, u, v, w, T0, S0, Tin, Sin, Tout, Sout, JT, JS, Jw, F, beta_dt,
                                     (tile % ntx) * TC, (tile / ntx) * PLAIN_TR, stochastic != 0, seed);
 }
 
@@ -355,9 +379,17 @@ batch_stage_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __re
         const Real* Sin = stage == 1 ? rc.S.p : stage == 2 ? rc.S1.p : rc.S2.p;
         Real* Tout = stage == 1 ? rc.T1.p : stage == 2 ? rc.T2.p : rc.T.p;
         Real* Sout = stage == 1 ? rc.S1.p : stage == 2 ? rc.S2.p : rc.S.p;
+        TracerForcing F = {0, 0.0, 0.0, 0.0, nullptr};
+        if (rc.p.radiative_forcing == NE_RADIATIVE_FORCING_TWO_COLOR) {
+            F.two_color = 1;
+            F.first_fraction = rc.p.first_color_fraction;
+            F.kappa1 = rc.p.first_absorption_coefficient;
+            F.kappa2 = rc.p.second_absorption_coefficient;
+            F.J0 = rc.n.J_sw.p;
+        }
         __syncthreads();   // the previous item's reads of the LDS are complete
         tile_body<Real, 0, TR>(sh, g, L, rc.u.p, rc.v.p, rc.w.p, rc.T.p, rc.S.p, Tin, Sin, Tout, Sout, rc.n.J_T.p,
-                               rc.n.J_S.p, rc.n.J_w.p, beta_dt, (it.index % ntx) * TC, (it.index / ntx) * TR,
+                               rc.n.J_S.p, rc.n.J_w.p, F, beta_dt, (it.index % ntx) * TC, (it.index / ntx) * TR,
                                stage == 3, step_seed(rc.time));
     }
 }
@@ -468,31 +500,32 @@ template <typename Real>
 hipError_t launch_rk_stage(const GridDev& g, View3<const Real> u, View3<const Real> v, View3<const Real> w,
                            View3<const Real> T0, View3<const Real> S0, View3<const Real> Tin, View3<const Real> Sin,
                            View3<Real> Tout, View3<Real> Sout, View3<const double> JT, View3<const double> JS,
-                           View3<const double> Jw, double beta_dt, bool stochastic, double time, hipStream_t s) {
+                           View3<const double> Jw, TracerForcing forcing, double beta_dt, bool stochastic,
+                           double time, hipStream_t s) {
     TracerLayout L = host_layout<Real>(Tin, u, v, w, JT);
     hipLaunchKernelGGL((tracer_tile_kernel<Real, 0>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, PLAIN_TR), 0, s, g, L,
-                       u.p, v.p, w.p, T0.p, S0.p, Tin.p, Sin.p, Tout.p, Sout.p, JT.p, JS.p, Jw.p, beta_dt,
+                       u.p, v.p, w.p, T0.p, S0.p, Tin.p, Sin.p, Tout.p, Sout.p, JT.p, JS.p, Jw.p, forcing, beta_dt,
                        stochastic ? 1 : 0, step_seed(time));
     return hipGetLastError();
 }
 template hipError_t launch_rk_stage<double>(const GridDev&, View3<const double>, View3<const double>,
                                             View3<const double>, View3<const double>, View3<const double>,
                                             View3<const double>, View3<const double>, View3<double>, View3<double>,
-                                            View3<const double>, View3<const double>, View3<const double>, double,
-                                            bool, double, hipStream_t);
+                                            View3<const double>, View3<const double>, View3<const double>,
+                                            TracerForcing, double, bool, double, hipStream_t);
 template hipError_t launch_rk_stage<float>(const GridDev&, View3<const float>, View3<const float>,
                                            View3<const float>, View3<const float>, View3<const float>,
                                            View3<const float>, View3<const float>, View3<float>, View3<float>,
-                                           View3<const double>, View3<const double>, View3<const double>, double,
-                                           bool, double, hipStream_t);
+                                           View3<const double>, View3<const double>, View3<const double>,
+                                           TracerForcing, double, bool, double, hipStream_t);
 
 hipError_t launch_tendency(const GridDev& g, View3<const double> u, View3<const double> v, View3<const double> w,
                            View3<const double> T, View3<const double> S, View3<const double> JT,
-                           View3<const double> JS, View3<const double> Jw, View3<double> GT, View3<double> GS,
-                           hipStream_t s) {
+                           View3<const double> JS, View3<const double> Jw, TracerForcing forcing, View3<double> GT,
+                           View3<double> GS, hipStream_t s) {
     TracerLayout L = host_layout<double>(T, u, v, w, JT);
     hipLaunchKernelGGL((tracer_tile_kernel<double, 1>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, PLAIN_TR), 0, s, g,
-                       L, u.p, v.p, w.p, T.p, S.p, T.p, S.p, GT.p, GS.p, JT.p, JS.p, Jw.p, 0.0, 0, 0u);
+                       L, u.p, v.p, w.p, T.p, S.p, T.p, S.p, GT.p, GS.p, JT.p, JS.p, Jw.p, forcing, 0.0, 0, 0u);
     return hipGetLastError();
 }
 

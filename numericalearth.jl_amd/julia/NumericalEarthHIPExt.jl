@@ -52,6 +52,7 @@ struct NEGrid
     topology_x :: Int32; topology_y :: Int32; advection :: Int32
     dz :: Ptr{Float64}; z_center :: Ptr{Float64}; dy_xface :: Ptr{Float64}; dx_yface :: Ptr{Float64}
     Az :: Ptr{Float64}; lambda_c :: Ptr{Float64}; phi_c :: Ptr{Float64}; kmin :: Ptr{Int32}
+    z_face :: Ptr{Float64}
 end
 
 struct NEExchangeState; u::NEView; v::NEView; T::NEView; p::NEView; q::NEView; rain::NEView; snow::NEView; end
@@ -61,13 +62,14 @@ struct NEInterfaceFluxes
     friction_velocity::NEView; temperature_scale::NEView; water_vapor_scale::NEView
     interface_temperature::NEView; iterations::NEView
 end
-struct NENetOceanFluxes; tau_x::NEView; tau_y::NEView; J_T::NEView; J_S::NEView; J_w::NEView; J_H::NEView; end
+struct NENetOceanFluxes; tau_x::NEView; tau_y::NEView; J_T::NEView; J_S::NEView; J_w::NEView; J_H::NEView
+    shortwave_surface_flux::NEView; end   # J₀ of TwoColorRadiation (surface_flux field), or a NULL view
 struct NERadiativeFluxes; upwelling_longwave::NEView; downwelling_longwave::NEView; downwelling_shortwave::NEView; end
 
-# ne_params is a 568-byte block in header order (checked against the header by the
+# ne_params is a 592-byte block in header order (checked against the header by the
 # repository's layout test); it is filled by ne_params_default and then overwritten
 # field by field from the flux formulation.
-const NE_PARAMS_BYTES = 568
+const NE_PARAMS_BYTES = 592
 struct NEParams
     bytes :: NTuple{NE_PARAMS_BYTES, UInt8}
 end

@@ -29,6 +29,7 @@ STATUS_NAMES = {
 
 NE_PERIODIC, NE_BOUNDED, NE_FLAT = 0, 1, 2
 NE_FP64, NE_MIXED_FP32 = 0, 1
+NE_RADIATIVE_FORCING_NONE, NE_RADIATIVE_FORCING_TWO_COLOR = 0, 1
 NE_STABILITY_NONE, NE_STABILITY_EDSON = 0, 1
 NE_SIMILARITY_LOG, NE_SIMILARITY_COARE_LOG = 0, 1
 NE_ROUGHNESS_CONSTANT, NE_ROUGHNESS_EDSON = 0, 1
@@ -53,7 +54,7 @@ class Grid(C.Structure):
         ("Nx", _i32), ("Ny", _i32), ("Nz", _i32),
         ("topology_x", _i32), ("topology_y", _i32), ("advection", _i32),
         ("dz", _p), ("z_center", _p), ("dy_xface", _p), ("dx_yface", _p), ("Az", _p),
-        ("lambda_c", _p), ("phi_c", _p), ("kmin", _p),
+        ("lambda_c", _p), ("phi_c", _p), ("kmin", _p), ("z_face", _p),
     ]
 
 
@@ -102,7 +103,8 @@ class InterfaceFluxes(C.Structure):
 
 
 class NetOceanFluxes(C.Structure):
-    _fields_ = [("tau_x", View), ("tau_y", View), ("J_T", View), ("J_S", View), ("J_w", View), ("J_H", View)]
+    _fields_ = [("tau_x", View), ("tau_y", View), ("J_T", View), ("J_S", View), ("J_w", View), ("J_H", View),
+                ("shortwave_surface_flux", View)]
 
 
 class RadiativeFluxes(C.Structure):
@@ -162,12 +164,15 @@ class Params(C.Structure):
         ("ocean_heat_capacity", _d),
         ("freshwater_density", _d),
         ("radiation", _i32),
-        ("reserved1", _i32),
+        ("radiative_forcing", _i32),
         ("stefan_boltzmann_constant", _d),
         ("ocean_albedo", _d),
         ("ocean_emissivity", _d),
         ("freshwater_melting_temperature", _d),
         ("liquidus_slope", _d),
+        ("first_color_fraction", _d),
+        ("first_absorption_coefficient", _d),
+        ("second_absorption_coefficient", _d),
     ]
 
 

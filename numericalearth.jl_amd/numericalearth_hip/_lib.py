@@ -46,9 +46,10 @@ SIGNATURES = {
                                          P(abi.ExchangeState), abi.View, P(abi.NetOceanFluxes), vp],
     "ne_apply_air_sea_radiation_f64": [vp, P(abi.Grid), P(abi.Params), abi.View, abi.View, abi.View,
                                        P(abi.NetOceanFluxes), P(abi.RadiativeFluxes), vp],
-    "ne_tracer_tendency_weno5_f64": [vp, P(abi.Grid), P(abi.OceanState), P(abi.NetOceanFluxes), abi.View,
-                                     abi.View, vp],
-    "ne_tracer_rk_stage_f64": [vp, P(abi.Grid)] + [abi.View] * 9 + [P(abi.NetOceanFluxes), C.c_double, vp],
+    "ne_tracer_tendency_weno5_f64": [vp, P(abi.Grid), P(abi.Params), P(abi.OceanState), P(abi.NetOceanFluxes),
+                                     abi.View, abi.View, vp],
+    "ne_tracer_rk_stage_f64": [vp, P(abi.Grid), P(abi.Params)] + [abi.View] * 9 + [P(abi.NetOceanFluxes),
+                                                                                 C.c_double, vp],
     "ne_budget_f64": [vp, P(abi.Grid), abi.View, C.c_double, vp, vp],
     "ne_budget_f32": [vp, P(abi.Grid), abi.View, C.c_double, vp, vp],
     "ne_model_create": [vp, P(abi.ModelDesc), P(vp), vp],

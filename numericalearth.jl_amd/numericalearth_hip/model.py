@@ -92,10 +92,12 @@ class DeviceGrid:
         self.lam = _dev(grid.lambda_c, device, f64)
         self.phi = _dev(grid.phi_c, device, f64)
         self.bottom = _dev(grid.bottom_height, device, f64)
+        self.zf = _dev(grid.z_face, device, f64)
         self.kmin = torch.empty((self.Ny, self.Nx), dtype=torch.int32, device=device)
         self.struct = abi.Grid(self.Nx, self.Ny, self.Nz, spec.topology_x, spec.topology_y, spec.advection,
                                self.dz.data_ptr(), self.zc.data_ptr(), self.dyx.data_ptr(), self.dxy.data_ptr(),
-                               self.Az.data_ptr(), self.lam.data_ptr(), self.phi.data_ptr(), self.kmin.data_ptr())
+                               self.Az.data_ptr(), self.lam.data_ptr(), self.phi.data_ptr(), self.kmin.data_ptr(),
+                               self.zf.data_ptr())
         check(ctx.lib.ne_grid_immersed_levels(ctx.handle, C.byref(self.struct), torch_view(self.bottom),
                                               self.kmin.data_ptr(), stream_handle()), "ne_grid_immersed_levels")
 
@@ -150,6 +152,7 @@ class OceanOnlyModel:
         self.fluxes = {k: z2() for k in FLUX_NAMES}
         self.iterations = torch.zeros((Ny, Nx), dtype=torch.int32, device=dev)
         self.net = {k: z2() for k in NET_NAMES}
+        self.shortwave_surface_flux = z2()    # J₀ of TwoColorRadiation
         self.radiative = {k: z2() for k in RAD_NAMES}
         self.frazil_heat = z2()
         self._atimes = np.ascontiguousarray(case.atmos_times, dtype=np.float64)
@@ -170,7 +173,8 @@ class OceanOnlyModel:
         d.ocean = abi.OceanState(torch_view(self.u), torch_view(self.v), torch_view(self.w), torch_view(self.T),
                                  torch_view(self.S))
         d.fluxes = abi.InterfaceFluxes(*(torch_view(self.fluxes[k]) for k in FLUX_NAMES), torch_view(self.iterations))
-        d.net = abi.NetOceanFluxes(*(torch_view(self.net[k]) for k in NET_NAMES))
+        d.net = abi.NetOceanFluxes(*(torch_view(self.net[k]) for k in NET_NAMES),
+                                   torch_view(self.shortwave_surface_flux))
         d.radiative = abi.RadiativeFluxes(*(torch_view(self.radiative[k]) for k in RAD_NAMES))
         d.frazil_heat = torch_view(self.frazil_heat)
         d.atmosphere_times = self._atimes.ctypes.data_as(C.POINTER(C.c_double))
@@ -325,14 +329,16 @@ class Phases:
         GT = torch.zeros_like(T)
         GS = torch.zeros_like(S)
         o = abi.OceanState(torch_view(m.u), torch_view(m.v), torch_view(m.w), torch_view(T), torch_view(S))
-        check(self.lib.ne_tracer_tendency_weno5_f64(self.ctx, C.byref(m.grid.struct), C.byref(o), C.byref(m.desc.net),
-                                                    torch_view(GT), torch_view(GS), stream_handle()),
+        check(self.lib.ne_tracer_tendency_weno5_f64(self.ctx, C.byref(m.grid.struct), C.byref(m.params), C.byref(o),
+                                                    C.byref(m.desc.net), torch_view(GT), torch_view(GS),
+                                                    stream_handle()),
               "ne_tracer_tendency_weno5_f64")
         return GT, GS
 
     def rk_stage(self, T0, S0, Tin, Sin, Tout, Sout, beta_dt: float):
         m = self.m
-        check(self.lib.ne_tracer_rk_stage_f64(self.ctx, C.byref(m.grid.struct), torch_view(m.u), torch_view(m.v),
+        check(self.lib.ne_tracer_rk_stage_f64(self.ctx, C.byref(m.grid.struct), C.byref(m.params), torch_view(m.u),
+                                              torch_view(m.v),
                                               torch_view(m.w), torch_view(T0), torch_view(S0), torch_view(Tin),
                                               torch_view(Sin), torch_view(Tout), torch_view(Sout),
                                               C.byref(m.desc.net), float(beta_dt), stream_handle()),

@@ -204,6 +204,21 @@ class SurfaceRadiationProperties:  # prescribed_radiation.jl:68 (ocean surface d
 
 
 @dataclass
+class TwoColorRadiation:
+    """TwoColorRadiation (Oceans/radiative_forcing.jl:5-46): shortwave that penetrates the ocean as two
+    colours decaying by Beer's law, I(z) = ε₁ I₀ exp(κ₁ z) + (1 − ε₁) I₀ exp(κ₂ z)."""
+    first_color_fraction: float = 0.58
+    first_absorption_coefficient: float = 1 / 0.35
+    second_absorption_coefficient: float = 1 / 23
+
+
+def default_radiative_forcing() -> TwoColorRadiation:
+    """Paulson & Simpson (1977) Jerlov Type I (ocean_simulation.jl:158-167)."""
+    return TwoColorRadiation(first_color_fraction=0.58, first_absorption_coefficient=1 / 0.35,
+                             second_absorption_coefficient=1 / 23)
+
+
+@dataclass
 class ComponentInterfaces:
     """The keyword arguments of ComponentInterfaces (component_interfaces.jl:390-419) on this path."""
     atmosphere_ocean_fluxes: Any = field(default_factory=SimilarityTheoryFluxes)
@@ -231,7 +246,8 @@ def to_params(fluxes: SimilarityTheoryFluxes | None = None, interfaces: Componen
               thermodynamics: AtmosphereThermodynamicsParameters | None = None,
               radiation: SurfaceRadiationProperties | None = SurfaceRadiationProperties(),
               stefan_boltzmann_constant: float = 5.670374419e-8,
-              liquidus: LinearLiquidus | None = None) -> abi.Params:
+              liquidus: LinearLiquidus | None = None,
+              radiative_forcing: TwoColorRadiation | None = None) -> abi.Params:
     """Lower the reference formulation objects to an ``ne_params`` block."""
     interfaces = interfaces or ComponentInterfaces()
     fluxes = fluxes or interfaces.atmosphere_ocean_fluxes
@@ -376,6 +392,19 @@ def to_params(fluxes: SimilarityTheoryFluxes | None = None, interfaces: Componen
     p.stefan_boltzmann_constant = float(stefan_boltzmann_constant)
     p.freshwater_melting_temperature = float(liq.freshwater_melting_temperature)
     p.liquidus_slope = float(liq.slope)
+    tcr = default_radiative_forcing()
+    p.first_color_fraction = float(tcr.first_color_fraction)
+    p.first_absorption_coefficient = float(tcr.first_absorption_coefficient)
+    p.second_absorption_coefficient = float(tcr.second_absorption_coefficient)
+    if radiative_forcing is None:
+        p.radiative_forcing = abi.NE_RADIATIVE_FORCING_NONE
+    elif isinstance(radiative_forcing, TwoColorRadiation):
+        p.radiative_forcing = abi.NE_RADIATIVE_FORCING_TWO_COLOR
+        p.first_color_fraction = float(radiative_forcing.first_color_fraction)
+        p.first_absorption_coefficient = float(radiative_forcing.first_absorption_coefficient)
+        p.second_absorption_coefficient = float(radiative_forcing.second_absorption_coefficient)
+    else:
+        raise NotImplementedError(f"radiative forcing {type(radiative_forcing).__name__}")
     return p
 
 

@@ -336,6 +336,11 @@ static inline int active(const ne_grid* g, int i, int j, int k) {
     return c >= 0 && k >= g->kmin[c];
 }
 
+/* TwoColorRadiation in use (radiative_forcing.jl:84-91): shortwave penetrates below the surface. */
+static int two_color(const ne_params* p, const ne_net_ocean_fluxes* n) {
+    return p && p->radiative_forcing == NE_RADIATIVE_FORCING_TWO_COLOR && n->shortwave_surface_flux.data;
+}
+
 static inline int wx(const ne_grid* g, int i) { return g->topology_x == NE_PERIODIC ? wrap(i, g->Nx) : i; }
 static inline int wy(const ne_grid* g, int j) { return g->topology_y == NE_PERIODIC ? wrap(j, g->Ny) : j; }
 
@@ -546,7 +551,13 @@ void neo_apply_air_sea_radiation(const ne_grid* g, const ne_params* p, ne_view s
             Ila = Ila * (1.0 - aice);
             Itsw = Itsw * (1.0 - aice);
             double Iup_ocean = Iup * (1.0 - aice);
-            double Qss = Itsw; /* shortwave_radiative_forcing with no TwoColorRadiation */
+            /* shortwave_radiative_forcing (radiative_forcing.jl:84-91): with TwoColorRadiation the
+               transmitted shortwave becomes its surface flux J₀ and leaves Jᵀ */
+            double Qss = Itsw;
+            if (two_color(p, n)) {
+                V2(n->shortwave_surface_flux, i, j) = -Itsw / (p->ocean_reference_density * p->ocean_heat_capacity);
+                Qss = 0.0;
+            }
             double SQ = Iup_ocean + Ila + Qss;
             double JTrad = SQ * rinv * cinv;
             int inactive = !active(g, i, j, kN);
@@ -655,8 +666,33 @@ static double flux_z(const ne_grid* g, ne_view w, ne_view c, int i, int j, int k
 /* Gc = −(1/V)(δx(Aₓuc̃) + δy(A_y vc̃) + δz(A_z wc̃)) − top boundary flux·Az/V at k = Nz.
  * If `scale` is given it receives Σ|terms|/V, the magnitude of the terms summed into G
  * (the scale against which the rounding of G is measured). */
+/* TwoColorRadiation (Oceans/radiative_forcing.jl:52-71). Flux of one colour through face kf
+   (0-based; face kf is the bottom of cell kf, face Nz the surface η): J₀ exp(κ (z − η)), and
+   zero where the cell below the face is inactive, so the radiation that reaches the bottom
+   is absorbed by the lowest active cell. */
+static double beers_law_radiation(const ne_grid* g, int i, int j, int kf, double J0, double kappa) {
+    double z = g->z_face[kf];
+    double eta = g->z_face[g->Nz];
+    double J = J0 * exp(kappa * (z - eta));
+    return active(g, i, j, kf - 1) ? J : 0.0;
+}
+
+/* ε₁ ∂z J₁ + (1 − ε₁) ∂z J₂ at cell k (∂zᶜᶜᶜ: (J(k+1) − J(k)) / Δz(k)); *mag receives the
+   magnitude of the terms. */
+static double two_color_forcing(const ne_grid* g, const ne_params* p, int i, int j, int k, double J0,
+                                double* mag) {
+    double e1 = p->first_color_fraction, k1 = p->first_absorption_coefficient, k2 = p->second_absorption_coefficient;
+    double dz = g->dz[k];
+    double a1 = beers_law_radiation(g, i, j, k + 1, J0, k1), b1 = beers_law_radiation(g, i, j, k, J0, k1);
+    double a2 = beers_law_radiation(g, i, j, k + 1, J0, k2), b2 = beers_law_radiation(g, i, j, k, J0, k2);
+    double dJ1dz = (a1 - b1) / dz;
+    double dJ2dz = (a2 - b2) / dz;
+    if (mag) *mag = (e1 * (fabs(a1) + fabs(b1)) + (1.0 - e1) * (fabs(a2) + fabs(b2))) / dz;
+    return e1 * dJ1dz + (1.0 - e1) * dJ2dz;
+}
+
 static double tendency(const ne_grid* g, ne_view u, ne_view v, ne_view w, ne_view c,
-                       int i, int j, int k, double top_flux, double* scale) {
+                       int i, int j, int k, double top_flux, double forcing, double forcing_mag, double* scale) {
     double div = 0.0, mag = 0.0;
     if (g->advection) {
         double fxe = flux_x(g, u, c, i + 1, j, k), fxw = flux_x(g, u, c, i, j, k);
@@ -670,6 +706,8 @@ static double tendency(const ne_grid* g, ne_view u, ne_view v, ne_view w, ne_vie
     }
     double V = g->Az[j] * g->dz[k];
     double G = -(1.0 / V * div);
+    G = G + forcing;
+    mag = mag + forcing_mag * V;
     if (k == g->Nz - 1) {
         G = G - top_flux * g->Az[j] / V;
         mag = mag + fabs(top_flux * g->Az[j]);
@@ -678,14 +716,15 @@ static double tendency(const ne_grid* g, ne_view u, ne_view v, ne_view w, ne_vie
     return G;
 }
 
-void neo_tracer_tendency(const ne_grid* g, const ne_ocean_state* o, const ne_net_ocean_fluxes* n,
-                         ne_view GT, ne_view GS) {
+void neo_tracer_tendency(const ne_grid* g, const ne_params* p, const ne_ocean_state* o,
+                         const ne_net_ocean_fluxes* n, ne_view GT, ne_view GS) {
     ne_view none = {NULL, 0, 0, 0};
-    neo_tracer_tendency_scaled(g, o, n, GT, GS, none, none);
+    neo_tracer_tendency_scaled(g, p, o, n, GT, GS, none, none);
 }
 
-void neo_tracer_tendency_scaled(const ne_grid* g, const ne_ocean_state* o, const ne_net_ocean_fluxes* n,
-                                ne_view GT, ne_view GS, ne_view sT, ne_view sS) {
+void neo_tracer_tendency_scaled(const ne_grid* g, const ne_params* p, const ne_ocean_state* o,
+                                const ne_net_ocean_fluxes* n, ne_view GT, ne_view GS, ne_view sT, ne_view sS) {
+    int tc = two_color(p, n);
     int kN = g->Nz - 1;
 #pragma omp parallel for NEO_THREADS
     for (int j = 0; j < g->Ny; j++)
@@ -704,18 +743,21 @@ void neo_tracer_tendency_scaled(const ne_grid* g, const ne_ocean_state* o, const
                         FT = V2(n->J_T, i, j) + (0.0 + 0.0);
                         FS = V2(n->J_S, i, j) + ((V3(o->S, i, j, kN) * V2(n->J_w, i, j) - 0.0) + 0.0);
                     }
-                    V3(GT, i, j, k) = tendency(g, o->u, o->v, o->w, o->T, i, j, k, FT, &scT);
-                    V3(GS, i, j, k) = tendency(g, o->u, o->v, o->w, o->S, i, j, k, FS, &scS);
+                    double F = 0.0, Fm = 0.0;
+                    if (tc) F = two_color_forcing(g, p, i, j, k, V2(n->shortwave_surface_flux, i, j), &Fm);
+                    V3(GT, i, j, k) = tendency(g, o->u, o->v, o->w, o->T, i, j, k, FT, F, Fm, &scT);
+                    V3(GS, i, j, k) = tendency(g, o->u, o->v, o->w, o->S, i, j, k, FS, 0.0, 0.0, &scS);
                 }
                 if (sT.data) V3(sT, i, j, k) = scT;
                 if (sS.data) V3(sS, i, j, k) = scS;
             }
 }
 
-void neo_rk_stage(const ne_grid* g, ne_view u, ne_view v, ne_view w, ne_view T0, ne_view S0,
+void neo_rk_stage(const ne_grid* g, const ne_params* p, ne_view u, ne_view v, ne_view w, ne_view T0, ne_view S0,
                   ne_view Tin, ne_view Sin, ne_view Tout, ne_view Sout,
                   const ne_net_ocean_fluxes* n, double beta_dt) {
     int kN = g->Nz - 1;
+    int tc = two_color(p, n);
     /* Tout may alias T0: each cell reads T0 only at itself, and writes after all
        stencil reads of Tin, which never aliases Tout. Rows are independent. */
 #pragma omp parallel for NEO_THREADS
@@ -728,8 +770,9 @@ void neo_rk_stage(const ne_grid* g, ne_view u, ne_view v, ne_view w, ne_view T0,
                     FT = V2(n->J_T, i, j) + (0.0 + 0.0);
                     FS = V2(n->J_S, i, j) + ((V3(Sin, i, j, kN) * V2(n->J_w, i, j) - 0.0) + 0.0);
                 }
-                double GTv = tendency(g, u, v, w, Tin, i, j, k, FT, NULL);
-                double GSv = tendency(g, u, v, w, Sin, i, j, k, FS, NULL);
+                double F = tc ? two_color_forcing(g, p, i, j, k, V2(n->shortwave_surface_flux, i, j), NULL) : 0.0;
+                double GTv = tendency(g, u, v, w, Tin, i, j, k, FT, F, 0.0, NULL);
+                double GSv = tendency(g, u, v, w, Sin, i, j, k, FS, 0.0, 0.0, NULL);
                 V3(Tout, i, j, k) = V3(T0, i, j, k) + beta_dt * GTv;
                 V3(Sout, i, j, k) = V3(S0, i, j, k) + beta_dt * GSv;
             }
@@ -774,7 +817,7 @@ void neo_model_ocean_step(neo_model* m, double dt) {
     const ne_grid* g = &m->grid;
     ne_ocean_state* o = &m->ocean;
     const double beta[3] = {1.0 / 3.0, 0.5, 1.0};
-    neo_rk_stage(g, o->u, o->v, o->w, o->T, o->S, o->T, o->S, m->T1, m->S1, &m->net, beta[0] * dt);
-    neo_rk_stage(g, o->u, o->v, o->w, o->T, o->S, m->T1, m->S1, m->T2, m->S2, &m->net, beta[1] * dt);
-    neo_rk_stage(g, o->u, o->v, o->w, o->T, o->S, m->T2, m->S2, o->T, o->S, &m->net, beta[2] * dt);
+    neo_rk_stage(g, &m->params, o->u, o->v, o->w, o->T, o->S, o->T, o->S, m->T1, m->S1, &m->net, beta[0] * dt);
+    neo_rk_stage(g, &m->params, o->u, o->v, o->w, o->T, o->S, m->T1, m->S1, m->T2, m->S2, &m->net, beta[1] * dt);
+    neo_rk_stage(g, &m->params, o->u, o->v, o->w, o->T, o->S, m->T2, m->S2, o->T, o->S, &m->net, beta[2] * dt);
 }

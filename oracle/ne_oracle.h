@@ -52,12 +52,15 @@ void neo_assemble_net_ocean_fluxes(const ne_grid* g, const ne_params* p, const n
                                    const ne_exchange_state* a, ne_view sst, const ne_net_ocean_fluxes* net);
 void neo_apply_air_sea_radiation(const ne_grid* g, const ne_params* p, ne_view sw, ne_view lw,
                                  ne_view Ts, const ne_net_ocean_fluxes* net, const ne_radiative_fluxes* rad);
-void neo_tracer_tendency(const ne_grid* g, const ne_ocean_state* o, const ne_net_ocean_fluxes* net,
-                         ne_view GT, ne_view GS);
+/* params may be NULL (no tracer forcing); with params->radiative_forcing two-colour the T
+ * tendency includes the TwoColorRadiation forcing of net->shortwave_surface_flux. */
+void neo_tracer_tendency(const ne_grid* g, const ne_params* params, const ne_ocean_state* o,
+                         const ne_net_ocean_fluxes* net, ne_view GT, ne_view GS);
 /* As above, and writes Σ|terms|/V of every cell (the magnitude of the flux terms summed into G). */
-void neo_tracer_tendency_scaled(const ne_grid* g, const ne_ocean_state* o, const ne_net_ocean_fluxes* net,
-                                ne_view GT, ne_view GS, ne_view scale_T, ne_view scale_S);
-void neo_rk_stage(const ne_grid* g, ne_view u, ne_view v, ne_view w, ne_view T0, ne_view S0,
+void neo_tracer_tendency_scaled(const ne_grid* g, const ne_params* params, const ne_ocean_state* o,
+                                const ne_net_ocean_fluxes* net, ne_view GT, ne_view GS, ne_view scale_T,
+                                ne_view scale_S);
+void neo_rk_stage(const ne_grid* g, const ne_params* params, ne_view u, ne_view v, ne_view w, ne_view T0, ne_view S0,
                   ne_view Tin, ne_view Sin, ne_view Tout, ne_view Sout,
                   const ne_net_ocean_fluxes* net, double beta_dt);
 void neo_budget(const ne_grid* g, ne_view c, double weight, double out[2]);

@@ -72,10 +72,12 @@ def lib():
         L.neo_model_update_state.argtypes = [C.c_void_p, C.c_double]
         L.neo_model_ocean_step.argtypes = [C.c_void_p, C.c_double]
         L.neo_budget.argtypes = [P(abi.Grid), abi.View, C.c_double, P(C.c_double)]
-        L.neo_tracer_tendency.argtypes = [P(abi.Grid), P(abi.OceanState), P(abi.NetOceanFluxes), abi.View, abi.View]
-        L.neo_tracer_tendency_scaled.argtypes = [P(abi.Grid), P(abi.OceanState), P(abi.NetOceanFluxes), abi.View,
-                                                 abi.View, abi.View, abi.View]
-        L.neo_rk_stage.argtypes = [P(abi.Grid)] + [abi.View] * 9 + [P(abi.NetOceanFluxes), C.c_double]
+        L.neo_tracer_tendency.argtypes = [P(abi.Grid), P(abi.Params), P(abi.OceanState), P(abi.NetOceanFluxes),
+                                          abi.View, abi.View]
+        L.neo_tracer_tendency_scaled.argtypes = [P(abi.Grid), P(abi.Params), P(abi.OceanState),
+                                                 P(abi.NetOceanFluxes), abi.View, abi.View, abi.View, abi.View]
+        L.neo_rk_stage.argtypes = [P(abi.Grid), P(abi.Params)] + [abi.View] * 9 + [P(abi.NetOceanFluxes),
+                                                                                  C.c_double]
         L.neo_ao_flux_solve.argtypes = [P(abi.Grid), P(abi.Params), P(abi.ExchangeState), P(abi.OceanState),
                                         P(abi.InterfaceFluxes)]
         L.neo_interp_atmos.argtypes = [P(abi.Grid), P(abi.PrescribedAtmosphereDesc), P(C.c_double), P(C.c_double),
@@ -168,9 +170,11 @@ class OracleModel:
         self.dz, self.zc, self.dyx, self.dxy, self.Az = (K(g.dz), K(g.z_center), K(g.dy_xface), K(g.dx_yface), K(g.Az))
         self.lam, self.phi = K(g.lambda_c), K(g.phi_c)
         self.kmin = K(g.kmin.astype(np.int32))
+        self.zf = K(g.z_face)
         self.grid = abi.Grid(Nx, Ny, Nz, spec.topology_x, spec.topology_y, spec.advection,
                              self.dz.ctypes.data, self.zc.ctypes.data, self.dyx.ctypes.data, self.dxy.ctypes.data,
-                             self.Az.ctypes.data, self.lam.ctypes.data, self.phi.ctypes.data, self.kmin.ctypes.data)
+                             self.Az.ctypes.data, self.lam.ctypes.data, self.phi.ctypes.data, self.kmin.ctypes.data,
+                             self.zf.ctypes.data)
         self.T = K(case.T.copy())
         self.S = K(case.S.copy())
         self.u, self.v, self.w = K(case.u.copy()), K(case.v.copy()), K(case.w.copy())
@@ -199,7 +203,9 @@ class OracleModel:
         self.iterations = K(np.zeros((Ny, Nx), np.int32))
         self.fluxes = abi.InterfaceFluxes(*(np_view(self.flux_arrays[k]) for k in FLUX_NAMES), np_view(self.iterations))
         self.net_arrays = {k: z2() for k in NET_NAMES}
-        self.net = abi.NetOceanFluxes(*(np_view(self.net_arrays[k]) for k in NET_NAMES))
+        self.shortwave_surface_flux = z2()
+        self.net = abi.NetOceanFluxes(*(np_view(self.net_arrays[k]) for k in NET_NAMES),
+                                      np_view(self.shortwave_surface_flux))
         self.rad_arrays = {k: z2() for k in RAD_NAMES}
         self.radiative = abi.RadiativeFluxes(*(np_view(self.rad_arrays[k]) for k in RAD_NAMES))
         self.frazil = z2()
@@ -276,8 +282,8 @@ class OracleModel:
             T = np.ascontiguousarray(T)
             S = np.ascontiguousarray(S)
             o = abi.OceanState(o.u, o.v, o.w, np_view(T), np_view(S))
-        lib().neo_tracer_tendency_scaled(C.byref(self.grid), C.byref(o), C.byref(self.net), np_view(GT), np_view(GS),
-                                         np_view(sT), np_view(sS))
+        lib().neo_tracer_tendency_scaled(C.byref(self.grid), C.byref(self.params), C.byref(o), C.byref(self.net),
+                                         np_view(GT), np_view(GS), np_view(sT), np_view(sS))
         return (GT, GS, sT, sS) if scales else (GT, GS)
 
     def budget(self, c: np.ndarray, weight: float = 1.0):

@@ -68,6 +68,12 @@ def test_null_arguments_are_rejected_without_a_device():
     assert L.ne_batch_info(None, None, None, None, None) == abi.NE_ERR_NULL
 
 
+def test_julia_mirror_has_the_header_params_size():
+    text = (REPO / "numericalearth.jl_amd" / "julia" / "NumericalEarthHIPExt.jl").read_text()
+    m = re.search(r"const NE_PARAMS_BYTES = (\d+)", text)
+    assert m and int(m.group(1)) == C.sizeof(abi.Params)
+
+
 def test_struct_layouts_match_header(tmp_path):
     lines = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(void) {"]
     for cname, cls in STRUCTS.items():

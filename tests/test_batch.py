@@ -23,13 +23,17 @@ def _specs(n=8, seed=5):
     return specs
 
 
-def _build(specs, precision, general=(), n_times=1):
-    """Members; those whose index is in `general` get a non-default Edson constant (general flux path)."""
+def _build(specs, precision, general=(), n_times=1, two_color=()):
+    """Members; those whose index is in `general` get a non-default Edson constant (general flux path),
+    those in `two_color` the TwoColorRadiation forcing."""
+    from numericalearth_hip import _ctypes_abi as abi_
     out = []
     for k, s in enumerate(specs):
         p = member_params(s)
         if k in general:
             p.edson_momentum[0] = 45.0
+        if k in two_color:
+            p.radiative_forcing = abi_.NE_RADIATIVE_FORCING_TWO_COLOR
         case = member_case(s) if n_times == 1 else _case_times(s, n_times)
         out.append(OceanOnlyModel(case, params=p, precision=precision))
     return out
@@ -42,7 +46,7 @@ def _case_times(s, n_times):
 
 
 def _state(m):
-    d = {"T": m.T, "S": m.S}
+    d = {"T": m.T, "S": m.S, "J0": m.shortwave_surface_flux}
     d.update({f"flux.{k}": v for k, v in m.fluxes.items()})
     d.update({f"net.{k}": v for k, v in m.net.items()})
     d["frazil"] = m.frazil_heat
@@ -63,8 +67,8 @@ def _assert_same(a_models, b_models, what):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_batch_equals_members_stepped_alone(require_gpu, precision):
     specs = _specs()
-    alone = _build(specs, precision, general=(2,))
-    batched = _build(specs, precision, general=(2,))
+    alone = _build(specs, precision, general=(2,), two_color=(4, 5))
+    batched = _build(specs, precision, general=(2,), two_color=(4, 5))
     ens = Ensemble(batched)
     print("[batch]", ens.info(), ens.tuning, flush=True)
     for _ in range(3):

@@ -179,6 +179,72 @@ def test_tendency_matches_oracle(require_gpu, shape, topology_x):
     assert np.abs(GT_o).max() > 0
 
 
+def test_full_size_snapshot_matches_oracle(require_gpu):
+    """The quarter-degree workload (1440×560×10): the construction-time flux solve, the WENO5
+    tendency of the resulting state and one coupled step match the oracle to 1e-12 (fluxes
+    elementwise, the tendency relative to its terms, the stepped state elementwise)."""
+    shape = (1440, 560, 10)
+    case = make_case(*shape, seed=0)
+    print(f"[full] land={case.meta['land_fraction']:.4f}", flush=True)
+    om = O.OracleModel(case).reconcile()
+    gm = OceanOnlyModel(case)
+    torch.cuda.synchronize()
+    _compare_fluxes(gm, om)
+    GT_o, GS_o, sT, sS = om.tendency(scales=True)
+    GT_g, GS_g = Phases(gm).tendency()
+    torch.cuda.synchronize()
+    eT = assert_rel_terms(_np(GT_g), GT_o, sT, RTOL, "G_T")
+    eS = assert_rel_terms(_np(GS_g), GS_o, sS, RTOL, "G_S")
+    print(f"[full] tendency G_T {eT:.2e} G_S {eS:.2e}", flush=True)
+    om.time_step(1200.0)
+    gm.time_step(1200.0)
+    torch.cuda.synchronize()
+    assert_rel(_np(gm.T), om.T, RTOL, "T")
+    assert_rel(_np(gm.S), om.S, RTOL, "S")
+    _compare_fluxes(gm, om)
+    scales = _net_term_scales(case, om)
+    for k in O.NET_NAMES:
+        assert_rel_terms(_np(gm.net[k]), om.net_arrays[k], scales.get(k, 0.0), RTOL, k)
+
+
+def test_tendency_two_color_matches_oracle(require_gpu):
+    """WENO5 tendency with the TwoColorRadiation forcing of T (20 m levels, so the shortwave reaches
+    several levels and the bottom of shallow columns)."""
+    from numericalearth_hip.params import TwoColorRadiation
+    p = to_params(radiative_forcing=TwoColorRadiation())
+    case, om, gm = _pair(SMALL, seed=14, params=p, z=(-200.0, 0.0))
+    om.reconcile()
+    _sync_net(gm, om)
+    gm.shortwave_surface_flux.copy_(torch.from_numpy(om.shortwave_surface_flux))
+    assert np.abs(om.shortwave_surface_flux).max() > 0
+    GT_o, GS_o, sT, sS = om.tendency(scales=True)
+    GT_g, GS_g = Phases(gm).tendency()
+    torch.cuda.synchronize()
+    eT = assert_rel_terms(_np(GT_g), GT_o, sT, RTOL, "G_T")
+    eS = assert_rel_terms(_np(GS_g), GS_o, sS, RTOL, "G_S")
+    print(f"[two-colour] tendency G_T {eT:.2e} G_S {eS:.2e}", flush=True)
+
+
+def test_model_two_color_steps_match_oracle(require_gpu):
+    from numericalearth_hip.params import TwoColorRadiation
+    p = to_params(radiative_forcing=TwoColorRadiation())
+    case, om, gm = _pair(SMALL, seed=15, params=p, z=(-200.0, 0.0))
+    om.reconcile()
+    torch.cuda.synchronize()
+    assert_rel(_np(gm.shortwave_surface_flux), om.shortwave_surface_flux, RTOL, "J0")
+    for step in range(3):
+        om.time_step(1200.0)
+        gm.time_step(1200.0)
+        torch.cuda.synchronize()
+        assert_rel(_np(gm.T), om.T, RTOL, "T")
+        assert_rel(_np(gm.S), om.S, RTOL, "S")
+        _compare_fluxes(gm, om)
+        scales = _net_term_scales(case, om)
+        for k in O.NET_NAMES:
+            assert_rel_terms(_np(gm.net[k]), om.net_arrays[k], scales.get(k, 0.0), RTOL, k)
+        assert_rel(_np(gm.shortwave_surface_flux), om.shortwave_surface_flux, RTOL, "J0")
+
+
 def test_rk_stage_matches_oracle(require_gpu):
     case, om, gm = _pair(seed=7)
     om.reconcile()

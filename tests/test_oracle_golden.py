@@ -366,3 +366,84 @@ def test_interpolation_invariants():
     assert np.allclose(m.exchange_arrays["T"][inner], expect[inner], rtol=0, atol=1e-13)
     # Neumann (edge-value) extension at the bounded latitude edges
     assert np.all(m.exchange_arrays["T"][~inner] == np.where(fj[~inner] < 0, 3.0, 3.0 + 0.5 * (a.Ny - 1))[:, None])
+
+
+# ---------------------------------------------------------------------------
+# TwoColorRadiation (Oceans/radiative_forcing.jl:5-106)
+# ---------------------------------------------------------------------------
+
+def _two_color_model(Nx, Ny, Nz, seed, z=(-3000.0, 0.0), tcr=None):
+    from numericalearth_hip.params import TwoColorRadiation, to_params
+    case = make_case(Nx, Ny, Nz, seed=seed, z=z)
+    p = to_params(radiative_forcing=tcr or TwoColorRadiation())
+    return case, O.OracleModel(case, params=p).reconcile()
+
+
+def test_two_color_top_cell_absorbs_the_two_band_fraction():
+    """Paulson & Simpson (1977) two-band law: a layer of thickness Δz at the surface absorbs
+    1 − ε₁e^{−κ₁Δz} − (1 − ε₁)e^{−κ₂Δz} of the penetrating shortwave. On 1 m levels the top cell's
+    heating V·F equals that fraction of Az·J₀, and J₀ = (1 − α)SW↓/(ρₒcₒ) > 0."""
+    from numericalearth_hip.params import default_radiative_forcing
+    case, m = _two_color_model(12, 8, 40, seed=3, z=(-40.0, 0.0))
+    p = m.params
+    tcr = default_radiative_forcing()
+    J0 = m.shortwave_surface_flux
+    SW = m.sw
+    np.testing.assert_allclose(J0, (1 - p.ocean_albedo) * SW / (p.ocean_reference_density * p.ocean_heat_capacity),
+                               rtol=1e-15)
+    zero = {k: np.zeros_like(v) for k, v in m.net_arrays.items()}
+    for k, v in zero.items():       # isolate the forcing: no surface fluxes, uniform tracer (no advection)
+        m.net_arrays[k][...] = v
+    c = np.full_like(m.T, 10.0)
+    GT, _ = m.tendency(c, c.copy())
+    g = case.grid
+    dz = g.dz[-1]
+    frac = 1 - tcr.first_color_fraction * math.exp(-tcr.first_absorption_coefficient * dz) \
+        - (1 - tcr.first_color_fraction) * math.exp(-tcr.second_absorption_coefficient * dz)
+    wet = g.kmin < g.spec.Nz
+    heat_top = GT[-1] * dz          # V·F / Az
+    np.testing.assert_allclose(heat_top[wet], frac * J0[wet], rtol=1e-12)
+    assert 0.5 < frac < 0.6
+
+
+def test_two_color_column_absorbs_everything_down_to_the_bottom():
+    """Σ_k V·F over a column is Az·J₀: all the penetrating shortwave is absorbed, the remainder by the
+    lowest active cell (the flux through the face below it is zero) — over immersed bottoms, and on a
+    10 m deep column where a quarter of the shortwave reaches the bottom face."""
+    for z, Nz in (((-3000.0, 0.0), 30), ((-10.0, 0.0), 5)):
+        case, m = _two_color_model(16, 10, Nz, seed=4, z=z)
+        for k in m.net_arrays:
+            m.net_arrays[k][...] = 0.0
+        c = np.full_like(m.T, 7.0)
+        GT, _ = m.tendency(c, c.copy())
+        g = case.grid
+        V = g.Az[None, :, None] * g.dz[:, None, None]
+        col = np.sum(V * GT, axis=0)
+        wet = g.kmin < g.spec.Nz
+        expect = g.Az[:, None] * m.shortwave_surface_flux
+        np.testing.assert_allclose(col[wet], expect[wet], rtol=1e-12)
+        if z[0] == -3000.0:
+            assert np.any(wet & (g.kmin > 0))
+        else:
+            p = m.params
+            reach = p.first_color_fraction * math.exp(-p.first_absorption_coefficient * 10.0) + \
+                (1 - p.first_color_fraction) * math.exp(-p.second_absorption_coefficient * 10.0)
+            assert reach > 0.25
+            # the bottom cell absorbs what reaches its top face, including the part that would leave
+            zt = g.z_face[1]
+            top = p.first_color_fraction * math.exp(p.first_absorption_coefficient * zt) + \
+                (1 - p.first_color_fraction) * math.exp(p.second_absorption_coefficient * zt)
+            np.testing.assert_allclose((V[0] * GT[0])[wet], (g.Az[:, None] * m.shortwave_surface_flux * top)[wet],
+                                       rtol=1e-12)
+
+
+def test_two_color_moves_shortwave_out_of_the_surface_flux():
+    """With TwoColorRadiation the transmitted shortwave leaves Jᵀ and becomes J₀:
+    Jᵀ(none) − Jᵀ(two-colour) = −J₀ on active columns (shortwave_radiative_forcing)."""
+    from numericalearth_hip.params import to_params
+    case, m2 = _two_color_model(20, 12, 6, seed=6)
+    m1 = O.OracleModel(case, params=to_params()).reconcile()
+    wet = case.grid.kmin < case.grid.spec.Nz
+    diff = m1.net_arrays["J_T"] - m2.net_arrays["J_T"]
+    np.testing.assert_allclose(diff[wet], -m2.shortwave_surface_flux[wet], rtol=1e-12, atol=1e-22)
+    assert np.all(m1.shortwave_surface_flux == 0.0)

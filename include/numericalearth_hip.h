@@ -119,9 +119,14 @@ typedef struct ne_prescribed_radiation {
 } ne_prescribed_radiation;
 
 /* Atmosphere state interpolated onto the exchange grid (2-D (C,C,Nothing) fields,
- * src/Atmospheres/prescribed_atmosphere_regridder.jl:7-13). */
+ * src/Atmospheres/prescribed_atmosphere_regridder.jl:7-13), and the radiation
+ * exchanger's interpolated downwelling shortwave / longwave
+ * (atmosphere_ocean_fluxes.jl:12-15), which only the SkinTemperature flux balance reads
+ * (interface_states.jl:547-553); with no radiation, or with BulkTemperature, they may
+ * have data == NULL. */
 typedef struct ne_exchange_state {
     ne_view u, v, T, p, q, rain, snow;
+    ne_view downwelling_shortwave, downwelling_longwave;
 } ne_exchange_state;
 
 /* Ocean state: u (Face,Center,Center), v (Center,Face,Center), w (Center,Center,Face),
@@ -242,9 +247,48 @@ typedef struct ne_params {
     double  first_color_fraction;         /* 0.58 */
     double  first_absorption_coefficient; /* 1/0.35 m⁻¹ */
     double  second_absorption_coefficient;/* 1/23 m⁻¹ */
+    /* Turbulent flux formulation and interface temperature. CoefficientBasedFluxes
+     * (InterfaceComputations/coefficient_based_turbulent_fluxes.jl:142-371) shares the stop
+     * criteria fields above (its defaults: tolerance 1e-8, maxiter 20, :215-229). */
+    int32_t turbulent_fluxes;             /* NE_FLUXES_SIMILARITY_THEORY or NE_FLUXES_COEFFICIENT_BASED */
+    int32_t transfer_coefficients;        /* NE_TRANSFER_SCALES: one coefficient each for momentum,
+                                             temperature, water vapour (:273-285);
+                                             NE_TRANSFER_LARGE_YEAGER (:288-340) */
+    int32_t transfer_kind[3];             /* NE_TRANSFER_SCALES entries: NE_COEFFICIENT_CONSTANT or
+                                             NE_COEFFICIENT_POLYNOMIAL (the polynomial below, :269-271) */
+    int32_t interface_temperature;        /* NE_TEMPERATURE_BULK (interface_states.jl:333) or
+                                             NE_TEMPERATURE_SKIN (SkinTemperature(DiffusiveFlux(κ, δ)), :355-577) */
+    double  transfer_coefficient[3];      /* constant Cd, Cₕ, Cq (default 1e-3 each) */
+    /* PolynomialNeutralDragCoefficient (:20-54): C_dN = (a/U + b + cU − dU⁶)/1000 */
+    double  drag_polynomial[4];           /* a b c d: 2.7, 0.142, 1/13.09, 3.14807e-10 */
+    double  high_wind_speed_threshold;    /* 33 m/s */
+    double  high_wind_drag_coefficient;   /* 2.34e-3 */
+    double  minimum_wind_speed;           /* 0.5 m/s */
+    /* LargeYeagerTransferCoefficients (:82-108) */
+    double  large_yeager_von_karman_constant;   /* 0.4 */
+    double  large_yeager_reference_height;      /* h₀ = 10 m */
+    double  stable_heat_transfer_coefficient;   /* 18 */
+    double  unstable_heat_transfer_coefficient; /* 32.7 */
+    double  moisture_transfer_coefficient;      /* 34.6 */
+    /* large_yeager_stability_functions (similarity_theory_turbulent_fluxes.jl:772-778):
+     * SplitStabilityFunction(stable = LinearStable, unstable = Paulson) */
+    double  paulson_momentum[2];          /* a b: 16, π/2 (:683-699) */
+    double  paulson_scalar;               /* a: 16 (:701-710) */
+    double  linear_stable_momentum[2];    /* c ζmax: 5, 10 (:742-752) */
+    double  linear_stable_scalar[2];      /* c ζmax: 5, 10 */
+    /* SkinTemperature(DiffusiveFlux(κ, δ); max_ΔT) (interface_states.jl:355-444, :536-577) */
+    double  skin_diffusivity;             /* κ, m² s⁻¹ */
+    double  skin_depth;                   /* δ, m */
+    double  skin_max_temperature_change;  /* max_ΔT = 5 K */
+    double  reference_sublimation_enthalpy; /* ℒₛ₀ = 2834400 (thermodynamic_parameters.jl:150) */
+    double  water_ice_heat_capacity;      /* cₚᵢ = 2100 (thermodynamic_parameters.jl:102) */
 } ne_params;
 
 enum { NE_RADIATIVE_FORCING_NONE = 0, NE_RADIATIVE_FORCING_TWO_COLOR = 1 };
+enum { NE_FLUXES_SIMILARITY_THEORY = 0, NE_FLUXES_COEFFICIENT_BASED = 1 };
+enum { NE_TRANSFER_SCALES = 0, NE_TRANSFER_LARGE_YEAGER = 1 };
+enum { NE_COEFFICIENT_CONSTANT = 0, NE_COEFFICIENT_POLYNOMIAL = 1 };
+enum { NE_TEMPERATURE_BULK = 0, NE_TEMPERATURE_SKIN = 1 };
 
 /* Counters written by the flux-solve kernels (reference: non-convergence is silent
  * and NaNs propagate to the NaNChecker, earth_system_model.jl:483-495). */

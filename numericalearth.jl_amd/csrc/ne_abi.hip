@@ -99,6 +99,15 @@ int check_params(const ne_params* p) {
     if (p->stability_functions != NE_STABILITY_NONE && p->stability_functions != NE_STABILITY_EDSON)
         return NE_ERR_UNSUPPORTED;
     if (p->maxiter < 0) return NE_ERR_SHAPE;
+    if (p->turbulent_fluxes != NE_FLUXES_SIMILARITY_THEORY && p->turbulent_fluxes != NE_FLUXES_COEFFICIENT_BASED)
+        return NE_ERR_UNSUPPORTED;
+    if (p->transfer_coefficients != NE_TRANSFER_SCALES && p->transfer_coefficients != NE_TRANSFER_LARGE_YEAGER)
+        return NE_ERR_UNSUPPORTED;
+    for (int n = 0; n < 3; n++)
+        if (p->transfer_kind[n] != NE_COEFFICIENT_CONSTANT && p->transfer_kind[n] != NE_COEFFICIENT_POLYNOMIAL)
+            return NE_ERR_UNSUPPORTED;
+    if (p->interface_temperature != NE_TEMPERATURE_BULK && p->interface_temperature != NE_TEMPERATURE_SKIN)
+        return NE_ERR_UNSUPPORTED;
     return NE_OK;
 }
 
@@ -211,6 +220,39 @@ int ne_params_default(ne_params* p) {
     p->first_color_fraction = 0.58;
     p->first_absorption_coefficient = 1.0 / 0.35;
     p->second_absorption_coefficient = 1.0 / 23.0;
+    // CoefficientBasedFluxes / LargeYeagerTransferCoefficients / SkinTemperature defaults
+    // (coefficient_based_turbulent_fluxes.jl:30-37, :92-99, :215-219; interface_states.jl:364)
+    p->turbulent_fluxes = NE_FLUXES_SIMILARITY_THEORY;
+    p->transfer_coefficients = NE_TRANSFER_SCALES;
+    for (int n = 0; n < 3; n++) {
+        p->transfer_kind[n] = NE_COEFFICIENT_CONSTANT;
+        p->transfer_coefficient[n] = 1e-3;
+    }
+    p->interface_temperature = NE_TEMPERATURE_BULK;
+    p->drag_polynomial[0] = 2.7;
+    p->drag_polynomial[1] = 0.142;
+    p->drag_polynomial[2] = 1.0 / 13.09;
+    p->drag_polynomial[3] = 3.14807e-10;
+    p->high_wind_speed_threshold = 33.0;
+    p->high_wind_drag_coefficient = 2.34e-3;
+    p->minimum_wind_speed = 0.5;
+    p->large_yeager_von_karman_constant = 0.4;
+    p->large_yeager_reference_height = 10.0;
+    p->stable_heat_transfer_coefficient = 18.0;
+    p->unstable_heat_transfer_coefficient = 32.7;
+    p->moisture_transfer_coefficient = 34.6;
+    p->paulson_momentum[0] = 16.0;
+    p->paulson_momentum[1] = 3.14159265358979323846 / 2.0;
+    p->paulson_scalar = 16.0;
+    p->linear_stable_momentum[0] = 5.0;
+    p->linear_stable_momentum[1] = 10.0;
+    p->linear_stable_scalar[0] = 5.0;
+    p->linear_stable_scalar[1] = 10.0;
+    p->skin_diffusivity = 0.0;
+    p->skin_depth = 0.0;
+    p->skin_max_temperature_change = 5.0;
+    p->reference_sublimation_enthalpy = 2834400.0;
+    p->water_ice_heat_capacity = 2100.0;
     return NE_OK;
 }
 
@@ -410,7 +452,8 @@ static int model_update_state(ne_model* m, double dt_frazil, hipStream_t s, hipE
     if (d.precision == NE_FP64) {
         AtmosDev<double> a = atmos_dev<double>(d.atmosphere);
         RadDev<double> r = rad_dev<double>(d.radiation);
-        e = launch_surface_solve<double>(g, d.params, a, d.atmos_fi, d.atmos_fj, view<const double>(d.ocean.T),
+        e = launch_surface_solve<double>(g, d.params, a, d.atmos_fi, d.atmos_fj, r, d.radiation_fi, d.radiation_fj,
+                                         view<const double>(d.ocean.T),
                                          view<const double>(d.ocean.S), view<const double>(d.ocean.u),
                                          view<const double>(d.ocean.v), f, m->ctx->counters, s);
         if (e != hipSuccess) return NE_ERR_DEVICE;
@@ -421,7 +464,8 @@ static int model_update_state(ne_model* m, double dt_frazil, hipStream_t s, hipE
     } else {
         AtmosDev<float> a = atmos_dev<float>(d.atmosphere);
         RadDev<float> r = rad_dev<float>(d.radiation);
-        e = launch_surface_solve<float>(g, d.params, a, d.atmos_fi, d.atmos_fj, view<const float>(d.ocean.T),
+        e = launch_surface_solve<float>(g, d.params, a, d.atmos_fi, d.atmos_fj, r, d.radiation_fi, d.radiation_fj,
+                                         view<const float>(d.ocean.T),
                                         view<const float>(d.ocean.S), view<const float>(d.ocean.u),
                                         view<const float>(d.ocean.v), f, m->ctx->counters, s);
         if (e != hipSuccess) return NE_ERR_DEVICE;

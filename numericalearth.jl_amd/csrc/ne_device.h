@@ -292,9 +292,12 @@ __device__ __forceinline__ void psi_pair_default(T z, T& pm, T& ps) {
     ps = -(x * tsqrt(x)) - (T)(2.0 / 3.0) * (z - (T)14.28) * e - (T)8.525;
 }
 
-// True when the flux formulation uses the reference's default Edson constants with the
-// logarithmic similarity profile (the case the *_default functions above implement).
+// True when the flux formulation is SimilarityTheoryFluxes with BulkTemperature and the
+// reference's default Edson constants with the logarithmic similarity profile (the case
+// the *_default functions above implement).
 __host__ inline bool edson_default_constants(const ne_params& p) {
+    if (p.turbulent_fluxes != NE_FLUXES_SIMILARITY_THEORY || p.interface_temperature != NE_TEMPERATURE_BULK)
+        return false;
     const double pi = 3.14159265358979323846;
     const double em[11] = {50.0, 0.35, 0.7, 0.75, 5.0 / 0.35, 15.0, 2.0, pi / 2.0, 10.15, 3.0, pi / sqrt(3.0)};
     const double es[12] = {50.0, 0.35, 2.0 / 3.0, 3.0 / 2.0, 14.28, 8.525, 15.0, 2.0, 0.0, 34.15, 3.0, pi / sqrt(3.0)};
@@ -354,6 +357,192 @@ struct ColumnFluxes {
     bool converged;
 };
 
+// ---------------------------------------------------------------------------
+// Coefficient-based fluxes and the skin temperature (the formulations other than
+// SimilarityTheoryFluxes + BulkTemperature). FP64, one iteration as the reference
+// writes it: the skin temperature, then qₛ, Δθ, Δq from it, then the scales.
+// ---------------------------------------------------------------------------
+
+// x^n, n > 3, as Julia's Float64^Integer: power by squaring in double-double with
+// error-free products (fma), rounded once.
+__device__ __forceinline__ double pow_int(double x, int n) {
+    double yh = 1.0, yl = 0.0, xh = x, xl = 0.0;
+    while (n > 0) {
+        if (n & 1) {
+            double ph = yh * xh;
+            double pl = fma(yh, xh, -ph) + (yh * xl + yl * xh);
+            yh = ph + pl;
+            yl = pl - (yh - ph);
+        }
+        n >>= 1;
+        if (n > 0) {
+            double sh = xh * xh;
+            double sl = fma(xh, xh, -sh) + 2.0 * xh * xl;
+            xh = sh + sl;
+            xl = sl - (xh - sh);
+        }
+    }
+    return yh + yl;
+}
+
+// latent_heat_sublim = ℒₛ₀ + (cₚᵥ − cₚᵢ)(T − T₀) (taken at interface_states.jl:544)
+__device__ __forceinline__ double latent_heat_sublim(const ne_params& p, double T) {
+    double dcp = p.water_vapor_heat_capacity - p.water_ice_heat_capacity;
+    return p.reference_sublimation_enthalpy + dcp * (T - p.reference_temperature);
+}
+
+// PolynomialNeutralDragCoefficient (coefficient_based_turbulent_fluxes.jl:48-54)
+__device__ __forceinline__ double neutral_drag(const ne_params& p, double U) {
+    U = jmax(U, p.minimum_wind_speed);
+    if (!(U < p.high_wind_speed_threshold)) return p.high_wind_drag_coefficient;
+    const double* c = p.drag_polynomial;
+    return (c[0] / U + c[1] + c[2] * U - c[3] * pow_int(U, 6)) / 1000.0;
+}
+
+// SplitStabilityFunction(LinearStable, Paulson) of large_yeager_stability_functions
+// (similarity_theory_turbulent_fluxes.jl:688-710, :742-752, :772-778); only the branch
+// selected by sign(ζ) is evaluated.
+__device__ __forceinline__ double psi_large_yeager(const ne_params& p, bool scalar, double zeta) {
+    if (zeta > 0.0) {
+        const double* ls = scalar ? p.linear_stable_scalar : p.linear_stable_momentum;
+        return -ls[0] * jmin(zeta, ls[1]);
+    }
+    double a = scalar ? p.paulson_scalar : p.paulson_momentum[0];
+    double z = sqrt(sqrt(1.0 - a * jmin(0.0, zeta)));
+    if (scalar) return 2.0 * log((1.0 + z * z) / 2.0);
+    return 2.0 * log((1.0 + z) / 2.0) + log((1.0 + z * z) / 2.0) + -2.0 * atan(z) + p.paulson_momentum[1];
+}
+
+// iterate_interface_fluxes(::CoefficientBasedFluxes) (coefficient_based_turbulent_fluxes.jl:346-371)
+// with evaluate_coefficients (:269-340).
+__device__ __forceinline__ void coefficient_scales(const ne_params& p, double du, double dv, double Ts, double qsurf,
+                                                   double dtheta, double dq, double dh, double& us, double& ts,
+                                                   double& qs) {
+    const bool ly = p.transfer_coefficients == NE_TRANSFER_LARGE_YEAGER;
+    double dU = jmax(sqrt(du * du + dv * dv), ly ? p.minimum_wind_speed : 0.0);
+    double Cd, Ch, Cq;
+    if (ly) {
+        const double kappa = p.large_yeager_von_karman_constant;
+        const double g = p.gravitational_acceleration;
+        double Tv = virtual_temperature(p, Ts, qsurf);
+        double delta = Rv_over_Rd(p) - 1.0;
+        double bstar = g / Tv * (ts * (1.0 + delta * qsurf) + delta * Tv * qs);
+        double L = bstar == 0.0 ? (double)INFINITY : us * us / (kappa * bstar);
+        double zeta = dh / L;
+        double psim = psi_large_yeager(p, false, zeta);
+        double psih = psi_large_yeager(p, true, zeta);
+        double lh = log(dh / p.large_yeager_reference_height);
+        double Cdp = us == 0.0 ? neutral_drag(p, dU) : us * us / (dU * dU);
+        double UN10 = jmax(dU / (1.0 + sqrt(Cdp) / kappa * (lh - psim)), p.minimum_wind_speed);
+        double CdN = neutral_drag(p, UN10);
+        double sCdN = sqrt(CdN);
+        double ChN = sCdN / 1000.0 *
+                     (zeta > 0.0 ? p.stable_heat_transfer_coefficient : p.unstable_heat_transfer_coefficient);
+        double CqN = sCdN / 1000.0 * p.moisture_transfer_coefficient;
+        double xim = sCdN / kappa * (lh - psim);
+        Cd = CdN / ((1.0 + xim) * (1.0 + xim));
+        double xih = sCdN / kappa * (lh - psih);
+        double ratio = sqrt(Cd) / sCdN;
+        Ch = ChN * ratio / (1.0 + ChN * xih);
+        Cq = CqN * ratio / (1.0 + CqN * xih);
+    } else {
+        Cd = p.transfer_kind[0] == NE_COEFFICIENT_POLYNOMIAL ? neutral_drag(p, dU) : p.transfer_coefficient[0];
+        Ch = p.transfer_kind[1] == NE_COEFFICIENT_POLYNOMIAL ? neutral_drag(p, dU) : p.transfer_coefficient[1];
+        Cq = p.transfer_kind[2] == NE_COEFFICIENT_POLYNOMIAL ? neutral_drag(p, dU) : p.transfer_coefficient[2];
+    }
+    double sCd = sqrt(Cd);
+    us = sCd * dU;
+    ts = Cd == 0.0 ? 0.0 : Ch / sCd * dtheta;
+    qs = Cd == 0.0 ? 0.0 : Cq / sCd * dq;
+}
+
+// flux_balance_temperature(::SkinTemperature{<:DiffusiveFlux}) (interface_states.jl:425-448)
+// inside compute_interface_temperature(::SkinTemperature) (:536-577), constant diffusivity.
+// Tm: the previous iterate's interface temperature; Ti: the top-cell temperature (K).
+__device__ __forceinline__ double skin_temperature(const ne_params& p, double rhoa, double ca, double Li,
+                                                   double thetaa, double Ti, double Tm, double us, double ts,
+                                                   double qs, double sigma, double alpha, double eps, double SW,
+                                                   double LW) {
+    double Iup = sigma * eps * pow_int(Tm, 4);
+    double Qd = -(1.0 - alpha) * SW - eps * LW;
+    double QT = -rhoa * ca * us * ts;
+    double Qv = -rhoa * Li * us * qs;
+    double kappa = p.skin_diffusivity, delta = p.skin_depth;
+    double lambda = 1.0 / (p.ocean_reference_density * p.ocean_heat_capacity);
+    double JT = (Qv + Iup + Qd) * lambda;
+    double dT = thetaa - Tm;
+    double OmT = QT * lambda;
+    double D = kappa * dT - OmT * delta;
+    double Tstar = D == 0.0 ? Tm : (Ti * kappa * dT - (JT * dT + OmT * thetaa) * delta) / D;
+    double x = Tstar - Ti, m = p.skin_max_temperature_change;
+    return Ti + (x > m ? m : (x < -m ? -m : x));
+}
+
+// The fixed point of compute_interface_state (compute_interface_state.jl:31-58) for these
+// formulations. Returns the scales and the interface temperature (K) through the references.
+__device__ __noinline__ int flux_iterate_general(const ne_params& p, double ua, double va, double Ta, double pa,
+                                                 double qa, double To, double su, double sv, double SW, double LW,
+                                                 double& us, double& ts, double& qs, double& sT, bool& reached) {
+    const double g = p.gravitational_acceleration;
+    const double za = p.surface_layer_height;
+    const bool converge = p.stop_criteria == NE_STOP_CONVERGENCE;
+    const bool skin = p.interface_temperature == NE_TEMPERATURE_SKIN;
+    const bool coeff = p.turbulent_fluxes == NE_FLUXES_COEFFICIENT_BASED;
+    const double thetaa = Ta + g * za / cp_m(p, qa);  // surface_atmosphere_temperature (interface_states.jl:308-317)
+    double rhoa = 0.0, Li = 0.0, ca = 0.0;
+    double sigma = 0.0, alpha = 0.0, eps = 0.0;
+    if (skin) {
+        rhoa = air_density(p, Ta, pa, qa);
+        ca = cp_m(p, qa);
+        Li = latent_heat_sublim(p, Ta);
+        if (p.radiation) { sigma = p.stefan_boltzmann_constant; alpha = p.ocean_albedo; eps = p.ocean_emissivity; }
+        else { SW = 0.0; LW = 0.0; }
+    }
+    double du, dv;
+    if (p.velocity_difference == NE_VELOCITY_RELATIVE) { du = ua - su; dv = va - sv; }
+    else { du = ua; dv = va; }
+    const double kappa = p.von_karman_constant;
+    const double delta = Rv_over_Rd(p) - 1.0;
+    int it = 0;
+    for (;;) {
+        double Ts = skin ? skin_temperature(p, rhoa, ca, Li, thetaa, To, sT, us, ts, qs, sigma, alpha, eps, SW, LW)
+                         : sT;
+        double qsurf = surface_specific_humidity(p, pa, Ts);
+        double dq = qa - qsurf;
+        double dtheta = thetaa - Ts;
+        double nus, nts, nqs;
+        if (coeff) {
+            nus = us; nts = ts; nqs = qs;
+            coefficient_scales(p, du, dv, Ts, qsurf, dtheta, dq, za, nus, nts, nqs);
+        } else {
+            // iterate_interface_fluxes(::SimilarityTheoryFluxes) (similarity:315-385)
+            double Tv = virtual_temperature(p, Ts, qsurf);
+            double bstar = g / Tv * (ts * (1.0 + delta * qsurf) + delta * Tv * qs);
+            double U = sqrt(du * du + dv * dv + gustiness_squared(p, us, bstar, p.boundary_layer_height));
+            double lu = momentum_roughness(p, us);
+            double lq = scalar_roughness(p, p.water_vapor_roughness, p.water_vapor_roughness_length, lu, us);
+            double lt = scalar_roughness(p, p.temperature_roughness, p.temperature_roughness_length, lu, us);
+            double dh = jmax(za - p.zero_plane_displacement, 2.0 * lu);
+            double L = bstar == 0.0 ? (double)INFINITY : us * us / (kappa * bstar);
+            nus = kappa / similarity_profile(p, false, dh, lu, L) * U;
+            nts = kappa / similarity_profile(p, true, dh, lt, L) * dtheta;
+            nqs = kappa / similarity_profile(p, true, dh, lq, L) * dq;
+        }
+        it += 1;
+        bool go;
+        if (converge) {
+            double drift = fabs(nus - us) + fabs(nts - ts) + fabs(nqs - qs);
+            reached = it >= p.maxiter;
+            go = !((drift < p.tolerance) | reached);
+        } else {
+            go = it < p.maxiter;
+        }
+        us = nus; ts = nts; qs = nqs; sT = Ts;
+        if (!go) break;
+    }
+    return it;
+}
+
 // One column of _compute_atmosphere_ocean_interface_state! (atmosphere_ocean_fluxes.jl:80-197)
 // with compute_interface_state (compute_interface_state.jl:31-58) and
 // iterate_interface_fluxes(::SimilarityTheoryFluxes) (similarity:315-385).
@@ -376,7 +565,7 @@ __device__ __forceinline__ float tdiv<float>(float a, float b) { return __fdivid
 template <bool Fast, bool Mixed = false>
 __device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p, double ua, double va, double Ta,
                                                     double pa, double qa, double To_C, double uo, double vo,
-                                                    bool inactive) {
+                                                    bool inactive, double SW = 0.0, double LW = 0.0) {
     const double g = p.gravitational_acceleration;
     const double za = p.surface_layer_height;
     const double hbl = p.boundary_layer_height;
@@ -388,7 +577,11 @@ __device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p, double u
     const bool converge = p.stop_criteria == NE_STOP_CONVERGENCE;
     bool reached = false;
 
-    if (!inactive && (converge || p.maxiter > 0)) {
+    const bool general = !Fast && (p.turbulent_fluxes != NE_FLUXES_SIMILARITY_THEORY ||
+                                   p.interface_temperature != NE_TEMPERATURE_BULK);
+    if (general && !inactive && (converge || p.maxiter > 0)) {
+        it = flux_iterate_general(p, ua, va, Ta, pa, qa, To, su, sv, SW, LW, us, ts, qs, sT, reached);
+    } else if (!inactive && (converge || p.maxiter > 0)) {
         // Iteration invariants: BulkTemperature keeps Tₛ = Tₒ, hence qₛ is fixed.
         const double Ts = To;
         const double qsurf = surface_specific_humidity(p, pa, Ts);

@@ -117,9 +117,9 @@ hipError_t launch_radiation(const GridDev& g, const ne_params& p, View3<const do
 // and adds radiation.
 template <typename Real>
 hipError_t launch_surface_solve(const GridDev& g, const ne_params& p, const AtmosDev<Real>& a,
-                                const double* fi, const double* fj, View3<const Real> T, View3<const Real> S,
-                                View3<const Real> u, View3<const Real> v, const FluxOut& out, ne_counters* c,
-                                hipStream_t s);
+                                const double* fi, const double* fj, const RadDev<Real>& r, const double* rfi,
+                                const double* rfj, View3<const Real> T, View3<const Real> S, View3<const Real> u,
+                                View3<const Real> v, const FluxOut& out, ne_counters* c, hipStream_t s);
 template <typename Real>
 hipError_t launch_surface_finalize(const GridDev& g, const ne_params& p, const AtmosDev<Real>& a,
                                    const double* afi, const double* afj, const RadDev<Real>& r,

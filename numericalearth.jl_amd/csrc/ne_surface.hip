@@ -144,7 +144,8 @@ __global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(NE_
 flux_solve_exchange_kernel(GridDev g, ne_params p, View3<const double> ua, View3<const double> va,
                                            View3<const double> Ta, View3<const double> pa, View3<const double> qa,
                                            View3<const double> T, View3<const double> S, View3<const double> u,
-                                           View3<const double> v, FluxOut out, ne_counters* counters) {
+                                           View3<const double> v, View3<const double> sw,
+                                           View3<const double> lw, FluxOut out, ne_counters* counters) {
     int i = blockIdx.x * BX + threadIdx.x;
     int j = blockIdx.y * BY + threadIdx.y;
     bool in = i < g.Nx && j < g.Ny;
@@ -154,8 +155,13 @@ flux_solve_exchange_kernel(GridDev g, ne_params p, View3<const double> ua, View3
         double To, So, uo, vo;
         ocean_surface<double>(g, T, S, u, v, i, j, To, So, uo, vo);
         inactive = !g.active(i, j, g.Nz - 1);
+        double SW = 0.0, LW = 0.0;
+        if (!Fast && p.interface_temperature == NE_TEMPERATURE_SKIN && p.radiation) {
+            SW = sw.at2(i, j);
+            LW = lw.at2(i, j);
+        }
         r = flux_column<Fast>(p, ua.at2(i, j), va.at2(i, j), Ta.at2(i, j), pa.at2(i, j), qa.at2(i, j), To, uo, vo,
-                              inactive);
+                              inactive, SW, LW);
         store_fluxes(out, i, j, r);
     }
     bool solved = in && !inactive;
@@ -164,10 +170,14 @@ flux_solve_exchange_kernel(GridDev g, ne_params p, View3<const double> ua, View3
 }
 
 // One column of K1: interpolate the atmosphere at the column, solve, store the fluxes.
+// The skin-temperature balance also reads the downwelling radiation, interpolated here
+// exactly as K2 interpolates it.
 template <typename Real, bool Fast>
 __device__ __forceinline__ ColumnFluxes surface_solve_column(const GridDev& g, const ne_params& p,
                                                              const AtmosDev<Real>& a, const double* __restrict__ fi,
-                                                             const double* __restrict__ fj,
+                                                             const double* __restrict__ fj, const RadDev<Real>& rd,
+                                                             const double* __restrict__ rfi,
+                                                             const double* __restrict__ rfj,
                                                              const View3<const Real>& T, const View3<const Real>& S,
                                                              const View3<const Real>& u, const View3<const Real>& v,
                                                              const FluxOut& out, int i, int j, bool& inactive) {
@@ -181,9 +191,16 @@ __device__ __forceinline__ ColumnFluxes surface_solve_column(const GridDev& g, c
     double To, So, uo, vo;
     ocean_surface<Real>(g, T, S, u, v, i, j, To, So, uo, vo);
     inactive = !g.active(i, j, g.Nz - 1);
+    double SW = 0.0, LW = 0.0;
+    if (!Fast && p.interface_temperature == NE_TEMPERATURE_SKIN && p.radiation) {
+        Neighbours rx = neighbours(rfi[i], rd.Nx, rd.tx);
+        Neighbours ry = neighbours(rfj[j], rd.Ny, rd.ty);
+        SW = interp_field(rd.sw, rx, ry, rd.time);
+        LW = interp_field(rd.lw, rx, ry, rd.time);
+    }
     // FP32 storage (the mixed form) also takes the FP32 pre-iteration of the fixed point
-    ColumnFluxes r =
-        flux_column<Fast, NE_MIXED_PREITERATION && sizeof(Real) == 4>(p, ua, va, Ta, pa, qa, To, uo, vo, inactive);
+    ColumnFluxes r = flux_column<Fast, NE_MIXED_PREITERATION && sizeof(Real) == 4>(p, ua, va, Ta, pa, qa, To, uo, vo,
+                                                                                   inactive, SW, LW);
     store_fluxes(out, i, j, r);
     return r;
 }
@@ -192,14 +209,15 @@ __device__ __forceinline__ ColumnFluxes surface_solve_column(const GridDev& g, c
 template <typename Real, bool Fast>
 __global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(NE_FLUX_WAVES_PER_EU)))
 surface_solve_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __restrict__ fi,
-                     const double* __restrict__ fj, View3<const Real> T, View3<const Real> S, View3<const Real> u,
+                     const double* __restrict__ fj, RadDev<Real> rd, const double* __restrict__ rfi,
+                     const double* __restrict__ rfj, View3<const Real> T, View3<const Real> S, View3<const Real> u,
                      View3<const Real> v, FluxOut out, ne_counters* counters) {
     int i = blockIdx.x * BX + threadIdx.x;
     int j = blockIdx.y * BY + threadIdx.y;
     bool in = i < g.Nx && j < g.Ny;
     ColumnFluxes r;
     bool inactive = true;
-    if (in) r = surface_solve_column<Real, Fast>(g, p, a, fi, fj, T, S, u, v, out, i, j, inactive);
+    if (in) r = surface_solve_column<Real, Fast>(g, p, a, fi, fj, rd, rfi, rfj, T, S, u, v, out, i, j, inactive);
     bool solved = in && !inactive;
     accumulate_counters(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
                         solved && !finite3(r.us, r.ts, r.qs));
@@ -223,8 +241,8 @@ batch_surface_solve_kernel(const BatchRec<Real>* __restrict__ recs, const WorkIt
         ColumnFluxes r;
         bool inactive = true;
         if (in)
-            r = surface_solve_column<Real, Fast>(g, rc.p, rc.a, rc.afi, rc.afj, cview(rc.T), cview(rc.S), rc.u, rc.v,
-                                                 rc.f, i, j, inactive);
+            r = surface_solve_column<Real, Fast>(g, rc.p, rc.a, rc.afi, rc.afj, rc.r, rc.rfi, rc.rfj, cview(rc.T),
+                                                 cview(rc.S), rc.u, rc.v, rc.f, i, j, inactive);
         bool solved = in && !inactive;
         accumulate_counters(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
                             solved && !finite3(r.us, r.ts, r.qs));
@@ -530,16 +548,20 @@ template hipError_t launch_interp_radiation<float>(const GridDev&, const RadDev<
 hipError_t launch_flux_solve_exchange(const GridDev& g, const ne_params& p, const ne_exchange_state& a,
                                       const ne_ocean_state& o, const FluxOut& out, ne_counters* c,
                                       hipStream_t s) {
+    View3<const double> sw = view<const double>(a.downwelling_shortwave);
+    View3<const double> lw = view<const double>(a.downwelling_longwave);
     if (edson_default_constants(p))
         hipLaunchKernelGGL(flux_solve_exchange_kernel<true>, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p,
                            view<const double>(a.u), view<const double>(a.v), view<const double>(a.T),
                            view<const double>(a.p), view<const double>(a.q), view<const double>(o.T),
-                           view<const double>(o.S), view<const double>(o.u), view<const double>(o.v), out, c);
+                           view<const double>(o.S), view<const double>(o.u), view<const double>(o.v), sw, lw, out,
+                           c);
     else
         hipLaunchKernelGGL(flux_solve_exchange_kernel<false>, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p,
                            view<const double>(a.u), view<const double>(a.v), view<const double>(a.T),
                            view<const double>(a.p), view<const double>(a.q), view<const double>(o.T),
-                           view<const double>(o.S), view<const double>(o.u), view<const double>(o.v), out, c);
+                           view<const double>(o.S), view<const double>(o.u), view<const double>(o.v), sw, lw, out,
+                           c);
     return hipGetLastError();
 }
 
@@ -565,24 +587,27 @@ hipError_t launch_radiation(const GridDev& g, const ne_params& p, View3<const do
 
 template <typename Real>
 hipError_t launch_surface_solve(const GridDev& g, const ne_params& p, const AtmosDev<Real>& a, const double* fi,
-                                const double* fj, View3<const Real> T, View3<const Real> S, View3<const Real> u,
-                                View3<const Real> v, const FluxOut& out, ne_counters* c, hipStream_t s) {
+                                const double* fj, const RadDev<Real>& r, const double* rfi, const double* rfj,
+                                View3<const Real> T, View3<const Real> S, View3<const Real> u, View3<const Real> v,
+                                const FluxOut& out, ne_counters* c, hipStream_t s) {
     if (edson_default_constants(p))
         hipLaunchKernelGGL((surface_solve_kernel<Real, true>), grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, a, fi,
-                           fj, T, S, u, v, out, c);
+                           fj, r, rfi, rfj, T, S, u, v, out, c);
     else
         hipLaunchKernelGGL((surface_solve_kernel<Real, false>), grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, a, fi,
-                           fj, T, S, u, v, out, c);
+                           fj, r, rfi, rfj, T, S, u, v, out, c);
     return hipGetLastError();
 }
 template hipError_t launch_surface_solve<double>(const GridDev&, const ne_params&, const AtmosDev<double>&,
-                                                 const double*, const double*, View3<const double>,
-                                                 View3<const double>, View3<const double>, View3<const double>,
-                                                 const FluxOut&, ne_counters*, hipStream_t);
+                                                 const double*, const double*, const RadDev<double>&, const double*,
+                                                 const double*, View3<const double>, View3<const double>,
+                                                 View3<const double>, View3<const double>, const FluxOut&,
+                                                 ne_counters*, hipStream_t);
 template hipError_t launch_surface_solve<float>(const GridDev&, const ne_params&, const AtmosDev<float>&,
-                                                const double*, const double*, View3<const float>,
-                                                View3<const float>, View3<const float>, View3<const float>,
-                                                const FluxOut&, ne_counters*, hipStream_t);
+                                                const double*, const double*, const RadDev<float>&, const double*,
+                                                const double*, View3<const float>, View3<const float>,
+                                                View3<const float>, View3<const float>, const FluxOut&,
+                                                ne_counters*, hipStream_t);
 
 template <typename Real>
 hipError_t launch_surface_finalize(const GridDev& g, const ne_params& p, const AtmosDev<Real>& a,

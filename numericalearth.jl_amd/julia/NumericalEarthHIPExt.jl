@@ -55,7 +55,10 @@ struct NEGrid
     z_face :: Ptr{Float64}
 end
 
-struct NEExchangeState; u::NEView; v::NEView; T::NEView; p::NEView; q::NEView; rain::NEView; snow::NEView; end
+struct NEExchangeState
+    u::NEView; v::NEView; T::NEView; p::NEView; q::NEView; rain::NEView; snow::NEView
+    downwelling_shortwave::NEView; downwelling_longwave::NEView
+end
 struct NEOceanState; u::NEView; v::NEView; w::NEView; T::NEView; S::NEView; end
 struct NEInterfaceFluxes
     latent_heat::NEView; sensible_heat::NEView; water_vapor::NEView; x_momentum::NEView; y_momentum::NEView
@@ -69,7 +72,7 @@ struct NERadiativeFluxes; upwelling_longwave::NEView; downwelling_longwave::NEVi
 # ne_params is a 592-byte block in header order (checked against the header by the
 # repository's layout test); it is filled by ne_params_default and then overwritten
 # field by field from the flux formulation.
-const NE_PARAMS_BYTES = 592
+const NE_PARAMS_BYTES = 832
 struct NEParams
     bytes :: NTuple{NE_PARAMS_BYTES, UInt8}
 end
@@ -111,8 +114,10 @@ function InterfaceComputations.compute_atmosphere_ocean_fluxes!(coupled_model::H
     ocean = coupled_model.ocean.model
     grid = ne_grid(coupled_model)
     params = ne_params(coupled_model)
+    rad = ex.radiation   # radiation exchanger (atmosphere_ocean_fluxes.jl:12-15); nothing without radiation
+    sw, lw = isnothing(rad) ? (NULLVIEW, NULLVIEW) : (ne_view(rad.state.ℐꜜˢʷ), ne_view(rad.state.ℐꜜˡʷ))
     state = NEExchangeState(ne_view(atmos.u), ne_view(atmos.v), ne_view(atmos.T), ne_view(atmos.p),
-                            ne_view(atmos.q), ne_view(atmos.Jʳⁿ), ne_view(atmos.Jˢⁿ))
+                            ne_view(atmos.q), ne_view(atmos.Jʳⁿ), ne_view(atmos.Jˢⁿ), sw, lw)
     oc = NEOceanState(ne_view(ocean.velocities.u), ne_view(ocean.velocities.v), ne_view(ocean.velocities.w),
                       ne_view(ocean.tracers.T), ne_view(ocean.tracers.S))
     f = ao.fluxes

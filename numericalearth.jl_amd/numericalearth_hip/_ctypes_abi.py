@@ -30,6 +30,10 @@ STATUS_NAMES = {
 NE_PERIODIC, NE_BOUNDED, NE_FLAT = 0, 1, 2
 NE_FP64, NE_MIXED_FP32 = 0, 1
 NE_RADIATIVE_FORCING_NONE, NE_RADIATIVE_FORCING_TWO_COLOR = 0, 1
+NE_FLUXES_SIMILARITY_THEORY, NE_FLUXES_COEFFICIENT_BASED = 0, 1
+NE_TRANSFER_SCALES, NE_TRANSFER_LARGE_YEAGER = 0, 1
+NE_COEFFICIENT_CONSTANT, NE_COEFFICIENT_POLYNOMIAL = 0, 1
+NE_TEMPERATURE_BULK, NE_TEMPERATURE_SKIN = 0, 1
 NE_STABILITY_NONE, NE_STABILITY_EDSON = 0, 1
 NE_SIMILARITY_LOG, NE_SIMILARITY_COARE_LOG = 0, 1
 NE_ROUGHNESS_CONSTANT, NE_ROUGHNESS_EDSON = 0, 1
@@ -86,7 +90,8 @@ class PrescribedRadiationDesc(C.Structure):
 
 
 class ExchangeState(C.Structure):
-    _fields_ = [("u", View), ("v", View), ("T", View), ("p", View), ("q", View), ("rain", View), ("snow", View)]
+    _fields_ = [("u", View), ("v", View), ("T", View), ("p", View), ("q", View), ("rain", View), ("snow", View),
+                ("downwelling_shortwave", View), ("downwelling_longwave", View)]
 
 
 class OceanState(C.Structure):
@@ -173,6 +178,29 @@ class Params(C.Structure):
         ("first_color_fraction", _d),
         ("first_absorption_coefficient", _d),
         ("second_absorption_coefficient", _d),
+        ("turbulent_fluxes", C.c_int32),
+        ("transfer_coefficients", C.c_int32),
+        ("transfer_kind", C.c_int32 * 3),
+        ("interface_temperature", C.c_int32),
+        ("transfer_coefficient", _d * 3),
+        ("drag_polynomial", _d * 4),
+        ("high_wind_speed_threshold", _d),
+        ("high_wind_drag_coefficient", _d),
+        ("minimum_wind_speed", _d),
+        ("large_yeager_von_karman_constant", _d),
+        ("large_yeager_reference_height", _d),
+        ("stable_heat_transfer_coefficient", _d),
+        ("unstable_heat_transfer_coefficient", _d),
+        ("moisture_transfer_coefficient", _d),
+        ("paulson_momentum", _d * 2),
+        ("paulson_scalar", _d),
+        ("linear_stable_momentum", _d * 2),
+        ("linear_stable_scalar", _d * 2),
+        ("skin_diffusivity", _d),
+        ("skin_depth", _d),
+        ("skin_max_temperature_change", _d),
+        ("reference_sublimation_enthalpy", _d),
+        ("water_ice_heat_capacity", _d),
     ]
 
 

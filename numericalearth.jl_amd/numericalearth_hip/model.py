@@ -278,7 +278,11 @@ class Phases:
         self.sw, self.lw = z2(), z2()
 
     def _ex(self):
-        return abi.ExchangeState(*(torch_view(self.exchange[k]) for k in EXCHANGE_NAMES))
+        ex = abi.ExchangeState(*(torch_view(self.exchange[k]) for k in EXCHANGE_NAMES))
+        if self.m.desc.params.radiation:   # the radiation exchanger's state (skin-temperature balance)
+            ex.downwelling_shortwave = torch_view(self.sw)
+            ex.downwelling_longwave = torch_view(self.lw)
+        return ex
 
     def interp(self, atmos_time: abi.TimeInterp | None = None, rad_time: abi.TimeInterp | None = None):
         m = self.m

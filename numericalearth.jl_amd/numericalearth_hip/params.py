@@ -153,6 +153,106 @@ class SimilarityTheoryFluxes:  # similarity_theory_turbulent_fluxes.jl:174-214
             self.stability_functions = SimilarityScales(z, z, z)
 
 
+# ---------------------------------------------------------------------------
+# Coefficient-based fluxes (coefficient_based_turbulent_fluxes.jl) and the Large-Yeager
+# stability functions (similarity_theory_turbulent_fluxes.jl:680-778)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class PaulsonMomentumStabilityFunction:  # similarity_theory_turbulent_fluxes.jl:683-686
+    a: float = 16.0
+    b: float = math.pi / 2
+
+
+@dataclass
+class PaulsonScalarStabilityFunction:  # similarity_theory_turbulent_fluxes.jl:701-703
+    a: float = 16.0
+
+
+@dataclass
+class LinearStableStabilityFunction:  # similarity_theory_turbulent_fluxes.jl:742-745
+    coefficient: float = 5.0
+    maximum_stability_parameter: float = 10.0
+
+
+@dataclass
+class SplitStabilityFunction:  # similarity_theory_turbulent_fluxes.jl:712-715
+    stable: Any
+    unstable: Any
+
+
+def large_yeager_stability_functions():  # similarity_theory_turbulent_fluxes.jl:772-778
+    """Paulson (1970) for unstable, ψ = −5ζ bounded at ζ ≤ 10 for stable stratification."""
+    stable = LinearStableStabilityFunction()
+    momentum = SplitStabilityFunction(stable, PaulsonMomentumStabilityFunction())
+    scalar = SplitStabilityFunction(stable, PaulsonScalarStabilityFunction())
+    return SimilarityScales(momentum, scalar, scalar)
+
+
+@dataclass
+class PolynomialNeutralDragCoefficient:  # coefficient_based_turbulent_fluxes.jl:20-46
+    """C_dN(U) = (a/U + b + cU − dU⁶)/1000 below the high-wind threshold (Large & Yeager 2004)."""
+    a: float = 2.7
+    b: float = 0.142
+    c: float = 1 / 13.09
+    d: float = 3.14807e-10
+    high_wind_speed_threshold: float = 33.0
+    high_wind_drag_coefficient: float = 2.34e-3
+    minimum_wind_speed: float = 0.5
+
+    def values(self):
+        return (self.a, self.b, self.c, self.d, self.high_wind_speed_threshold, self.high_wind_drag_coefficient,
+                self.minimum_wind_speed)
+
+
+@dataclass
+class LargeYeagerTransferCoefficients:  # coefficient_based_turbulent_fluxes.jl:82-108
+    von_karman_constant: float = 0.4
+    neutral_drag_coefficient: Any = field(default_factory=PolynomialNeutralDragCoefficient)
+    stability_functions: Any = field(default_factory=large_yeager_stability_functions)
+    reference_height: float = 10.0
+    stable_heat_transfer_coefficient: float = 18.0
+    unstable_heat_transfer_coefficient: float = 32.7
+    moisture_transfer_coefficient: float = 34.6
+
+
+_TRANSFER_KEYS = ("momentum", "temperature", "water_vapor")
+
+
+@dataclass
+class CoefficientBasedFluxes:  # coefficient_based_turbulent_fluxes.jl:142-229
+    """Bulk fluxes from transfer coefficients: a 3-tuple, a dict with keys momentum,
+    temperature, water_vapor, a SimilarityScales (entries constant numbers or a
+    PolynomialNeutralDragCoefficient) or a LargeYeagerTransferCoefficients."""
+    transfer_coefficients: Any = (1e-3, 1e-3, 1e-3)
+    solver_stop_criteria: Any = None
+    solver_tolerance: float = 1e-8
+    solver_maxiter: int = 20
+
+    def __post_init__(self):
+        tc = self.transfer_coefficients
+        if isinstance(tc, dict):          # validate_coefficients(::NamedTuple) (:234-246)
+            missing = [k for k in _TRANSFER_KEYS if k not in tc]
+            if missing:
+                raise ValueError(f"Transfer coefficients must contain keys {_TRANSFER_KEYS}. Missing keys: "
+                                 f"{tuple(missing)}. Received: {tuple(tc)}")
+        elif isinstance(tc, (tuple, list)):  # validate_coefficients(::Tuple) (:248-258)
+            if len(tc) != 3:
+                raise ValueError("Transfer coefficients must be a tuple of length 3: (momentum, temperature, "
+                                 f"water_vapor). Got length {len(tc)} with value {tuple(tc)}.")
+        if self.solver_stop_criteria is None:
+            self.solver_stop_criteria = ConvergenceStopCriteria(self.solver_tolerance, self.solver_maxiter)
+
+    def coefficients(self):
+        """(momentum, temperature, water vapour) entries of a scales-type coefficient set."""
+        tc = self.transfer_coefficients
+        if isinstance(tc, dict):
+            return tuple(tc[k] for k in _TRANSFER_KEYS)
+        if isinstance(tc, SimilarityScales):
+            return (tc.momentum, tc.temperature, tc.water_vapor)
+        return tuple(tc)
+
+
 @dataclass
 class ImpureSaturationSpecificHumidity:  # interface_states.jl:20-47; default component_interfaces.jl:353-358
     phase: str = "liquid"
@@ -167,6 +267,27 @@ class FixedSpecificHumidity:
 
 class BulkTemperature:  # interface_states.jl:330-333
     pass
+
+
+@dataclass
+class InteriorDiffusivity:  # interface_states.jl:378-392
+    minimum_diffusivity: float = 1.4e-7
+
+
+@dataclass
+class DiffusiveFlux:  # interface_states.jl:366-376
+    """Internal flux J = −κ (Tₛ − Tᵢ)/δ between the top-cell temperature, δ below the
+    interface, and the interface temperature."""
+    κ: Any
+    δ: float
+
+
+@dataclass
+class SkinTemperature:  # interface_states.jl:355-364
+    """Interface temperature from the flux balance F(Tₛ) − Jᵀ = 0 (semi-implicit in the
+    sensible heat flux, each iterate's step clamped to max_ΔT)."""
+    internal_flux: Any
+    max_ΔT: float = 5.0
 
 
 class RelativeVelocity:  # interface_states.jl:287
@@ -186,6 +307,8 @@ class AtmosphereThermodynamicsParameters:  # src/Atmospheres/thermodynamic_param
     water_vapor_heat_capacity: float = 1859.0
     liquid_water_heat_capacity: float = 4181.0
     reference_vaporization_enthalpy: float = 2500800.0
+    reference_sublimation_enthalpy: float = 2834400.0
+    water_ice_heat_capacity: float = 2100.0
     reference_temperature: float = 273.16
     triple_point_temperature: float = 273.16
     triple_point_pressure: float = 611.657
@@ -241,7 +364,98 @@ def _roughness(kind_obj, which: str):
     raise NotImplementedError(f"roughness length {type(kind_obj).__name__} is not implemented by the HIP path")
 
 
-def to_params(fluxes: SimilarityTheoryFluxes | None = None, interfaces: ComponentInterfaces | None = None, *,
+def _set_polynomial(p: abi.Params, poly: PolynomialNeutralDragCoefficient) -> None:
+    for n in range(4):
+        p.drag_polynomial[n] = float(poly.values()[n])
+    p.high_wind_speed_threshold = float(poly.high_wind_speed_threshold)
+    p.high_wind_drag_coefficient = float(poly.high_wind_drag_coefficient)
+    p.minimum_wind_speed = float(poly.minimum_wind_speed)
+
+
+def _lower_coefficients(p: abi.Params, fluxes: CoefficientBasedFluxes | None) -> None:
+    """The coefficient-based block; with fluxes None, its defaults (as ne_params_default)."""
+    ly_default = LargeYeagerTransferCoefficients()
+    _set_polynomial(p, ly_default.neutral_drag_coefficient)
+    p.large_yeager_von_karman_constant = ly_default.von_karman_constant
+    p.large_yeager_reference_height = ly_default.reference_height
+    p.stable_heat_transfer_coefficient = ly_default.stable_heat_transfer_coefficient
+    p.unstable_heat_transfer_coefficient = ly_default.unstable_heat_transfer_coefficient
+    p.moisture_transfer_coefficient = ly_default.moisture_transfer_coefficient
+    pm, ps, ls = PaulsonMomentumStabilityFunction(), PaulsonScalarStabilityFunction(), LinearStableStabilityFunction()
+    p.paulson_momentum[0], p.paulson_momentum[1], p.paulson_scalar = pm.a, pm.b, ps.a
+    for arr in (p.linear_stable_momentum, p.linear_stable_scalar):
+        arr[0], arr[1] = ls.coefficient, ls.maximum_stability_parameter
+    p.turbulent_fluxes = abi.NE_FLUXES_SIMILARITY_THEORY
+    p.transfer_coefficients = abi.NE_TRANSFER_SCALES
+    for n in range(3):
+        p.transfer_kind[n] = abi.NE_COEFFICIENT_CONSTANT
+        p.transfer_coefficient[n] = 1e-3
+    if fluxes is None:
+        return
+    p.turbulent_fluxes = abi.NE_FLUXES_COEFFICIENT_BASED
+    tc = fluxes.transfer_coefficients
+    if isinstance(tc, LargeYeagerTransferCoefficients):
+        p.transfer_coefficients = abi.NE_TRANSFER_LARGE_YEAGER
+        if not isinstance(tc.neutral_drag_coefficient, PolynomialNeutralDragCoefficient):
+            raise NotImplementedError(f"neutral drag {type(tc.neutral_drag_coefficient).__name__}")
+        _set_polynomial(p, tc.neutral_drag_coefficient)
+        p.large_yeager_von_karman_constant = float(tc.von_karman_constant)
+        p.large_yeager_reference_height = float(tc.reference_height)
+        p.stable_heat_transfer_coefficient = float(tc.stable_heat_transfer_coefficient)
+        p.unstable_heat_transfer_coefficient = float(tc.unstable_heat_transfer_coefficient)
+        p.moisture_transfer_coefficient = float(tc.moisture_transfer_coefficient)
+        sf = tc.stability_functions
+        for name, ψ, unstable_type in (("momentum", sf.momentum, PaulsonMomentumStabilityFunction),
+                                       ("temperature", sf.temperature, PaulsonScalarStabilityFunction)):
+            if not (isinstance(ψ, SplitStabilityFunction) and isinstance(ψ.stable, LinearStableStabilityFunction)
+                    and isinstance(ψ.unstable, unstable_type)):
+                raise NotImplementedError(f"Large-Yeager {name} stability function other than "
+                                          f"SplitStabilityFunction(LinearStable, {unstable_type.__name__})")
+            ls = p.linear_stable_momentum if name == "momentum" else p.linear_stable_scalar
+            ls[0], ls[1] = float(ψ.stable.coefficient), float(ψ.stable.maximum_stability_parameter)
+        p.paulson_momentum[0] = float(sf.momentum.unstable.a)
+        p.paulson_momentum[1] = float(sf.momentum.unstable.b)
+        p.paulson_scalar = float(sf.temperature.unstable.a)
+        return
+    poly = None
+    for n, c in enumerate(fluxes.coefficients()):
+        if isinstance(c, bool) or not isinstance(c, (int, float, PolynomialNeutralDragCoefficient)):
+            raise NotImplementedError(f"{_TRANSFER_KEYS[n]} transfer coefficient of type {type(c).__name__} "
+                                      "(the HIP path takes numbers or a PolynomialNeutralDragCoefficient)")
+        if isinstance(c, PolynomialNeutralDragCoefficient):
+            if poly is not None and c.values() != poly.values():
+                raise NotImplementedError("two different PolynomialNeutralDragCoefficient parameter sets")
+            poly = c
+            p.transfer_kind[n] = abi.NE_COEFFICIENT_POLYNOMIAL
+            p.transfer_coefficient[n] = 0.0
+        else:
+            p.transfer_coefficient[n] = float(c)
+    if poly is not None:
+        _set_polynomial(p, poly)
+
+
+def _lower_interface_temperature(p: abi.Params, it) -> None:
+    p.interface_temperature = abi.NE_TEMPERATURE_BULK
+    p.skin_diffusivity, p.skin_depth, p.skin_max_temperature_change = 0.0, 0.0, 5.0
+    if isinstance(it, BulkTemperature):
+        return
+    if not isinstance(it, SkinTemperature):
+        raise NotImplementedError(f"interface temperature {type(it).__name__}")
+    F = it.internal_flux
+    if not isinstance(F, DiffusiveFlux):
+        raise NotImplementedError(f"skin-temperature internal flux {type(F).__name__} (sea-ice conductive fluxes "
+                                  "have no counterpart on an ocean-only path)")
+    if isinstance(F.κ, bool) or not isinstance(F.κ, (int, float)):
+        raise NotImplementedError(f"skin diffusivity {type(F.κ).__name__}: a diffusivity assessed from the "
+                                  "interior model needs a turbulence closure, which this path's ocean has not")
+    p.interface_temperature = abi.NE_TEMPERATURE_SKIN
+    p.skin_diffusivity = float(F.κ)
+    p.skin_depth = float(F.δ)
+    p.skin_max_temperature_change = float(it.max_ΔT)
+
+
+def to_params(fluxes: SimilarityTheoryFluxes | CoefficientBasedFluxes | None = None,
+              interfaces: ComponentInterfaces | None = None, *,
               surface_layer_height: float = 10.0, boundary_layer_height: float = 512.0,
               thermodynamics: AtmosphereThermodynamicsParameters | None = None,
               radiation: SurfaceRadiationProperties | None = SurfaceRadiationProperties(),
@@ -251,11 +465,32 @@ def to_params(fluxes: SimilarityTheoryFluxes | None = None, interfaces: Componen
     """Lower the reference formulation objects to an ``ne_params`` block."""
     interfaces = interfaces or ComponentInterfaces()
     fluxes = fluxes or interfaces.atmosphere_ocean_fluxes
-    if not isinstance(fluxes, SimilarityTheoryFluxes):
+    if isinstance(fluxes, CoefficientBasedFluxes):
+        p = _lower_similarity(SimilarityTheoryFluxes())   # the unused similarity block keeps its defaults
+        _lower_coefficients(p, fluxes)
+    elif isinstance(fluxes, SimilarityTheoryFluxes):
+        p = _lower_similarity(fluxes)
+        _lower_coefficients(p, None)
+    else:
         raise NotImplementedError(f"{type(fluxes).__name__} is not implemented by the HIP path "
-                                  "(only SimilarityTheoryFluxes)")
-    th = thermodynamics or AtmosphereThermodynamicsParameters()
-    liq = liquidus or LinearLiquidus()
+                                  "(SimilarityTheoryFluxes or CoefficientBasedFluxes)")
+    sc = fluxes.solver_stop_criteria
+    if isinstance(sc, ConvergenceStopCriteria):
+        p.stop_criteria = abi.NE_STOP_CONVERGENCE
+        p.maxiter = int(sc.maxiter)
+        p.tolerance = float(sc.tolerance)
+    elif isinstance(sc, FixedIterations):
+        p.stop_criteria = abi.NE_STOP_FIXED_ITERATIONS
+        p.maxiter = int(sc.iterations)
+        p.tolerance = 0.0
+    else:
+        raise NotImplementedError(f"solver stop criteria {type(sc).__name__}")
+    _lower_interface_temperature(p, interfaces.atmosphere_ocean_interface_temperature)
+    return _lower_interfaces(p, interfaces, surface_layer_height, boundary_layer_height, thermodynamics, radiation,
+                             stefan_boltzmann_constant, liquidus, radiative_forcing)
+
+
+def _lower_similarity(fluxes: SimilarityTheoryFluxes) -> abi.Params:
     p = abi.Params()
     p.von_karman_constant = fluxes.von_karman_constant
     p.turbulent_prandtl_number = fluxes.turbulent_prandtl_number
@@ -339,21 +574,13 @@ def to_params(fluxes: SimilarityTheoryFluxes | None = None, interfaces: Componen
     if not isinstance(fluxes.zero_plane_displacement, (int, float)):
         raise NotImplementedError("field-valued zero-plane displacement")
     p.zero_plane_displacement = float(fluxes.zero_plane_displacement)
+    return p
 
-    sc = fluxes.solver_stop_criteria
-    if isinstance(sc, ConvergenceStopCriteria):
-        p.stop_criteria = abi.NE_STOP_CONVERGENCE
-        p.maxiter = int(sc.maxiter)
-        p.tolerance = float(sc.tolerance)
-    elif isinstance(sc, FixedIterations):
-        p.stop_criteria = abi.NE_STOP_FIXED_ITERATIONS
-        p.maxiter = int(sc.iterations)
-        p.tolerance = 0.0
-    else:
-        raise NotImplementedError(f"solver stop criteria {type(sc).__name__}")
 
-    if not isinstance(interfaces.atmosphere_ocean_interface_temperature, BulkTemperature):
-        raise NotImplementedError("interface temperature formulations other than BulkTemperature")
+def _lower_interfaces(p, interfaces, surface_layer_height, boundary_layer_height, thermodynamics, radiation,
+                      stefan_boltzmann_constant, liquidus, radiative_forcing) -> abi.Params:
+    th = thermodynamics or AtmosphereThermodynamicsParameters()
+    liq = liquidus or LinearLiquidus()
     q = interfaces.atmosphere_ocean_interface_specific_humidity
     if isinstance(q, ImpureSaturationSpecificHumidity):
         if q.phase != "liquid":
@@ -378,6 +605,7 @@ def to_params(fluxes: SimilarityTheoryFluxes | None = None, interfaces: Componen
     p.gravitational_acceleration = float(interfaces.gravitational_acceleration)
     for name in ("gas_constant", "dry_air_molar_mass", "water_molar_mass", "dry_air_adiabatic_exponent",
                  "water_vapor_heat_capacity", "liquid_water_heat_capacity", "reference_vaporization_enthalpy",
+                 "reference_sublimation_enthalpy", "water_ice_heat_capacity",
                  "reference_temperature", "triple_point_temperature", "triple_point_pressure"):
         setattr(p, name, float(getattr(th, name)))
     p.ocean_reference_density = float(interfaces.ocean_reference_density)

@@ -87,6 +87,36 @@ double neo_latent_heat_vapor(const ne_params* p, double T) {
     return p->reference_vaporization_enthalpy + dcp * (T - p->reference_temperature);
 }
 
+/* latent_heat_sublim = ℒₛ₀ + (cₚᵥ − cₚᵢ)(T − T₀), the sublimation counterpart of the above;
+ * the SkinTemperature balance takes it for the latent heat (interface_states.jl:544). */
+double neo_latent_heat_sublim(const ne_params* p, double T) {
+    double dcp = p->water_vapor_heat_capacity - p->water_ice_heat_capacity;
+    return p->reference_sublimation_enthalpy + dcp * (T - p->reference_temperature);
+}
+
+/* x^n for a literal integer n > 3 as Julia evaluates Float64^Integer: power by squaring
+ * carried in double-double (error-free products through fma), rounded once at the end,
+ * so the result is the correctly rounded power up to a rare last-ulp tie. */
+double neo_pow_int(double x, int n) {
+    double yh = 1.0, yl = 0.0, xh = x, xl = 0.0;
+    while (n > 0) {
+        if (n & 1) {
+            double ph = yh * xh;
+            double pl = fma(yh, xh, -ph) + (yh * xl + yl * xh);
+            yh = ph + pl;
+            yl = pl - (yh - ph);
+        }
+        n >>= 1;
+        if (n > 0) {
+            double sh = xh * xh;
+            double sl = fma(xh, xh, -sh) + 2.0 * xh * xl;
+            xh = sh + sl;
+            xl = sl - (xh - sh);
+        }
+    }
+    return yh + yl;
+}
+
 double neo_virtual_temperature(const ne_params* p, double T, double q) {
     return gas_constant_air(p, q) / R_d(p) * T;
 }
@@ -205,6 +235,79 @@ static double similarity_profile(const ne_params* p, int scalar, double h, doubl
     return log(h / l) - psih + psil;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Coefficient-based fluxes (coefficient_based_turbulent_fluxes.jl)           */
+/* ------------------------------------------------------------------------- */
+
+/* PolynomialNeutralDragCoefficient functor (:48-54) */
+double neo_neutral_drag(const ne_params* p, double U) {
+    const double* c = p->drag_polynomial;
+    U = jmax(U, p->minimum_wind_speed);
+    double poly = (c[0] / U + c[1] + c[2] * U - c[3] * neo_pow_int(U, 6)) / 1000.0;
+    return U < p->high_wind_speed_threshold ? poly : p->high_wind_drag_coefficient;
+}
+
+/* stability_profile of the SplitStabilityFunction(LinearStable, Paulson) pairs of
+ * large_yeager_stability_functions (similarity_theory_turbulent_fluxes.jl:772-778):
+ * LinearStableStabilityFunction (:742-752) for ζ > 0, Paulson (:688-710) otherwise.
+ * scalar = 0: momentum, 1: temperature / water vapour. */
+double neo_psi_large_yeager(const ne_params* p, int scalar, double zeta) {
+    const double* ls = scalar ? p->linear_stable_scalar : p->linear_stable_momentum;
+    double zp = jmax(0.0, zeta);
+    double stable = -ls[0] * jmin(zp, ls[1]);
+    double a = scalar ? p->paulson_scalar : p->paulson_momentum[0];
+    double zm = jmin(0.0, zeta);
+    double z = sqrt(sqrt(1.0 - a * zm));
+    double unstable;
+    if (scalar) {
+        unstable = 2.0 * log((1.0 + z * z) / 2.0);
+    } else {
+        double psi1 = 2.0 * log((1.0 + z) / 2.0);
+        double psi2 = log((1.0 + z * z) / 2.0);
+        double psi3 = -2.0 * atan(z);
+        unstable = psi1 + psi2 + psi3 + p->paulson_momentum[1];
+    }
+    return zeta > 0.0 ? stable : unstable;
+}
+
+/* evaluate_coefficients(::LargeYeagerTransferCoefficients) (:288-340). us, ts, qs: the
+ * previous iterate's scales. */
+void neo_large_yeager_coefficients(const ne_params* p, double dU, double Ts, double qsurf, double dh,
+                                   double us, double ts, double qs, double C[3]) {
+    double kappa = p->large_yeager_von_karman_constant;
+    double g = p->gravitational_acceleration;
+    double h0 = p->large_yeager_reference_height;
+    double Umin = p->minimum_wind_speed;
+    dU = jmax(dU, Umin);
+    /* buoyancy_scale (similarity:417-425) */
+    double Tv = neo_virtual_temperature(p, Ts, qsurf);
+    double delta = Rv_over_Rd(p) - 1.0;
+    double bstar = g / Tv * (ts * (1.0 + delta * qsurf) + delta * Tv * qs);
+    double L = bstar == 0.0 ? INFINITY : us * us / (kappa * bstar);
+    double zeta = dh / L;
+    double psim = neo_psi_large_yeager(p, 0, zeta);
+    double psih = neo_psi_large_yeager(p, 1, zeta);
+    double Cdp = us == 0.0 ? neo_neutral_drag(p, dU) : us * us / (dU * dU);
+    double UN10 = dU / (1.0 + sqrt(Cdp) / kappa * (log(dh / h0) - psim));
+    UN10 = jmax(UN10, Umin);
+    double CdN = neo_neutral_drag(p, UN10);
+    int stable = zeta > 0.0;
+    double ChN = sqrt(CdN) / 1000.0 *
+                 (stable ? p->stable_heat_transfer_coefficient : p->unstable_heat_transfer_coefficient);
+    double CqN = sqrt(CdN) / 1000.0 * p->moisture_transfer_coefficient;
+    double xim = sqrt(CdN) / kappa * (log(dh / h0) - psim);
+    double Cd = CdN / ((1.0 + xim) * (1.0 + xim));
+    double xih = sqrt(CdN) / kappa * (log(dh / h0) - psih);
+    double ratio = sqrt(Cd) / sqrt(CdN);
+    C[0] = Cd;
+    C[1] = ChN * ratio / (1.0 + ChN * xih);
+    C[2] = CqN * ratio / (1.0 + CqN * xih);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Interface state iteration (compute_interface_state.jl)                    */
+/* ------------------------------------------------------------------------- */
+
 typedef struct {
     double us, ts, qs;   /* u★ θ★ q★ */
     double u, v;         /* interface velocities */
@@ -213,26 +316,61 @@ typedef struct {
 
 typedef struct { double z, u, v, T, p, q, hbl; } atmos_state;
 
-/* iterate_interface_state (compute_interface_state.jl:69-122) with BulkTemperature
- * (interface_states.jl:333), ImpureSaturationSpecificHumidity (:587-588) and
- * iterate_interface_fluxes(::SimilarityTheoryFluxes) (similarity:315-385). */
-static iface_state iterate_state(const ne_params* p, const atmos_state* A, iface_state s) {
-    double g = p->gravitational_acceleration;
-    double Ts = s.T;
-    double qs = neo_surface_specific_humidity(p, A->p, Ts);
-    double dq = A->q - qs;
-    double thetaa = A->T + g * A->z / neo_cp_m(p, A->q); /* interface_states.jl:308-317 */
-    double dtheta = thetaa - Ts;
-    double dh = A->z;
+/* air_sea_interface_radiation_state (Radiations/air_sea_interface_radiation_state.jl:8-33):
+ * zero when there is no radiation. */
+typedef struct { double sigma, alpha, eps, SW, LW; } rad_state;
 
-    double ustar = s.us, tstar = s.ts, qstar = s.qs;
+/* surface_atmosphere_temperature (interface_states.jl:308-317) */
+static double surface_atmosphere_temperature(const ne_params* p, const atmos_state* A) {
+    return A->T + p->gravitational_acceleration * A->z / neo_cp_m(p, A->q);
+}
+
+/* compute_interface_temperature(::SkinTemperature) (interface_states.jl:536-577) with
+ * flux_balance_temperature(::SkinTemperature{<:DiffusiveFlux}) (:425-448) for a constant
+ * diffusivity. Ti: interior (top-cell) temperature in K. */
+static double skin_temperature(const ne_params* p, const atmos_state* A, const rad_state* R, double Ti,
+                               const iface_state* s) {
+    double rhoa = neo_air_density(p, A->T, A->p, A->q);
+    double ca = neo_cp_m(p, A->q);
+    double Li = neo_latent_heat_sublim(p, A->T);
+    double Tm = s->T;
+    double Iup = R->sigma * R->eps * neo_pow_int(Tm, 4);
+    double Qd = -(1.0 - R->alpha) * R->SW - R->eps * R->LW;
+    double QT = -rhoa * ca * s->us * s->ts;
+    double Qv = -rhoa * Li * s->us * s->qs;
+    /* flux_balance_temperature */
+    double kappa = p->skin_diffusivity;
+    double delta = p->skin_depth;
+    double rho = p->ocean_reference_density;
+    double c = p->ocean_heat_capacity;
+    double Qa = Qv + Iup + Qd;
+    double lambda = 1.0 / (rho * c);
+    double JT = Qa * lambda;
+    double Ta = surface_atmosphere_temperature(p, A);
+    double dT = Ta - Tm;
+    double OmT = QT * lambda;
+    double D = kappa * dT - OmT * delta;
+    double Tstar = (Ti * kappa * dT - (JT * dT + OmT * Ta) * delta) / D;
+    Tstar = D == 0.0 ? Tm : Tstar;
+    double maxdT = p->skin_max_temperature_change;
+    /* clamp(x, lo, hi) = x > hi ? hi : x < lo ? lo : x (Julia Base) */
+    double x = Tstar - Ti;
+    double cl = x > maxdT ? maxdT : (x < -maxdT ? -maxdT : x);
+    return Ti + cl;
+}
+
+/* iterate_interface_fluxes(::SimilarityTheoryFluxes) (similarity:315-385) */
+static void similarity_scales(const ne_params* p, const atmos_state* A, const iface_state* s, double Ts,
+                              double qs, double dtheta, double dq, double dh, double out[3]) {
+    double g = p->gravitational_acceleration;
+    double ustar = s->us, tstar = s->ts, qstar = s->qs;
     /* buoyancy_scale (similarity:417-425) */
     double Tv = neo_virtual_temperature(p, Ts, qs);
     double delta = Rv_over_Rd(p) - 1.0;
     double bstar = g / Tv * (tstar * (1.0 + delta * qs) + delta * Tv * qstar);
     double Usg2 = neo_gustiness_squared(p, ustar, bstar, A->hbl);
     double du, dv;
-    if (p->velocity_difference == NE_VELOCITY_RELATIVE) { du = A->u - s.u; dv = A->v - s.v; }
+    if (p->velocity_difference == NE_VELOCITY_RELATIVE) { du = A->u - s->u; dv = A->v - s->v; }
     else { du = A->u; dv = A->v; }
     double U = sqrt(du * du + dv * dv + Usg2);
     double lu = neo_momentum_roughness(p, ustar);
@@ -244,23 +382,76 @@ static iface_state iterate_state(const ne_params* p, const atmos_state* A, iface
     double chiu = kappa / similarity_profile(p, 0, dh, lu, L);
     double chit = kappa / similarity_profile(p, 1, dh, lt, L);
     double chiq = kappa / similarity_profile(p, 1, dh, lq, L);
+    out[0] = chiu * U;
+    out[1] = chit * dtheta;
+    out[2] = chiq * dq;
+}
+
+/* iterate_interface_fluxes(::CoefficientBasedFluxes) (coefficient_based_turbulent_fluxes.jl:346-371)
+ * with evaluate_coefficients (:269-340). */
+static void coefficient_scales(const ne_params* p, const atmos_state* A, const iface_state* s, double Ts,
+                               double qs, double dtheta, double dq, double dh, double out[3]) {
+    double du, dv;
+    if (p->velocity_difference == NE_VELOCITY_RELATIVE) { du = A->u - s->u; dv = A->v - s->v; }
+    else { du = A->u; dv = A->v; }
+    int ly = p->transfer_coefficients == NE_TRANSFER_LARGE_YEAGER;
+    double Umin = ly ? p->minimum_wind_speed : 0.0; /* minimum_wind_speed (:265-267) */
+    double dU = jmax(sqrt(du * du + dv * dv), Umin);
+    double C[3];
+    if (ly) {
+        neo_large_yeager_coefficients(p, dU, Ts, qs, dh, s->us, s->ts, s->qs, C);
+    } else {
+        for (int n = 0; n < 3; n++)
+            C[n] = p->transfer_kind[n] == NE_COEFFICIENT_POLYNOMIAL ? neo_neutral_drag(p, dU)
+                                                                    : p->transfer_coefficient[n];
+    }
+    double Cd = C[0];
+    out[0] = sqrt(Cd) * dU;
+    out[1] = Cd == 0.0 ? 0.0 : C[1] / sqrt(Cd) * dtheta;
+    out[2] = Cd == 0.0 ? 0.0 : C[2] / sqrt(Cd) * dq;
+}
+
+/* iterate_interface_state (compute_interface_state.jl:69-122) with BulkTemperature
+ * (interface_states.jl:333) or SkinTemperature, ImpureSaturationSpecificHumidity
+ * (:587-588) or a fixed humidity, and the similarity-theory or coefficient-based fluxes. */
+static iface_state iterate_state(const ne_params* p, const atmos_state* A, const rad_state* R, double Ti,
+                                 iface_state s) {
+    double Ts = p->interface_temperature == NE_TEMPERATURE_SKIN ? skin_temperature(p, A, R, Ti, &s) : s.T;
+    double qs = neo_surface_specific_humidity(p, A->p, Ts);
+    double dq = A->q - qs;
+    double thetaa = surface_atmosphere_temperature(p, A);
+    double dtheta = thetaa - Ts;
+    double dh = A->z;
+    double scales[3];
+    if (p->turbulent_fluxes == NE_FLUXES_COEFFICIENT_BASED)
+        coefficient_scales(p, A, &s, Ts, qs, dtheta, dq, dh, scales);
+    else
+        similarity_scales(p, A, &s, Ts, qs, dtheta, dq, dh, scales);
     iface_state n = s;
-    n.us = chiu * U;
-    n.ts = chit * dtheta;
-    n.qs = chiq * dq;
+    n.us = scales[0];
+    n.ts = scales[1];
+    n.qs = scales[2];
     n.T = Ts;
     n.q = qs;
     return n;
 }
 
 /* One column of _compute_atmosphere_ocean_interface_state! (atmosphere_ocean_fluxes.jl:80-197). */
-int neo_flux_column(const ne_params* p, const double in[9], int inactive, double out[9]) {
+int neo_flux_column(const ne_params* p, const double in[11], int inactive, double out[9]) {
     atmos_state A;
     A.u = in[0]; A.v = in[1]; A.T = in[2]; A.p = in[3]; A.q = in[4];
     A.z = p->surface_layer_height;
     A.hbl = p->boundary_layer_height;
     double To = in[5] + CELSIUS_TO_KELVIN; /* convert_to_kelvin, components.jl:11 */
     double uo = in[7], vo = in[8];
+    rad_state R = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (p->radiation) {
+        R.sigma = p->stefan_boltzmann_constant;
+        R.alpha = p->ocean_albedo;
+        R.eps = p->ocean_emissivity;
+        R.SW = in[9];
+        R.LW = in[10];
+    }
 
     double qs0 = neo_surface_specific_humidity(p, A.p, To);
     iface_state s0 = {1e-4, 1e-4, 1e-4, uo, vo, To, qs0};
@@ -286,7 +477,7 @@ int neo_flux_column(const ne_params* p, const double in[9], int inactive, double
             }
             if (!iterating) break;
             prev = s;
-            s = iterate_state(p, &A, prev);
+            s = iterate_state(p, &A, &R, To, prev);
             it += 1;
         }
     }
@@ -437,7 +628,7 @@ void neo_ao_flux_solve(const ne_grid* g, const ne_params* p, const ne_exchange_s
 #pragma omp parallel for schedule(dynamic, 4) NEO_THREADS
     for (int j = 0; j < g->Ny; j++) {
         for (int i = 0; i < g->Nx; i++) {
-            double in[9], out[9];
+            double in[11], out[9];
             in[0] = V2(a->u, i, j); in[1] = V2(a->v, i, j); in[2] = V2(a->T, i, j);
             in[3] = V2(a->p, i, j); in[4] = V2(a->q, i, j);
             in[5] = V3(o->T, i, j, kN);
@@ -447,6 +638,8 @@ void neo_ao_flux_solve(const ne_grid* g, const ne_params* p, const ne_exchange_s
             int jp = g->topology_y == NE_PERIODIC ? wrap(j + 1, g->Ny) : j + 1;
             in[7] = (V3(o->u, i, j, kN) + V3(o->u, ip, j, kN)) / 2.0;
             in[8] = (V3(o->v, i, j, kN) + V3(o->v, i, jp, kN)) / 2.0;
+            in[9] = a->downwelling_shortwave.data ? V2(a->downwelling_shortwave, i, j) : 0.0;
+            in[10] = a->downwelling_longwave.data ? V2(a->downwelling_longwave, i, j) : 0.0;
             int inactive = !active(g, i, j, kN);
             int it = neo_flux_column(p, in, inactive, out);
             V2(f->latent_heat, i, j) = out[0];
@@ -804,6 +997,10 @@ void neo_model_update_state(neo_model* m, double dt_frazil) {
     if (m->params.radiation)
         neo_interp_radiation(g, &m->radiation, m->radiation_fi, m->radiation_fj, m->sw, m->lw);
     neo_interp_atmos(g, &m->atmosphere, m->atmos_fi, m->atmos_fj, &m->exchange);
+    /* the radiation exchanger's state, read by the SkinTemperature balance */
+    ne_view none = {NULL, 0, 0, 0};
+    m->exchange.downwelling_shortwave = m->params.radiation ? m->sw : none;
+    m->exchange.downwelling_longwave = m->params.radiation ? m->lw : none;
     neo_ao_flux_solve(g, &m->params, &m->exchange, &m->ocean, &m->fluxes);
     neo_freeze_clamp(g, &m->params, m->ocean.T, m->ocean.S, m->frazil_heat, dt_frazil);
     neo_assemble_net_ocean_fluxes(g, &m->params, &m->fluxes, &m->exchange, m->ocean.T, &m->net);

@@ -31,9 +31,20 @@ double neo_gustiness_squared(const ne_params* p, double ustar, double bstar, dou
 double neo_momentum_roughness(const ne_params* p, double ustar);
 double neo_scalar_roughness(const ne_params* p, double lu, double ustar, int which);
 
-/* One column of the flux solve. in: ua va Ta pa qa To_C So uo vo; inactive flag.
- * out[9]: Qv Qt Jv rho_tau_x rho_tau_y ustar thetastar qstar Ts_C. Returns iterations. */
-int neo_flux_column(const ne_params* p, const double in[9], int inactive, double out[9]);
+/* Coefficient-based fluxes: the polynomial neutral drag, the Large-Yeager stability
+ * profiles (scalar = 0 momentum, 1 scalar) and transfer coefficients C = (Cd, Ch, Cq). */
+double neo_neutral_drag(const ne_params* p, double U);
+double neo_psi_large_yeager(const ne_params* p, int scalar, double zeta);
+void neo_large_yeager_coefficients(const ne_params* p, double dU, double Ts, double qsurf, double dh,
+                                   double us, double ts, double qs, double C[3]);
+double neo_latent_heat_sublim(const ne_params* p, double T);
+double neo_pow_int(double x, int n);
+
+/* One column of the flux solve. in: ua va Ta pa qa To_C So uo vo SW LW (the downwelling
+ * radiation, read by the skin-temperature balance when params->radiation is set);
+ * inactive flag. out[9]: Qv Qt Jv rho_tau_x rho_tau_y ustar thetastar qstar Ts_C.
+ * Returns iterations. */
+int neo_flux_column(const ne_params* p, const double in[11], int inactive, double out[9]);
 
 /* Phase restatements over a grid (host views). */
 void neo_immersed_levels(const ne_grid* g, ne_view bottom_height, int32_t* kmin);

@@ -59,6 +59,15 @@ def lib():
         for name in ("neo_psi_momentum", "neo_psi_scalar", "neo_momentum_roughness"):
             getattr(L, name).argtypes = [P(abi.Params), C.c_double]
             getattr(L, name).restype = C.c_double
+        L.neo_latent_heat_sublim.argtypes = [P(abi.Params), C.c_double]
+        L.neo_latent_heat_sublim.restype = C.c_double
+        L.neo_neutral_drag.argtypes = [P(abi.Params), C.c_double]
+        L.neo_neutral_drag.restype = C.c_double
+        L.neo_psi_large_yeager.argtypes = [P(abi.Params), C.c_int, C.c_double]
+        L.neo_psi_large_yeager.restype = C.c_double
+        L.neo_large_yeager_coefficients.argtypes = [P(abi.Params)] + [C.c_double] * 7 + [P(C.c_double)]
+        L.neo_pow_int.argtypes = [C.c_double, C.c_int]
+        L.neo_pow_int.restype = C.c_double
         L.neo_gustiness_squared.argtypes = [P(abi.Params), C.c_double, C.c_double, C.c_double]
         L.neo_gustiness_squared.restype = C.c_double
         L.neo_scalar_roughness.argtypes = [P(abi.Params), C.c_double, C.c_double, C.c_int]
@@ -199,6 +208,9 @@ class OracleModel:
         self.exchange_arrays = {k: z2() for k in EXCHANGE_NAMES}
         self.exchange = abi.ExchangeState(*(np_view(self.exchange_arrays[k]) for k in EXCHANGE_NAMES))
         self.sw, self.lw = z2(), z2()
+        if self.params.radiation:  # the radiation exchanger's state, read by the skin-temperature balance
+            self.exchange.downwelling_shortwave = np_view(self.sw)
+            self.exchange.downwelling_longwave = np_view(self.lw)
         self.flux_arrays = {k: z2() for k in FLUX_NAMES}
         self.iterations = K(np.zeros((Ny, Nx), np.int32))
         self.fluxes = abi.InterfaceFluxes(*(np_view(self.flux_arrays[k]) for k in FLUX_NAMES), np_view(self.iterations))

@@ -23,9 +23,10 @@ def _specs(n=8, seed=5):
     return specs
 
 
-def _build(specs, precision, general=(), n_times=1, two_color=()):
+def _build(specs, precision, general=(), n_times=1, two_color=(), large_yeager=(), skin=()):
     """Members; those whose index is in `general` get a non-default Edson constant (general flux path),
-    those in `two_color` the TwoColorRadiation forcing."""
+    those in `two_color` the TwoColorRadiation forcing, those in `large_yeager` the Large-Yeager
+    coefficient-based fluxes, those in `skin` the skin temperature."""
     from numericalearth_hip import _ctypes_abi as abi_
     out = []
     for k, s in enumerate(specs):
@@ -34,6 +35,13 @@ def _build(specs, precision, general=(), n_times=1, two_color=()):
             p.edson_momentum[0] = 45.0
         if k in two_color:
             p.radiative_forcing = abi_.NE_RADIATIVE_FORCING_TWO_COLOR
+        if k in large_yeager:
+            p.turbulent_fluxes = abi_.NE_FLUXES_COEFFICIENT_BASED
+            p.transfer_coefficients = abi_.NE_TRANSFER_LARGE_YEAGER
+            p.maxiter = 20
+        if k in skin:
+            p.interface_temperature = abi_.NE_TEMPERATURE_SKIN
+            p.skin_diffusivity, p.skin_depth = 1e-4, 1.0
         case = member_case(s) if n_times == 1 else _case_times(s, n_times)
         out.append(OceanOnlyModel(case, params=p, precision=precision))
     return out
@@ -67,8 +75,9 @@ def _assert_same(a_models, b_models, what):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_batch_equals_members_stepped_alone(require_gpu, precision):
     specs = _specs()
-    alone = _build(specs, precision, general=(2,), two_color=(4, 5))
-    batched = _build(specs, precision, general=(2,), two_color=(4, 5))
+    mix = dict(general=(2,), two_color=(4, 5), large_yeager=(1, 6), skin=(6, 7))
+    alone = _build(specs, precision, **mix)
+    batched = _build(specs, precision, **mix)
     ens = Ensemble(batched)
     print("[batch]", ens.info(), ens.tuning, flush=True)
     for _ in range(3):

@@ -18,8 +18,10 @@ import oracle as O
 from numericalearth_hip import _ctypes_abi as abi
 from numericalearth_hip.generators import make_case
 from numericalearth_hip.model import OceanOnlyModel, Phases
-from numericalearth_hip.params import (ComponentInterfaces, FixedIterations, FixedSpecificHumidity,
-                                       SimilarityTheoryFluxes, to_params)
+from numericalearth_hip.params import (CoefficientBasedFluxes, ComponentInterfaces, DiffusiveFlux, FixedIterations,
+                                       FixedSpecificHumidity, LargeYeagerTransferCoefficients,
+                                       PolynomialNeutralDragCoefficient, SimilarityScales, SimilarityTheoryFluxes,
+                                       SkinTemperature, to_params)
 
 pytestmark = pytest.mark.gpu
 
@@ -121,6 +123,61 @@ def test_flux_solve_neutral_and_fixed_humidity(require_gpu):
     case, om, gm = _pair(params=p, seed=4)
     _solve_both(case, om, gm)
     _compare_fluxes(gm, om, max_iteration_mismatch_frac=0.0)
+
+
+COEFFICIENT_FORMS = {
+    "constant": CoefficientBasedFluxes(transfer_coefficients=(1.3e-3, 1.1e-3, 1.2e-3)),
+    "polynomial": CoefficientBasedFluxes(
+        transfer_coefficients=SimilarityScales(PolynomialNeutralDragCoefficient(), 1e-3, 1.2e-3)),
+    "large_yeager": CoefficientBasedFluxes(transfer_coefficients=LargeYeagerTransferCoefficients()),
+    "large_yeager_fixed5": CoefficientBasedFluxes(transfer_coefficients=LargeYeagerTransferCoefficients(),
+                                                  solver_stop_criteria=FixedIterations(5)),
+}
+
+
+@pytest.mark.parametrize("form", sorted(COEFFICIENT_FORMS))
+def test_flux_solve_coefficient_based(require_gpu, form):
+    p = to_params(COEFFICIENT_FORMS[form])
+    case, om, gm = _pair(params=p, seed=21)
+    _solve_both(case, om, gm)
+    _compare_fluxes(gm, om)
+
+
+SKIN = ComponentInterfaces(atmosphere_ocean_interface_temperature=SkinTemperature(DiffusiveFlux(1e-4, 1.0)))
+
+
+@pytest.mark.parametrize("form", ["similarity", "large_yeager"])
+@pytest.mark.parametrize("radiation", [True, False])
+def test_flux_solve_skin_temperature(require_gpu, form, radiation):
+    fl = SimilarityTheoryFluxes() if form == "similarity" else COEFFICIENT_FORMS["large_yeager"]
+    p = to_params(fl, SKIN)
+    case, om, gm = _pair(params=p, seed=22, radiation=radiation)
+    _solve_both(case, om, gm)
+    _compare_fluxes(gm, om)
+    Ts, To = _np(gm.fluxes["interface_temperature"]), case.T[-1]
+    wet = case.grid.kmin < case.grid.spec.Nz
+    assert np.any(Ts[wet] != To[wet]) and np.all(np.abs(Ts[wet] - To[wet]) <= 5.0)
+
+
+@pytest.mark.parametrize("form", ["large_yeager_skin", "constant"])
+def test_model_coefficient_steps_match_oracle(require_gpu, form):
+    """The fused step (solve with the radiation interpolated in K1 for the skin balance) over 3 steps."""
+    p = (to_params(COEFFICIENT_FORMS["large_yeager"], SKIN) if form == "large_yeager_skin"
+         else to_params(COEFFICIENT_FORMS["constant"]))
+    case, om, gm = _pair(seed=23, params=p)
+    om.reconcile()
+    torch.cuda.synchronize()
+    _compare_fluxes(gm, om)
+    for step in range(3):
+        om.time_step(1200.0)
+        gm.time_step(1200.0)
+        torch.cuda.synchronize()
+        assert_rel(_np(gm.T), om.T, RTOL, "T")
+        assert_rel(_np(gm.S), om.S, RTOL, "S")
+        _compare_fluxes(gm, om)
+        scales = _net_term_scales(case, om)
+        for k in O.NET_NAMES:
+            assert_rel_terms(_np(gm.net[k]), om.net_arrays[k], scales.get(k, 0.0), RTOL, k)
 
 
 def test_assembly_radiation_freeze_bit_exact(require_gpu):

@@ -53,8 +53,8 @@ def test_mahrt_sun():
     assert mahrt_sun_subgrid_velocity(25e3) > mahrt_sun_subgrid_velocity(10e3) > 0
 
 
-def _column(p, ua, va, Ta, pa, qa, ToC, So=30.0, uo=0.0, vo=0.0, inactive=0):
-    inp = (C.c_double * 9)(ua, va, Ta, pa, qa, ToC, So, uo, vo)
+def _column(p, ua, va, Ta, pa, qa, ToC, So=30.0, uo=0.0, vo=0.0, inactive=0, SW=0.0, LW=0.0):
+    inp = (C.c_double * 11)(ua, va, Ta, pa, qa, ToC, So, uo, vo, SW, LW)
     out = (C.c_double * 9)()
     it = L.neo_flux_column(C.byref(p), inp, inactive, out)
     return list(out), it

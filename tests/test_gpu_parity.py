@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import assert_rel, assert_rel_terms, rel_err
+from helpers import assert_rel, assert_rel_envelope, assert_rel_terms, rel_err
 
 import oracle as O
 from numericalearth_hip import _ctypes_abi as abi
@@ -86,7 +86,13 @@ def _solve_both(case, om, gm):
     return ph
 
 
-def _compare_fluxes(gm, om, rtol=RTOL, max_iteration_mismatch_frac=1e-4):
+# Amplification allowed over the oracle's own one-ulp response (see _flux_envelope).
+ENVELOPE_FACTOR = 64.0
+
+
+def _compare_fluxes(gm, om, rtol=RTOL, max_iteration_mismatch_frac=1e-4, envelope=None):
+    """GPU interface fluxes against the oracle's: elementwise within rtol (and rtol of the field's
+    largest value), or, with an envelope, within max(rtol·|b|, ENVELOPE_FACTOR·envelope)."""
     it_g = _np(gm.iterations)
     it_o = om.iterations
     mismatch = it_g != it_o
@@ -98,8 +104,53 @@ def _compare_fluxes(gm, om, rtol=RTOL, max_iteration_mismatch_frac=1e-4):
     for k in O.FLUX_NAMES:
         a = _np(gm.fluxes[k])[ok]
         b = om.flux_arrays[k][ok]
-        worst[k] = assert_rel(a, b, rtol, k)
+        if envelope is None:
+            worst[k] = assert_rel(a, b, rtol, k)
+        else:
+            worst[k] = assert_rel_envelope(a, b, ENVELOPE_FACTOR * envelope[k][ok], rtol, k)
     print("[flux] worst relative errors:", {k: f"{v[0]:.2e}" for k, v in worst.items()}, flush=True)
+
+
+def _oracle_on_state(om, T, S, time=None, nudge=0):
+    """A fresh oracle model with om's case and parameters holding the state (T, S): atmosphere and
+    radiation interpolated at `time` (om's by default), the air temperature moved by `nudge` ulps
+    (±1), and the fluxes solved."""
+    o = O.OracleModel(om.case, params=abi.Params.from_buffer_copy(om.params), radiation=bool(om.params.radiation))
+    o.T[...] = T
+    o.S[...] = S
+    o.time = om.time if time is None else time
+    o.interp()
+    if nudge:
+        Ta = o.exchange_arrays["T"]
+        Ta[...] = np.nextafter(Ta, np.inf if nudge > 0 else -np.inf)
+    o.flux_solve()
+    return o
+
+
+def _flux_envelope(om, T=None, S=None):
+    """Per-element response of the oracle's column solve to a one-ulp change of the air temperature,
+    up and down. Where θₐ − Tₛ nearly cancels (a column with almost no air-sea temperature
+    difference), or the skin balance's denominator κΔT − Ωᵀδ does, any one-ulp difference inside the
+    column (a transcendental's last bit) is amplified by the same factor; this measures it."""
+    T = om.T if T is None else T
+    S = om.S if S is None else S
+    base = _oracle_on_state(om, T, S)
+    env = {k: np.zeros_like(v) for k, v in base.flux_arrays.items()}
+    for nudge in (1, -1):
+        o = _oracle_on_state(om, T, S, nudge=nudge)
+        for k in env:
+            env[k] = np.maximum(env[k], np.abs(o.flux_arrays[k] - base.flux_arrays[k]))
+    return env
+
+
+def _compare_fluxes_on_gpu_state(gm, om, envelope=False):
+    """The GPU's fluxes after a step against the oracle's flux solve of the GPU's own stepped state:
+    the two steps' states differ in the last bits, and a column with almost no air-sea temperature
+    difference turns that into a large relative difference of its sensible heat, which is not a
+    difference of the flux solve."""
+    T, S = _np(gm.T).astype(np.float64), _np(gm.S).astype(np.float64)
+    ref = _oracle_on_state(om, T, S)
+    _compare_fluxes(gm, ref, envelope=_flux_envelope(om, T, S) if envelope else None)
 
 
 def test_flux_solve_matches_oracle(require_gpu):
@@ -140,7 +191,8 @@ def test_flux_solve_coefficient_based(require_gpu, form):
     p = to_params(COEFFICIENT_FORMS[form])
     case, om, gm = _pair(params=p, seed=21)
     _solve_both(case, om, gm)
-    _compare_fluxes(gm, om)
+    # Large-Yeager columns that do not converge within maxiter amplify last-bit differences
+    _compare_fluxes(gm, om, envelope=_flux_envelope(om) if form.startswith("large_yeager") else None)
 
 
 SKIN = ComponentInterfaces(atmosphere_ocean_interface_temperature=SkinTemperature(DiffusiveFlux(1e-4, 1.0)))
@@ -153,7 +205,7 @@ def test_flux_solve_skin_temperature(require_gpu, form, radiation):
     p = to_params(fl, SKIN)
     case, om, gm = _pair(params=p, seed=22, radiation=radiation)
     _solve_both(case, om, gm)
-    _compare_fluxes(gm, om)
+    _compare_fluxes(gm, om, envelope=_flux_envelope(om))
     Ts, To = _np(gm.fluxes["interface_temperature"]), case.T[-1]
     wet = case.grid.kmin < case.grid.spec.Nz
     assert np.any(Ts[wet] != To[wet]) and np.all(np.abs(Ts[wet] - To[wet]) <= 5.0)
@@ -165,16 +217,17 @@ def test_model_coefficient_steps_match_oracle(require_gpu, form):
     p = (to_params(COEFFICIENT_FORMS["large_yeager"], SKIN) if form == "large_yeager_skin"
          else to_params(COEFFICIENT_FORMS["constant"]))
     case, om, gm = _pair(seed=23, params=p)
+    skin = form == "large_yeager_skin"
     om.reconcile()
     torch.cuda.synchronize()
-    _compare_fluxes(gm, om)
+    _compare_fluxes(gm, om, envelope=_flux_envelope(om, case.T, case.S) if skin else None)
     for step in range(3):
         om.time_step(1200.0)
         gm.time_step(1200.0)
         torch.cuda.synchronize()
         assert_rel(_np(gm.T), om.T, RTOL, "T")
         assert_rel(_np(gm.S), om.S, RTOL, "S")
-        _compare_fluxes(gm, om)
+        _compare_fluxes_on_gpu_state(gm, om, envelope=skin)
         scales = _net_term_scales(case, om)
         for k in O.NET_NAMES:
             assert_rel_terms(_np(gm.net[k]), om.net_arrays[k], scales.get(k, 0.0), RTOL, k)
@@ -258,7 +311,7 @@ def test_full_size_snapshot_matches_oracle(require_gpu):
     torch.cuda.synchronize()
     assert_rel(_np(gm.T), om.T, RTOL, "T")
     assert_rel(_np(gm.S), om.S, RTOL, "S")
-    _compare_fluxes(gm, om)
+    _compare_fluxes_on_gpu_state(gm, om)
     scales = _net_term_scales(case, om)
     for k in O.NET_NAMES:
         assert_rel_terms(_np(gm.net[k]), om.net_arrays[k], scales.get(k, 0.0), RTOL, k)
@@ -295,7 +348,7 @@ def test_model_two_color_steps_match_oracle(require_gpu):
         torch.cuda.synchronize()
         assert_rel(_np(gm.T), om.T, RTOL, "T")
         assert_rel(_np(gm.S), om.S, RTOL, "S")
-        _compare_fluxes(gm, om)
+        _compare_fluxes_on_gpu_state(gm, om)
         scales = _net_term_scales(case, om)
         for k in O.NET_NAMES:
             assert_rel_terms(_np(gm.net[k]), om.net_arrays[k], scales.get(k, 0.0), RTOL, k)
@@ -333,7 +386,7 @@ def test_model_steps_match_oracle(require_gpu, precision):
         print(f"[model] step {step + 1}", flush=True)
         assert_rel(_np(gm.T), om.T, RTOL, "T")
         assert_rel(_np(gm.S), om.S, RTOL, "S")
-        _compare_fluxes(gm, om)
+        _compare_fluxes_on_gpu_state(gm, om)
         scales = _net_term_scales(case, om)
         for k in O.NET_NAMES:
             assert_rel_terms(_np(gm.net[k]), om.net_arrays[k], scales.get(k, 0.0), RTOL, k)

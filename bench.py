@@ -19,6 +19,7 @@ sys.path.insert(0, str(REPO / "numericalearth.jl_amd"))
 
 METRIC = "cell-updates/s & %HBM peak at 1440×560×10 (FP64 vs FP32); p99 latency batched"
 HBM_PEAK_GBS = 8000.0
+FP64_VALU_PEAK_TFLOPS = 78.6   # MI355X dense FP64 vector peak
 WORKLOADS = {
     "quarter_degree_1440x560x10": (1440, 560, 10),
     "cpu_reference_90x40x10": (90, 40, 10),
@@ -63,6 +64,22 @@ def measured_traffic(workload: str, precision: str):
         return None
     m = re.search(rf"traffic_bytes_per_launch\[{precision}\] = ([0-9.eE+-]+)", f.read_text())
     return float(m.group(1)) if m else None
+
+
+def measured_valu(workload: str, precision: str):
+    """Issued FP64 lane-flops per flux-solve launch and the VALU-busy fraction of the flux-solve
+    and stage kernels, from the committed SQ counter summary, or None."""
+    import re
+    f = REPO / "profiles" / "surface_solve_and_stage_valu_1440x560x10.md"
+    if workload != "quarter_degree_1440x560x10" or not f.exists():
+        return None
+    text = f.read_text()
+    out = {}
+    for key in ("fp64_lane_flops_per_launch", "valu_busy_surface_solve", "valu_busy_rk_stage"):
+        m = re.search(rf"{key}\[{precision}\] = ([0-9.eE+-]+)", text)
+        if m:
+            out[key] = float(m.group(1))
+    return out or None
 
 
 def copy_bandwidth(gib: float, reps: int) -> dict:
@@ -355,6 +372,22 @@ def main():
                        "nonconverged_columns": counters["nonconverged_columns"],
                        "nan_columns": counters["nan_columns"]},
     }
+    valu = measured_valu(args.workload, args.precision)
+    if valu and args.precision == "fp64" and "fp64_lane_flops_per_launch" in valu:
+        # the flux solve is FP64-VALU bound: issued FP64 lane-flops (ADD, MUL = 1, FMA = 2 per lane,
+        # counted by SQ_INSTS_VALU_*_F64 over every issued wavefront instruction) per live launch time
+        solve_s = kernel_ms["surface_solve"] / nrec * 1e-3
+        tf = valu["fp64_lane_flops_per_launch"] / solve_s / 1e12
+        out["flux_solve"]["roofline"] = {"kernel": "surface_solve (interpolation + Monin-Obukhov fixed point)",
+                                         "bound": "fp64-valu", "achieved": tf, "peak": FP64_VALU_PEAK_TFLOPS,
+                                         "unit": "TFLOP/s", "frac": tf / FP64_VALU_PEAK_TFLOPS,
+                                         "valu_busy": valu.get("valu_busy_surface_solve")}
+    elif valu and "valu_busy_surface_solve" in valu:
+        # the mixed form's solve iterates in FP32 first: its bound is the VALU issue rate, not FP64 flops
+        out["flux_solve"]["roofline"] = {"kernel": "surface_solve (interpolation + Monin-Obukhov fixed point)",
+                                         "bound": "valu", "valu_busy": valu["valu_busy_surface_solve"]}
+    if valu and "valu_busy_rk_stage" in valu:
+        out["roofline"]["valu_busy"] = valu["valu_busy_rk_stage"]
     model.close()
     copy = copy_bandwidth(2.0, 20)
     out["roofline"]["measured_copy_gbs"] = copy["gbs"]

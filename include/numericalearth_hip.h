@@ -401,6 +401,16 @@ int ne_tracer_rk_stage_f64(ne_ctx* ctx, const ne_grid* grid, const ne_params* pa
                            ne_view T_out, ne_view S_out,
                            const ne_net_ocean_fluxes* net, double beta_dt, void* stream);
 
+/* Vertical velocity from continuity, for callers that evolve u and v: replaces Oceananigans'
+ * compute_w_from_continuity! of the HydrostaticFreeSurfaceModel that ocean_simulation builds
+ * (ocean_simulation.jl:487-499; Oceananigans not vendored). Per column, w = 0 at the bottom
+ * face and w[k+1] = w[k] − (δx(Ax u) + δy(Ay v))/Az on active cells with Ax = Δy·Δz,
+ * Ay = Δx·Δz of the cell's faces; faces above inactive cells carry w = 0. u and v are zero on
+ * immersed and peripheral faces (the model's convention, as Oceananigans masks them).
+ * Writes every face of w (Nz+1 levels). */
+int ne_compute_w_from_continuity_f64(ne_ctx* ctx, const ne_grid* grid, ne_view u, ne_view v, ne_view w,
+                                     void* stream);
+
 /* Compensated (Neumaier, double-double combine) sum over active cells of
  * weight·V·c, V = Az·Δz. Writes {sum, compensation} (2 doubles) to device `out`. */
 int ne_budget_f64(ne_ctx* ctx, const ne_grid* grid, ne_view c, double weight,
@@ -459,6 +469,11 @@ int ne_model_time_step(ne_model* model, double dt, void* stream);
 
 /* update_state! alone (phases 1-4). */
 int ne_model_update_state(ne_model* model, void* stream);
+
+/* Recompute the model's w from its u and v by continuity (ne_compute_w_from_continuity_f64,
+ * in the model's precision, accumulated in FP64), for callers that set new horizontal
+ * velocities between steps. */
+int ne_model_update_vertical_velocity(ne_model* model, void* stream);
 
 /* Clock (seconds) and iteration count of the model. Host only. */
 int ne_model_clock(const ne_model* model, double* time, int64_t* iteration);

@@ -418,6 +418,15 @@ int ne_tracer_rk_stage_f64(ne_ctx* c, const ne_grid* g, const ne_params* p, ne_v
                                            tracer_forcing(p, *net), beta_dt, false, 0.0, as_stream(stream)));
 }
 
+int ne_compute_w_from_continuity_f64(ne_ctx* c, const ne_grid* g, ne_view u, ne_view v, ne_view w, void* stream) {
+    if (!c) return NE_ERR_NULL;
+    int r;
+    if ((r = check_grid(g)) || (r = check_view3(u)) || (r = check_view3(v)) || (r = check_view3(w))) return r;
+    if (w.data == u.data || w.data == v.data) return NE_ERR_SHAPE;
+    return dev_err(launch_w_continuity<double>(grid_dev(*g), view<const double>(u), view<const double>(v),
+                                               view<double>(w), as_stream(stream)));
+}
+
 int ne_budget_f64(ne_ctx* c, const ne_grid* g, ne_view cv, double weight, double* out, void* stream) {
     if (!c || !out) return NE_ERR_NULL;
     int r;
@@ -647,6 +656,21 @@ int ne_model_update_state(ne_model* m, void* stream) {
     if (!m) return NE_ERR_NULL;
     if (m->batch) return NE_ERR_STATE;
     return model_update_state(m, m->iteration == 0 ? INFINITY : m->last_dt, as_stream(stream));
+}
+
+int ne_model_update_vertical_velocity(ne_model* m, void* stream) {
+    if (!m) return NE_ERR_NULL;
+    if (m->batch) return NE_ERR_STATE;
+    const ne_model_desc& d = m->d;
+    const GridDev g = grid_dev(d.grid);
+    hipError_t e;
+    if (d.precision == NE_FP64)
+        e = launch_w_continuity<double>(g, view<const double>(d.ocean.u), view<const double>(d.ocean.v),
+                                        view<double>(d.ocean.w), as_stream(stream));
+    else
+        e = launch_w_continuity<float>(g, view<const float>(d.ocean.u), view<const float>(d.ocean.v),
+                                       view<float>(d.ocean.w), as_stream(stream));
+    return dev_err(e);
 }
 
 int ne_model_clock(const ne_model* m, double* time, int64_t* iteration) {

@@ -157,6 +157,10 @@ hipError_t launch_tendency(const GridDev& g, View3<const double> u, View3<const 
                            View3<const double> T, View3<const double> S, View3<const double> JT,
                            View3<const double> JS, View3<const double> Jw, TracerForcing forcing, View3<double> GT,
                            View3<double> GS, hipStream_t s);
+// Vertical velocity from continuity (all Nz+1 faces of w).
+template <typename Real>
+hipError_t launch_w_continuity(const GridDev& g, View3<const Real> u, View3<const Real> v, View3<Real> w,
+                               hipStream_t s);
 template <typename Real>
 hipError_t launch_budget(ne_ctx* ctx, const GridDev& g, View3<const Real> c, double weight, double* out,
                          hipStream_t s);

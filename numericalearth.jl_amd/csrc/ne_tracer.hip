@@ -1,15 +1,18 @@
 // ne_tracer.hip — WENO5 tracer-advection tendency on the immersed LatLon grid, fused
 // with the top flux boundary condition and the split Runge-Kutta stage update.
 //
-// Tiling (gfx950): a block of 512 work-items = 8 wavefronts owns a tile of 63 × 8
-// columns and marches up the active levels. At each level the tile's T and S plane
-// plus a 3-cell halo (69 × 14 cells) is staged in LDS; every x- and y-face flux of the
-// tile is reconstructed once (lane ℓ of wavefront r owns the west face of column ℓ and
-// the south face of row r, lane 63 only a face), written to LDS, and differenced by the
-// owning cell. The vertical face flux at the top of level k is carried in registers as
-// the bottom flux of level k+1, with the column's vertical stencil in a register window.
-// Tiles are ordered so that each XCD (blocks are dealt round-robin to the 8 XCDs)
-// receives a contiguous band of tiles, keeping halo re-reads inside one L2.
+// Tiling (gfx950): a block of 64·TR work-items = TR wavefronts (TR = 4 for one model) owns
+// a tile of 63 × TR columns and marches up the active levels. At each level the tile's T
+// and S plane plus a 3-cell halo (69 × (TR+6) cells) is staged in LDS; every x- and y-face
+// flux of the tile is reconstructed once (lane ℓ of wavefront r owns the west face of
+// column ℓ and the south face of row r, lane 63 only a face; T, then S), written to LDS,
+// and differenced by the owning cell. The vertical face flux at the top of level k is
+// carried in registers as the bottom flux of level k+1, with the column's vertical stencil
+// in a register window. Row-dependent values are wavefront-uniform (one wavefront per row)
+// and kept in scalar registers. Tiles are ordered so that each XCD (blocks are dealt
+// round-robin to the 8 XCDs) receives a contiguous band of tiles, keeping halo re-reads
+// inside one L2. The FP64 kernel fits 128 VGPRs (4 wavefronts per SIMD, so 4 blocks of 4
+// per CU); the FP32-storage kernel is asked for 6 wavefronts per SIMD.
 #include "ne_internal.h"
 
 namespace ne {
@@ -18,7 +21,15 @@ namespace {
 
 constexpr int TW = 64;            // work-items per tile row (x faces per row)
 constexpr int TC = 63;            // cells per tile row
-constexpr int PLAIN_TR = 8;       // cell rows per tile of the single-model kernel
+#ifndef NE_PLAIN_TR
+#define NE_PLAIN_TR 4
+#endif
+// Minimum wavefronts per SIMD requested for the FP32-storage stage kernels (the FP64 ones are
+// fastest at the compiler's choice, which fits them in 128 VGPRs without spilling).
+#ifndef NE_STAGE_WAVES_PER_EU_F32
+#define NE_STAGE_WAVES_PER_EU_F32 6
+#endif
+constexpr int PLAIN_TR = NE_PLAIN_TR;  // cell rows per tile of the single-model kernel
 constexpr int NXCD = 8;
 constexpr int KMIN_OUTSIDE = 1 << 30;  // kmin of a column outside a bounded domain
 
@@ -95,6 +106,15 @@ __device__ __forceinline__ R round_state(double x, uint32_t key, uint32_t seed, 
     return r;
 }
 
+// A wave-uniform value as a scalar (the tile kernels' blocks are 64 wide, so every value that
+// depends on the row only is uniform over a wavefront): keeps it out of vector registers.
+__device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ double wave_uniform(double x) {
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(x));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(x));
+    return __hiloint2double(hi, lo);
+}
+
 template <typename T>
 __device__ __forceinline__ T ldb(const T* base, uint32_t off) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
@@ -125,7 +145,7 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
     constexpr uint32_t ES = sizeof(R);
     constexpr int NEVER = 1 << 30;
     const int lane = threadIdx.x;
-    const int row = threadIdx.y;
+    const int row = wave_uniform((int)threadIdx.y);   // one wavefront per row
     const int tid = lane + row * TW;
     const int Nz = g.Nz;
     const bool adv = g.advection != 0;   // single-column grids carry no advection
@@ -173,10 +193,12 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
     const int fjs = g.wy(j0 + row);
     const int fjn = g.wy(j0 + TR);
     const int cy = ycol ? ci : 0;
-    const double Axf = g.dy_xface[xrow ? cj : 0];
-    const double Ays = (j0 + row <= g.Ny) ? g.dx_yface[fjs < 0 ? 0 : (fjs > g.Ny ? g.Ny : fjs)] : 0.0;
-    const double Ayn = (j0 + TR <= g.Ny) ? g.dx_yface[fjn < 0 ? 0 : (fjn > g.Ny ? g.Ny : fjn)] : 0.0;
-    const double Azc = cell ? g.Az[cj] : 0.0;
+    const double Axf = wave_uniform(g.dy_xface[xrow ? cj : 0]);
+    const double Ays =
+        wave_uniform((j0 + row <= g.Ny) ? g.dx_yface[fjs < 0 ? 0 : (fjs > g.Ny ? g.Ny : fjs)] : 0.0);
+    const double Ayn =
+        wave_uniform((j0 + TR <= g.Ny) ? g.dx_yface[fjn < 0 ? 0 : (fjn > g.Ny ? g.Ny : fjn)] : 0.0);
+    const double Azc = wave_uniform(xrow ? g.Az[cj] : 0.0);   // used on cells only
     const uint32_t uoff = xrow ? (uint32_t)fiw * ES + (uint32_t)cj * L.u_sy : 0u;
     const uint32_t voff_s = ycol ? (uint32_t)cy * ES + (uint32_t)fjs * L.v_sy : 0u;
     const uint32_t voff_n = ycol ? (uint32_t)cy * ES + (uint32_t)fjn * L.v_sy : 0u;
@@ -203,64 +225,80 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
     }
     R FzT = (R)0, FzS = (R)0;  // flux through the bottom face of the current level
 
-    // TwoColorRadiation: J₀ of the column and the Beer's-law factors exp(κ (z − η)) of the
-    // bottom face of the current level (uniform over the tile)
+    // TwoColorRadiation: J₀ of the column and the surface height η of the Beer's-law factors
     const bool tc = F.two_color != 0;
     const double J0 = (tc && cell) ? ldb(F.J0, joff) : 0.0;
     const double eta = tc ? g.zf[Nz] : 0.0;
-    double E1b = tc ? exp(F.kappa1 * (g.zf[kstart] - eta)) : 0.0;
-    double E2b = tc ? exp(F.kappa2 * (g.zf[kstart] - eta)) : 0.0;
 
+    // Inputs of one level: its plane (staged in LDS below), the velocities of the faces this
+    // work-item reconstructs and c⁰ of its cell. (Loading level k+1 during level k was measured
+    // slower: the extra live registers cost more occupancy than the overlap gains.)
+    const bool ynorth = row == TR - 1;
+    R nT[NPL], nS[NPL];
+    R nu = (R)0, nvs = (R)0, nvn = (R)0, nw = (R)0, nT0 = (R)0, nS0 = (R)0;
+    auto load_level = [&](int k) {
+        const uint32_t kc = (uint32_t)k * L.c_sz;
+#pragma unroll
+        for (int q = 0; q < NPL; q++) {
+            const bool a = tid + q * NTHR < PN && k >= pkm[q];
+            nT[q] = a ? ldb(Tin, poff[q] + kc) : (R)0;
+            nS[q] = a ? ldb(Sin, poff[q] + kc) : (R)0;
+        }
+        nu = order_at(k, X1, X2, X3) > 0 ? ldb(u, uoff + (uint32_t)k * L.u_sz) : (R)0;
+        nvs = order_at(k, S1, S2, S3) > 0 ? ldb(v, voff_s + (uint32_t)k * L.v_sz) : (R)0;
+        nvn = (ynorth && order_at(k, N1, N2, N3) > 0) ? ldb(v, voff_n + (uint32_t)k * L.v_sz) : (R)0;
+        const bool zf = cell && adv && k >= mykmin && k <= Nz - 2;
+        nw = zf ? ldb(w, woff + (uint32_t)(k + 1) * L.w_sz) : (R)0;
+        const bool upd = MODE == 0 && cell && k >= mykmin;
+        nT0 = upd ? ldb(T0, coff + kc) : (R)0;
+        nS0 = upd ? ldb(S0, coff + kc) : (R)0;
+    };
     for (int k = kstart; k < Nz; k++) {
         const uint32_t kc = (uint32_t)k * L.c_sz;
-        // 1. stage the plane of level k
+        // 1. stage the plane of level k and take its velocities
+        load_level(k);
 #pragma unroll
         for (int q = 0; q < NPL; q++) {
             const int n = tid + q * NTHR;
             if (n < PN) {
-                const bool a = k >= pkm[q];
-                sh.cT[n] = a ? ldb(Tin, poff[q] + kc) : (R)0;
-                sh.cS[n] = a ? ldb(Sin, poff[q] + kc) : (R)0;
+                sh.cT[n] = nT[q];
+                sh.cS[n] = nS[q];
             }
         }
+        const R uk = nu, vsk = nvs, vnk = nvn, wk = nw, T0k = nT0, S0k = nS0;
         __syncthreads();
         const R dzk = (R)g.dz[k];
 
-        // 2a. x face owned by this lane: cells i0+lane-3 … i0+lane+2 of row j0+row
+        // 2a. x face owned by this lane: cells i0+lane-3 … i0+lane+2 of row j0+row; T, then S
+        // (one tracer at a time keeps a single reconstruction's temporaries live)
         {
-            R fT = (R)0, fS = (R)0;
             const int order = order_at(k, X1, X2, X3);
-            if (order > 0) {
-                const int base = (row + 3) * PXW + lane;
-                R vel = ldb(u, uoff + (uint32_t)k * L.u_sz);
-                R A = (R)Axf * dzk;
-                const R* t = &sh.cT[base];
-                const R* s = &sh.cS[base];
-                fT = A * vel * face_value(t[0], t[1], t[2], t[3], t[4], t[5], order, vel);
-                fS = A * vel * face_value(s[0], s[1], s[2], s[3], s[4], s[5], order, vel);
+            const R A = (R)Axf * dzk;
+            const int base = (row + 3) * PXW + lane;
+#pragma unroll 1
+            for (int tr = 0; tr < 2; tr++) {
+                const R* t = (tr ? sh.cS : sh.cT) + base;
+                R f = (R)0;
+                if (order > 0) f = A * uk * face_value(t[0], t[1], t[2], t[3], t[4], t[5], order, uk);
+                (tr ? sh.fxS : sh.fxT)[row * TW + lane] = f;
             }
-            sh.fxT[row * TW + lane] = fT;
-            sh.fxS[row * TW + lane] = fS;
         }
         // 2b. y faces: the south face of row j0+row, and for the last wavefront the north face of the tile
         if (ycol) {
 #pragma unroll 1
-            for (int pass = 0; pass < (row == TR - 1 ? 2 : 1); pass++) {
-                const int r0 = row + pass;                // plane row of the first stencil cell
-                const double Ay = pass ? Ayn : Ays;
-                R fT = (R)0, fS = (R)0;
-                const int order = pass ? order_at(k, N1, N2, N3) : order_at(k, S1, S2, S3);
+            for (int pass = 0; pass < (ynorth ? 4 : 2); pass++) {
+                const int side = pass >> 1, tr = pass & 1;
+                const int r0 = row + side;                // plane row of the first stencil cell
+                const double Ay = side ? Ayn : Ays;
+                const int order = side ? order_at(k, N1, N2, N3) : order_at(k, S1, S2, S3);
+                R f = (R)0;
                 if (order > 0) {
-                    const int base = r0 * PXW + lane + 3;
-                    R vel = ldb(v, (pass ? voff_n : voff_s) + (uint32_t)k * L.v_sz);
-                    R A = (R)Ay * dzk;
-                    const R* t = &sh.cT[base];
-                    const R* s = &sh.cS[base];
-                    fT = A * vel * face_value(t[0], t[PXW], t[2 * PXW], t[3 * PXW], t[4 * PXW], t[5 * PXW], order, vel);
-                    fS = A * vel * face_value(s[0], s[PXW], s[2 * PXW], s[3 * PXW], s[4 * PXW], s[5 * PXW], order, vel);
+                    const R vel = side ? vnk : vsk;
+                    const R A = (R)Ay * dzk;
+                    const R* t = (tr ? sh.cS : sh.cT) + r0 * PXW + lane + 3;
+                    f = A * vel * face_value(t[0], t[PXW], t[2 * PXW], t[3 * PXW], t[4 * PXW], t[5 * PXW], order, vel);
                 }
-                sh.fyT[(r0)*TC + lane] = fT;
-                sh.fyS[(r0)*TC + lane] = fS;
+                (tr ? sh.fyS : sh.fyT)[r0 * TC + lane] = f;
             }
         }
         // 2c. z face at the top of level k (cells k-2 … k+3 of the owned column)
@@ -269,10 +307,9 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
             const int order = (k >= mykmin && k <= Nz - 2 ? 1 : 0) + (k >= mykmin + 1 && k <= Nz - 3 ? 1 : 0) +
                               (k >= mykmin + 2 && k <= Nz - 4 ? 1 : 0);
             if (order > 0) {
-                R vel = ldb(w, woff + (uint32_t)(k + 1) * L.w_sz);
                 R A = (R)Azc;
-                FztT = A * vel * face_value(zT[0], zT[1], zT[2], zT[3], zT[4], zT[5], order, vel);
-                FztS = A * vel * face_value(zS[0], zS[1], zS[2], zS[3], zS[4], zS[5], order, vel);
+                FztT = A * wk * face_value(zT[0], zT[1], zT[2], zT[3], zT[4], zT[5], order, wk);
+                FztS = A * wk * face_value(zS[0], zS[1], zS[2], zS[3], zS[4], zS[5], order, wk);
             }
         }
         __syncthreads();
@@ -289,7 +326,9 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
             R GS = -((R)1 / V * div_S);
             if (tc) {
                 // ε₁ ∂z J₁ + (1 − ε₁) ∂z J₂; the face below the lowest active cell carries no flux
+                // (the level's two faces; uniform over the tile, recomputed rather than carried)
                 const double E1t = exp(F.kappa1 * (g.zf[k + 1] - eta)), E2t = exp(F.kappa2 * (g.zf[k + 1] - eta));
+                const double E1b = exp(F.kappa1 * (g.zf[k] - eta)), E2b = exp(F.kappa2 * (g.zf[k] - eta));
                 const bool below = k - 1 >= mykmin;
                 const double a1 = J0 * E1t, b1 = below ? J0 * E1b : 0.0;
                 const double a2 = J0 * E2t, b2 = below ? J0 * E2b : 0.0;
@@ -311,10 +350,8 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                 // The update is formed in FP64 and rounded once to the storage type (for FP32
                 // storage this keeps the per-stage rounding of the state at half an ulp).
                 const uint32_t key = 2u * ((uint32_t)ci + (uint32_t)g.Nx * ((uint32_t)cj + (uint32_t)g.Ny * k));
-                stb(Tout, coff + kc, round_state<R>((double)ldb(T0, coff + kc) + beta_dt * (double)GT, key, seed,
-                                                    stochastic));
-                stb(Sout, coff + kc, round_state<R>((double)ldb(S0, coff + kc) + beta_dt * (double)GS, key + 1u,
-                                                    seed, stochastic));
+                stb(Tout, coff + kc, round_state<R>((double)T0k + beta_dt * (double)GT, key, seed, stochastic));
+                stb(Sout, coff + kc, round_state<R>((double)S0k + beta_dt * (double)GS, key + 1u, seed, stochastic));
             } else {
                 stb(Tout, coff + kc, GT);
                 stb(Sout, coff + kc, GS);
@@ -322,10 +359,6 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
         }
         FzT = FztT;
         FzS = FztS;
-        if (tc) {
-            E1b = exp(F.kappa1 * (g.zf[k + 1] - eta));
-            E2b = exp(F.kappa2 * (g.zf[k + 1] - eta));
-        }
         // 4. slide the vertical window up one level
         if (cell) {
 #pragma unroll
@@ -342,6 +375,7 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
 // round-robin to the 8 XCDs) receives a contiguous band of tiles.
 template <typename Real, int MODE>
 __global__ void __launch_bounds__(TW * PLAIN_TR)
+__attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? NE_STAGE_WAVES_PER_EU_F32 : 1)))
 tracer_tile_kernel(GridDev g, TracerLayout L, const Real* u, const Real* v, const Real* w, const Real* T0,
                    const Real* S0, const Real* Tin, const Real* Sin, Real* Tout, Real* Sout, const double* JT,
                    const double* JS, const double* Jw, TracerForcing F, double beta_dt, int stochastic,
@@ -362,6 +396,7 @@ tracer_tile_kernel(GridDev g, TracerLayout L, const Real* u, const Real* v, cons
 // stage 1: T¹ = T + β₁Δt G(T); stage 2: T² = T + β₂Δt G(T¹); stage 3: T = T + β₃Δt G(T²).
 template <typename Real, int TR>
 __global__ void __launch_bounds__(TW * TR)
+__attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? NE_STAGE_WAVES_PER_EU_F32 : 1)))
 batch_stage_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __restrict__ work, int nwork,
                    int stage, double beta_dt) {
     __shared__ TileShared<Real, TR> sh;
@@ -391,6 +426,33 @@ batch_stage_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __re
         tile_body<Real, 0, TR>(sh, g, L, rc.u.p, rc.v.p, rc.w.p, rc.T.p, rc.S.p, Tin, Sin, Tout, Sout, rc.n.J_T.p,
                                rc.n.J_S.p, rc.n.J_w.p, F, beta_dt, (it.index % ntx) * TC, (it.index / ntx) * TR,
                                stage == 3, step_seed(rc.time));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Vertical velocity from continuity: one work-item per column marching up the levels,
+// i fastest so that every level's u, v and w rows are read and written coalesced. The
+// running w is kept in FP64 and rounded once per face to the storage type.
+// ---------------------------------------------------------------------------
+
+template <typename Real>
+__global__ void __launch_bounds__(256) w_continuity_kernel(GridDev g, View3<const Real> u, View3<const Real> v,
+                                                           View3<Real> w) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    if (i >= g.Nx || j >= g.Ny) return;
+    const int ip = g.tx == NE_PERIODIC ? wrapi(i + 1, g.Nx) : i + 1;
+    const int jp = g.ty == NE_PERIODIC ? wrapi(j + 1, g.Ny) : j + 1;
+    const double dyx = g.dy_xface[j], dxs = g.dx_yface[j], dxn = g.dx_yface[jp], Az = g.Az[j];
+    double wk = 0.0;
+    w(i, j, 0) = (Real)0;
+    for (int k = 0; k < g.Nz; k++) {
+        const double dz = g.dz[k];
+        const double Ax = dyx * dz, Ays = dxs * dz, Ayn = dxn * dz;
+        const double div = (Ax * (double)u(ip, j, k) - Ax * (double)u(i, j, k)) +
+                           (Ayn * (double)v(i, jp, k) - Ays * (double)v(i, j, k));
+        wk = g.active(i, j, k) ? wk - div / Az : 0.0;
+        w(i, j, k + 1) = (Real)wk;
     }
 }
 
@@ -554,6 +616,18 @@ template hipError_t launch_budget<double>(ne_ctx*, const GridDev&, View3<const d
                                           hipStream_t);
 template hipError_t launch_budget<float>(ne_ctx*, const GridDev&, View3<const float>, double, double*,
                                          hipStream_t);
+
+template <typename Real>
+hipError_t launch_w_continuity(const GridDev& g, View3<const Real> u, View3<const Real> v, View3<Real> w,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(w_continuity_kernel<Real>, dim3((g.Nx + 63) / 64, (g.Ny + 3) / 4), dim3(64, 4), 0, s, g, u, v,
+                       w);
+    return hipGetLastError();
+}
+template hipError_t launch_w_continuity<double>(const GridDev&, View3<const double>, View3<const double>,
+                                                View3<double>, hipStream_t);
+template hipError_t launch_w_continuity<float>(const GridDev&, View3<const float>, View3<const float>, View3<float>,
+                                               hipStream_t);
 
 int batch_tiles_x(int Nx) { return tiles_x(Nx); }
 int batch_tiles_y(int Ny, int tile_rows) { return tiles_y(Ny, tile_rows); }

@@ -200,6 +200,11 @@ class OceanOnlyModel:
     def update_state(self, stream=None) -> None:
         check(self.lib.ne_model_update_state(self.handle, stream_handle(stream)), "ne_model_update_state")
 
+    def update_vertical_velocity(self, stream=None) -> None:
+        """w from continuity of the current u and v (after the caller sets new horizontal velocities)."""
+        check(self.lib.ne_model_update_vertical_velocity(self.handle, stream_handle(stream)),
+              "ne_model_update_vertical_velocity")
+
     @property
     def clock(self):
         t = C.c_double()
@@ -338,6 +343,12 @@ class Phases:
                                                     stream_handle()),
               "ne_tracer_tendency_weno5_f64")
         return GT, GS
+
+    def compute_w(self, u: torch.Tensor, v: torch.Tensor, w: torch.Tensor) -> None:
+        m = self.m
+        check(self.lib.ne_compute_w_from_continuity_f64(self.ctx, C.byref(m.grid.struct), torch_view(u),
+                                                        torch_view(v), torch_view(w), stream_handle()),
+              "ne_compute_w_from_continuity_f64")
 
     def rk_stage(self, T0, S0, Tin, Sin, Tout, Sout, beta_dt: float):
         m = self.m

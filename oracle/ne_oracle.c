@@ -1009,6 +1009,31 @@ void neo_model_update_state(neo_model* m, double dt_frazil) {
                                     &m->net, &m->radiative);
 }
 
+/* compute_w_from_continuity! of the HydrostaticFreeSurfaceModel (Oceananigans, not vendored;
+ * ocean_simulation.jl:487-499 builds the model): w = 0 at the bottom face of every column,
+ * w[k+1] = w[k] − (δx(Ax u) + δy(Ay v))/Az on active cells, Ax = Δy·Δz and Ay = Δx·Δz of the
+ * cell's faces; faces above an inactive cell carry 0. */
+void neo_compute_w_from_continuity(const ne_grid* g, ne_view u, ne_view v, ne_view w) {
+#pragma omp parallel for schedule(static) NEO_THREADS
+    for (int j = 0; j < g->Ny; j++) {
+        for (int i = 0; i < g->Nx; i++) {
+            int ip = g->topology_x == NE_PERIODIC ? wrap(i + 1, g->Nx) : i + 1;
+            int jp = g->topology_y == NE_PERIODIC ? wrap(j + 1, g->Ny) : j + 1;
+            double wk = 0.0;
+            V3(w, i, j, 0) = 0.0;
+            for (int k = 0; k < g->Nz; k++) {
+                double Ax = g->dy_xface[j] * g->dz[k];
+                double Ays = g->dx_yface[j] * g->dz[k];
+                double Ayn = g->dx_yface[jp] * g->dz[k];
+                double div = (Ax * V3(u, ip, j, k) - Ax * V3(u, i, j, k)) +
+                             (Ayn * V3(v, i, jp, k) - Ays * V3(v, i, j, k));
+                wk = active(g, i, j, k) ? wk - div / g->Az[j] : 0.0;
+                V3(w, i, j, k + 1) = wk;
+            }
+        }
+    }
+}
+
 /* SplitRungeKutta3: c⁽ᵐ⁾ = c⁰ + βₘ Δt G(c⁽ᵐ⁻¹⁾), β = (1/3, 1/2, 1). */
 void neo_model_ocean_step(neo_model* m, double dt) {
     const ne_grid* g = &m->grid;

@@ -75,6 +75,7 @@ void neo_rk_stage(const ne_grid* g, const ne_params* params, ne_view u, ne_view 
                   ne_view Tin, ne_view Sin, ne_view Tout, ne_view Sout,
                   const ne_net_ocean_fluxes* net, double beta_dt);
 void neo_budget(const ne_grid* g, ne_view c, double weight, double out[2]);
+void neo_compute_w_from_continuity(const ne_grid* g, ne_view u, ne_view v, ne_view w);
 
 /* WENO reconstructions (left-biased: upwind value is the middle-left one). */
 double neo_weno5z(double c0, double c1, double c2, double c3, double c4);

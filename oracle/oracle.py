@@ -100,6 +100,7 @@ def lib():
         L.neo_apply_air_sea_radiation.argtypes = [P(abi.Grid), P(abi.Params), abi.View, abi.View, abi.View,
                                                   P(abi.NetOceanFluxes), P(abi.RadiativeFluxes)]
         L.neo_immersed_levels.argtypes = [P(abi.Grid), abi.View, P(C.c_int32)]
+        L.neo_compute_w_from_continuity.argtypes = [P(abi.Grid), abi.View, abi.View, abi.View]
         _lib = L
     return _lib
 

@@ -64,7 +64,9 @@ typedef struct ne_view {
 /* Immersed LatitudeLongitudeGrid (exchange grid == ocean grid), device arrays.
  * Reference: Oceananigans LatitudeLongitudeGrid + ImmersedBoundaryGrid(GridFittedBottom)
  * as built by the reference's examples (README.md:78-108). Metrics are per
- * latitude row because LatLon metrics depend on j only. */
+ * latitude row because LatLon metrics depend on j only. Nz ≤ NE_MAX_LEVELS (grids with
+ * more levels are refused with NE_ERR_UNSUPPORTED). */
+#define NE_MAX_LEVELS 1023
 typedef struct ne_grid {
     int32_t Nx, Ny, Nz;
     int32_t topology_x;          /* NE_PERIODIC or NE_BOUNDED                       */

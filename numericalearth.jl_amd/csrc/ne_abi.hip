@@ -18,6 +18,7 @@ inline int dev_err(hipError_t e) { return e == hipSuccess ? NE_OK : NE_ERR_DEVIC
 int check_grid(const ne_grid* g) {
     if (!g) return NE_ERR_NULL;
     if (g->Nx <= 0 || g->Ny <= 0 || g->Nz <= 0) return NE_ERR_SHAPE;
+    if (g->Nz > NE_MAX_LEVELS) return NE_ERR_UNSUPPORTED;
     if (g->topology_x != NE_PERIODIC && g->topology_x != NE_BOUNDED) return NE_ERR_UNSUPPORTED;
     if (g->topology_y != NE_PERIODIC && g->topology_y != NE_BOUNDED) return NE_ERR_UNSUPPORTED;
     if (!g->dz || !g->z_center || !g->dy_xface || !g->dx_yface || !g->Az || !g->lambda_c || !g->phi_c || !g->kmin)
@@ -303,6 +304,7 @@ int ne_ctx_counters(ne_ctx* c, ne_counters* out, int reset, void* stream) {
 int ne_grid_immersed_levels(ne_ctx* c, const ne_grid* g, ne_view bottom, int32_t* kmin, void* stream) {
     if (!c || !g || !kmin) return NE_ERR_NULL;
     if (g->Nx <= 0 || g->Ny <= 0 || g->Nz <= 0) return NE_ERR_SHAPE;
+    if (g->Nz > NE_MAX_LEVELS) return NE_ERR_UNSUPPORTED;
     if (!g->z_center || !bottom.data) return NE_ERR_NULL;
     GridDev d = grid_dev(*g);
     return dev_err(launch_immersed_levels(d, view<const double>(bottom), kmin, as_stream(stream)));

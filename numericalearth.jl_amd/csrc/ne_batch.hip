@@ -70,8 +70,9 @@ struct ne_batch {
 
     // views into the arena
     void* recs = nullptr;
-    WorkItem* stage_work[3] = {nullptr, nullptr, nullptr};    // per tile-row choice
-    int stage_nwork[3] = {0, 0, 0};
+    // per forcing (without / with the two-colour shortwave) and tile-row choice
+    WorkItem* stage_work[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    int stage_nwork[2][3] = {{0, 0, 0}, {0, 0, 0}};
     // per column-block choice: every member (finalize), and the members of each flux path (solve)
     WorkItem* surface_work[3] = {nullptr, nullptr, nullptr};
     int surface_nwork[3] = {0, 0, 0};
@@ -108,13 +109,16 @@ hipError_t enqueue_step(ne_batch* b, double dt, hipStream_t s, ne_counters* coun
     if (ti < 0 || si < 0) return hipErrorInvalidValue;
     const double beta[3] = {1.0 / 3.0, 0.5, 1.0};
     hipError_t e;
-    int nst = b->stage_nwork[ti];
-    int sblocks = t.stage_blocks > 0 ? std::min(t.stage_blocks, round8(nst)) : round8(nst);
-    sblocks = std::max(sblocks, NXCD);
-    for (int st = 0; st < 3; st++) {
-        e = launch_batch_stage<Real>(recs, b->stage_work[ti], nst, t.tile_rows, sblocks, st + 1, beta[st] * dt, s);
-        if (e != hipSuccess) return e;
-    }
+    for (int st = 0; st < 3; st++)
+        for (int tc = 0; tc < 2; tc++) {
+            const int nst = b->stage_nwork[tc][ti];
+            if (nst == 0) continue;
+            int sblocks = t.stage_blocks > 0 ? std::min(t.stage_blocks, round8(nst)) : round8(nst);
+            sblocks = std::max(sblocks, NXCD);
+            e = launch_batch_stage<Real>(recs, b->stage_work[tc][ti], nst, t.tile_rows, sblocks, st + 1,
+                                         beta[st] * dt, tc == 1, s);
+            if (e != hipSuccess) return e;
+        }
     hipLaunchKernelGGL(batch_clock_kernel<Real>, dim3((n + 63) / 64), dim3(64), 0, s,
                        static_cast<BatchRec<Real>*>(b->recs), n, dt);
     if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -174,9 +178,10 @@ BatchRec<Real> member_record(const ne_model* m, const double* atimes_dev, const 
     return r;
 }
 
-// Host-side layout of the arena: records | times | 3 stage lists | 3 surface lists | 2×3 solve lists.
+// Host-side layout of the arena: records | times | 2×3 stage lists | 3 surface lists | 2×3 solve lists.
 struct Layout {
-    size_t recs = 0, times = 0, stage[3] = {0, 0, 0}, surface[3] = {0, 0, 0}, solve[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    size_t recs = 0, times = 0, stage[2][3] = {{0, 0, 0}, {0, 0, 0}}, surface[3] = {0, 0, 0},
+           solve[2][3] = {{0, 0, 0}, {0, 0, 0}};
     size_t total = 0;
 };
 
@@ -184,15 +189,16 @@ template <typename Real>
 int rebuild(ne_batch* b, hipStream_t s) {
     const int n = (int)b->members.size();
     // work lists and snapshot times on the host
-    std::vector<WorkItem> stage[3], surface[3], solve[2][3];
+    std::vector<WorkItem> stage[2][3], surface[3], solve[2][3];
     std::vector<double> times;
     std::vector<size_t> aoff(n), roff(n);
     for (int k = 0; k < n; k++) {
         const ne_model* m = b->members[k];
         const ne_grid& g = m->d.grid;
+        const int tc = m->d.params.radiative_forcing == NE_RADIATIVE_FORCING_TWO_COLOR ? 1 : 0;
         for (int c = 0; c < 3; c++) {
             int nt = batch_tiles_x(g.Nx) * batch_tiles_y(g.Ny, TILE_ROWS[c]);
-            for (int t = 0; t < nt; t++) stage[c].push_back(WorkItem{k, t});
+            for (int t = 0; t < nt; t++) stage[tc][c].push_back(WorkItem{k, t});
         }
         int fast = edson_default_constants(m->d.params) ? 1 : 0;
         int cols = g.Nx * g.Ny;
@@ -213,8 +219,10 @@ int rebuild(ne_batch* b, hipStream_t s) {
     L.times = off;
     off = align256(off + times.size() * sizeof(double));
     for (int c = 0; c < 3; c++) {
-        L.stage[c] = off;
-        off = align256(off + stage[c].size() * sizeof(WorkItem));
+        for (int f = 0; f < 2; f++) {
+            L.stage[f][c] = off;
+            off = align256(off + stage[f][c].size() * sizeof(WorkItem));
+        }
         L.surface[c] = off;
         off = align256(off + surface[c].size() * sizeof(WorkItem));
         for (int f = 0; f < 2; f++) {
@@ -264,7 +272,7 @@ int rebuild(ne_batch* b, hipStream_t s) {
         *count = (int)w.size();
     };
     for (int c = 0; c < 3; c++) {
-        put(L.stage[c], stage[c], &b->stage_work[c], &b->stage_nwork[c]);
+        for (int f = 0; f < 2; f++) put(L.stage[f][c], stage[f][c], &b->stage_work[f][c], &b->stage_nwork[f][c]);
         put(L.surface[c], surface[c], &b->surface_work[c], &b->surface_nwork[c]);
         for (int f = 0; f < 2; f++) put(L.solve[f][c], solve[f][c], &b->solve_work[f][c], &b->solve_nwork[f][c]);
     }
@@ -332,15 +340,23 @@ int autotune(ne_batch* b, hipStream_t s, ne_batch_tuning* out) {
     // the state untouched. Grids: 1, 2 or 4 persistent blocks per CU, or one block per item (0).
     double best_stage = INFINITY;
     for (int c = 0; c < 3; c++) {
-        int nwork = b->stage_nwork[c];
+        const int nwork = b->stage_nwork[0][c] + b->stage_nwork[1][c];
         if (nwork == 0) continue;
         const int per_cu[4] = {1, 2, 4, 0};
         for (int pc : per_cu) {
-            int blocks = pc ? std::max(NXCD, std::min(round8(nwork), pc * b->cus)) : round8(nwork);
+            const int blocks = pc ? std::max(NXCD, std::min(round8(nwork), pc * b->cus)) : round8(nwork);
             double ms;
             auto f = [&]() {
-                return launch_batch_stage<Real>(recs, b->stage_work[c], nwork, TILE_ROWS[c], blocks, 1,
-                                                1.0 / 3.0 * 1200.0, s);
+                // both forcing lists, each with the grid the step would give it
+                for (int tc = 0; tc < 2; tc++) {
+                    const int nw = b->stage_nwork[tc][c];
+                    if (nw == 0) continue;
+                    const int bl = std::max(NXCD, pc ? std::min(blocks, round8(nw)) : round8(nw));
+                    hipError_t e = launch_batch_stage<Real>(recs, b->stage_work[tc][c], nw, TILE_ROWS[c], bl, 1,
+                                                            1.0 / 3.0 * 1200.0, tc == 1, s);
+                    if (e != hipSuccess) return e;
+                }
+                return hipSuccess;
             };
             int r = time_launches(s, reps, f, &ms);
             if (r) return r;
@@ -560,7 +576,7 @@ int ne_batch_info(const ne_batch* b, int* members, int64_t* cells, int* stage_it
     }
     // counts of the current upload (the next step rebuilds the lists if membership changed)
     int ti = tr_index(b->tuning.tile_rows), si = sb_index(b->tuning.surface_block);
-    if (stage_items) *stage_items = ti >= 0 ? b->stage_nwork[ti] : 0;
+    if (stage_items) *stage_items = ti >= 0 ? b->stage_nwork[0][ti] + b->stage_nwork[1][ti] : 0;
     if (surface_items) *surface_items = si >= 0 ? b->surface_nwork[si] : 0;
     return NE_OK;
 }

@@ -221,7 +221,7 @@ struct WorkItem {
 // deal the work list out in 8 contiguous chunks, one per XCD.
 template <typename Real>
 hipError_t launch_batch_stage(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int tile_rows,
-                              int blocks, int stage, double beta_dt, hipStream_t s);
+                              int blocks, int stage, double beta_dt, bool two_color, hipStream_t s);
 template <typename Real>
 hipError_t launch_batch_surface_solve(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int block,
                                       int blocks, bool fast, ne_counters* c, hipStream_t s);

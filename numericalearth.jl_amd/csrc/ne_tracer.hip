@@ -56,16 +56,22 @@ inline int grid_blocks(int Nx, int Ny) {
 
 // Face order at level k from the three thresholds of its stencil: the face is
 // reconstructed with order 5 (3) if k ≥ K3 (K2), upwind if k ≥ K1, and carries no flux
-// otherwise (K1 ≤ K2 ≤ K3 are the largest kmin over the 2, 4 and 6 cells around it).
-__device__ __forceinline__ int order_at(int k, int K1, int K2, int K3) {
-    return (k >= K1 ? 1 : 0) + (k >= K2 ? 1 : 0) + (k >= K3 ? 1 : 0);
+// otherwise (K1 ≤ K2 ≤ K3 are the largest kmin over the 2, 4 and 6 cells around it). The
+// three are packed in 10-bit fields of one register (grids have at most NE_MAX_LEVELS =
+// 1023 levels, so 1023 — "never" — exceeds every level).
+constexpr uint32_t NEVER_LEVEL = 1023u;
+__device__ __forceinline__ int order_at(int k, uint32_t P) {
+    return (k >= (int)(P & 1023u) ? 1 : 0) + (k >= (int)((P >> 10) & 1023u) ? 1 : 0) + (k >= (int)(P >> 20) ? 1 : 0);
 }
 
-__device__ __forceinline__ void thresholds(const int* km, int stride, int& K1, int& K2, int& K3) {
-    K1 = max(km[2 * stride], km[3 * stride]);
-    K2 = max(K1, max(km[stride], km[4 * stride]));
-    K3 = max(K2, max(km[0], km[5 * stride]));
+__device__ __forceinline__ uint32_t thresholds(const int* km, int stride) {
+    const int K1 = max(km[2 * stride], km[3 * stride]);
+    const int K2 = max(K1, max(km[stride], km[4 * stride]));
+    const int K3 = max(K2, max(km[0], km[5 * stride]));
+    auto field = [](int K) { return K < (int)NEVER_LEVEL ? (uint32_t)K : NEVER_LEVEL; };
+    return field(K1) | field(K2) << 10 | field(K3) << 20;
 }
+constexpr uint32_t NEVER_FACE = NEVER_LEVEL | NEVER_LEVEL << 10 | NEVER_LEVEL << 20;
 
 // Integer hash (Wellons' "lowbias32" finaliser) for the stochastic rounding below.
 __device__ __forceinline__ uint32_t hash32(uint32_t x) {
@@ -129,7 +135,7 @@ __device__ __forceinline__ void stb(T* base, uint32_t off, T v) {
 // cells). Every work-item of the block calls this with the same tile. Every address is a
 // 32-bit byte offset from a uniform base (the host checks that each view spans < 4 GiB),
 // computed once per tile; a level adds its uniform k-stride.
-template <typename Real, int MODE, int TR>
+template <typename Real, int MODE, int TR, bool TWO_COLOR>
 __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDev& g, const TracerLayout& L,
                                           const Real* __restrict__ u, const Real* __restrict__ v,
                                           const Real* __restrict__ w, const Real* T0, const Real* S0,
@@ -181,14 +187,12 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
     }
     const bool xrow = cj < g.Ny;                 // this wavefront's row is inside the domain
     const bool ycol = lane < TC && ci < g.Nx;
-    int X1 = NEVER, X2 = NEVER, X3 = NEVER;      // x face owned by this lane (west face of column i0+lane)
-    if (xrow && adv) thresholds(&sh.km[(row + 3) * PXW + lane], 1, X1, X2, X3);
-    int S1 = NEVER, S2 = NEVER, S3 = NEVER;      // south y face of row j0+row
-    int N1 = NEVER, N2 = NEVER, N3 = NEVER;      // north y face of the tile (last wavefront)
-    if (ycol && adv) {
-        thresholds(&sh.km[row * PXW + lane + 3], PXW, S1, S2, S3);
-        if (row == TR - 1) thresholds(&sh.km[(row + 1) * PXW + lane + 3], PXW, N1, N2, N3);
-    }
+    // x face owned by this lane (west face of column i0+lane), south y face of row j0+row,
+    // north y face of the tile (last wavefront)
+    const uint32_t Xk = (xrow && adv) ? thresholds(&sh.km[(row + 3) * PXW + lane], 1) : NEVER_FACE;
+    const uint32_t Sk = (ycol && adv) ? thresholds(&sh.km[row * PXW + lane + 3], PXW) : NEVER_FACE;
+    const uint32_t Nk =
+        (ycol && adv && row == TR - 1) ? thresholds(&sh.km[(row + 1) * PXW + lane + 3], PXW) : NEVER_FACE;
     const int fiw = g.wx(i0 + lane);
     const int fjs = g.wy(j0 + row);
     const int fjn = g.wy(j0 + TR);
@@ -201,10 +205,10 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
     const double Azc = wave_uniform(xrow ? g.Az[cj] : 0.0);   // used on cells only
     const uint32_t uoff = xrow ? (uint32_t)fiw * ES + (uint32_t)cj * L.u_sy : 0u;
     const uint32_t voff_s = ycol ? (uint32_t)cy * ES + (uint32_t)fjs * L.v_sy : 0u;
-    const uint32_t voff_n = ycol ? (uint32_t)cy * ES + (uint32_t)fjn * L.v_sy : 0u;
+    const uint32_t dvn = (uint32_t)(fjn - fjs) * L.v_sy;           // north face: voff_s + dvn (uniform)
     const uint32_t coff = cell ? (uint32_t)ci * ES + (uint32_t)cj * L.c_sy : 0u;
-    const uint32_t woff = cell ? (uint32_t)ci * ES + (uint32_t)cj * L.w_sy : 0u;
-    const uint32_t joff = cell ? (uint32_t)ci * 8u + (uint32_t)cj * L.j_sy : 0u;
+    const uint32_t dw = (uint32_t)cj * (L.w_sy - L.c_sy);           // w of the cell: coff + dw (uniform)
+    auto joff = [&]() { return (uint32_t)ci * 8u + (uint32_t)cj * L.j_sy; };   // 2-D fluxes (cells only)
     __syncthreads();
     const int kstart = sh.kstart;
     if (MODE == 1 && cell)
@@ -226,8 +230,8 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
     R FzT = (R)0, FzS = (R)0;  // flux through the bottom face of the current level
 
     // TwoColorRadiation: J₀ of the column and the surface height η of the Beer's-law factors
-    const bool tc = F.two_color != 0;
-    const double J0 = (tc && cell) ? ldb(F.J0, joff) : 0.0;
+    const bool tc = TWO_COLOR && F.two_color != 0;   // instances without it carry none of its registers
+    const double J0 = (tc && cell) ? ldb(F.J0, joff()) : 0.0;
     const double eta = tc ? g.zf[Nz] : 0.0;
 
     // Inputs of one level: its plane (staged in LDS below), the velocities of the faces this
@@ -235,7 +239,6 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
     // slower: the extra live registers cost more occupancy than the overlap gains.)
     const bool ynorth = row == TR - 1;
     R nT[NPL], nS[NPL];
-    R nu = (R)0, nvs = (R)0, nvn = (R)0, nw = (R)0, nT0 = (R)0, nS0 = (R)0;
     auto load_level = [&](int k) {
         const uint32_t kc = (uint32_t)k * L.c_sz;
 #pragma unroll
@@ -244,14 +247,6 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
             nT[q] = a ? ldb(Tin, poff[q] + kc) : (R)0;
             nS[q] = a ? ldb(Sin, poff[q] + kc) : (R)0;
         }
-        nu = order_at(k, X1, X2, X3) > 0 ? ldb(u, uoff + (uint32_t)k * L.u_sz) : (R)0;
-        nvs = order_at(k, S1, S2, S3) > 0 ? ldb(v, voff_s + (uint32_t)k * L.v_sz) : (R)0;
-        nvn = (ynorth && order_at(k, N1, N2, N3) > 0) ? ldb(v, voff_n + (uint32_t)k * L.v_sz) : (R)0;
-        const bool zf = cell && adv && k >= mykmin && k <= Nz - 2;
-        nw = zf ? ldb(w, woff + (uint32_t)(k + 1) * L.w_sz) : (R)0;
-        const bool upd = MODE == 0 && cell && k >= mykmin;
-        nT0 = upd ? ldb(T0, coff + kc) : (R)0;
-        nS0 = upd ? ldb(S0, coff + kc) : (R)0;
     };
     for (int k = kstart; k < Nz; k++) {
         const uint32_t kc = (uint32_t)k * L.c_sz;
@@ -265,21 +260,23 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                 sh.cS[n] = nS[q];
             }
         }
-        const R uk = nu, vsk = nvs, vnk = nvn, wk = nw, T0k = nT0, S0k = nS0;
         __syncthreads();
         const R dzk = (R)g.dz[k];
 
         // 2a. x face owned by this lane: cells i0+lane-3 … i0+lane+2 of row j0+row; T, then S
         // (one tracer at a time keeps a single reconstruction's temporaries live)
         {
-            const int order = order_at(k, X1, X2, X3);
+            const int order = order_at(k, Xk);
             const R A = (R)Axf * dzk;
             const int base = (row + 3) * PXW + lane;
 #pragma unroll 1
             for (int tr = 0; tr < 2; tr++) {
                 const R* t = (tr ? sh.cS : sh.cT) + base;
                 R f = (R)0;
-                if (order > 0) f = A * uk * face_value(t[0], t[1], t[2], t[3], t[4], t[5], order, uk);
+                if (order > 0) {
+                    const R uk = ldb(u, uoff + (uint32_t)k * L.u_sz);
+                    f = A * uk * face_value(t[0], t[1], t[2], t[3], t[4], t[5], order, uk);
+                }
                 (tr ? sh.fxS : sh.fxT)[row * TW + lane] = f;
             }
         }
@@ -290,10 +287,10 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                 const int side = pass >> 1, tr = pass & 1;
                 const int r0 = row + side;                // plane row of the first stencil cell
                 const double Ay = side ? Ayn : Ays;
-                const int order = side ? order_at(k, N1, N2, N3) : order_at(k, S1, S2, S3);
+                const int order = order_at(k, side ? Nk : Sk);
                 R f = (R)0;
                 if (order > 0) {
-                    const R vel = side ? vnk : vsk;
+                    const R vel = ldb(v, voff_s + (side ? dvn : 0u) + (uint32_t)k * L.v_sz);
                     const R A = (R)Ay * dzk;
                     const R* t = (tr ? sh.cS : sh.cT) + r0 * PXW + lane + 3;
                     f = A * vel * face_value(t[0], t[PXW], t[2 * PXW], t[3 * PXW], t[4 * PXW], t[5 * PXW], order, vel);
@@ -301,18 +298,17 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                 (tr ? sh.fyS : sh.fyT)[r0 * TC + lane] = f;
             }
         }
-        // 2c. z face at the top of level k (cells k-2 … k+3 of the owned column)
+        // 2c. z face at the top of level k (cells k-2 … k+3 of the owned column): T here, S after
+        // the barrier, so that the two reconstructions are not live at the same time
+        const int zorder = (cell && adv) ? (k >= mykmin && k <= Nz - 2 ? 1 : 0) +
+                                               (k >= mykmin + 1 && k <= Nz - 3 ? 1 : 0) +
+                                               (k >= mykmin + 2 && k <= Nz - 4 ? 1 : 0)
+                                         : 0;
         R FztT = (R)0, FztS = (R)0;
-        if (cell && adv) {
-            const int order = (k >= mykmin && k <= Nz - 2 ? 1 : 0) + (k >= mykmin + 1 && k <= Nz - 3 ? 1 : 0) +
-                              (k >= mykmin + 2 && k <= Nz - 4 ? 1 : 0);
-            if (order > 0) {
-                R A = (R)Azc;
-                FztT = A * wk * face_value(zT[0], zT[1], zT[2], zT[3], zT[4], zT[5], order, wk);
-                FztS = A * wk * face_value(zS[0], zS[1], zS[2], zS[3], zS[4], zS[5], order, wk);
-            }
-        }
+        const R wk = zorder > 0 ? ldb(w, coff + dw + (uint32_t)(k + 1) * L.w_sz) : (R)0;
+        if (zorder > 0) FztT = (R)Azc * wk * face_value(zT[0], zT[1], zT[2], zT[3], zT[4], zT[5], zorder, wk);
         __syncthreads();
+        if (zorder > 0) FztS = (R)Azc * wk * face_value(zS[0], zS[1], zS[2], zS[3], zS[4], zS[5], zorder, wk);
 
         // 3. difference the face fluxes, add the top boundary flux, update
         if (cell && k >= mykmin) {
@@ -341,8 +337,9 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                 // MultipleFluxes(Jᵀ, FreshwaterExchange{:T}) = Jᵀ + 0 on a static grid;
                 // MultipleFluxes(Jˢ, FreshwaterExchange{:S}) = Jˢ + (Sᴺ Jʷ − 0)
                 // (multiple_surface_fluxes.jl:27-31; ocean_simulation.jl:80-91)
-                double FT = ldb(JT, joff) + (0.0 + 0.0);
-                double FS = ldb(JS, joff) + (((double)zS[2] * ldb(Jw, joff) - 0.0) + 0.0);
+                const uint32_t jo = joff();
+                double FT = ldb(JT, jo) + (0.0 + 0.0);
+                double FS = ldb(JS, jo) + (((double)zS[2] * ldb(Jw, jo) - 0.0) + 0.0);
                 GT = GT - (R)(FT * Azc) / V;
                 GS = GS - (R)(FS * Azc) / V;
             }
@@ -350,8 +347,10 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                 // The update is formed in FP64 and rounded once to the storage type (for FP32
                 // storage this keeps the per-stage rounding of the state at half an ulp).
                 const uint32_t key = 2u * ((uint32_t)ci + (uint32_t)g.Nx * ((uint32_t)cj + (uint32_t)g.Ny * k));
-                stb(Tout, coff + kc, round_state<R>((double)T0k + beta_dt * (double)GT, key, seed, stochastic));
-                stb(Sout, coff + kc, round_state<R>((double)S0k + beta_dt * (double)GS, key + 1u, seed, stochastic));
+                stb(Tout, coff + kc, round_state<R>((double)ldb(T0, coff + kc) + beta_dt * (double)GT, key, seed,
+                                                    stochastic));
+                stb(Sout, coff + kc, round_state<R>((double)ldb(S0, coff + kc) + beta_dt * (double)GS, key + 1u,
+                                                    seed, stochastic));
             } else {
                 stb(Tout, coff + kc, GT);
                 stb(Sout, coff + kc, GS);
@@ -373,7 +372,7 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
 
 // Single model: one block per tile, tiles ordered so that each XCD (blocks are dealt
 // round-robin to the 8 XCDs) receives a contiguous band of tiles.
-template <typename Real, int MODE>
+template <typename Real, int MODE, bool TWO_COLOR>
 __global__ void __launch_bounds__(TW * PLAIN_TR)
 __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? NE_STAGE_WAVES_PER_EU_F32 : 1)))
 tracer_tile_kernel(GridDev g, TracerLayout L, const Real* u, const Real* v, const Real* w, const Real* T0,
@@ -386,7 +385,7 @@ tracer_tile_kernel(GridDev g, TracerLayout L, const Real* u, const Real* v, cons
     const int per = (ntiles + NXCD - 1) / NXCD;
     const int tile = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
     if (tile >= ntiles) return;
-    tile_body<Real, MODE, PLAIN_TR>(sh, g, L, u, v, w, T0, S0, Tin, Sin, Tout, Sout, JT, JS, Jw, F, beta_dt,
+    tile_body<Real, MODE, PLAIN_TR, TWO_COLOR>(sh, g, L, u, v, w, T0, S0, Tin, Sin, Tout, Sout, JT, JS, Jw, F, beta_dt,
                                     (tile % ntx) * TC, (tile / ntx) * PLAIN_TR, stochastic != 0, seed);
 }
 
@@ -394,7 +393,7 @@ tracer_tile_kernel(GridDev g, TracerLayout L, const Real* u, const Real* v, cons
 // contiguous chunks, chunk x going to the blocks of XCD x, which stride through it; each
 // member's tiles are contiguous in the list, so a member's halo re-reads stay in one L2.
 // stage 1: T¹ = T + β₁Δt G(T); stage 2: T² = T + β₂Δt G(T¹); stage 3: T = T + β₃Δt G(T²).
-template <typename Real, int TR>
+template <typename Real, int TR, bool TWO_COLOR>
 __global__ void __launch_bounds__(TW * TR)
 __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? NE_STAGE_WAVES_PER_EU_F32 : 1)))
 batch_stage_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __restrict__ work, int nwork,
@@ -423,7 +422,7 @@ batch_stage_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __re
             F.J0 = rc.n.J_sw.p;
         }
         __syncthreads();   // the previous item's reads of the LDS are complete
-        tile_body<Real, 0, TR>(sh, g, L, rc.u.p, rc.v.p, rc.w.p, rc.T.p, rc.S.p, Tin, Sin, Tout, Sout, rc.n.J_T.p,
+        tile_body<Real, 0, TR, TWO_COLOR>(sh, g, L, rc.u.p, rc.v.p, rc.w.p, rc.T.p, rc.S.p, Tin, Sin, Tout, Sout, rc.n.J_T.p,
                                rc.n.J_S.p, rc.n.J_w.p, F, beta_dt, (it.index % ntx) * TC, (it.index / ntx) * TR,
                                stage == 3, step_seed(rc.time));
     }
@@ -565,9 +564,14 @@ hipError_t launch_rk_stage(const GridDev& g, View3<const Real> u, View3<const Re
                            View3<const double> Jw, TracerForcing forcing, double beta_dt, bool stochastic,
                            double time, hipStream_t s) {
     TracerLayout L = host_layout<Real>(Tin, u, v, w, JT);
-    hipLaunchKernelGGL((tracer_tile_kernel<Real, 0>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, PLAIN_TR), 0, s, g, L,
-                       u.p, v.p, w.p, T0.p, S0.p, Tin.p, Sin.p, Tout.p, Sout.p, JT.p, JS.p, Jw.p, forcing, beta_dt,
-                       stochastic ? 1 : 0, step_seed(time));
+    if (forcing.two_color)
+        hipLaunchKernelGGL((tracer_tile_kernel<Real, 0, true>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, PLAIN_TR), 0,
+                           s, g, L, u.p, v.p, w.p, T0.p, S0.p, Tin.p, Sin.p, Tout.p, Sout.p, JT.p, JS.p, Jw.p,
+                           forcing, beta_dt, stochastic ? 1 : 0, step_seed(time));
+    else
+        hipLaunchKernelGGL((tracer_tile_kernel<Real, 0, false>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, PLAIN_TR), 0,
+                           s, g, L, u.p, v.p, w.p, T0.p, S0.p, Tin.p, Sin.p, Tout.p, Sout.p, JT.p, JS.p, Jw.p,
+                           forcing, beta_dt, stochastic ? 1 : 0, step_seed(time));
     return hipGetLastError();
 }
 template hipError_t launch_rk_stage<double>(const GridDev&, View3<const double>, View3<const double>,
@@ -586,8 +590,14 @@ hipError_t launch_tendency(const GridDev& g, View3<const double> u, View3<const 
                            View3<const double> JS, View3<const double> Jw, TracerForcing forcing, View3<double> GT,
                            View3<double> GS, hipStream_t s) {
     TracerLayout L = host_layout<double>(T, u, v, w, JT);
-    hipLaunchKernelGGL((tracer_tile_kernel<double, 1>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, PLAIN_TR), 0, s, g,
-                       L, u.p, v.p, w.p, T.p, S.p, T.p, S.p, GT.p, GS.p, JT.p, JS.p, Jw.p, forcing, 0.0, 0, 0u);
+    if (forcing.two_color)
+        hipLaunchKernelGGL((tracer_tile_kernel<double, 1, true>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, PLAIN_TR), 0,
+                           s, g, L, u.p, v.p, w.p, T.p, S.p, T.p, S.p, GT.p, GS.p, JT.p, JS.p, Jw.p, forcing, 0.0, 0,
+                           0u);
+    else
+        hipLaunchKernelGGL((tracer_tile_kernel<double, 1, false>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, PLAIN_TR),
+                           0, s, g, L, u.p, v.p, w.p, T.p, S.p, T.p, S.p, GT.p, GS.p, JT.p, JS.p, Jw.p, forcing, 0.0,
+                           0, 0u);
     return hipGetLastError();
 }
 
@@ -632,32 +642,39 @@ template hipError_t launch_w_continuity<float>(const GridDev&, View3<const float
 int batch_tiles_x(int Nx) { return tiles_x(Nx); }
 int batch_tiles_y(int Ny, int tile_rows) { return tiles_y(Ny, tile_rows); }
 
-template <typename Real>
-hipError_t launch_batch_stage(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int tile_rows,
-                              int blocks, int stage, double beta_dt, hipStream_t s) {
-    if (nwork <= 0) return hipSuccess;
-    if (blocks <= 0 || blocks % NXCD != 0) return hipErrorInvalidValue;
+template <typename Real, bool TWO_COLOR>
+hipError_t launch_batch_stage_tc(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int tile_rows,
+                                 int blocks, int stage, double beta_dt, hipStream_t s) {
     switch (tile_rows) {
     case 4:
-        hipLaunchKernelGGL((batch_stage_kernel<Real, 4>), dim3(blocks), dim3(TW, 4), 0, s, recs, work, nwork, stage,
-                           beta_dt);
+        hipLaunchKernelGGL((batch_stage_kernel<Real, 4, TWO_COLOR>), dim3(blocks), dim3(TW, 4), 0, s, recs, work,
+                           nwork, stage, beta_dt);
         break;
     case 8:
-        hipLaunchKernelGGL((batch_stage_kernel<Real, 8>), dim3(blocks), dim3(TW, 8), 0, s, recs, work, nwork, stage,
-                           beta_dt);
+        hipLaunchKernelGGL((batch_stage_kernel<Real, 8, TWO_COLOR>), dim3(blocks), dim3(TW, 8), 0, s, recs, work,
+                           nwork, stage, beta_dt);
         break;
     case 16:
-        hipLaunchKernelGGL((batch_stage_kernel<Real, 16>), dim3(blocks), dim3(TW, 16), 0, s, recs, work, nwork,
-                           stage, beta_dt);
+        hipLaunchKernelGGL((batch_stage_kernel<Real, 16, TWO_COLOR>), dim3(blocks), dim3(TW, 16), 0, s, recs, work,
+                           nwork, stage, beta_dt);
         break;
     default:
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
+
+template <typename Real>
+hipError_t launch_batch_stage(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int tile_rows,
+                              int blocks, int stage, double beta_dt, bool two_color, hipStream_t s) {
+    if (nwork <= 0) return hipSuccess;
+    if (blocks <= 0 || blocks % NXCD != 0) return hipErrorInvalidValue;
+    return two_color ? launch_batch_stage_tc<Real, true>(recs, work, nwork, tile_rows, blocks, stage, beta_dt, s)
+                     : launch_batch_stage_tc<Real, false>(recs, work, nwork, tile_rows, blocks, stage, beta_dt, s);
+}
 template hipError_t launch_batch_stage<double>(const BatchRec<double>*, const WorkItem*, int, int, int, int, double,
-                                               hipStream_t);
+                                               bool, hipStream_t);
 template hipError_t launch_batch_stage<float>(const BatchRec<float>*, const WorkItem*, int, int, int, int, double,
-                                              hipStream_t);
+                                              bool, hipStream_t);
 
 }  // namespace ne

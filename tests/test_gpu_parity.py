@@ -111,35 +111,40 @@ def _compare_fluxes(gm, om, rtol=RTOL, max_iteration_mismatch_frac=1e-4, envelop
     print("[flux] worst relative errors:", {k: f"{v[0]:.2e}" for k, v in worst.items()}, flush=True)
 
 
-def _oracle_on_state(om, T, S, time=None, nudge=0):
+def _oracle_on_state(om, T, S, time=None, nudge=None):
     """A fresh oracle model with om's case and parameters holding the state (T, S): atmosphere and
-    radiation interpolated at `time` (om's by default), the air temperature moved by `nudge` ulps
-    (±1), and the fluxes solved."""
+    radiation interpolated at `time` (om's by default), one input moved by one ulp if `nudge` =
+    (name, ±1) names one (an interpolated atmosphere field, or "sst", the top level of T), and the
+    fluxes solved."""
     o = O.OracleModel(om.case, params=abi.Params.from_buffer_copy(om.params), radiation=bool(om.params.radiation))
     o.T[...] = T
     o.S[...] = S
+    if nudge and nudge[0] == "sst":
+        o.T[-1] = np.nextafter(o.T[-1], np.inf * nudge[1])
     o.time = om.time if time is None else time
     o.interp()
-    if nudge:
-        Ta = o.exchange_arrays["T"]
-        Ta[...] = np.nextafter(Ta, np.inf if nudge > 0 else -np.inf)
+    if nudge and nudge[0] != "sst":
+        x = o.exchange_arrays[nudge[0]]
+        x[...] = np.nextafter(x, np.inf * nudge[1])
     o.flux_solve()
     return o
 
 
 def _flux_envelope(om, T=None, S=None):
-    """Per-element response of the oracle's column solve to a one-ulp change of the air temperature,
-    up and down. Where θₐ − Tₛ nearly cancels (a column with almost no air-sea temperature
-    difference), or the skin balance's denominator κΔT − Ωᵀδ does, any one-ulp difference inside the
-    column (a transcendental's last bit) is amplified by the same factor; this measures it."""
+    """Per-element response of the oracle's column solve to a one-ulp change of each of its main inputs
+    (air temperature, humidity, wind, SST), up and down. Where θₐ − Tₛ nearly cancels (a column with
+    almost no air-sea temperature difference), or the skin balance's denominator κΔT − Ωᵀδ does, or
+    an iteration stops short of its fixed point, any one-ulp difference inside the column (a
+    transcendental's last bit) is amplified by the same factor; this measures it."""
     T = om.T if T is None else T
     S = om.S if S is None else S
     base = _oracle_on_state(om, T, S)
     env = {k: np.zeros_like(v) for k, v in base.flux_arrays.items()}
-    for nudge in (1, -1):
-        o = _oracle_on_state(om, T, S, nudge=nudge)
-        for k in env:
-            env[k] = np.maximum(env[k], np.abs(o.flux_arrays[k] - base.flux_arrays[k]))
+    for name in ("T", "q", "u", "sst"):
+        for sign in (1, -1):
+            o = _oracle_on_state(om, T, S, nudge=(name, sign))
+            for k in env:
+                env[k] = np.maximum(env[k], np.abs(o.flux_arrays[k] - base.flux_arrays[k]))
     return env
 
 

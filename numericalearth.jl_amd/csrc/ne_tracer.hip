@@ -11,8 +11,9 @@
 // in a register window. Row-dependent values are wavefront-uniform (one wavefront per row)
 // and kept in scalar registers. Tiles are ordered so that each XCD (blocks are dealt
 // round-robin to the 8 XCDs) receives a contiguous band of tiles, keeping halo re-reads
-// inside one L2. The FP64 kernel fits 128 VGPRs (4 wavefronts per SIMD, so 4 blocks of 4
-// per CU); the FP32-storage kernel is asked for 6 wavefronts per SIMD.
+// inside one L2. Register pressure sets the occupancy (LDS does not): the FP64 kernel runs 5
+// wavefronts per SIMD, the FP32-storage kernel 6 or more; instances without the two-colour
+// forcing carry none of its registers.
 #include "ne_internal.h"
 
 namespace ne {
@@ -24,10 +25,13 @@ constexpr int TC = 63;            // cells per tile row
 #ifndef NE_PLAIN_TR
 #define NE_PLAIN_TR 4
 #endif
-// Minimum wavefronts per SIMD requested for the FP32-storage stage kernels (the FP64 ones are
-// fastest at the compiler's choice, which fits them in 128 VGPRs without spilling).
+// Minimum wavefronts per SIMD requested for the stage kernels, per storage type (measured:
+// FP64 5 — 95 VGPRs with a 20-byte spill beats 4 without; FP32 6).
 #ifndef NE_STAGE_WAVES_PER_EU_F32
 #define NE_STAGE_WAVES_PER_EU_F32 6
+#endif
+#ifndef NE_STAGE_WAVES_PER_EU_F64
+#define NE_STAGE_WAVES_PER_EU_F64 5
 #endif
 constexpr int PLAIN_TR = NE_PLAIN_TR;  // cell rows per tile of the single-model kernel
 constexpr int NXCD = 8;
@@ -234,11 +238,17 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
     const double J0 = (tc && cell) ? ldb(F.J0, joff()) : 0.0;
     const double eta = tc ? g.zf[Nz] : 0.0;
 
-    // Inputs of one level: its plane (staged in LDS below), the velocities of the faces this
-    // work-item reconstructs and c⁰ of its cell. (Loading level k+1 during level k was measured
-    // slower: the extra live registers cost more occupancy than the overlap gains.)
+    // Inputs of one level, loaded at its start so that the first barrier's wait covers their
+    // latency: its plane (staged in LDS below), the velocities of the faces this work-item
+    // reconstructs and, for FP32 storage, c⁰ of its cell (FP64 loads c⁰ at the update: its two
+    // extra registers would cost a wavefront per SIMD). Measured alternatives, all slower:
+    // loading level k+1 during level k (more live registers, lower occupancy), loading the
+    // velocities at their point of use (exposed latency), the S vertical reconstruction after
+    // the second barrier.
+    constexpr bool EARLY_C0 = sizeof(R) == 4;
     const bool ynorth = row == TR - 1;
     R nT[NPL], nS[NPL];
+    R nu = (R)0, nv = (R)0, nvn = (R)0, nw = (R)0, nT0 = (R)0, nS0 = (R)0;
     auto load_level = [&](int k) {
         const uint32_t kc = (uint32_t)k * L.c_sz;
 #pragma unroll
@@ -246,6 +256,16 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
             const bool a = tid + q * NTHR < PN && k >= pkm[q];
             nT[q] = a ? ldb(Tin, poff[q] + kc) : (R)0;
             nS[q] = a ? ldb(Sin, poff[q] + kc) : (R)0;
+        }
+        nu = order_at(k, Xk) > 0 ? ldb(u, uoff + (uint32_t)k * L.u_sz) : (R)0;
+        nv = order_at(k, Sk) > 0 ? ldb(v, voff_s + (uint32_t)k * L.v_sz) : (R)0;
+        nvn = order_at(k, Nk) > 0 ? ldb(v, voff_s + dvn + (uint32_t)k * L.v_sz) : (R)0;
+        const bool zf = cell && adv && k >= mykmin && k <= Nz - 2;
+        nw = zf ? ldb(w, coff + dw + (uint32_t)(k + 1) * L.w_sz) : (R)0;
+        if (EARLY_C0 && MODE == 0) {
+            const bool upd = cell && k >= mykmin;
+            nT0 = upd ? ldb(T0, coff + kc) : (R)0;
+            nS0 = upd ? ldb(S0, coff + kc) : (R)0;
         }
     };
     for (int k = kstart; k < Nz; k++) {
@@ -274,8 +294,7 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                 const R* t = (tr ? sh.cS : sh.cT) + base;
                 R f = (R)0;
                 if (order > 0) {
-                    const R uk = ldb(u, uoff + (uint32_t)k * L.u_sz);
-                    f = A * uk * face_value(t[0], t[1], t[2], t[3], t[4], t[5], order, uk);
+                    f = A * nu * face_value(t[0], t[1], t[2], t[3], t[4], t[5], order, nu);
                 }
                 (tr ? sh.fxS : sh.fxT)[row * TW + lane] = f;
             }
@@ -290,7 +309,7 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                 const int order = order_at(k, side ? Nk : Sk);
                 R f = (R)0;
                 if (order > 0) {
-                    const R vel = ldb(v, voff_s + (side ? dvn : 0u) + (uint32_t)k * L.v_sz);
+                    const R vel = side ? nvn : nv;
                     const R A = (R)Ay * dzk;
                     const R* t = (tr ? sh.cS : sh.cT) + r0 * PXW + lane + 3;
                     f = A * vel * face_value(t[0], t[PXW], t[2 * PXW], t[3 * PXW], t[4 * PXW], t[5 * PXW], order, vel);
@@ -298,17 +317,17 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                 (tr ? sh.fyS : sh.fyT)[r0 * TC + lane] = f;
             }
         }
-        // 2c. z face at the top of level k (cells k-2 … k+3 of the owned column): T here, S after
-        // the barrier, so that the two reconstructions are not live at the same time
+        // 2c. z face at the top of level k (cells k-2 … k+3 of the owned column)
         const int zorder = (cell && adv) ? (k >= mykmin && k <= Nz - 2 ? 1 : 0) +
                                                (k >= mykmin + 1 && k <= Nz - 3 ? 1 : 0) +
                                                (k >= mykmin + 2 && k <= Nz - 4 ? 1 : 0)
                                          : 0;
         R FztT = (R)0, FztS = (R)0;
-        const R wk = zorder > 0 ? ldb(w, coff + dw + (uint32_t)(k + 1) * L.w_sz) : (R)0;
-        if (zorder > 0) FztT = (R)Azc * wk * face_value(zT[0], zT[1], zT[2], zT[3], zT[4], zT[5], zorder, wk);
+        if (zorder > 0) {
+            FztT = (R)Azc * nw * face_value(zT[0], zT[1], zT[2], zT[3], zT[4], zT[5], zorder, nw);
+            FztS = (R)Azc * nw * face_value(zS[0], zS[1], zS[2], zS[3], zS[4], zS[5], zorder, nw);
+        }
         __syncthreads();
-        if (zorder > 0) FztS = (R)Azc * wk * face_value(zS[0], zS[1], zS[2], zS[3], zS[4], zS[5], zorder, wk);
 
         // 3. difference the face fluxes, add the top boundary flux, update
         if (cell && k >= mykmin) {
@@ -347,9 +366,11 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                 // The update is formed in FP64 and rounded once to the storage type (for FP32
                 // storage this keeps the per-stage rounding of the state at half an ulp).
                 const uint32_t key = 2u * ((uint32_t)ci + (uint32_t)g.Nx * ((uint32_t)cj + (uint32_t)g.Ny * k));
-                stb(Tout, coff + kc, round_state<R>((double)ldb(T0, coff + kc) + beta_dt * (double)GT, key, seed,
+                const R c0T = EARLY_C0 ? nT0 : ldb(T0, coff + kc);
+                const R c0S = EARLY_C0 ? nS0 : ldb(S0, coff + kc);
+                stb(Tout, coff + kc, round_state<R>((double)c0T + beta_dt * (double)GT, key, seed,
                                                     stochastic));
-                stb(Sout, coff + kc, round_state<R>((double)ldb(S0, coff + kc) + beta_dt * (double)GS, key + 1u,
+                stb(Sout, coff + kc, round_state<R>((double)c0S + beta_dt * (double)GS, key + 1u,
                                                     seed, stochastic));
             } else {
                 stb(Tout, coff + kc, GT);
@@ -374,7 +395,7 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
 // round-robin to the 8 XCDs) receives a contiguous band of tiles.
 template <typename Real, int MODE, bool TWO_COLOR>
 __global__ void __launch_bounds__(TW * PLAIN_TR)
-__attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? NE_STAGE_WAVES_PER_EU_F32 : 1)))
+__attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? NE_STAGE_WAVES_PER_EU_F32 : NE_STAGE_WAVES_PER_EU_F64)))
 tracer_tile_kernel(GridDev g, TracerLayout L, const Real* u, const Real* v, const Real* w, const Real* T0,
                    const Real* S0, const Real* Tin, const Real* Sin, Real* Tout, Real* Sout, const double* JT,
                    const double* JS, const double* Jw, TracerForcing F, double beta_dt, int stochastic,
@@ -395,7 +416,7 @@ tracer_tile_kernel(GridDev g, TracerLayout L, const Real* u, const Real* v, cons
 // stage 1: T¹ = T + β₁Δt G(T); stage 2: T² = T + β₂Δt G(T¹); stage 3: T = T + β₃Δt G(T²).
 template <typename Real, int TR, bool TWO_COLOR>
 __global__ void __launch_bounds__(TW * TR)
-__attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? NE_STAGE_WAVES_PER_EU_F32 : 1)))
+__attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? NE_STAGE_WAVES_PER_EU_F32 : NE_STAGE_WAVES_PER_EU_F64)))
 batch_stage_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __restrict__ work, int nwork,
                    int stage, double beta_dt) {
     __shared__ TileShared<Real, TR> sh;

@@ -39,16 +39,19 @@ def assert_rel(a, b, rtol, name=""):
 
 
 def assert_rel_envelope(a, b, envelope, rtol, name=""):
-    """|a − b| ≤ max(rtol·|b|, envelope) elementwise and ≤ rtol·max|b| over the field, where
-    envelope is a per-element bound on the rounding-level response of b (see the caller)."""
+    """|a − b| ≤ max(rtol·|b|, envelope) elementwise, and what exceeds the envelope ≤ rtol·max|b|
+    over the field, where envelope is a per-element bound on the rounding-level response of b
+    (see the caller)."""
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
+    env = np.asarray(envelope, dtype=np.float64)
     assert np.all(np.isfinite(a)) == np.all(np.isfinite(b)), f"{name}: finiteness differs"
     diff = np.abs(a - b)
-    bound = np.maximum(rtol * np.abs(b), np.asarray(envelope, dtype=np.float64))
+    bound = np.maximum(rtol * np.abs(b), env)
     over = diff > bound
     scale = float(np.max(np.abs(b))) if b.size else 0.0
-    field = float(diff.max(initial=0.0) / scale) if scale > 0 else float(diff.max(initial=0.0))
+    excess = np.maximum(diff - env, 0.0)
+    field = float(excess.max(initial=0.0) / scale) if scale > 0 else float(excess.max(initial=0.0))
     assert not over.any(), (f"{name}: {int(over.sum())} elements beyond max(rtol·|b|, envelope); worst "
                             f"|Δ| {float(diff[over].max()):.3e} against bound {float(bound[over].min()):.3e}")
     assert field <= rtol, f"{name}: field-scaled error {field:.3e} > {rtol:.1e}"

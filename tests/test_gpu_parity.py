@@ -145,6 +145,18 @@ def _flux_envelope(om, T=None, S=None):
             o = _oracle_on_state(om, T, S, nudge=(name, sign))
             for k in env:
                 env[k] = np.maximum(env[k], np.abs(o.flux_arrays[k] - base.flux_arrays[k]))
+    # θ★ and the sensible heat are proportional to Δθ = θₐ − Tₛ: a one-ulp change of a computed
+    # interface temperature Tₛ (the skin balance's result) changes them by |Tₛ/Δθ| ulps, a factor
+    # the input nudges above need not excite
+    p = base.params
+    cp_d = p.gas_constant / p.dry_air_molar_mass / p.dry_air_adiabatic_exponent
+    qa, Ta = base.exchange_arrays["q"], base.exchange_arrays["T"]
+    theta = Ta + p.gravitational_acceleration * p.surface_layer_height / (cp_d + (p.water_vapor_heat_capacity - cp_d) * qa)
+    Ts = base.flux_arrays["interface_temperature"] + 273.15
+    with np.errstate(divide="ignore", invalid="ignore"):
+        cond = np.where(theta != Ts, np.abs(Ts) / np.abs(theta - Ts), 0.0)
+    for k in ("temperature_scale", "sensible_heat"):
+        env[k] = np.maximum(env[k], np.abs(base.flux_arrays[k]) * cond * 2.0 ** -52)
     return env
 
 

@@ -452,7 +452,25 @@ typedef struct ne_model_desc {
                                              buffers (ne_model_workspace_bytes); NULL: the model
                                              allocates them itself */
     int64_t workspace_bytes;
+    /* Snapshot time indexing (FieldTimeSeries time_indexing; JRA55 atmospheres are Cyclical,
+     * DataWrangling/JRA55/JRA55_prescribed_atmosphere.jl:17) and in-memory windows (the
+     * InMemory(N) backend of a FieldTimeSeries, JRA55_prescribed_atmosphere.jl:42). */
+    int32_t time_indexing;                /* NE_TIME_CLAMPED or NE_TIME_CYCLICAL */
+    int32_t atmosphere_slots;             /* 0: every snapshot resident in the atmosphere views;
+                                             N ≥ 2: the views hold N snapshot slots, snapshot n in
+                                             slot n mod N, uploaded by the caller
+                                             (ne_model_snapshots_at, ne_model_snapshot_resident) */
+    int32_t radiation_slots;              /* the same for the radiation views */
+    int32_t reserved2;
+    double  time_period;                  /* NE_TIME_CYCLICAL: the period (s), ≥ last − first time */
 } ne_model_desc;
+
+/* Time indexing of the snapshot series. CLAMPED: linear between the two snapshots around t,
+ * the first / last snapshot outside the series (the reference's Linear indexing inside the
+ * series). CYCLICAL: Oceananigans Cyclical(period): t is taken modulo the period from the
+ * first snapshot time, and between the last snapshot and the first one a period later the
+ * two are interpolated across the wrap. */
+enum { NE_TIME_CLAMPED = 0, NE_TIME_CYCLICAL = 1 };
 
 /* Bytes of device workspace a model of this descriptor needs (four stage buffers of T, S). */
 int ne_model_workspace_bytes(const ne_model_desc* desc, int64_t* bytes);
@@ -476,6 +494,43 @@ int ne_model_update_state(ne_model* model, void* stream);
  * in the model's precision, accumulated in FP64), for callers that set new horizontal
  * velocities between steps. */
 int ne_model_update_vertical_velocity(ne_model* model, void* stream);
+
+/* Snapshot time interpolation of a series (Oceananigans.OutputReaders
+ * interpolating_time_indices for Linear / Cyclical time indexing, the function a
+ * FieldTimeSeries evaluates at a time; cpu_interpolating_time_indices,
+ * src/Atmospheres/interpolate_atmospheric_state.jl:57-60): `interp` receives the two 0-based
+ * snapshots and the weight of the second, `positions` their stream positions (see
+ * ne_model_snapshots_at). The same function the models and the device clock evaluate.
+ * NE_ERR_SHAPE for an empty series or a CYCLICAL period that is not positive or shorter
+ * than the series. Host only; needs no device. */
+int ne_snapshot_time_indices(const double* times, int32_t n_times, int32_t time_indexing, double time_period,
+                             double time, ne_time_interp* interp, int64_t positions[2]);
+
+/* Stream positions of the snapshots the model interpolates between at model time `time`:
+ * out[0], out[1] of the atmosphere, out[2], out[3] of the radiation (INT64_MIN for a series
+ * with no snapshots). A position p is snapshot p mod n_times of its series (floor modulo) and
+ * fills slot p mod slots of windowed views. With CLAMPED indexing p is the snapshot index; with
+ * CYCLICAL indexing p = cycle · n_times + index, so the interpolation across the wrap reads two
+ * consecutive positions (the last snapshot and the first one of the next cycle) and a ring of
+ * two slots never has both in one slot. A caller streaming snapshots into windowed views
+ * (atmosphere_slots / radiation_slots) asks this for the time after the next step and uploads
+ * the missing ones first. Host only.
+ * (FieldTimeSeries time_indices / update_field_time_series!, Oceananigans.OutputReaders;
+ * the reference's InMemory windows are refilled through set!, JRA55_field_time_series.jl:64-80.) */
+int ne_model_snapshots_at(const ne_model* model, double time, int64_t out[4]);
+
+/* Record that the snapshot at stream position p of the atmosphere (which = 0) or the
+ * radiation (which = 1) now fills its slot p mod slots of the windowed views; the caller
+ * orders the upload before the next step on that step's stream. A step (or update_state)
+ * whose snapshots are not resident returns NE_ERR_STATE before enqueueing anything. At
+ * creation the slots are taken to hold the N consecutive positions from p₁, the first one
+ * read at time 0 (with CLAMPED indexing the run is moved to fit inside the series), as an
+ * in-memory window is filled from the snapshot the current time interpolates from; the
+ * caller fills them before ne_model_create (ne_snapshot_time_indices at time 0 gives p₁).
+ * Windowed models cannot join a batch
+ * (NE_ERR_UNSUPPORTED). NE_ERR_STATE for a series that is not windowed, NE_ERR_SHAPE for a
+ * CLAMPED position outside 0 … n_times − 1. Host only. */
+int ne_model_snapshot_resident(ne_model* model, int32_t which, int64_t position);
 
 /* Clock (seconds) and iteration count of the model. Host only. */
 int ne_model_clock(const ne_model* model, double* time, int64_t* iteration);

@@ -1,7 +1,9 @@
 // ne_abi.hip — extern "C" entry points of include/numericalearth_hip.h.
 #include <math.h>
+#include <cmath>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
 #include <vector>
 
@@ -112,8 +114,83 @@ int check_params(const ne_params* p) {
     return NE_OK;
 }
 
-ne_time_interp time_indices(const std::vector<double>& times, double t) {
-    return ne::time_indices(times.data(), (int)times.size(), t);
+ne_time_interp time_indices(const std::vector<double>& times, double t, const ne_model_desc& d) {
+    return ne::time_indices(times.data(), (int)times.size(), t, d.time_indexing, d.time_period);
+}
+
+constexpr int64_t NO_POSITION = INT64_MIN;
+
+int64_t floor_mod(int64_t a, int64_t n) {
+    int64_t r = a % n;
+    return r < 0 ? r + n : r;
+}
+
+// Stream positions of the two snapshots the interpolation at t reads: the snapshot index
+// itself, or with cyclical indexing cycle·n + index, so that the wrap from the last snapshot
+// to the first one of the next cycle reads two consecutive positions.
+struct SeriesAt {
+    ne_time_interp ti;
+    int64_t p1, p2;
+};
+
+SeriesAt series_at(const double* times, int n, double t, int indexing, double period) {
+    SeriesAt s;
+    s.ti = ne::time_indices(times, n, t, indexing, period);
+    s.p1 = s.ti.n1;
+    s.p2 = s.ti.n2;
+    if (indexing == NE_TIME_CYCLICAL && n > 1) {
+        const double x = t - times[0];
+        double tau = std::fmod(x, period);   // as in ne::time_indices
+        if (tau != 0.0 && ((tau > 0.0) != (period > 0.0))) tau += period;
+        const int64_t cycle = std::llround((x - tau) / period);
+        s.p1 = cycle * n + s.ti.n1;
+        s.p2 = (s.ti.n2 < s.ti.n1 ? cycle + 1 : cycle) * n + s.ti.n2;
+    }
+    return s;
+}
+
+SeriesAt series_at(const std::vector<double>& times, double t, const ne_model_desc& d) {
+    return series_at(times.data(), (int)times.size(), t, d.time_indexing, d.time_period);
+}
+
+// Positions a freshly created window of `slots` holds: from the first position read at
+// time t, N consecutive ones (kept inside the series with clamped indexing), as an in-memory
+// window is filled from the snapshot the current time interpolates from.
+void seed_window(std::vector<int64_t>& resident, const std::vector<double>& times, double t,
+                 const ne_model_desc& d) {
+    const int64_t N = (int64_t)resident.size(), nt = (int64_t)times.size();
+    if (N == 0 || nt == 0) return;
+    int64_t start = series_at(times, t, d).p1;
+    int64_t count = N;
+    if (d.time_indexing != NE_TIME_CYCLICAL) {
+        start = std::max<int64_t>(0, std::min(start, nt - N));
+        count = std::min(N, nt - start);
+    }
+    for (int64_t p = start; p < start + count; p++) resident[(size_t)floor_mod(p, N)] = p;
+}
+
+// Interpolation indices of a windowed series mapped to its slots (position p in slot
+// p mod slots); NE_ERR_STATE if a snapshot the interpolation reads is not resident.
+int to_slots(SeriesAt& s, int slots, const std::vector<int64_t>& resident) {
+    if (slots <= 0) return NE_OK;
+    const int s1 = (int)floor_mod(s.p1, slots), s2 = (int)floor_mod(s.p2, slots);
+    if (resident[s1] != s.p1 || resident[s2] != s.p2) return NE_ERR_STATE;
+    s.ti.n1 = s1;
+    s.ti.n2 = s2;
+    return NE_OK;
+}
+
+// The model's snapshot views at model time t (windowed series mapped to their slots); with
+// apply == false only whether they are resident.
+int model_time_views(ne_model* m, double t, bool apply = true) {
+    ne_model_desc& d = m->d;
+    SeriesAt a = series_at(m->atimes, t, d), r = series_at(m->rtimes, t, d);
+    int e = to_slots(a, d.atmosphere_slots, m->aslot);
+    if (!e && d.params.radiation) e = to_slots(r, d.radiation_slots, m->rslot);
+    if (e || !apply) return e;
+    d.atmosphere.time = a.ti;
+    d.radiation.time = r.ti;
+    return NE_OK;
 }
 
 // Bytes spanned by a view of nx × ny × nz elements (0 if a stride is not positive).
@@ -453,8 +530,8 @@ int ne_budget_f32(ne_ctx* c, const ne_grid* g, ne_view cv, double weight, double
 static int model_update_state(ne_model* m, double dt_frazil, hipStream_t s, hipEvent_t* ev = nullptr) {
     ne_model_desc& d = m->d;
     GridDev g = grid_dev(d.grid);
-    d.atmosphere.time = time_indices(m->atimes, m->time);
-    d.radiation.time = time_indices(m->rtimes, m->time);
+    int tv = model_time_views(m, m->time);
+    if (tv) return tv;
     FluxOut f = flux_out(d.fluxes);
     NetOut n = net_out(d.net);
     RadOut ro = rad_out(d.radiative);
@@ -545,6 +622,17 @@ int ne_model_create(ne_ctx* c, const ne_model_desc* desc, ne_model** out, void* 
             return r;
     }
     if (desc->n_radiation_times > 0 && !desc->radiation_times) return NE_ERR_NULL;
+    if (desc->time_indexing != NE_TIME_CLAMPED && desc->time_indexing != NE_TIME_CYCLICAL) return NE_ERR_UNSUPPORTED;
+    if (desc->time_indexing == NE_TIME_CYCLICAL) {
+        if (!(desc->time_period > 0.0)) return NE_ERR_SHAPE;
+        for (int n = 0; n < 2; n++) {
+            const double* t = n ? desc->radiation_times : desc->atmosphere_times;
+            const int nt = n ? desc->n_radiation_times : desc->n_atmosphere_times;
+            if (nt > 1 && !(desc->time_period >= t[nt - 1] - t[0])) return NE_ERR_SHAPE;
+        }
+    }
+    if (desc->atmosphere_slots < 0 || desc->radiation_slots < 0) return NE_ERR_SHAPE;
+    if (desc->atmosphere_slots == 1 || desc->radiation_slots == 1) return NE_ERR_SHAPE;   // two are read at once
 
     ne_model* m = new (std::nothrow) ne_model();
     if (!m) return NE_ERR_CAPACITY;
@@ -552,6 +640,10 @@ int ne_model_create(ne_ctx* c, const ne_model_desc* desc, ne_model** out, void* 
     m->d = *desc;
     m->atimes.assign(desc->atmosphere_times, desc->atmosphere_times + desc->n_atmosphere_times);
     m->rtimes.assign(desc->radiation_times, desc->radiation_times + desc->n_radiation_times);
+    m->aslot.assign((size_t)desc->atmosphere_slots, NO_POSITION);
+    m->rslot.assign((size_t)desc->radiation_slots, NO_POSITION);
+    seed_window(m->aslot, m->atimes, m->time, *desc);
+    seed_window(m->rslot, m->rtimes, m->time, *desc);
     const ne_grid& g = desc->grid;
     size_t n = stage_buffer_bytes(*desc);
     void** bufs[4] = {&m->T1, &m->S1, &m->T2, &m->S2};
@@ -611,9 +703,12 @@ int ne_model_time_step(ne_model* m, double dt, void* stream) {
     if (m->batch) return NE_ERR_STATE;
     if (!(dt > 0)) return NE_ERR_SHAPE;
     hipStream_t s = as_stream(stream);
+    // the snapshots of the step's end must be resident before anything is enqueued
+    int r = model_time_views(m, m->time + dt, false);
+    if (r) return r;
     hipEvent_t* ev = nullptr;
     if (m->timing && m->recorded < m->max_steps) ev = &m->events[(size_t)m->recorded * 6];
-    int r = m->d.precision == NE_FP64 ? model_ocean_step<double>(m, dt, s, ev) : model_ocean_step<float>(m, dt, s, ev);
+    r = m->d.precision == NE_FP64 ? model_ocean_step<double>(m, dt, s, ev) : model_ocean_step<float>(m, dt, s, ev);
     if (r) return r;
     m->time += dt;
     m->iteration += 1;
@@ -673,6 +768,43 @@ int ne_model_update_vertical_velocity(ne_model* m, void* stream) {
         e = launch_w_continuity<float>(g, view<const float>(d.ocean.u), view<const float>(d.ocean.v),
                                        view<float>(d.ocean.w), as_stream(stream));
     return dev_err(e);
+}
+
+int ne_snapshot_time_indices(const double* times, int32_t n_times, int32_t time_indexing, double time_period,
+                             double time, ne_time_interp* interp, int64_t positions[2]) {
+    if (!interp || !positions || (n_times > 0 && !times)) return NE_ERR_NULL;
+    if (n_times <= 0) return NE_ERR_SHAPE;
+    if (time_indexing != NE_TIME_CLAMPED && time_indexing != NE_TIME_CYCLICAL) return NE_ERR_UNSUPPORTED;
+    if (time_indexing == NE_TIME_CYCLICAL && !(time_period > 0.0 && time_period >= times[n_times - 1] - times[0]))
+        return NE_ERR_SHAPE;
+    SeriesAt s = series_at(times, n_times, time, time_indexing, time_period);
+    *interp = s.ti;
+    positions[0] = s.p1;
+    positions[1] = s.p2;
+    return NE_OK;
+}
+
+int ne_model_snapshots_at(const ne_model* m, double time, int64_t out[4]) {
+    if (!m || !out) return NE_ERR_NULL;
+    const ne_model_desc& d = m->d;
+    SeriesAt a = series_at(m->atimes, time, d), r = series_at(m->rtimes, time, d);
+    out[0] = m->atimes.empty() ? NO_POSITION : a.p1;
+    out[1] = m->atimes.empty() ? NO_POSITION : a.p2;
+    out[2] = m->rtimes.empty() ? NO_POSITION : r.p1;
+    out[3] = m->rtimes.empty() ? NO_POSITION : r.p2;
+    return NE_OK;
+}
+
+int ne_model_snapshot_resident(ne_model* m, int32_t which, int64_t position) {
+    if (!m) return NE_ERR_NULL;
+    if (which != 0 && which != 1) return NE_ERR_SHAPE;
+    std::vector<int64_t>& slot = which ? m->rslot : m->aslot;
+    const int64_t nt = (int64_t)(which ? m->rtimes.size() : m->atimes.size());
+    if (slot.empty()) return NE_ERR_STATE;           // not a windowed series
+    if (position == NO_POSITION) return NE_ERR_SHAPE;
+    if (m->d.time_indexing != NE_TIME_CYCLICAL && (position < 0 || position >= nt)) return NE_ERR_SHAPE;
+    slot[(size_t)floor_mod(position, (int64_t)slot.size())] = position;
+    return NE_OK;
 }
 
 int ne_model_clock(const ne_model* m, double* time, int64_t* iteration) {

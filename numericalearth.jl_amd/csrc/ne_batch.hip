@@ -45,8 +45,8 @@ __global__ void batch_clock_kernel(BatchRec<Real>* recs, int n, double dt) {
     BatchRec<Real>& r = recs[m];
     double t = r.time + dt;
     r.time = t;
-    r.a.time = time_indices(r.atimes, r.nat, t);
-    r.r.time = time_indices(r.rtimes, r.nrt, t);
+    r.a.time = time_indices(r.atimes, r.nat, t, r.time_indexing, r.time_period);
+    r.r.time = time_indices(r.rtimes, r.nrt, t, r.time_indexing, r.time_period);
 }
 
 }  // namespace
@@ -147,11 +147,13 @@ BatchRec<Real> member_record(const ne_model* m, const double* atimes_dev, const 
     r.g = grid_dev(d.grid);
     r.p = d.params;
     r.a = atmos_dev<Real>(d.atmosphere);
-    r.a.time = time_indices(m->atimes.data(), (int)m->atimes.size(), m->time);
+    r.a.time = time_indices(m->atimes.data(), (int)m->atimes.size(), m->time, d.time_indexing, d.time_period);
     if (d.params.radiation) {
         r.r = rad_dev<Real>(d.radiation);
-        r.r.time = time_indices(m->rtimes.data(), (int)m->rtimes.size(), m->time);
+        r.r.time = time_indices(m->rtimes.data(), (int)m->rtimes.size(), m->time, d.time_indexing, d.time_period);
     }
+    r.time_indexing = d.time_indexing;
+    r.time_period = d.time_period;
     r.afi = d.atmos_fi;
     r.afj = d.atmos_fj;
     r.rfi = d.radiation_fi;
@@ -415,6 +417,8 @@ int check_member(const ne_batch* b, const ne_model* m) {
     if (!m) return NE_ERR_NULL;
     if (m->ctx != b->ctx || m->batch || m->timing) return NE_ERR_STATE;
     if (m->d.precision != b->precision) return NE_ERR_UNSUPPORTED;
+    // the device clock indexes whole series: windowed snapshot views stay with single models
+    if (m->d.atmosphere_slots > 0 || m->d.radiation_slots > 0) return NE_ERR_UNSUPPORTED;
     return NE_OK;
 }
 

@@ -766,9 +766,10 @@ __device__ __forceinline__ double interp_field(const View3<const Real>& d, const
     return (1.0 - t.tilde_n) * a + t.tilde_n * b;
 }
 
-// cpu_interpolating_time_indices with Linear time indexing, clamped at both ends
-// (interpolate_atmospheric_state.jl:57-60). The same arithmetic on the host and on the
-// device, so a device-side clock reproduces the host's indices exactly.
+// cpu_interpolating_time_indices (interpolate_atmospheric_state.jl:57-60): linear between
+// the two snapshots around t inside the series, clamped to the first / last snapshot
+// outside it. The same arithmetic on the host and on the device, so a device-side clock
+// reproduces the host's indices exactly.
 __host__ __device__ inline ne_time_interp time_indices(const double* times, int n, double t) {
     ne_time_interp r = {0, 0, 0.0};
     if (n <= 1 || t <= times[0]) return r;
@@ -781,6 +782,25 @@ __host__ __device__ inline ne_time_interp time_indices(const double* times, int 
     r.n1 = k;
     r.n2 = k + 1;
     r.tilde_n = (t - times[k]) / (times[k + 1] - times[k]);
+    return r;
+}
+
+// Oceananigans Cyclical(period) indexing: τ = mod(t − t¹, period) (Julia's mod: the result
+// has the sign of the period), t' = t¹ + τ; inside the series as above, and between the last
+// snapshot and the first one a period later, ñ = (t' − tᴺ)/(period − (tᴺ − t¹)) from the last
+// snapshot to the first.
+__host__ __device__ inline ne_time_interp time_indices(const double* times, int n, double t, int indexing,
+                                                       double period) {
+    if (indexing != NE_TIME_CYCLICAL || n <= 1) return time_indices(times, n, t);
+    const double t1 = times[0], tN = times[n - 1];
+    double tau = fmod(t - t1, period);
+    if (tau != 0.0 && ((tau > 0.0) != (period > 0.0))) tau += period;
+    const double tm = tau + t1;
+    if (tm <= tN) return time_indices(times, n, tm);
+    ne_time_interp r;
+    r.n1 = n - 1;
+    r.n2 = 0;
+    r.tilde_n = (tm - tN) / (period - (tN - t1));
     return r;
 }
 

@@ -32,6 +32,8 @@ struct ne_model {
     std::vector<hipEvent_t> events;
     bool owns_buffers = true;           // stage buffers allocated by the model (no workspace given)
     struct ne_batch* batch = nullptr;   // the batch this model belongs to, if any
+    // windowed snapshot views: the snapshot held by each slot (−1: none)
+    std::vector<int64_t> aslot, rslot;   // stream position held by each snapshot slot (windowed series)
 };
 
 namespace ne {
@@ -193,6 +195,8 @@ struct BatchRec {
     const double* atimes;   // device copies of the snapshot times
     const double* rtimes;
     int nat, nrt;
+    int time_indexing;      // NE_TIME_CLAMPED or NE_TIME_CYCLICAL
+    double time_period;
     int fast;               // default Edson constants: the shared-ψ flux path
     double time;           // device clock of the member (advanced by the clock kernel)
 };

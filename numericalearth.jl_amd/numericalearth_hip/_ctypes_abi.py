@@ -40,6 +40,8 @@ NE_ROUGHNESS_CONSTANT, NE_ROUGHNESS_EDSON = 0, 1
 NE_STOP_CONVERGENCE, NE_STOP_FIXED_ITERATIONS = 0, 1
 NE_HUMIDITY_IMPURE_SATURATION, NE_HUMIDITY_FIXED = 0, 1
 NE_VELOCITY_RELATIVE, NE_VELOCITY_WIND = 0, 1
+NE_TIME_CLAMPED, NE_TIME_CYCLICAL = 0, 1
+NO_POSITION = -(2 ** 63)   # ne_model_snapshots_at: a series with no snapshots
 
 _d = C.c_double
 _i32 = C.c_int32
@@ -234,6 +236,11 @@ class ModelDesc(C.Structure):
         ("radiation_times", _pd),
         ("workspace", _p),
         ("workspace_bytes", C.c_int64),
+        ("time_indexing", _i32),
+        ("atmosphere_slots", _i32),
+        ("radiation_slots", _i32),
+        ("reserved2", _i32),
+        ("time_period", C.c_double),
     ]
 
 

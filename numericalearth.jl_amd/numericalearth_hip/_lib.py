@@ -61,6 +61,10 @@ SIGNATURES = {
     "ne_model_timing_end": [vp, P(C.c_double), P(C.c_int)],
     "ne_model_workspace_bytes": [P(abi.ModelDesc), P(C.c_int64)],
     "ne_model_update_vertical_velocity": [vp, vp],
+    "ne_snapshot_time_indices": [P(C.c_double), C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                 P(abi.TimeInterp), P(C.c_int64)],
+    "ne_model_snapshots_at": [vp, C.c_double, P(C.c_int64)],
+    "ne_model_snapshot_resident": [vp, C.c_int32, C.c_int64],
     "ne_compute_w_from_continuity_f64": [vp, P(abi.Grid), abi.View, abi.View, abi.View, vp],
     "ne_batch_create": [vp, C.c_int32, C.c_int, P(vp), P(vp), vp],
     "ne_batch_destroy": [vp],

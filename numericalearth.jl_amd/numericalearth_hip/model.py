@@ -20,6 +20,7 @@ from . import _ctypes_abi as abi
 from ._lib import check, load
 from .generators import SurfaceCase
 from .params import default_params
+from .time_series import ArraySnapshots, SnapshotStream, initial_window, resolve_time_indexing
 
 FLUX_NAMES = ("latent_heat", "sensible_heat", "water_vapor", "x_momentum", "y_momentum",
               "friction_velocity", "temperature_scale", "water_vapor_scale", "interface_temperature")
@@ -115,7 +116,13 @@ class OceanOnlyModel:
     """
 
     def __init__(self, case: SurfaceCase, *, params: abi.Params | None = None, precision: str = "fp64",
-                 radiation: bool = True, device: int = 0):
+                 radiation: bool = True, device: int = 0, time_indexing=None, atmosphere_window: int | None = None,
+                 radiation_window: int | None = None, atmosphere_source=None, radiation_source=None,
+                 lookahead: int | None = None):
+        """time_indexing: Clamp() (default) or Cyclical(period) of both snapshot series.
+        atmosphere_window / radiation_window: keep only that many snapshots of the series on the device
+        (a ring of slots, time_indices_in_memory) and stream the others from `atmosphere_source` /
+        `radiation_source` (an ArraySnapshots or NpySnapshotStore; the case's arrays by default)."""
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.case = case
@@ -134,8 +141,36 @@ class OceanOnlyModel:
         self.u = _dev(case.u, dev, R)
         self.v = _dev(case.v, dev, R)
         self.w = _dev(case.w, dev, R)
-        self.atm = {k: _dev(v, dev, R) for k, v in case.atmos.items()}
-        self.rad = {k: _dev(v, dev, R) for k, v in case.rad.items()}
+        self._atimes = np.ascontiguousarray(case.atmos_times, dtype=np.float64)
+        self._rtimes = np.ascontiguousarray(case.rad_times, dtype=np.float64)
+        self.time_indexing = time_indexing
+        self.time_indexing_kind, self.time_period = resolve_time_indexing(time_indexing, self._atimes)
+        sources = {"atmosphere": atmosphere_source or ArraySnapshots(case.atmos, self._atimes),
+                   "radiation": radiation_source or ArraySnapshots(case.rad, self._rtimes)}
+        windows = {"atmosphere": atmosphere_window, "radiation": radiation_window if radiation else None}
+        resident = {}
+        for series, arrays, times in (("atmosphere", case.atmos, self._atimes), ("radiation", case.rad, self._rtimes)):
+            N = windows[series]
+            if N is None:
+                continue
+            if int(N) < 2:
+                raise ValueError(f"{series}_window must hold at least two snapshots")
+            src = sources[series]
+            if src.n_times != len(times) or not np.array_equal(src.times, times):
+                raise ValueError(f"the {series} source's times differ from the case's")
+            resident[series] = initial_window(times, int(N), time_indexing)
+        self.atm, self.rad = {}, {}
+        for series, arrays, out in (("atmosphere", case.atmos, self.atm), ("radiation", case.rad, self.rad)):
+            if series not in resident:
+                out.update({k: _dev(v, dev, R) for k, v in arrays.items()})
+                continue
+            src, N = sources[series], int(windows[series])
+            stage = np.empty(src.shape[1:], dtype=np.float64 if R == torch.float64 else np.float32)
+            for k in arrays:
+                out[k] = torch.zeros((N,) + src.shape[1:], dtype=R, device=dev)
+                for p in resident[series]:       # the window as created: position p in slot p mod N
+                    src.read(p % src.n_times, k, stage)
+                    out[k][p % N].copy_(torch.from_numpy(stage))
         self.src_atmos = _source(case.atmos_grid)
         self.src_rad = _source(case.rad_grid)
         self.afi = torch.empty(Nx, dtype=f64, device=dev)
@@ -155,8 +190,6 @@ class OceanOnlyModel:
         self.shortwave_surface_flux = z2()    # J₀ of TwoColorRadiation
         self.radiative = {k: z2() for k in RAD_NAMES}
         self.frazil_heat = z2()
-        self._atimes = np.ascontiguousarray(case.atmos_times, dtype=np.float64)
-        self._rtimes = np.ascontiguousarray(case.rad_times, dtype=np.float64)
         d = abi.ModelDesc()
         d.precision = abi.NE_FP64 if precision == "fp64" else abi.NE_MIXED_FP32
         d.fuse_freeze_clamp = 0
@@ -187,18 +220,42 @@ class OceanOnlyModel:
         check(self.lib.ne_model_workspace_bytes(C.byref(d), C.byref(nbytes)), "ne_model_workspace_bytes")
         self._workspace = torch.empty(nbytes.value, dtype=torch.uint8, device=dev)
         d.workspace, d.workspace_bytes = self._workspace.data_ptr(), nbytes.value
+        d.time_indexing, d.time_period = self.time_indexing_kind, self.time_period
+        d.atmosphere_slots = int(atmosphere_window) if "atmosphere" in resident else 0
+        d.radiation_slots = int(radiation_window) if "radiation" in resident else 0
         self.desc = d
         h = C.c_void_p()
         check(self.lib.ne_model_create(self.ctx.handle, C.byref(d), C.byref(h), s), "ne_model_create")
         self.handle = h
         self._budget_out = torch.zeros(2, dtype=f64, device=dev)
+        self.streams = {}
+        for which, series, fields in ((0, "atmosphere", self.atm), (1, "radiation", self.rad)):
+            if series in resident:
+                self.streams[series] = SnapshotStream(self, which, sources[series], fields, resident[series],
+                                                      lookahead=lookahead)
 
     # reference API ------------------------------------------------------------
     def time_step(self, dt: float, stream=None) -> None:
+        if self.streams:
+            self._prepare(self.clock[0] + float(dt), stream)
         check(self.lib.ne_model_time_step(self.handle, float(dt), stream_handle(stream)), "ne_model_time_step")
 
     def update_state(self, stream=None) -> None:
+        if self.streams:
+            self._prepare(self.clock[0], stream)
         check(self.lib.ne_model_update_state(self.handle, stream_handle(stream)), "ne_model_update_state")
+
+    def snapshots_at(self, time: float) -> tuple[int, int, int, int]:
+        """Stream positions of the atmosphere's and the radiation's two snapshots read at `time`."""
+        out = (C.c_int64 * 4)()
+        check(self.lib.ne_model_snapshots_at(self.handle, float(time), out), "ne_model_snapshots_at")
+        return tuple(out)
+
+    def _prepare(self, time: float, stream) -> None:
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        pos = self.snapshots_at(time)
+        for series, st in self.streams.items():
+            st.prepare(pos[0:2] if series == "atmosphere" else pos[2:4], s)
 
     def update_vertical_velocity(self, stream=None) -> None:
         """w from continuity of the current u and v (after the caller sets new horizontal velocities)."""
@@ -236,6 +293,8 @@ class OceanOnlyModel:
         return self.budget(self.T, self.params.ocean_reference_density * self.params.ocean_heat_capacity)
 
     def close(self) -> None:
+        for st in getattr(self, "streams", {}).values():
+            st.close()
         if getattr(self, "handle", None):
             torch.cuda.synchronize(self.device)
             check(self.lib.ne_model_destroy(self.handle), "ne_model_destroy")

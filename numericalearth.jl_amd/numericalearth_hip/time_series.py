@@ -1,0 +1,236 @@
+"""Snapshot time indexing of prescribed forcing, snapshot stores, and host→device streaming.
+
+``Clamp`` and ``Cyclical`` name Oceananigans.OutputReaders' time indexing of a
+FieldTimeSeries; the reference builds its JRA55 atmosphere and radiation series with
+``Cyclical()`` (DataWrangling/JRA55/JRA55_prescribed_atmosphere.jl:17,
+JRA55_prescribed_radiation.jl:14) and evaluates the indices on the host
+(``cpu_interpolating_time_indices``, Atmospheres/interpolate_atmospheric_state.jl:57-60).
+``interpolating_time_indices`` is that function, evaluated by the library
+(``ne_snapshot_time_indices``) exactly as the models and the batched device clock evaluate it.
+
+The reference keeps ``time_indices_in_memory`` snapshots of each series on the device (an
+InMemory(N) backend) and refills the whole window through ``set!`` when the clock leaves it
+(JRA55_field_time_series.jl:64-80). Here a window is a ring of N slots: the snapshot at stream
+position p (the snapshot index, or cycle·Nt + index with cyclical indexing) fills slot p mod N,
+and ``SnapshotStream`` uploads one snapshot at a time on a copy stream, ahead of the step that
+first reads it, while the steps before it run.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from concurrent.futures import Future, ThreadPoolExecutor
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import _ctypes_abi as abi
+from ._lib import check, load
+
+
+@dataclass(frozen=True)
+class Clamp:
+    """Linear between the two snapshots around t; the first / last snapshot outside the series."""
+
+
+@dataclass(frozen=True)
+class Cyclical:
+    """Oceananigans Cyclical(period): t is taken modulo the period from the first snapshot time,
+    and between the last snapshot and the first one a period later the two are interpolated.
+    ``period=None`` is Cyclical() with the period inferred from the times as Oceananigans infers
+    it: last − first plus the last interval."""
+
+    period: float | None = None
+
+
+def resolve_time_indexing(indexing, times) -> tuple[int, float]:
+    """(NE_TIME_*, period) of a Clamp / Cyclical object (None is Clamp())."""
+    if indexing is None or isinstance(indexing, Clamp):
+        return abi.NE_TIME_CLAMPED, 0.0
+    if isinstance(indexing, Cyclical):
+        period = indexing.period
+        if period is None:
+            times = np.asarray(times, dtype=np.float64)
+            if len(times) < 2:
+                raise ValueError("Cyclical() infers its period from at least two snapshot times")
+            period = float(times[-1] - times[0] + (times[-1] - times[-2]))
+        return abi.NE_TIME_CYCLICAL, float(period)
+    raise TypeError(f"time indexing must be Clamp() or Cyclical(period), not {indexing!r}")
+
+
+def interpolating_time_indices(times, t: float, indexing=None) -> tuple[abi.TimeInterp, tuple[int, int]]:
+    """The two snapshots read at time t, the weight of the second, and their stream positions."""
+    times = np.ascontiguousarray(times, dtype=np.float64)
+    kind, period = resolve_time_indexing(indexing, times)
+    ti = abi.TimeInterp()
+    pos = (C.c_int64 * 2)()
+    check(load().ne_snapshot_time_indices(times.ctypes.data_as(C.POINTER(C.c_double)), len(times), kind, period,
+                                          float(t), C.byref(ti), pos), "ne_snapshot_time_indices")
+    return ti, (pos[0], pos[1])
+
+
+def initial_window(times, slots: int, indexing=None, time: float = 0.0) -> list[int]:
+    """Stream positions a freshly created window of `slots` holds: N consecutive positions from the
+    first one read at `time` (moved to fit inside the series with clamped indexing)."""
+    nt = len(times)
+    _, (start, _) = interpolating_time_indices(times, time, indexing)
+    count = slots
+    if resolve_time_indexing(indexing, times)[0] != abi.NE_TIME_CYCLICAL:
+        start = max(0, min(start, nt - slots))
+        count = min(slots, nt - start)
+    return list(range(start, start + count))
+
+
+# -- snapshot sources -----------------------------------------------------------------------------
+class ArraySnapshots:
+    """Snapshots of named fields held in host arrays shaped (Nt, Ny, Nx), i fastest."""
+
+    def __init__(self, fields: dict, times):
+        self.fields = fields
+        self.times = np.ascontiguousarray(times, dtype=np.float64)
+        shapes = {np.shape(a) for a in fields.values()}
+        if len(shapes) != 1:
+            raise ValueError(f"snapshot fields differ in shape: {shapes}")
+        self.shape = shapes.pop()
+        if self.shape[0] != len(self.times):
+            raise ValueError(f"{self.shape[0]} snapshots but {len(self.times)} times")
+
+    @property
+    def n_times(self) -> int:
+        return len(self.times)
+
+    def read(self, n: int, name: str, out: np.ndarray) -> None:
+        """Copy snapshot n of `name` into `out` (converted to out's dtype)."""
+        np.copyto(out, self.fields[name][n], casting="same_kind")
+
+
+class NpySnapshotStore(ArraySnapshots):
+    """A directory of ``<field>.npy`` arrays (Nt, Ny, Nx) and ``times.npy``, memory-mapped read-only, so a
+    series far larger than host memory is read one snapshot at a time."""
+
+    def __init__(self, directory, names):
+        d = Path(directory)
+        fields = {k: np.load(d / f"{k}.npy", mmap_mode="r") for k in names}
+        super().__init__(fields, np.load(d / "times.npy"))
+
+    @staticmethod
+    def write(directory, fields: dict, times) -> "Path":
+        d = Path(directory)
+        d.mkdir(parents=True, exist_ok=True)
+        for k, a in fields.items():
+            np.save(d / f"{k}.npy", np.ascontiguousarray(a))
+        np.save(d / "times.npy", np.ascontiguousarray(times, dtype=np.float64))
+        return d
+
+
+# -- streaming ------------------------------------------------------------------------------------
+class SnapshotStream:
+    """Uploads the snapshots of one windowed series of a model into its ring of slots.
+
+    ``prepare(time, stream)`` is called before each step (or update_state) that reads the series at
+    `time`: snapshots it reads that are not resident are read (on a worker thread), staged in pinned
+    host memory and copied on a copy stream that first waits for the work already on `stream` (the
+    slot may still be read there); `stream` then waits for the copy. The following `lookahead`
+    positions are read and copied ahead in the same way without making `stream` wait, so their copy
+    overlaps the steps before them. Positions are marked resident in the model when their copy is
+    enqueued; every step of the model must be prepared (OceanOnlyModel.time_step does it)."""
+
+    def __init__(self, model, which: int, source: ArraySnapshots, device_fields: dict, resident: list[int],
+                 lookahead: int | None = None, workers: int = 1):
+        self.model = model
+        self.which = which
+        self.source = source
+        self.fields = device_fields
+        self.names = tuple(device_fields)
+        first = next(iter(device_fields.values()))
+        self.slots = first.shape[0]
+        self.device = first.device
+        self.dtype = first.dtype
+        self.cyclical = model.time_indexing_kind == abi.NE_TIME_CYCLICAL
+        self.lookahead = max(0, self.slots - 2) if lookahead is None else min(int(lookahead), self.slots - 2)
+        self.copy_stream = torch.cuda.Stream(device=self.device)
+        self.pool = ThreadPoolExecutor(max_workers=workers)
+        self.slot_position = {int(p % self.slots): int(p) for p in resident}
+        self.reads: dict[int, Future] = {}     # position -> Future of the pinned staging buffer
+        self.copies: dict[int, torch.cuda.Event] = {}   # position -> event after its copy
+        self.free: list[tuple[torch.Tensor, torch.cuda.Event | None]] = []
+        self.uploads = 0                        # snapshots copied to the device
+        self.stalls = 0                         # prepares that waited for a read
+
+    def _snapshot(self, p: int) -> int:
+        return p % self.source.n_times
+
+    def _valid(self, p: int) -> bool:
+        return self.cyclical or 0 <= p < self.source.n_times
+
+    def _staging(self) -> torch.Tensor:
+        if self.free:
+            buf, ev = self.free.pop()
+            if ev is not None:
+                ev.synchronize()   # its previous copy has left the buffer
+            return buf
+        shape = (len(self.names),) + tuple(next(iter(self.fields.values())).shape[1:])
+        return torch.empty(shape, dtype=self.dtype, pin_memory=True)
+
+    def _read(self, p: int, buf: torch.Tensor) -> torch.Tensor:
+        n = self._snapshot(p)
+        host = buf.numpy()
+        for k, name in enumerate(self.names):
+            self.source.read(n, name, host[k])
+        return buf
+
+    def _request(self, p: int) -> None:
+        if p not in self.reads and p not in self.copies and self.slot_position.get(p % self.slots) != p:
+            self.reads[p] = self.pool.submit(self._read, p, self._staging())
+
+    def _copy(self, p: int, stream: torch.cuda.Stream) -> None:
+        fut = self.reads.pop(p)
+        if not fut.done():
+            self.stalls += 1
+        buf = fut.result()
+        slot = p % self.slots
+        ready = torch.cuda.Event()
+        ready.record(stream)                    # what is enqueued so far may still read the slot
+        cs = self.copy_stream
+        cs.wait_event(ready)
+        with torch.cuda.stream(cs):
+            for k, name in enumerate(self.names):
+                self.fields[name][slot].copy_(buf[k], non_blocking=True)
+        done = torch.cuda.Event()
+        done.record(cs)
+        self.copies = {q: e for q, e in self.copies.items() if q % self.slots != slot}
+        self.copies[p] = done
+        self.free.append((buf, done))
+        self.slot_position[slot] = p
+        check(load().ne_model_snapshot_resident(self.model.handle, self.which, p), "ne_model_snapshot_resident")
+        self.uploads += 1
+
+    def prepare(self, positions, stream: torch.cuda.Stream) -> None:
+        """Make the positions read next resident and ordered before `stream`'s next work."""
+        need = sorted({int(p) for p in positions})
+        for p in need:
+            if self.slot_position.get(p % self.slots) != p:
+                self._request(p)
+                self._copy(p, stream)
+        for p in need:
+            ev = self.copies.pop(p, None)
+            if ev is not None:
+                stream.wait_event(ev)
+        # read ahead, and copy what has been read (the copy waits for the work already enqueued)
+        ahead = [q for q in range(need[-1] + 1, need[-1] + 1 + self.lookahead) if self._valid(q)]
+        for q in ahead:
+            self._request(q)
+        for q in ahead:
+            fut = self.reads.get(q)
+            if fut is not None and fut.done():
+                self._copy(q, stream)
+
+    def close(self) -> None:
+        self.pool.shutdown(wait=True)
+        self.reads.clear()
+
+
+def default_workers() -> int:
+    return max(1, min(4, (os.cpu_count() or 1) // 4))

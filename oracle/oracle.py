@@ -149,9 +149,19 @@ EXCHANGE_NAMES = ("u", "v", "T", "p", "q", "rain", "snow")
 RAD_NAMES = ("upwelling_longwave", "downwelling_longwave", "downwelling_shortwave")
 
 
-def time_indices(times: np.ndarray, t: float):
-    """cpu_interpolating_time_indices, Linear indexing clamped at the ends."""
+def time_indices(times: np.ndarray, t: float, indexing: int = abi.NE_TIME_CLAMPED, period: float | None = None):
+    """cpu_interpolating_time_indices (interpolate_atmospheric_state.jl:57-60), Linear indexing
+    clamped at the ends; with cyclical indexing Oceananigans' Cyclical(period) (the indexing the
+    reference gives its JRA55 series, DataWrangling/JRA55/JRA55_prescribed_atmosphere.jl:17):
+    t is reduced to t¹ + mod(t − t¹, period) (Julia's floored mod, which Python's float % is),
+    and between the last snapshot and the first one a period later the two are interpolated."""
     n = len(times)
+    if indexing == abi.NE_TIME_CYCLICAL and n > 1:
+        t1, tN = float(times[0]), float(times[-1])
+        tm = (t - t1) % period + t1
+        if tm > tN:
+            return n - 1, 0, (tm - tN) / (period - (tN - t1))
+        t = tm
     if n <= 1 or t <= times[0]:
         return 0, 0, 0.0
     if t >= times[-1]:
@@ -160,11 +170,25 @@ def time_indices(times: np.ndarray, t: float):
     return k, k + 1, (t - times[k]) / (times[k + 1] - times[k])
 
 
+def snapshot_positions(times: np.ndarray, t: float, indexing: int = abi.NE_TIME_CLAMPED,
+                       period: float | None = None):
+    """Stream positions of the two snapshots read at t: the snapshot indices, or with cyclical
+    indexing cycle·Nt + index, the second one in the next cycle across the wrap."""
+    n1, n2, _ = time_indices(times, t, indexing, period)
+    if indexing != abi.NE_TIME_CYCLICAL or len(times) <= 1:
+        return n1, n2
+    x = t - float(times[0])
+    cycle = round((x - x % period) / period)
+    return cycle * len(times) + n1, (cycle + (1 if n2 < n1 else 0)) * len(times) + n2
+
+
 class OracleModel:
     """CPU OceanOnlyModel on a SurfaceCase (copies of its arrays)."""
 
-    def __init__(self, case, params: abi.Params | None = None, radiation: bool = True):
+    def __init__(self, case, params: abi.Params | None = None, radiation: bool = True,
+                 time_indexing: int = abi.NE_TIME_CLAMPED, time_period: float | None = None):
         self.case = case
+        self.time_indexing, self.time_period = time_indexing, time_period
         g = case.grid
         spec = g.spec
         Nx, Ny, Nz = spec.Nx, spec.Ny, spec.Nz
@@ -234,9 +258,10 @@ class OracleModel:
 
     # -- clock -------------------------------------------------------------
     def _set_time_indices(self):
-        n1, n2, tn = time_indices(self.case.atmos_times, self.time)
+        ix = (self.time_indexing, self.time_period)
+        n1, n2, tn = time_indices(self.case.atmos_times, self.time, *ix)
         self._model.atmosphere.time = abi.TimeInterp(n1, n2, tn)
-        n1, n2, tn = time_indices(self.case.rad_times, self.time)
+        n1, n2, tn = time_indices(self.case.rad_times, self.time, *ix)
         self._model.radiation.time = abi.TimeInterp(n1, n2, tn)
 
     def update_state(self, dt_frazil: float | None = None):

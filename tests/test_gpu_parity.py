@@ -116,7 +116,8 @@ def _oracle_on_state(om, T, S, time=None, nudge=None):
     radiation interpolated at `time` (om's by default), one input moved by one ulp if `nudge` =
     (name, ±1) names one (an interpolated atmosphere field, or "sst", the top level of T), and the
     fluxes solved."""
-    o = O.OracleModel(om.case, params=abi.Params.from_buffer_copy(om.params), radiation=bool(om.params.radiation))
+    o = O.OracleModel(om.case, params=abi.Params.from_buffer_copy(om.params), radiation=bool(om.params.radiation),
+                      time_indexing=om.time_indexing, time_period=om.time_period)
     o.T[...] = T
     o.S[...] = S
     if nudge and nudge[0] == "sst":

@@ -50,6 +50,12 @@ def parse():
     ap.add_argument("--serve-cells", type=float, default=32e6, help="work budget: resident cells")
     ap.add_argument("--serve-pool", type=int, default=32, help="distinct pre-generated request inputs")
     ap.add_argument("--serve-plain-max", type=int, default=150, help="requests run on the plain path")
+    ap.add_argument("--stream-report", action="store_true",
+                    help="step time with the atmosphere and radiation streamed into rings of snapshot slots "
+                         "against the same series held whole")
+    ap.add_argument("--stream-every", type=int, default=9, help="steps between snapshots (9: 3-hourly at 20 min)")
+    ap.add_argument("--stream-window", type=int, default=3, help="snapshot slots per streamed series")
+    ap.add_argument("--stream-snapshots", type=int, default=16, help="snapshots in the (cyclical) series")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="only run a device-to-device copy of this many GiB (--steps times) and report its bandwidth")
     return ap.parse_args()
@@ -295,6 +301,56 @@ def cpu_baseline(case, dt: float, target_s: float) -> dict:
                               "sample": f"median of {n_one} steps after 1 warm-up step"}}
 
 
+def stream_report(args):
+    """ms per step with the prescribed series streamed from host memory into rings of slots, against the
+    same model holding the whole series in HBM, and the host-to-device rate of one snapshot."""
+    import torch
+    from numericalearth_hip.generators import make_case
+    from numericalearth_hip.model import OceanOnlyModel
+    from numericalearth_hip.time_series import Cyclical
+
+    Nx, Ny, Nz = WORKLOADS[args.workload]
+    case = make_case(Nx, Ny, Nz, seed=args.seed, n_times=args.stream_snapshots,
+                     dt_times=args.dt * args.stream_every)
+    out = {"report": "stream", "workload": args.workload, "precision": args.precision, "dt": args.dt,
+           "steps_between_snapshots": args.stream_every, "snapshots": args.stream_snapshots,
+           "window": args.stream_window, "steps": args.steps}
+    win = dict(atmosphere_window=args.stream_window, radiation_window=args.stream_window)
+    for label, kw in (("resident", {}), ("streamed", win), ("streamed_copy_stream", dict(win, copy_stream=True))):
+        m = OceanOnlyModel(case, precision=args.precision, time_indexing=Cyclical(), **kw)
+        for _ in range(args.warmup):
+            m.time_step(args.dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            m.time_step(args.dt)
+        torch.cuda.synchronize()
+        out[f"ms_per_step_{label}"] = 1e3 * (time.perf_counter() - t0) / args.steps
+        if m.streams and label == "streamed":
+            st = m.streams["atmosphere"]
+            out["uploads_atmosphere"] = st.uploads
+            out["stalled_prepares"] = st.stalls
+            out["snapshot_bytes"] = {k: len(s.names) * s.fields[s.names[0]][0].numel() * s.fields[s.names[0]].element_size()
+                                     for k, s in m.streams.items()}
+            out["device_bytes_resident_series"] = {k: sum(t.numel() * t.element_size() for t in s.fields.values())
+                                                   for k, s in m.streams.items()}
+        m.close()
+        del m
+    out["overhead"] = out["ms_per_step_streamed"] / out["ms_per_step_resident"] - 1.0
+    out["overhead_copy_stream"] = out["ms_per_step_streamed_copy_stream"] / out["ms_per_step_resident"] - 1.0
+    nbytes = out["snapshot_bytes"]["atmosphere"]
+    host = torch.empty(nbytes // 8, dtype=torch.float64, pin_memory=True)
+    dev = torch.empty_like(host, device="cuda")
+    dev.copy_(host, non_blocking=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        dev.copy_(host, non_blocking=True)
+    torch.cuda.synchronize()
+    out["h2d_gbs_one_snapshot"] = 20 * nbytes / (time.perf_counter() - t0) / 1e9
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -309,6 +365,9 @@ def main():
         return
     if args.serve:
         serving_report(args)
+        return
+    if args.stream_report:
+        stream_report(args)
         return
     if args.copy_gib > 0:
         print(json.dumps(dict(copy_bandwidth(args.copy_gib, args.steps), report="copy", gib=args.copy_gib)),

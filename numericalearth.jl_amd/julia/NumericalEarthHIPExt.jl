@@ -69,7 +69,7 @@ struct NENetOceanFluxes; tau_x::NEView; tau_y::NEView; J_T::NEView; J_S::NEView;
     shortwave_surface_flux::NEView; end   # J₀ of TwoColorRadiation (surface_flux field), or a NULL view
 struct NERadiativeFluxes; upwelling_longwave::NEView; downwelling_longwave::NEView; downwelling_shortwave::NEView; end
 
-# ne_params is a 592-byte block in header order (checked against the header by the
+# ne_params is an 832-byte block in header order (checked against the header by the
 # repository's layout test); it is filled by ne_params_default and then overwritten
 # field by field from the flux formulation.
 const NE_PARAMS_BYTES = 832

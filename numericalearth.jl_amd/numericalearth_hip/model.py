@@ -118,7 +118,7 @@ class OceanOnlyModel:
     def __init__(self, case: SurfaceCase, *, params: abi.Params | None = None, precision: str = "fp64",
                  radiation: bool = True, device: int = 0, time_indexing=None, atmosphere_window: int | None = None,
                  radiation_window: int | None = None, atmosphere_source=None, radiation_source=None,
-                 lookahead: int | None = None):
+                 lookahead: int | None = None, copy_stream: bool = False):
         """time_indexing: Clamp() (default) or Cyclical(period) of both snapshot series.
         atmosphere_window / radiation_window: keep only that many snapshots of the series on the device
         (a ring of slots, time_indices_in_memory) and stream the others from `atmosphere_source` /
@@ -232,7 +232,7 @@ class OceanOnlyModel:
         for which, series, fields in ((0, "atmosphere", self.atm), (1, "radiation", self.rad)):
             if series in resident:
                 self.streams[series] = SnapshotStream(self, which, sources[series], fields, resident[series],
-                                                      lookahead=lookahead)
+                                                      lookahead=lookahead, copy_stream=copy_stream)
 
     # reference API ------------------------------------------------------------
     def time_step(self, dt: float, stream=None) -> None:

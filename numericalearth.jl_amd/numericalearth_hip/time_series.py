@@ -129,16 +129,24 @@ class NpySnapshotStore(ArraySnapshots):
 class SnapshotStream:
     """Uploads the snapshots of one windowed series of a model into its ring of slots.
 
-    ``prepare(time, stream)`` is called before each step (or update_state) that reads the series at
-    `time`: snapshots it reads that are not resident are read (on a worker thread), staged in pinned
-    host memory and copied on a copy stream that first waits for the work already on `stream` (the
-    slot may still be read there); `stream` then waits for the copy. The following `lookahead`
-    positions are read and copied ahead in the same way without making `stream` wait, so their copy
-    overlaps the steps before them. Positions are marked resident in the model when their copy is
-    enqueued; every step of the model must be prepared (OceanOnlyModel.time_step does it)."""
+    ``prepare(positions, stream)`` is called before each step (or update_state) with the positions the
+    step reads: those not resident are copied into their slots, in stream order on `stream` itself (the
+    default), or with ``copy_stream=True`` on a stream of their own that first waits for the work already
+    on `stream` (the slot may still be read there), `stream` then waiting for the copy. In-stream copies
+    cost the step loop their own duration (a 7-field snapshot of 11 MB: about 0.25 ms); the cross-stream
+    event waits have been measured to cost more than that on ROCm (single steps several milliseconds
+    long), so the copy stream is an option, not the default. Snapshots are read
+    from the source by worker threads (one task per field) into pinned staging buffers `read_ahead`
+    positions ahead of the step that needs them; the next `lookahead` positions (at most slots − 2, so
+    the two slots the next step reads are never overwritten) are copied as soon as they have been read,
+    so the host's reads overlap the steps before them. The read_ahead + 2 pinned
+    staging buffers are allocated once; a buffer whose copy is still queued is passed over, so the host
+    waits for the device only when every buffer's copy is. Positions are marked resident in the model when
+    their copy is enqueued; every step of the model must be prepared (OceanOnlyModel.time_step does it)."""
 
     def __init__(self, model, which: int, source: ArraySnapshots, device_fields: dict, resident: list[int],
-                 lookahead: int | None = None, workers: int = 1):
+                 lookahead: int | None = None, read_ahead: int | None = None, workers: int | None = None,
+                 copy_stream: bool = False):
         self.model = model
         self.which = which
         self.source = source
@@ -150,51 +158,57 @@ class SnapshotStream:
         self.dtype = first.dtype
         self.cyclical = model.time_indexing_kind == abi.NE_TIME_CYCLICAL
         self.lookahead = max(0, self.slots - 2) if lookahead is None else min(int(lookahead), self.slots - 2)
-        self.copy_stream = torch.cuda.Stream(device=self.device)
-        self.pool = ThreadPoolExecutor(max_workers=workers)
+        self.read_ahead = max(self.lookahead + 2, 2) if read_ahead is None else max(int(read_ahead), self.lookahead)
+        self.copy_stream = torch.cuda.Stream(device=self.device) if copy_stream else None
+        self.pool = ThreadPoolExecutor(max_workers=workers or min(len(self.names), default_workers()))
         self.slot_position = {int(p % self.slots): int(p) for p in resident}
-        self.reads: dict[int, Future] = {}     # position -> Future of the pinned staging buffer
-        self.copies: dict[int, torch.cuda.Event] = {}   # position -> event after its copy
-        self.free: list[tuple[torch.Tensor, torch.cuda.Event | None]] = []
+        self.reads: dict[int, tuple[torch.Tensor, list[Future]]] = {}   # position -> staging buffer, field reads
+        self.copies: dict[int, torch.cuda.Event] = {}                    # position -> event after its copy
+        # pinned staging buffers, allocated here once (a pinned allocation of a snapshot takes
+        # milliseconds of host time, which inside the step loop would let the device run dry)
+        shape = (len(self.names),) + tuple(first.shape[1:])
+        self.n_staging = self.read_ahead + 2
+        self.free: list[tuple[torch.Tensor, torch.cuda.Event | None]] = [
+            (torch.empty(shape, dtype=self.dtype, pin_memory=True), None) for _ in range(self.n_staging)]
         self.uploads = 0                        # snapshots copied to the device
-        self.stalls = 0                         # prepares that waited for a read
-
-    def _snapshot(self, p: int) -> int:
-        return p % self.source.n_times
+        self.stalls = 0                         # copies that waited for their read
 
     def _valid(self, p: int) -> bool:
         return self.cyclical or 0 <= p < self.source.n_times
 
     def _staging(self) -> torch.Tensor:
-        if self.free:
-            buf, ev = self.free.pop()
-            if ev is not None:
-                ev.synchronize()   # its previous copy has left the buffer
-            return buf
-        shape = (len(self.names),) + tuple(next(iter(self.fields.values())).shape[1:])
-        return torch.empty(shape, dtype=self.dtype, pin_memory=True)
-
-    def _read(self, p: int, buf: torch.Tensor) -> torch.Tensor:
-        n = self._snapshot(p)
-        host = buf.numpy()
-        for k, name in enumerate(self.names):
-            self.source.read(n, name, host[k])
+        for k, (buf, ev) in enumerate(self.free):
+            if ev is None or ev.query():        # its previous copy has left the buffer
+                del self.free[k]
+                return buf
+        buf, ev = self.free.pop(0)              # every buffer's copy still queued: wait for the oldest
+        ev.synchronize()
         return buf
 
     def _request(self, p: int) -> None:
-        if p not in self.reads and p not in self.copies and self.slot_position.get(p % self.slots) != p:
-            self.reads[p] = self.pool.submit(self._read, p, self._staging())
+        if p in self.reads or p in self.copies or self.slot_position.get(p % self.slots) == p:
+            return
+        buf = self._staging()
+        host = buf.numpy()
+        n = p % self.source.n_times
+        futs = [self.pool.submit(self.source.read, n, name, host[k]) for k, name in enumerate(self.names)]
+        self.reads[p] = (buf, futs)
+
+    def _read_done(self, p: int) -> bool:
+        return p in self.reads and all(f.done() for f in self.reads[p][1])
 
     def _copy(self, p: int, stream: torch.cuda.Stream) -> None:
-        fut = self.reads.pop(p)
-        if not fut.done():
+        buf, futs = self.reads.pop(p)
+        if not all(f.done() for f in futs):
             self.stalls += 1
-        buf = fut.result()
+        for f in futs:
+            f.result()
         slot = p % self.slots
-        ready = torch.cuda.Event()
-        ready.record(stream)                    # what is enqueued so far may still read the slot
-        cs = self.copy_stream
-        cs.wait_event(ready)
+        cs = self.copy_stream or stream
+        if cs is not stream:
+            ready = torch.cuda.Event()
+            ready.record(stream)                # what is enqueued so far may still read the slot
+            cs.wait_event(ready)
         with torch.cuda.stream(cs):
             for k, name in enumerate(self.names):
                 self.fields[name][slot].copy_(buf[k], non_blocking=True)
@@ -218,13 +232,12 @@ class SnapshotStream:
             ev = self.copies.pop(p, None)
             if ev is not None:
                 stream.wait_event(ev)
-        # read ahead, and copy what has been read (the copy waits for the work already enqueued)
-        ahead = [q for q in range(need[-1] + 1, need[-1] + 1 + self.lookahead) if self._valid(q)]
-        for q in ahead:
-            self._request(q)
-        for q in ahead:
-            fut = self.reads.get(q)
-            if fut is not None and fut.done():
+        last = need[-1]
+        for q in range(last + 1, last + 1 + self.read_ahead):
+            if self._valid(q):
+                self._request(q)
+        for q in range(last + 1, last + 1 + self.lookahead):
+            if self._read_done(q):
                 self._copy(q, stream)
 
     def close(self) -> None:

@@ -23,7 +23,7 @@ def _specs(n=8, seed=5):
     return specs
 
 
-def _build(specs, precision, general=(), n_times=1, two_color=(), large_yeager=(), skin=()):
+def _build(specs, precision, general=(), n_times=1, two_color=(), large_yeager=(), skin=(), time_indexing=None):
     """Members; those whose index is in `general` get a non-default Edson constant (general flux path),
     those in `two_color` the TwoColorRadiation forcing, those in `large_yeager` the Large-Yeager
     coefficient-based fluxes, those in `skin` the skin temperature."""
@@ -43,7 +43,7 @@ def _build(specs, precision, general=(), n_times=1, two_color=(), large_yeager=(
             p.interface_temperature = abi_.NE_TEMPERATURE_SKIN
             p.skin_diffusivity, p.skin_depth = 1e-4, 1.0
         case = member_case(s) if n_times == 1 else _case_times(s, n_times)
-        out.append(OceanOnlyModel(case, params=p, precision=precision))
+        out.append(OceanOnlyModel(case, params=p, precision=precision, time_indexing=time_indexing))
     return out
 
 
@@ -162,6 +162,23 @@ def test_device_clock_crosses_snapshots(require_gpu):
         for m in alone:
             m.time_step(DT)
     _assert_same(alone, batched, "time interpolation")
+
+
+def test_device_clock_cycles_through_the_wrap(require_gpu):
+    from numericalearth_hip.time_series import Cyclical
+    specs = _specs(4, seed=13)
+    # members with different periods: 3 h inferred, and 4.5 h (a 1.5 h wrap interval)
+    alone = _build(specs[:2], "fp64", n_times=3, time_indexing=Cyclical())
+    alone += _build(specs[2:], "fp64", n_times=3, time_indexing=Cyclical(4.5 * 3600.0))
+    batched = _build(specs[:2], "fp64", n_times=3, time_indexing=Cyclical())
+    batched += _build(specs[2:], "fp64", n_times=3, time_indexing=Cyclical(4.5 * 3600.0))
+    ens = Ensemble(batched)
+    ens.use_graph(True)
+    for _ in range(24):  # 8 h: through the wrap of both periods, twice for the 3 h one
+        ens.time_step(DT)
+        for m in alone:
+            m.time_step(DT)
+    _assert_same(alone, batched, "cyclical time interpolation")
 
 
 def test_members_are_locked_while_batched(require_gpu):

@@ -1,7 +1,7 @@
 """Snapshot windows streamed from the host, and cyclical time indexing, on the GPU.
 
 A model that keeps only a ring of N snapshots on the device (snapshots uploaded one at a
-time by SnapshotStream, on a copy stream, ahead of the steps that read them) must step bit
+time by SnapshotStream, in stream order or on a copy stream, ahead of the steps that read them) must step bit
 for bit like the same model holding the whole series, through the wrap of a cyclical series
 and past the end of a clamped one. A cyclical model's fluxes inside the wrap interval match
 the oracle's with the same indexing.
@@ -45,18 +45,20 @@ def _assert_same(a, b, when):
         assert torch.equal(fa[k], fb[k]), f"{k} differs {when}"
 
 
-@pytest.mark.parametrize("precision,indexing,aw,rw,n_times", [
-    ("fp64", Clamp(), 2, 3, 5),
-    ("fp64", Cyclical(), 2, 2, 5),              # odd Nt, ring of two: the wrap reads positions Nt-1 and Nt
-    ("mixed", Cyclical(5.5 * HOUR), 3, 4, 5),
-    ("fp64", Cyclical(), 4, None, 6),           # radiation held whole
+@pytest.mark.parametrize("precision,indexing,aw,rw,n_times,copy_stream", [
+    ("fp64", Clamp(), 2, 3, 5, False),
+    ("fp64", Cyclical(), 2, 2, 5, False),       # odd Nt, ring of two: the wrap reads positions Nt-1 and Nt
+    ("mixed", Cyclical(5.5 * HOUR), 3, 4, 5, False),
+    ("fp64", Cyclical(), 4, None, 6, False),    # radiation held whole
+    ("fp64", Cyclical(), 3, 3, 5, True),        # copies on their own stream, ordered by events
 ])
-def test_streamed_window_steps_like_resident_series(require_gpu, precision, indexing, aw, rw, n_times):
-    print(f"[window] precision={precision} indexing={indexing} atmosphere={aw} radiation={rw}", flush=True)
+def test_streamed_window_steps_like_resident_series(require_gpu, precision, indexing, aw, rw, n_times, copy_stream):
+    print(f"[window] precision={precision} indexing={indexing} atmosphere={aw} radiation={rw} "
+          f"copy_stream={copy_stream}", flush=True)
     case = _case(n_times)
     full = OceanOnlyModel(case, precision=precision, time_indexing=indexing)
     win = OceanOnlyModel(case, precision=precision, time_indexing=indexing, atmosphere_window=aw,
-                         radiation_window=rw)
+                         radiation_window=rw, copy_stream=copy_stream)
     assert win.atm["T"].shape[0] == aw and full.atm["T"].shape[0] == n_times
     _assert_same(win, full, "after construction")
     for n in range(30):                          # 12.5 h: more than two cycles of the cyclical series

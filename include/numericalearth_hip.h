@@ -449,8 +449,8 @@ typedef struct ne_model_desc {
     int32_t n_radiation_times;
     const double* radiation_times;        /* HOST */
     void*   workspace;                    /* optional caller-owned device memory for the RK stage
-                                             buffers (ne_model_workspace_bytes); NULL: the model
-                                             allocates them itself */
+                                             buffers and freeze flags (ne_model_workspace_bytes);
+                                             NULL: the model allocates them itself */
     int64_t workspace_bytes;
     /* Snapshot time indexing (FieldTimeSeries time_indexing; JRA55 atmospheres are Cyclical,
      * DataWrangling/JRA55/JRA55_prescribed_atmosphere.jl:17) and in-memory windows (the
@@ -472,7 +472,9 @@ typedef struct ne_model_desc {
  * two are interpolated across the wrap. */
 enum { NE_TIME_CLAMPED = 0, NE_TIME_CYCLICAL = 1 };
 
-/* Bytes of device workspace a model of this descriptor needs (four stage buffers of T, S). */
+/* Bytes of device workspace a model of this descriptor needs: four RK stage buffers of T, S
+ * and one byte per column of freeze flags (the step's last stage marks the columns holding a
+ * cell below its freezing point, and the freeze clamp after the step visits only those). */
 int ne_model_workspace_bytes(const ne_model_desc* desc, int64_t* bytes);
 
 typedef struct ne_model ne_model;

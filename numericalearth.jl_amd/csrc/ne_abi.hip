@@ -231,6 +231,14 @@ size_t stage_buffer_bytes(const ne_model_desc& d) {
     return ((size_t)(n > 0 ? n : 0) + 255) & ~(size_t)255;
 }
 
+// The model's freeze flags: one byte per column, rounded up to 256 bytes.
+size_t freeze_flag_bytes(const ne_model_desc& d) {
+    return ((size_t)d.grid.Nx * (size_t)d.grid.Ny + 255) & ~(size_t)255;
+}
+
+// Device workspace of a model: the four RK stage buffers, then the freeze flags.
+size_t model_workspace_bytes(const ne_model_desc& d) { return 4 * stage_buffer_bytes(d) + freeze_flag_bytes(d); }
+
 }  // namespace
 
 extern "C" {
@@ -527,7 +535,10 @@ int ne_budget_f32(ne_ctx* c, const ne_grid* g, ne_view cv, double weight, double
 // ---------------------------------------------------------------------------
 
 // ev (optional): ev[3] is recorded before K1 by the caller, ev[4] after K1 and ev[5] after K2.
-static int model_update_state(ne_model* m, double dt_frazil, hipStream_t s, hipEvent_t* ev = nullptr) {
+// after_step: the state was written by the last RK stage, whose freeze flags name the columns
+// the freeze clamp has to visit; otherwise (construction, a caller's update_state) every column.
+static int model_update_state(ne_model* m, double dt_frazil, hipStream_t s, hipEvent_t* ev = nullptr,
+                              bool after_step = false) {
     ne_model_desc& d = m->d;
     GridDev g = grid_dev(d.grid);
     int tv = model_time_views(m, m->time);
@@ -536,6 +547,7 @@ static int model_update_state(ne_model* m, double dt_frazil, hipStream_t s, hipE
     NetOut n = net_out(d.net);
     RadOut ro = rad_out(d.radiative);
     View3<double> frazil = view<double>(d.frazil_heat);
+    const uint8_t* flags = after_step ? m->freeze_flags : nullptr;
     hipError_t e;
     if (d.precision == NE_FP64) {
         AtmosDev<double> a = atmos_dev<double>(d.atmosphere);
@@ -548,7 +560,7 @@ static int model_update_state(ne_model* m, double dt_frazil, hipStream_t s, hipE
         if (ev && hipEventRecord(ev[4], s) != hipSuccess) return NE_ERR_DEVICE;
         e = launch_surface_finalize<double>(g, d.params, a, d.atmos_fi, d.atmos_fj, r, d.radiation_fi,
                                             d.radiation_fj, view<double>(d.ocean.T), view<const double>(d.ocean.S),
-                                            f, n, ro, frazil, dt_frazil, s);
+                                            f, n, ro, frazil, dt_frazil, flags, s);
     } else {
         AtmosDev<float> a = atmos_dev<float>(d.atmosphere);
         RadDev<float> r = rad_dev<float>(d.radiation);
@@ -560,7 +572,7 @@ static int model_update_state(ne_model* m, double dt_frazil, hipStream_t s, hipE
         if (ev && hipEventRecord(ev[4], s) != hipSuccess) return NE_ERR_DEVICE;
         e = launch_surface_finalize<float>(g, d.params, a, d.atmos_fi, d.atmos_fj, r, d.radiation_fi,
                                            d.radiation_fj, view<float>(d.ocean.T), view<const float>(d.ocean.S), f,
-                                           n, ro, frazil, dt_frazil, s);
+                                           n, ro, frazil, dt_frazil, flags, s);
     }
     if (e == hipSuccess && ev && hipEventRecord(ev[5], s) != hipSuccess) return NE_ERR_DEVICE;
     return dev_err(e);
@@ -589,8 +601,12 @@ static int model_ocean_step(ne_model* m, double dt, hipStream_t s, hipEvent_t* e
                               s);
     if (e != hipSuccess) return NE_ERR_DEVICE;
     if (ev && hipEventRecord(ev[2], s) != hipSuccess) return NE_ERR_DEVICE;
+    TracerForcing F3 = F;   // the last stage flags the columns the freeze clamp visits
+    F3.freeze_flags = m->freeze_flags;
+    F3.melt0 = d.params.freshwater_melting_temperature;
+    F3.liquidus_slope = d.params.liquidus_slope;
     e = launch_rk_stage<Real>(g, u, v, w, T0, S0, view<const Real>(m->vT2), view<const Real>(m->vS2),
-                              view<Real>(d.ocean.T), view<Real>(d.ocean.S), JT, JS, Jw, F, beta[2] * dt, sr, m->time,
+                              view<Real>(d.ocean.T), view<Real>(d.ocean.S), JT, JS, Jw, F3, beta[2] * dt, sr, m->time,
                               s);
     if (e != hipSuccess) return NE_ERR_DEVICE;
     if (ev && hipEventRecord(ev[3], s) != hipSuccess) return NE_ERR_DEVICE;
@@ -649,12 +665,22 @@ int ne_model_create(ne_ctx* c, const ne_model_desc* desc, ne_model** out, void* 
     void** bufs[4] = {&m->T1, &m->S1, &m->T2, &m->S2};
     ne_view* views[4] = {&m->vT1, &m->vS1, &m->vT2, &m->vS2};
     if (desc->workspace) {
-        if (desc->workspace_bytes < (int64_t)(4 * n)) {
+        if (desc->workspace_bytes < (int64_t)model_workspace_bytes(*desc)) {
             delete m;
             return NE_ERR_CAPACITY;
         }
         m->owns_buffers = false;
         for (int b = 0; b < 4; b++) *bufs[b] = static_cast<char*>(desc->workspace) + b * n;
+        m->freeze_flags = reinterpret_cast<uint8_t*>(static_cast<char*>(desc->workspace) + 4 * n);
+    } else if (hipMalloc(reinterpret_cast<void**>(&m->freeze_flags), freeze_flag_bytes(*desc)) != hipSuccess) {
+        ne_model_destroy(m);
+        return NE_ERR_DEVICE;
+    }
+    // every column is visited until a step's last stage has flagged them (creation is never
+    // inside a captured graph, so the fill may be a memset)
+    if (hipMemsetAsync(m->freeze_flags, 1, freeze_flag_bytes(*desc), as_stream(stream)) != hipSuccess) {
+        ne_model_destroy(m);
+        return NE_ERR_DEVICE;
     }
     for (int b = 0; b < 4; b++) {
         if (m->owns_buffers && hipMalloc(bufs[b], n) != hipSuccess) {
@@ -681,7 +707,7 @@ int ne_model_workspace_bytes(const ne_model_desc* desc, int64_t* bytes) {
     int r = check_grid(&desc->grid);
     if (r || (r = check_view3(desc->ocean.T))) return r;
     if (desc->precision != NE_FP64 && desc->precision != NE_MIXED_FP32) return NE_ERR_UNSUPPORTED;
-    *bytes = (int64_t)(4 * stage_buffer_bytes(*desc));
+    *bytes = (int64_t)model_workspace_bytes(*desc);
     return NE_OK;
 }
 
@@ -690,9 +716,11 @@ int ne_model_destroy(ne_model* m) {
     if (m->batch) return NE_ERR_STATE;
     int r = NE_OK;
     void* bufs[4] = {m->T1, m->S1, m->T2, m->S2};
-    if (m->owns_buffers)
+    if (m->owns_buffers) {
         for (void* b : bufs)
             if (b && hipFree(b) != hipSuccess) r = NE_ERR_DEVICE;
+        if (m->freeze_flags && hipFree(m->freeze_flags) != hipSuccess) r = NE_ERR_DEVICE;
+    }
     for (hipEvent_t e : m->events) (void)hipEventDestroy(e);
     delete m;
     return r;
@@ -713,7 +741,7 @@ int ne_model_time_step(ne_model* m, double dt, void* stream) {
     m->time += dt;
     m->iteration += 1;
     m->last_dt = dt;
-    r = model_update_state(m, dt, s, ev);
+    r = model_update_state(m, dt, s, ev, true);
     if (!r && ev) m->recorded += 1;
     return r;
 }

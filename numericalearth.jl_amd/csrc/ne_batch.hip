@@ -171,6 +171,7 @@ BatchRec<Real> member_record(const ne_model* m, const double* atimes_dev, const 
     r.n = net_out(d.net);
     r.ro = rad_out(d.radiative);
     r.frazil = view<double>(d.frazil_heat);
+    r.freeze_flags = m->freeze_flags;
     r.atimes = atimes_dev;
     r.rtimes = rtimes_dev;
     r.nat = (int)m->atimes.size();

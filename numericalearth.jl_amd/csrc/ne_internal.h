@@ -23,6 +23,9 @@ struct ne_model {
     void* T2 = nullptr;
     void* S2 = nullptr;
     ne_view vT1, vS1, vT2, vS2;
+    // one byte per column, written by the step's last RK stage: 1 where a cell of the column is
+    // below its freezing point (or not known), so the freeze clamp that follows visits only those
+    uint8_t* freeze_flags = nullptr;
     double time = 0.0;
     double last_dt = 0.0;
     int64_t iteration = 0;
@@ -32,7 +35,6 @@ struct ne_model {
     std::vector<hipEvent_t> events;
     bool owns_buffers = true;           // stage buffers allocated by the model (no workspace given)
     struct ne_batch* batch = nullptr;   // the batch this model belongs to, if any
-    // windowed snapshot views: the snapshot held by each slot (−1: none)
     std::vector<int64_t> aslot, rslot;   // stream position held by each snapshot slot (windowed series)
 };
 
@@ -68,10 +70,14 @@ struct TracerForcing {
     int two_color;
     double first_fraction, kappa1, kappa2;
     const double* J0;
+    // freeze watch of the step's last stage (nullptr: none): flag[i + j·Nx] = 1 when a stored
+    // cell of the column is below T_m = melt0 − liquidus_slope·S (the freeze clamp's test)
+    uint8_t* freeze_flags;
+    double melt0, liquidus_slope;
 };
 
 inline TracerForcing tracer_forcing(const ne_params* p, const ne_net_ocean_fluxes& n) {
-    TracerForcing f = {0, 0.0, 0.0, 0.0, nullptr};
+    TracerForcing f = {0, 0.0, 0.0, 0.0, nullptr, nullptr, 0.0, 0.0};
     if (p && p->radiative_forcing == NE_RADIATIVE_FORCING_TWO_COLOR) {
         f.two_color = 1;
         f.first_fraction = p->first_color_fraction;
@@ -127,7 +133,7 @@ hipError_t launch_surface_finalize(const GridDev& g, const ne_params& p, const A
                                    const double* afi, const double* afj, const RadDev<Real>& r,
                                    const double* rfi, const double* rfj, View3<Real> T, View3<const Real> S,
                                    const FluxOut& f, const NetOut& n, const RadOut& ro, View3<double> frazil,
-                                   double dt_frazil, hipStream_t s);
+                                   double dt_frazil, const uint8_t* freeze_flags, hipStream_t s);
 
 // Tracer launchers (ne_tracer.hip).
 // stochastic: round the new state stochastically (FP32 storage only; the stage that writes
@@ -192,6 +198,7 @@ struct BatchRec {
     NetOut n;
     RadOut ro;
     View3<double> frazil;
+    uint8_t* freeze_flags;  // the member's freeze flags (written by stage 3, read by K2)
     const double* atimes;   // device copies of the snapshot times
     const double* rtimes;
     int nat, nrt;

@@ -386,8 +386,12 @@ __device__ __forceinline__ void surface_finalize_column(const GridDev& g, const 
                                                         const double* __restrict__ rfj, const View3<Real>& T,
                                                         const View3<const Real>& S, const FluxOut& f,
                                                         const NetOut& n, const RadOut& ro,
-                                                        const View3<double>& frazil, double dt_frazil, int i, int j) {
-    double dQ = freeze_column<Real>(g, p, T, S, i, j, dt_frazil);
+                                                        const View3<double>& frazil, double dt_frazil,
+                                                        const uint8_t* __restrict__ freeze_flags, int i, int j) {
+    // after a step, only columns its last stage flagged can hold a cell below freezing; the
+    // others would run the clamp without a change and with dQ = 0 (the same +0.0)
+    const bool visit = freeze_flags == nullptr || freeze_flags[(int64_t)i + (int64_t)j * g.Nx] != 0;
+    double dQ = visit ? freeze_column<Real>(g, p, T, S, i, j, dt_frazil) : 0.0;
     if (frazil.p) frazil.at2(i, j) = dQ;
 
     Neighbours x = neighbours(afi[i], a.Nx, a.tx);
@@ -427,11 +431,12 @@ __global__ void __launch_bounds__(BX* BY)
 surface_finalize_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __restrict__ afi,
                         const double* __restrict__ afj, RadDev<Real> rd, const double* __restrict__ rfi,
                         const double* __restrict__ rfj, View3<Real> T, View3<const Real> S, FluxOut f, NetOut n,
-                        RadOut ro, View3<double> frazil, double dt_frazil) {
+                        RadOut ro, View3<double> frazil, double dt_frazil, const uint8_t* freeze_flags) {
     int i = blockIdx.x * BX + threadIdx.x;
     int j = blockIdx.y * BY + threadIdx.y;
     if (i >= g.Nx || j >= g.Ny) return;
-    surface_finalize_column<Real>(g, p, a, afi, afj, rd, rfi, rfj, T, S, f, n, ro, frazil, dt_frazil, i, j);
+    surface_finalize_column<Real>(g, p, a, afi, afj, rd, rfi, rfj, T, S, f, n, ro, frazil, dt_frazil, freeze_flags,
+                                  i, j);
 }
 
 // Ensemble form of K2 over the same work list as the ensemble K1.
@@ -447,7 +452,7 @@ batch_surface_finalize_kernel(const BatchRec<Real>* __restrict__ recs, const Wor
         if (col >= g.Nx * g.Ny) continue;
         const int i = col % g.Nx, j = col / g.Nx;
         surface_finalize_column<Real>(g, rc.p, rc.a, rc.afi, rc.afj, rc.r, rc.rfi, rc.rfj, rc.T, cview(rc.S), rc.f,
-                                      rc.n, rc.ro, rc.frazil, dt_frazil, i, j);
+                                      rc.n, rc.ro, rc.frazil, dt_frazil, rc.freeze_flags, i, j);
     }
 }
 
@@ -614,21 +619,22 @@ hipError_t launch_surface_finalize(const GridDev& g, const ne_params& p, const A
                                    const double* afi, const double* afj, const RadDev<Real>& r, const double* rfi,
                                    const double* rfj, View3<Real> T, View3<const Real> S, const FluxOut& f,
                                    const NetOut& n, const RadOut& ro, View3<double> frazil, double dt_frazil,
-                                   hipStream_t s) {
+                                   const uint8_t* freeze_flags, hipStream_t s) {
     hipLaunchKernelGGL(surface_finalize_kernel<Real>, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, a, afi, afj, r,
-                       rfi, rfj, T, S, f, n, ro, frazil, dt_frazil);
+                       rfi, rfj, T, S, f, n, ro, frazil, dt_frazil, freeze_flags);
     return hipGetLastError();
 }
 template hipError_t launch_surface_finalize<double>(const GridDev&, const ne_params&, const AtmosDev<double>&,
                                                     const double*, const double*, const RadDev<double>&,
                                                     const double*, const double*, View3<double>,
                                                     View3<const double>, const FluxOut&, const NetOut&,
-                                                    const RadOut&, View3<double>, double, hipStream_t);
+                                                    const RadOut&, View3<double>, double, const uint8_t*,
+                                                    hipStream_t);
 template hipError_t launch_surface_finalize<float>(const GridDev&, const ne_params&, const AtmosDev<float>&,
                                                    const double*, const double*, const RadDev<float>&,
                                                    const double*, const double*, View3<float>, View3<const float>,
                                                    const FluxOut&, const NetOut&, const RadOut&, View3<double>,
-                                                   double, hipStream_t);
+                                                   double, const uint8_t*, hipStream_t);
 
 template <typename Real>
 hipError_t launch_batch_surface_solve(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int block,

@@ -139,7 +139,7 @@ __device__ __forceinline__ void stb(T* base, uint32_t off, T v) {
 // cells). Every work-item of the block calls this with the same tile. Every address is a
 // 32-bit byte offset from a uniform base (the host checks that each view spans < 4 GiB),
 // computed once per tile; a level adds its uniform k-stride.
-template <typename Real, int MODE, int TR, bool TWO_COLOR>
+template <typename Real, int MODE, int TR, bool TWO_COLOR, bool WATCH = false>
 __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDev& g, const TracerLayout& L,
                                           const Real* __restrict__ u, const Real* __restrict__ v,
                                           const Real* __restrict__ w, const Real* T0, const Real* S0,
@@ -220,7 +220,13 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
             stb(Tout, coff + (uint32_t)k * L.c_sz, (R)0);
             stb(Sout, coff + (uint32_t)k * L.c_sz, (R)0);
         }
-    if (kstart >= Nz) return;  // uniform: the whole tile is land
+    // freeze watch (the step's last stage; instances without it carry none of its code)
+    const bool watch = WATCH && MODE == 0 && F.freeze_flags != nullptr;   // uniform
+    bool frz = false;
+    if (kstart >= Nz) {  // uniform: the whole tile is land
+        if (watch && cell) F.freeze_flags[(int64_t)ci + (int64_t)cj * g.Nx] = 0;
+        return;
+    }
 
     // Vertical register window of the owned column: levels k-2 … k+3
     R zT[6], zS[6];
@@ -368,10 +374,12 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                 const uint32_t key = 2u * ((uint32_t)ci + (uint32_t)g.Nx * ((uint32_t)cj + (uint32_t)g.Ny * k));
                 const R c0T = EARLY_C0 ? nT0 : ldb(T0, coff + kc);
                 const R c0S = EARLY_C0 ? nS0 : ldb(S0, coff + kc);
-                stb(Tout, coff + kc, round_state<R>((double)c0T + beta_dt * (double)GT, key, seed,
-                                                    stochastic));
-                stb(Sout, coff + kc, round_state<R>((double)c0S + beta_dt * (double)GS, key + 1u,
-                                                    seed, stochastic));
+                const R Tn = round_state<R>((double)c0T + beta_dt * (double)GT, key, seed, stochastic);
+                const R Sn = round_state<R>((double)c0S + beta_dt * (double)GS, key + 1u, seed, stochastic);
+                stb(Tout, coff + kc, Tn);
+                stb(Sout, coff + kc, Sn);
+                // the freeze clamp's test on the stored values (freeze_column, ne_surface.hip)
+                if (watch) frz = frz || (double)Tn < F.melt0 - F.liquidus_slope * (double)Sn;
             } else {
                 stb(Tout, coff + kc, GT);
                 stb(Sout, coff + kc, GS);
@@ -389,11 +397,12 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
             zS[5] = a ? ldb(Sin, coff + (uint32_t)kk * L.c_sz) : (R)0;
         }
     }
+    if (watch && cell) F.freeze_flags[(int64_t)ci + (int64_t)cj * g.Nx] = frz ? 1 : 0;
 }
 
 // Single model: one block per tile, tiles ordered so that each XCD (blocks are dealt
 // round-robin to the 8 XCDs) receives a contiguous band of tiles.
-template <typename Real, int MODE, bool TWO_COLOR>
+template <typename Real, int MODE, bool TWO_COLOR, bool WATCH = false>
 __global__ void __launch_bounds__(TW * PLAIN_TR)
 __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? NE_STAGE_WAVES_PER_EU_F32 : NE_STAGE_WAVES_PER_EU_F64)))
 tracer_tile_kernel(GridDev g, TracerLayout L, const Real* u, const Real* v, const Real* w, const Real* T0,
@@ -406,15 +415,16 @@ tracer_tile_kernel(GridDev g, TracerLayout L, const Real* u, const Real* v, cons
     const int per = (ntiles + NXCD - 1) / NXCD;
     const int tile = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
     if (tile >= ntiles) return;
-    tile_body<Real, MODE, PLAIN_TR, TWO_COLOR>(sh, g, L, u, v, w, T0, S0, Tin, Sin, Tout, Sout, JT, JS, Jw, F, beta_dt,
-                                    (tile % ntx) * TC, (tile / ntx) * PLAIN_TR, stochastic != 0, seed);
+    tile_body<Real, MODE, PLAIN_TR, TWO_COLOR, WATCH>(sh, g, L, u, v, w, T0, S0, Tin, Sin, Tout, Sout, JT, JS, Jw, F,
+                                                     beta_dt, (tile % ntx) * TC, (tile / ntx) * PLAIN_TR,
+                                                     stochastic != 0, seed);
 }
 
 // Ensemble: a persistent grid walks the (member, tile) work list. The list is cut into 8
 // contiguous chunks, chunk x going to the blocks of XCD x, which stride through it; each
 // member's tiles are contiguous in the list, so a member's halo re-reads stay in one L2.
 // stage 1: T¹ = T + β₁Δt G(T); stage 2: T² = T + β₂Δt G(T¹); stage 3: T = T + β₃Δt G(T²).
-template <typename Real, int TR, bool TWO_COLOR>
+template <typename Real, int TR, bool TWO_COLOR, bool WATCH>
 __global__ void __launch_bounds__(TW * TR)
 __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? NE_STAGE_WAVES_PER_EU_F32 : NE_STAGE_WAVES_PER_EU_F64)))
 batch_stage_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __restrict__ work, int nwork,
@@ -434,7 +444,7 @@ batch_stage_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __re
         const Real* Sin = stage == 1 ? rc.S.p : stage == 2 ? rc.S1.p : rc.S2.p;
         Real* Tout = stage == 1 ? rc.T1.p : stage == 2 ? rc.T2.p : rc.T.p;
         Real* Sout = stage == 1 ? rc.S1.p : stage == 2 ? rc.S2.p : rc.S.p;
-        TracerForcing F = {0, 0.0, 0.0, 0.0, nullptr};
+        TracerForcing F = {0, 0.0, 0.0, 0.0, nullptr, nullptr, 0.0, 0.0};
         if (rc.p.radiative_forcing == NE_RADIATIVE_FORCING_TWO_COLOR) {
             F.two_color = 1;
             F.first_fraction = rc.p.first_color_fraction;
@@ -442,8 +452,13 @@ batch_stage_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __re
             F.kappa2 = rc.p.second_absorption_coefficient;
             F.J0 = rc.n.J_sw.p;
         }
+        if (WATCH) {   // the step's last stage flags the columns the freeze clamp visits
+            F.freeze_flags = rc.freeze_flags;
+            F.melt0 = rc.p.freshwater_melting_temperature;
+            F.liquidus_slope = rc.p.liquidus_slope;
+        }
         __syncthreads();   // the previous item's reads of the LDS are complete
-        tile_body<Real, 0, TR, TWO_COLOR>(sh, g, L, rc.u.p, rc.v.p, rc.w.p, rc.T.p, rc.S.p, Tin, Sin, Tout, Sout, rc.n.J_T.p,
+        tile_body<Real, 0, TR, TWO_COLOR, WATCH>(sh, g, L, rc.u.p, rc.v.p, rc.w.p, rc.T.p, rc.S.p, Tin, Sin, Tout, Sout, rc.n.J_T.p,
                                rc.n.J_S.p, rc.n.J_w.p, F, beta_dt, (it.index % ntx) * TC, (it.index / ntx) * TR,
                                stage == 3, step_seed(rc.time));
     }
@@ -585,14 +600,22 @@ hipError_t launch_rk_stage(const GridDev& g, View3<const Real> u, View3<const Re
                            View3<const double> Jw, TracerForcing forcing, double beta_dt, bool stochastic,
                            double time, hipStream_t s) {
     TracerLayout L = host_layout<Real>(Tin, u, v, w, JT);
-    if (forcing.two_color)
-        hipLaunchKernelGGL((tracer_tile_kernel<Real, 0, true>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, PLAIN_TR), 0,
-                           s, g, L, u.p, v.p, w.p, T0.p, S0.p, Tin.p, Sin.p, Tout.p, Sout.p, JT.p, JS.p, Jw.p,
-                           forcing, beta_dt, stochastic ? 1 : 0, step_seed(time));
+    const dim3 grid(grid_blocks(g.Nx, g.Ny)), block(TW, PLAIN_TR);
+    const int sr = stochastic ? 1 : 0;
+    const uint32_t seed = step_seed(time);
+    const bool watch = forcing.freeze_flags != nullptr;
+    if (forcing.two_color && watch)
+        hipLaunchKernelGGL((tracer_tile_kernel<Real, 0, true, true>), grid, block, 0, s, g, L, u.p, v.p, w.p, T0.p,
+                           S0.p, Tin.p, Sin.p, Tout.p, Sout.p, JT.p, JS.p, Jw.p, forcing, beta_dt, sr, seed);
+    else if (forcing.two_color)
+        hipLaunchKernelGGL((tracer_tile_kernel<Real, 0, true, false>), grid, block, 0, s, g, L, u.p, v.p, w.p, T0.p,
+                           S0.p, Tin.p, Sin.p, Tout.p, Sout.p, JT.p, JS.p, Jw.p, forcing, beta_dt, sr, seed);
+    else if (watch)
+        hipLaunchKernelGGL((tracer_tile_kernel<Real, 0, false, true>), grid, block, 0, s, g, L, u.p, v.p, w.p, T0.p,
+                           S0.p, Tin.p, Sin.p, Tout.p, Sout.p, JT.p, JS.p, Jw.p, forcing, beta_dt, sr, seed);
     else
-        hipLaunchKernelGGL((tracer_tile_kernel<Real, 0, false>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, PLAIN_TR), 0,
-                           s, g, L, u.p, v.p, w.p, T0.p, S0.p, Tin.p, Sin.p, Tout.p, Sout.p, JT.p, JS.p, Jw.p,
-                           forcing, beta_dt, stochastic ? 1 : 0, step_seed(time));
+        hipLaunchKernelGGL((tracer_tile_kernel<Real, 0, false, false>), grid, block, 0, s, g, L, u.p, v.p, w.p, T0.p,
+                           S0.p, Tin.p, Sin.p, Tout.p, Sout.p, JT.p, JS.p, Jw.p, forcing, beta_dt, sr, seed);
     return hipGetLastError();
 }
 template hipError_t launch_rk_stage<double>(const GridDev&, View3<const double>, View3<const double>,
@@ -663,26 +686,36 @@ template hipError_t launch_w_continuity<float>(const GridDev&, View3<const float
 int batch_tiles_x(int Nx) { return tiles_x(Nx); }
 int batch_tiles_y(int Ny, int tile_rows) { return tiles_y(Ny, tile_rows); }
 
-template <typename Real, bool TWO_COLOR>
-hipError_t launch_batch_stage_tc(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int tile_rows,
-                                 int blocks, int stage, double beta_dt, hipStream_t s) {
+template <typename Real, bool TWO_COLOR, bool WATCH>
+hipError_t launch_batch_stage_w(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int tile_rows,
+                                int blocks, int stage, double beta_dt, hipStream_t s) {
     switch (tile_rows) {
     case 4:
-        hipLaunchKernelGGL((batch_stage_kernel<Real, 4, TWO_COLOR>), dim3(blocks), dim3(TW, 4), 0, s, recs, work,
-                           nwork, stage, beta_dt);
+        hipLaunchKernelGGL((batch_stage_kernel<Real, 4, TWO_COLOR, WATCH>), dim3(blocks), dim3(TW, 4), 0, s, recs,
+                           work, nwork, stage, beta_dt);
         break;
     case 8:
-        hipLaunchKernelGGL((batch_stage_kernel<Real, 8, TWO_COLOR>), dim3(blocks), dim3(TW, 8), 0, s, recs, work,
-                           nwork, stage, beta_dt);
+        hipLaunchKernelGGL((batch_stage_kernel<Real, 8, TWO_COLOR, WATCH>), dim3(blocks), dim3(TW, 8), 0, s, recs,
+                           work, nwork, stage, beta_dt);
         break;
     case 16:
-        hipLaunchKernelGGL((batch_stage_kernel<Real, 16, TWO_COLOR>), dim3(blocks), dim3(TW, 16), 0, s, recs, work,
-                           nwork, stage, beta_dt);
+        hipLaunchKernelGGL((batch_stage_kernel<Real, 16, TWO_COLOR, WATCH>), dim3(blocks), dim3(TW, 16), 0, s, recs,
+                           work, nwork, stage, beta_dt);
         break;
     default:
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// Stage 3 (the step's last) is the instance that flags the columns the freeze clamp visits.
+template <typename Real, bool TWO_COLOR>
+hipError_t launch_batch_stage_tc(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int tile_rows,
+                                 int blocks, int stage, double beta_dt, hipStream_t s) {
+    return stage == 3 ? launch_batch_stage_w<Real, TWO_COLOR, true>(recs, work, nwork, tile_rows, blocks, stage,
+                                                                    beta_dt, s)
+                      : launch_batch_stage_w<Real, TWO_COLOR, false>(recs, work, nwork, tile_rows, blocks, stage,
+                                                                     beta_dt, s);
 }
 
 template <typename Real>

@@ -423,6 +423,45 @@ def test_model_steps_match_oracle(require_gpu, precision):
     assert gm.clock == (6000.0, 5)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_freezing_steps_clamp_every_flagged_column(require_gpu, precision):
+    """Cold, windy air over water 2 mK above its freezing point on 10 m levels: each step cools
+    surface cells below T_m and advection carries saltier (colder-freezing) water into fresher cells.
+    The step's last RK stage flags those columns and the freeze clamp after it visits only them; every
+    active cell must end at or above T_m, and in FP64 T, S and the frazil heat must equal the oracle's
+    (which clamps every column)."""
+    case = make_case(*SMALL, seed=21, z=(-100.0, 0.0))
+    p = to_params()
+    melt0, slope = p.freshwater_melting_temperature, p.liquidus_slope
+    case.T[...] = melt0 - slope * case.S + 0.002
+    case.atmos["T"][...] = 250.0
+    case.atmos["u"][...] = 15.0
+    print(f"[freeze] precision={precision} melt0={melt0} slope={slope}", flush=True)
+    gm = OceanOnlyModel(case, params=abi.Params.from_buffer_copy(p), precision=precision)
+    om = O.OracleModel(case, params=abi.Params.from_buffer_copy(p)) if precision == "fp64" else None
+    if om is not None:
+        om.reconcile()
+    active = np.arange(case.grid.spec.Nz)[:, None, None] >= case.grid.kmin[None]
+    frozen = 0
+    for step in range(3):
+        gm.time_step(1200.0)
+        torch.cuda.synchronize()
+        T, S = _np(gm.T).astype(np.float64), _np(gm.S).astype(np.float64)
+        # the clamp stores T_m (formed in FP64) rounded to the storage type
+        Tm = (melt0 - slope * S).astype(_np(gm.T).dtype).astype(np.float64)
+        assert np.all(T[active] >= Tm[active]), f"a cell below freezing after step {step + 1}"
+        fz = _np(gm.frazil_heat)
+        frozen += int(np.count_nonzero(fz))
+        if om is not None:
+            om.time_step(1200.0)
+            assert_rel(T, om.T, RTOL, "T")
+            assert_rel(S, om.S, RTOL, "S")
+            assert_rel(fz, om.frazil, RTOL, "frazil heat")
+            assert np.array_equal(fz == 0.0, om.frazil == 0.0)
+    assert frozen > 100, "the case froze too few columns to exercise the clamp"
+    gm.close()
+
+
 def _net_term_scales(case, om):
     """Magnitude of the terms summed into each net flux: τ is a face average of neighbouring
     ρτ/ρₒ (which may cancel), Jᵀ a sum of the turbulent and radiative heat fluxes over ρₒcₒ."""

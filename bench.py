@@ -76,12 +76,12 @@ def measured_valu(workload: str, precision: str):
     """Issued FP64 lane-flops per flux-solve launch and the VALU-busy fraction of the flux-solve
     and stage kernels, from the committed SQ counter summary, or None."""
     import re
-    f = REPO / "profiles" / "surface_solve_and_stage_valu_1440x560x10.md"
+    f = REPO / "profiles" / "surface_update_and_stage_valu_1440x560x10.md"
     if workload != "quarter_degree_1440x560x10" or not f.exists():
         return None
     text = f.read_text()
     out = {}
-    for key in ("fp64_lane_flops_per_launch", "valu_busy_surface_solve", "valu_busy_rk_stage"):
+    for key in ("fp64_lane_flops_per_launch", "valu_busy_surface_update", "valu_busy_rk_stage"):
         m = re.search(rf"{key}\[{precision}\] = ([0-9.eE+-]+)", text)
         if m:
             out[key] = float(m.group(1))
@@ -435,16 +435,16 @@ def main():
     if valu and args.precision == "fp64" and "fp64_lane_flops_per_launch" in valu:
         # the flux solve is FP64-VALU bound: issued FP64 lane-flops (ADD, MUL = 1, FMA = 2 per lane,
         # counted by SQ_INSTS_VALU_*_F64 over every issued wavefront instruction) per live launch time
-        solve_s = kernel_ms["surface_solve"] / nrec * 1e-3
+        solve_s = kernel_ms["surface_update"] / nrec * 1e-3
         tf = valu["fp64_lane_flops_per_launch"] / solve_s / 1e12
-        out["flux_solve"]["roofline"] = {"kernel": "surface_solve (interpolation + Monin-Obukhov fixed point)",
+        out["flux_solve"]["roofline"] = {"kernel": "surface_update (interpolation + Monin-Obukhov fixed point + column assembly)",
                                          "bound": "fp64-valu", "achieved": tf, "peak": FP64_VALU_PEAK_TFLOPS,
                                          "unit": "TFLOP/s", "frac": tf / FP64_VALU_PEAK_TFLOPS,
-                                         "valu_busy": valu.get("valu_busy_surface_solve")}
-    elif valu and "valu_busy_surface_solve" in valu:
+                                         "valu_busy": valu.get("valu_busy_surface_update")}
+    elif valu and "valu_busy_surface_update" in valu:
         # the mixed form's solve iterates in FP32 first: its bound is the VALU issue rate, not FP64 flops
-        out["flux_solve"]["roofline"] = {"kernel": "surface_solve (interpolation + Monin-Obukhov fixed point)",
-                                         "bound": "valu", "valu_busy": valu["valu_busy_surface_solve"]}
+        out["flux_solve"]["roofline"] = {"kernel": "surface_update (interpolation + Monin-Obukhov fixed point + column assembly)",
+                                         "bound": "valu", "valu_busy": valu["valu_busy_surface_update"]}
     if valu and "valu_busy_rk_stage" in valu:
         out["roofline"]["valu_busy"] = valu["valu_busy_rk_stage"]
     model.close()

@@ -542,8 +542,9 @@ int ne_model_clock(const ne_model* model, double* time, int64_t* iteration);
  * examples/near_global_ocean_simulation.jl:109-132). begin creates the events for up to
  * max_steps steps; end waits for them and writes the summed milliseconds of each kernel
  * to ms[NE_TIMED_KERNELS] and the number of steps recorded to *steps. Kernels, in order:
- * RK stage 1, RK stage 2, RK stage 3, surface solve (interpolation + flux solve),
- * surface finalize (freeze clamp + net fluxes + radiation). */
+ * RK stage 1, RK stage 2, RK stage 3, surface update (interpolation + flux solve, then per
+ * column the freeze clamp, net fluxes other than τ and radiation), surface τ (the momentum
+ * fluxes averaged onto the faces). */
 #define NE_TIMED_KERNELS 5
 int ne_model_timing_begin(ne_model* model, int max_steps);
 int ne_model_timing_end(ne_model* model, double* ms, int* steps);

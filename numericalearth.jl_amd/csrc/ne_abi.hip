@@ -552,28 +552,21 @@ static int model_update_state(ne_model* m, double dt_frazil, hipStream_t s, hipE
     if (d.precision == NE_FP64) {
         AtmosDev<double> a = atmos_dev<double>(d.atmosphere);
         RadDev<double> r = rad_dev<double>(d.radiation);
-        e = launch_surface_solve<double>(g, d.params, a, d.atmos_fi, d.atmos_fj, r, d.radiation_fi, d.radiation_fj,
-                                         view<const double>(d.ocean.T),
-                                         view<const double>(d.ocean.S), view<const double>(d.ocean.u),
-                                         view<const double>(d.ocean.v), f, m->ctx->counters, s);
-        if (e != hipSuccess) return NE_ERR_DEVICE;
-        if (ev && hipEventRecord(ev[4], s) != hipSuccess) return NE_ERR_DEVICE;
-        e = launch_surface_finalize<double>(g, d.params, a, d.atmos_fi, d.atmos_fj, r, d.radiation_fi,
-                                            d.radiation_fj, view<double>(d.ocean.T), view<const double>(d.ocean.S),
-                                            f, n, ro, frazil, dt_frazil, flags, s);
+        e = launch_surface_update<double>(g, d.params, a, d.atmos_fi, d.atmos_fj, r, d.radiation_fi, d.radiation_fj,
+                                          view<double>(d.ocean.T), view<const double>(d.ocean.S),
+                                          view<const double>(d.ocean.u), view<const double>(d.ocean.v), f, n, ro,
+                                          frazil, dt_frazil, flags, m->ctx->counters, s);
     } else {
         AtmosDev<float> a = atmos_dev<float>(d.atmosphere);
         RadDev<float> r = rad_dev<float>(d.radiation);
-        e = launch_surface_solve<float>(g, d.params, a, d.atmos_fi, d.atmos_fj, r, d.radiation_fi, d.radiation_fj,
-                                         view<const float>(d.ocean.T),
-                                        view<const float>(d.ocean.S), view<const float>(d.ocean.u),
-                                        view<const float>(d.ocean.v), f, m->ctx->counters, s);
-        if (e != hipSuccess) return NE_ERR_DEVICE;
-        if (ev && hipEventRecord(ev[4], s) != hipSuccess) return NE_ERR_DEVICE;
-        e = launch_surface_finalize<float>(g, d.params, a, d.atmos_fi, d.atmos_fj, r, d.radiation_fi,
-                                           d.radiation_fj, view<float>(d.ocean.T), view<const float>(d.ocean.S), f,
-                                           n, ro, frazil, dt_frazil, flags, s);
+        e = launch_surface_update<float>(g, d.params, a, d.atmos_fi, d.atmos_fj, r, d.radiation_fi, d.radiation_fj,
+                                         view<float>(d.ocean.T), view<const float>(d.ocean.S),
+                                         view<const float>(d.ocean.u), view<const float>(d.ocean.v), f, n, ro, frazil,
+                                         dt_frazil, flags, m->ctx->counters, s);
     }
+    if (e != hipSuccess) return NE_ERR_DEVICE;
+    if (ev && hipEventRecord(ev[4], s) != hipSuccess) return NE_ERR_DEVICE;
+    e = launch_surface_tau(g, d.params, f, n, s);
     if (e == hipSuccess && ev && hipEventRecord(ev[5], s) != hipSuccess) return NE_ERR_DEVICE;
     return dev_err(e);
 }

@@ -121,19 +121,16 @@ hipError_t launch_assemble(const GridDev& g, const ne_params& p, const ne_interf
 hipError_t launch_radiation(const GridDev& g, const ne_params& p, View3<const double> sw, View3<const double> lw,
                             View3<const double> Ts, const NetOut& n, const RadOut& r, hipStream_t s);
 
-// Fused update_state: K1 interpolates and solves every column, K2 clamps, assembles
-// and adds radiation.
+// Fused update_state: K1 interpolates, solves, clamps, assembles and adds radiation per
+// column; the τ kernel then averages the momentum fluxes onto the faces.
 template <typename Real>
-hipError_t launch_surface_solve(const GridDev& g, const ne_params& p, const AtmosDev<Real>& a,
-                                const double* fi, const double* fj, const RadDev<Real>& r, const double* rfi,
-                                const double* rfj, View3<const Real> T, View3<const Real> S, View3<const Real> u,
-                                View3<const Real> v, const FluxOut& out, ne_counters* c, hipStream_t s);
-template <typename Real>
-hipError_t launch_surface_finalize(const GridDev& g, const ne_params& p, const AtmosDev<Real>& a,
-                                   const double* afi, const double* afj, const RadDev<Real>& r,
-                                   const double* rfi, const double* rfj, View3<Real> T, View3<const Real> S,
-                                   const FluxOut& f, const NetOut& n, const RadOut& ro, View3<double> frazil,
-                                   double dt_frazil, const uint8_t* freeze_flags, hipStream_t s);
+hipError_t launch_surface_update(const GridDev& g, const ne_params& p, const AtmosDev<Real>& a, const double* fi,
+                                 const double* fj, const RadDev<Real>& r, const double* rfi, const double* rfj,
+                                 View3<Real> T, View3<const Real> S, View3<const Real> u, View3<const Real> v,
+                                 const FluxOut& out, const NetOut& n, const RadOut& ro, View3<double> frazil,
+                                 double dt_frazil, const uint8_t* freeze_flags, ne_counters* c, hipStream_t s);
+hipError_t launch_surface_tau(const GridDev& g, const ne_params& p, const FluxOut& f, const NetOut& n,
+                              hipStream_t s);
 
 // Tracer launchers (ne_tracer.hip).
 // stochastic: round the new state stochastically (FP32 storage only; the stage that writes

@@ -205,24 +205,6 @@ __device__ __forceinline__ ColumnFluxes surface_solve_column(const GridDev& g, c
     return r;
 }
 
-// K1 of the fused update_state: interpolate the atmosphere at the column and solve.
-template <typename Real, bool Fast>
-__global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(NE_FLUX_WAVES_PER_EU)))
-surface_solve_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __restrict__ fi,
-                     const double* __restrict__ fj, RadDev<Real> rd, const double* __restrict__ rfi,
-                     const double* __restrict__ rfj, View3<const Real> T, View3<const Real> S, View3<const Real> u,
-                     View3<const Real> v, FluxOut out, ne_counters* counters) {
-    int i = blockIdx.x * BX + threadIdx.x;
-    int j = blockIdx.y * BY + threadIdx.y;
-    bool in = i < g.Nx && j < g.Ny;
-    ColumnFluxes r;
-    bool inactive = true;
-    if (in) r = surface_solve_column<Real, Fast>(g, p, a, fi, fj, rd, rfi, rfj, T, S, u, v, out, i, j, inactive);
-    bool solved = in && !inactive;
-    accumulate_counters(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
-                        solved && !finite3(r.us, r.ts, r.qs));
-}
-
 // Ensemble form of K1: a persistent grid walks the (member, first column) work list; an
 // item is blockDim.x consecutive columns (i fastest) of one member. Members with the
 // default Edson constants are listed for the shared-ψ instance (Fast), the others for
@@ -280,6 +262,16 @@ struct NetColumn {
     double tau_x, tau_y, J_T, J_S, J_w, J_H;
 };
 
+// τ on the face between a cell and its west (south) neighbour: the face average of
+// (1 − ℵ)ρτ/ρₒ of the two cells plus the sea-ice term, 0 here (assemble_net_ocean_fluxes.jl:139-142).
+__device__ __forceinline__ double tau_face(const ne_params& p, double r, double r_nb) {
+    double rinv = 1.0 / p.ocean_reference_density;
+    double aice = 0.0;
+    double t_ao = (rinv * (1.0 - aice) * r + rinv * (1.0 - aice) * r_nb) / 2.0;
+    double t_io = 0.0 * rinv * 0.0;
+    return t_ao + t_io;
+}
+
 __device__ __forceinline__ NetColumn assemble_column(const GridDev& g, const ne_params& p, double Qt, double Qv,
                                                      double Jv, double rtx, double rtx_w, double rty,
                                                      double rty_s, double Jr, double Js, double To,
@@ -293,12 +285,9 @@ __device__ __forceinline__ NetColumn assemble_column(const GridDev& g, const ne_
     double Qin = 0.0, Jsio = 0.0, Jwio = 0.0;
     double JTao = SQao * rinv * cinv;
     double JTio = Qin * rinv * cinv;
-    double tx_ao = (rinv * (1.0 - aice) * rtx + rinv * (1.0 - aice) * rtx_w) / 2.0;
-    double ty_ao = (rinv * (1.0 - aice) * rty + rinv * (1.0 - aice) * rty_s) / 2.0;
-    double tx_io = 0.0 * rinv * 0.0, ty_io = 0.0 * rinv * 0.0;
     NetColumn n;
-    n.tau_x = inactive ? 0.0 : tx_ao + tx_io;
-    n.tau_y = inactive ? 0.0 : ty_ao + ty_io;
+    n.tau_x = inactive ? 0.0 : tau_face(p, rtx, rtx_w);
+    n.tau_y = inactive ? 0.0 : tau_face(p, rty, rty_s);
     n.J_T = inactive ? 0.0 : JTao + JTio;
     n.J_S = inactive ? 0.0 : Jsio;
     n.J_w = inactive ? 0.0 : Jwao + Jwio;
@@ -378,7 +367,7 @@ __global__ void radiation_kernel(GridDev g, ne_params p, View3<const double> sw,
 }
 
 // One column of K2: freeze clamp, net-flux assembly and radiation.
-template <typename Real>
+template <typename Real, bool TAU>
 __device__ __forceinline__ void surface_finalize_column(const GridDev& g, const ne_params& p, const AtmosDev<Real>& a,
                                                         const double* __restrict__ afi,
                                                         const double* __restrict__ afj, const RadDev<Real>& rd,
@@ -387,7 +376,8 @@ __device__ __forceinline__ void surface_finalize_column(const GridDev& g, const 
                                                         const View3<const Real>& S, const FluxOut& f,
                                                         const NetOut& n, const RadOut& ro,
                                                         const View3<double>& frazil, double dt_frazil,
-                                                        const uint8_t* __restrict__ freeze_flags, int i, int j) {
+                                                        const uint8_t* __restrict__ freeze_flags, double Qt,
+                                                        double Qv, double Jv, double Ts_C, int i, int j) {
     // after a step, only columns its last stage flagged can hold a cell below freezing; the
     // others would run the clamp without a change and with dQ = 0 (the same +0.0)
     const bool visit = freeze_flags == nullptr || freeze_flags[(int64_t)i + (int64_t)j * g.Nx] != 0;
@@ -398,11 +388,15 @@ __device__ __forceinline__ void surface_finalize_column(const GridDev& g, const 
     Neighbours y = neighbours(afj[j], a.Ny, a.ty);
     double Jr = interp_field(a.rain, x, y, a.time);
     double Js = interp_field(a.snow, x, y, a.time);
-    double rtx_w, rty_s;
-    neighbour_momentum(g, f.rtx, f.rty, i, j, rtx_w, rty_s);
+    double rtx = 0.0, rty = 0.0, rtx_w = 0.0, rty_s = 0.0;   // τ here, or in the τ kernel after all columns
+    if (TAU) {
+        rtx = f.rtx.at2(i, j);
+        rty = f.rty.at2(i, j);
+        neighbour_momentum(g, f.rtx, f.rty, i, j, rtx_w, rty_s);
+    }
     bool inactive = !g.active(i, j, g.Nz - 1);
-    NetColumn c = assemble_column(g, p, f.Qt.at2(i, j), f.Qv.at2(i, j), f.Jv.at2(i, j), f.rtx.at2(i, j), rtx_w,
-                                  f.rty.at2(i, j), rty_s, Jr, Js, (double)T(i, j, g.Nz - 1), inactive);
+    NetColumn c = assemble_column(g, p, Qt, Qv, Jv, rtx, rtx_w, rty, rty_s, Jr, Js, (double)T(i, j, g.Nz - 1),
+                                  inactive);
     double JT = c.J_T;
     if (p.radiation) {
         Neighbours rx = neighbours(rfi[i], rd.Nx, rd.tx);
@@ -410,33 +404,60 @@ __device__ __forceinline__ void surface_finalize_column(const GridDev& g, const 
         double SW = interp_field(rd.sw, rx, ry, rd.time);
         double LW = interp_field(rd.lw, rx, ry, rd.time);
         double Iup, Ila, Itsw;
-        double JTrad = radiation_column(p, f.Ts.at2(i, j), SW, LW, Iup, Ila, Itsw);
+        double JTrad = radiation_column(p, Ts_C, SW, LW, Iup, Ila, Itsw);
         JT = JT + (inactive ? 0.0 : JTrad);
         store_penetrating_shortwave(p, n, i, j, Itsw);
         if (ro.up_lw.p) ro.up_lw.at2(i, j) = Iup;
         if (ro.down_lw.p) ro.down_lw.at2(i, j) = -Ila;
         if (ro.down_sw.p) ro.down_sw.at2(i, j) = -Itsw;
     }
-    n.tau_x.at2(i, j) = c.tau_x;
-    n.tau_y.at2(i, j) = c.tau_y;
+    if (TAU) {
+        n.tau_x.at2(i, j) = c.tau_x;
+        n.tau_y.at2(i, j) = c.tau_y;
+    }
     n.J_T.at2(i, j) = JT;
     n.J_S.at2(i, j) = c.J_S;
     n.J_w.at2(i, j) = c.J_w;
     n.J_H.at2(i, j) = c.J_H;
 }
 
-// K2 of the fused update_state: freeze clamp, net-flux assembly and radiation per column.
-template <typename Real>
-__global__ void __launch_bounds__(BX* BY)
-surface_finalize_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __restrict__ afi,
-                        const double* __restrict__ afj, RadDev<Real> rd, const double* __restrict__ rfi,
-                        const double* __restrict__ rfj, View3<Real> T, View3<const Real> S, FluxOut f, NetOut n,
-                        RadOut ro, View3<double> frazil, double dt_frazil, const uint8_t* freeze_flags) {
+// K1 of the fused update_state: per column, interpolate the atmosphere and solve the fluxes,
+// then (with the solved values still in registers) the freeze clamp, the net fluxes other
+// than τ and the radiation, in the reference's phase order (fluxes from the unclamped SST,
+// clamp, assembly with the clamped SST, radiation).
+template <typename Real, bool Fast>
+__global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(NE_FLUX_WAVES_PER_EU)))
+surface_update_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __restrict__ fi,
+                      const double* __restrict__ fj, RadDev<Real> rd, const double* __restrict__ rfi,
+                      const double* __restrict__ rfj, View3<Real> T, View3<const Real> S, View3<const Real> u,
+                      View3<const Real> v, FluxOut out, NetOut n, RadOut ro, View3<double> frazil, double dt_frazil,
+                      const uint8_t* __restrict__ freeze_flags, ne_counters* counters) {
+    int i = blockIdx.x * BX + threadIdx.x;
+    int j = blockIdx.y * BY + threadIdx.y;
+    bool in = i < g.Nx && j < g.Ny;
+    ColumnFluxes r;
+    bool inactive = true;
+    if (in) {
+        r = surface_solve_column<Real, Fast>(g, p, a, fi, fj, rd, rfi, rfj, cview(T), S, u, v, out, i, j, inactive);
+        surface_finalize_column<Real, false>(g, p, a, fi, fj, rd, rfi, rfj, T, S, out, n, ro, frazil, dt_frazil,
+                                             freeze_flags, r.Qt, r.Qv, r.Jv, r.Ts_C, i, j);
+    }
+    bool solved = in && !inactive;
+    accumulate_counters(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
+                        solved && !finite3(r.us, r.ts, r.qs));
+}
+
+// After the fused K1: τ of every column from its own and its west / south neighbours'
+// momentum fluxes, which K1 wrote (the one part of the update that reads other columns).
+__global__ void __launch_bounds__(BX* BY) surface_tau_kernel(GridDev g, ne_params p, FluxOut f, NetOut n) {
     int i = blockIdx.x * BX + threadIdx.x;
     int j = blockIdx.y * BY + threadIdx.y;
     if (i >= g.Nx || j >= g.Ny) return;
-    surface_finalize_column<Real>(g, p, a, afi, afj, rd, rfi, rfj, T, S, f, n, ro, frazil, dt_frazil, freeze_flags,
-                                  i, j);
+    double rtx_w, rty_s;
+    neighbour_momentum(g, f.rtx, f.rty, i, j, rtx_w, rty_s);
+    bool inactive = !g.active(i, j, g.Nz - 1);
+    n.tau_x.at2(i, j) = inactive ? 0.0 : tau_face(p, f.rtx.at2(i, j), rtx_w);
+    n.tau_y.at2(i, j) = inactive ? 0.0 : tau_face(p, f.rty.at2(i, j), rty_s);
 }
 
 // Ensemble form of K2 over the same work list as the ensemble K1.
@@ -451,8 +472,10 @@ batch_surface_finalize_kernel(const BatchRec<Real>* __restrict__ recs, const Wor
         const int col = it.index + (int)threadIdx.x;
         if (col >= g.Nx * g.Ny) continue;
         const int i = col % g.Nx, j = col / g.Nx;
-        surface_finalize_column<Real>(g, rc.p, rc.a, rc.afi, rc.afj, rc.r, rc.rfi, rc.rfj, rc.T, cview(rc.S), rc.f,
-                                      rc.n, rc.ro, rc.frazil, dt_frazil, rc.freeze_flags, i, j);
+        surface_finalize_column<Real, true>(g, rc.p, rc.a, rc.afi, rc.afj, rc.r, rc.rfi, rc.rfj, rc.T, cview(rc.S),
+                                            rc.f, rc.n, rc.ro, rc.frazil, dt_frazil, rc.freeze_flags,
+                                            rc.f.Qt.at2(i, j), rc.f.Qv.at2(i, j), rc.f.Jv.at2(i, j),
+                                            rc.f.Ts.at2(i, j), i, j);
     }
 }
 
@@ -591,50 +614,37 @@ hipError_t launch_radiation(const GridDev& g, const ne_params& p, View3<const do
 }
 
 template <typename Real>
-hipError_t launch_surface_solve(const GridDev& g, const ne_params& p, const AtmosDev<Real>& a, const double* fi,
-                                const double* fj, const RadDev<Real>& r, const double* rfi, const double* rfj,
-                                View3<const Real> T, View3<const Real> S, View3<const Real> u, View3<const Real> v,
-                                const FluxOut& out, ne_counters* c, hipStream_t s) {
+hipError_t launch_surface_update(const GridDev& g, const ne_params& p, const AtmosDev<Real>& a, const double* fi,
+                                 const double* fj, const RadDev<Real>& r, const double* rfi, const double* rfj,
+                                 View3<Real> T, View3<const Real> S, View3<const Real> u, View3<const Real> v,
+                                 const FluxOut& out, const NetOut& n, const RadOut& ro, View3<double> frazil,
+                                 double dt_frazil, const uint8_t* freeze_flags, ne_counters* c, hipStream_t s) {
     if (edson_default_constants(p))
-        hipLaunchKernelGGL((surface_solve_kernel<Real, true>), grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, a, fi,
-                           fj, r, rfi, rfj, T, S, u, v, out, c);
+        hipLaunchKernelGGL((surface_update_kernel<Real, true>), grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, a, fi,
+                           fj, r, rfi, rfj, T, S, u, v, out, n, ro, frazil, dt_frazil, freeze_flags, c);
     else
-        hipLaunchKernelGGL((surface_solve_kernel<Real, false>), grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, a, fi,
-                           fj, r, rfi, rfj, T, S, u, v, out, c);
+        hipLaunchKernelGGL((surface_update_kernel<Real, false>), grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, a, fi,
+                           fj, r, rfi, rfj, T, S, u, v, out, n, ro, frazil, dt_frazil, freeze_flags, c);
     return hipGetLastError();
 }
-template hipError_t launch_surface_solve<double>(const GridDev&, const ne_params&, const AtmosDev<double>&,
-                                                 const double*, const double*, const RadDev<double>&, const double*,
-                                                 const double*, View3<const double>, View3<const double>,
-                                                 View3<const double>, View3<const double>, const FluxOut&,
-                                                 ne_counters*, hipStream_t);
-template hipError_t launch_surface_solve<float>(const GridDev&, const ne_params&, const AtmosDev<float>&,
-                                                const double*, const double*, const RadDev<float>&, const double*,
-                                                const double*, View3<const float>, View3<const float>,
-                                                View3<const float>, View3<const float>, const FluxOut&,
-                                                ne_counters*, hipStream_t);
+template hipError_t launch_surface_update<double>(const GridDev&, const ne_params&, const AtmosDev<double>&,
+                                                  const double*, const double*, const RadDev<double>&, const double*,
+                                                  const double*, View3<double>, View3<const double>,
+                                                  View3<const double>, View3<const double>, const FluxOut&,
+                                                  const NetOut&, const RadOut&, View3<double>, double,
+                                                  const uint8_t*, ne_counters*, hipStream_t);
+template hipError_t launch_surface_update<float>(const GridDev&, const ne_params&, const AtmosDev<float>&,
+                                                 const double*, const double*, const RadDev<float>&, const double*,
+                                                 const double*, View3<float>, View3<const float>,
+                                                 View3<const float>, View3<const float>, const FluxOut&,
+                                                 const NetOut&, const RadOut&, View3<double>, double,
+                                                 const uint8_t*, ne_counters*, hipStream_t);
 
-template <typename Real>
-hipError_t launch_surface_finalize(const GridDev& g, const ne_params& p, const AtmosDev<Real>& a,
-                                   const double* afi, const double* afj, const RadDev<Real>& r, const double* rfi,
-                                   const double* rfj, View3<Real> T, View3<const Real> S, const FluxOut& f,
-                                   const NetOut& n, const RadOut& ro, View3<double> frazil, double dt_frazil,
-                                   const uint8_t* freeze_flags, hipStream_t s) {
-    hipLaunchKernelGGL(surface_finalize_kernel<Real>, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, a, afi, afj, r,
-                       rfi, rfj, T, S, f, n, ro, frazil, dt_frazil, freeze_flags);
+hipError_t launch_surface_tau(const GridDev& g, const ne_params& p, const FluxOut& f, const NetOut& n,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(surface_tau_kernel, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, f, n);
     return hipGetLastError();
 }
-template hipError_t launch_surface_finalize<double>(const GridDev&, const ne_params&, const AtmosDev<double>&,
-                                                    const double*, const double*, const RadDev<double>&,
-                                                    const double*, const double*, View3<double>,
-                                                    View3<const double>, const FluxOut&, const NetOut&,
-                                                    const RadOut&, View3<double>, double, const uint8_t*,
-                                                    hipStream_t);
-template hipError_t launch_surface_finalize<float>(const GridDev&, const ne_params&, const AtmosDev<float>&,
-                                                   const double*, const double*, const RadDev<float>&,
-                                                   const double*, const double*, View3<float>, View3<const float>,
-                                                   const FluxOut&, const NetOut&, const RadOut&, View3<double>,
-                                                   double, const uint8_t*, hipStream_t);
 
 template <typename Real>
 hipError_t launch_batch_surface_solve(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int block,

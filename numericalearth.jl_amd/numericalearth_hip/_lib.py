@@ -78,7 +78,7 @@ SIGNATURES = {
     "ne_batch_info": [vp, P(C.c_int), P(C.c_int64), P(C.c_int), P(C.c_int)],
 }
 
-TIMED_KERNELS = ("rk_stage_1", "rk_stage_2", "rk_stage_3", "surface_solve", "surface_finalize")
+TIMED_KERNELS = ("rk_stage_1", "rk_stage_2", "rk_stage_3", "surface_update", "surface_tau")
 
 _lib = None
 

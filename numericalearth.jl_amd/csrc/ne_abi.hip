@@ -248,51 +248,15 @@ int ne_version(void) { return NE_ABI_VERSION; }
 int ne_params_default(ne_params* p) {
     if (!p) return NE_ERR_NULL;
     memset(p, 0, sizeof(*p));
+    set_default_surface_constants(*p);   // roughness, gustiness, stop criterion, thermodynamics
     const double pi = 3.14159265358979323846;
-    p->von_karman_constant = 0.4;
     p->turbulent_prandtl_number = 1.0;
-    p->convective_gustiness = 1;
-    p->mesoscale_gustiness = 0;
-    p->gustiness_parameter = 1.2;
-    p->minimum_gustiness = 0.01;
-    p->mesoscale_velocity = 0.0;
     p->stability_functions = NE_STABILITY_EDSON;
     p->similarity_form = NE_SIMILARITY_LOG;
     const double em[11] = {50.0, 0.35, 0.7, 0.75, 5.0 / 0.35, 15.0, 2.0, pi / 2.0, 10.15, 3.0, pi / sqrt(3.0)};
     const double es[12] = {50.0, 0.35, 2.0 / 3.0, 3.0 / 2.0, 14.28, 8.525, 15.0, 2.0, 0.0, 34.15, 3.0, pi / sqrt(3.0)};
     memcpy(p->edson_momentum, em, sizeof(em));
     memcpy(p->edson_scalar, es, sizeof(es));
-    p->momentum_roughness = NE_ROUGHNESS_EDSON;
-    p->temperature_roughness = NE_ROUGHNESS_EDSON;
-    p->water_vapor_roughness = NE_ROUGHNESS_EDSON;
-    p->roughness_gravitational_acceleration = 9.80665;
-    p->air_kinematic_viscosity = 1.5e-5;
-    p->charnock_parameter = 0.02;
-    p->smooth_wall_parameter = 0.11;
-    p->maximum_momentum_roughness = 1.0;
-    p->reynolds_scaling_A = 5.85e-5;
-    p->reynolds_scaling_b = 0.72;
-    p->maximum_scalar_roughness = 1.6e-4;
-    p->zero_plane_displacement = 0.0;
-    p->stop_criteria = NE_STOP_CONVERGENCE;
-    p->maxiter = 100;
-    p->tolerance = 1e-8;
-    p->specific_humidity = NE_HUMIDITY_IMPURE_SATURATION;
-    p->velocity_difference = NE_VELOCITY_RELATIVE;
-    p->water_mole_fraction = 0.98;
-    p->surface_layer_height = 10.0;
-    p->boundary_layer_height = 512.0;
-    p->gravitational_acceleration = 9.80665;
-    p->gas_constant = 8.3144598;
-    p->dry_air_molar_mass = 0.02897;
-    p->water_molar_mass = 0.018015;
-    p->dry_air_adiabatic_exponent = 2.0 / 7.0;
-    p->water_vapor_heat_capacity = 1859.0;
-    p->liquid_water_heat_capacity = 4181.0;
-    p->reference_vaporization_enthalpy = 2500800.0;
-    p->reference_temperature = 273.16;
-    p->triple_point_temperature = 273.16;
-    p->triple_point_pressure = 611.657;
     p->ocean_reference_density = 1020.0;
     p->ocean_heat_capacity = 3991.86795711963;
     p->freshwater_density = 1000.0;

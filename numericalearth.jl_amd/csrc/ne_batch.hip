@@ -76,8 +76,8 @@ struct ne_batch {
     // per column-block choice: every member (finalize), and the members of each flux path (solve)
     WorkItem* surface_work[3] = {nullptr, nullptr, nullptr};
     int surface_nwork[3] = {0, 0, 0};
-    WorkItem* solve_work[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};  // [fast][choice]
-    int solve_nwork[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    WorkItem* solve_work[3][3] = {};   // [flux path][choice]
+    int solve_nwork[3][3] = {};
 
     ne_batch_tuning tuning;
     int cus = 256;
@@ -122,10 +122,10 @@ hipError_t enqueue_step(ne_batch* b, double dt, hipStream_t s, ne_counters* coun
     hipLaunchKernelGGL(batch_clock_kernel<Real>, dim3((n + 63) / 64), dim3(64), 0, s,
                        static_cast<BatchRec<Real>*>(b->recs), n, dt);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    for (int fast = 1; fast >= 0; fast--) {
-        int nw = b->solve_nwork[fast][si];
-        e = launch_batch_surface_solve<Real>(recs, b->solve_work[fast][si], nw, t.surface_block,
-                                             std::min(blocks_for(t.surface_blocks, nw), std::max(nw, 1)), fast == 1,
+    for (int path = 2; path >= 0; path--) {
+        int nw = b->solve_nwork[path][si];
+        e = launch_batch_surface_solve<Real>(recs, b->solve_work[path][si], nw, t.surface_block,
+                                             std::min(blocks_for(t.surface_blocks, nw), std::max(nw, 1)), path,
                                              counters, s);
         if (e != hipSuccess) return e;
     }
@@ -176,15 +176,15 @@ BatchRec<Real> member_record(const ne_model* m, const double* atimes_dev, const 
     r.rtimes = rtimes_dev;
     r.nat = (int)m->atimes.size();
     r.nrt = (int)m->rtimes.size();
-    r.fast = edson_default_constants(d.params) ? 1 : 0;
+    r.fast = flux_path(d.params);
     r.time = m->time;
     return r;
 }
 
-// Host-side layout of the arena: records | times | 2×3 stage lists | 3 surface lists | 2×3 solve lists.
+// Host-side layout of the arena: records | times | 2×3 stage lists | 3 surface lists | 3×3 solve lists.
 struct Layout {
     size_t recs = 0, times = 0, stage[2][3] = {{0, 0, 0}, {0, 0, 0}}, surface[3] = {0, 0, 0},
-           solve[2][3] = {{0, 0, 0}, {0, 0, 0}};
+           solve[3][3] = {};
     size_t total = 0;
 };
 
@@ -192,7 +192,7 @@ template <typename Real>
 int rebuild(ne_batch* b, hipStream_t s) {
     const int n = (int)b->members.size();
     // work lists and snapshot times on the host
-    std::vector<WorkItem> stage[2][3], surface[3], solve[2][3];
+    std::vector<WorkItem> stage[2][3], surface[3], solve[3][3];
     std::vector<double> times;
     std::vector<size_t> aoff(n), roff(n);
     for (int k = 0; k < n; k++) {
@@ -203,12 +203,12 @@ int rebuild(ne_batch* b, hipStream_t s) {
             int nt = batch_tiles_x(g.Nx) * batch_tiles_y(g.Ny, TILE_ROWS[c]);
             for (int t = 0; t < nt; t++) stage[tc][c].push_back(WorkItem{k, t});
         }
-        int fast = edson_default_constants(m->d.params) ? 1 : 0;
+        int path = flux_path(m->d.params);
         int cols = g.Nx * g.Ny;
         for (int c = 0; c < 3; c++)
             for (int s0 = 0; s0 < cols; s0 += SURFACE_BLOCKS[c]) {
                 surface[c].push_back(WorkItem{k, s0});
-                solve[fast][c].push_back(WorkItem{k, s0});
+                solve[path][c].push_back(WorkItem{k, s0});
             }
         aoff[k] = times.size();
         times.insert(times.end(), m->atimes.begin(), m->atimes.end());
@@ -228,7 +228,7 @@ int rebuild(ne_batch* b, hipStream_t s) {
         }
         L.surface[c] = off;
         off = align256(off + surface[c].size() * sizeof(WorkItem));
-        for (int f = 0; f < 2; f++) {
+        for (int f = 0; f < 3; f++) {
             L.solve[f][c] = off;
             off = align256(off + solve[f][c].size() * sizeof(WorkItem));
         }
@@ -277,7 +277,7 @@ int rebuild(ne_batch* b, hipStream_t s) {
     for (int c = 0; c < 3; c++) {
         for (int f = 0; f < 2; f++) put(L.stage[f][c], stage[f][c], &b->stage_work[f][c], &b->stage_nwork[f][c]);
         put(L.surface[c], surface[c], &b->surface_work[c], &b->surface_nwork[c]);
-        for (int f = 0; f < 2; f++) put(L.solve[f][c], solve[f][c], &b->solve_work[f][c], &b->solve_nwork[f][c]);
+        for (int f = 0; f < 3; f++) put(L.solve[f][c], solve[f][c], &b->solve_work[f][c], &b->solve_nwork[f][c]);
     }
     b->recs = D + L.recs;
     if (hipMemcpyAsync(D, H, L.total, hipMemcpyHostToDevice, s) != hipSuccess) return NE_ERR_DEVICE;
@@ -381,10 +381,10 @@ int autotune(ne_batch* b, hipStream_t s, ne_batch_tuning* out) {
             double ms;
             auto f = [&]() {
                 hipError_t e = hipSuccess;
-                for (int fast = 1; fast >= 0 && e == hipSuccess; fast--) {
-                    int nw = b->solve_nwork[fast][c];
-                    e = launch_batch_surface_solve<Real>(recs, b->solve_work[fast][c], nw, SURFACE_BLOCKS[c],
-                                                         std::min(blocks_for(req, nw), std::max(nw, 1)), fast == 1,
+                for (int path = 2; path >= 0 && e == hipSuccess; path--) {
+                    int nw = b->solve_nwork[path][c];
+                    e = launch_batch_surface_solve<Real>(recs, b->solve_work[path][c], nw, SURFACE_BLOCKS[c],
+                                                         std::min(blocks_for(req, nw), std::max(nw, 1)), path,
                                                          nullptr, s);
                 }
                 return e;

@@ -4,6 +4,7 @@
 // (cited per function); the library is compiled with -ffp-contract=off so that the
 // FP64 path is the same sequence of IEEE operations as the reference arithmetic.
 #pragma once
+#include <string.h>
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -309,6 +310,69 @@ __host__ inline bool edson_default_constants(const ne_params& p) {
     return true;
 }
 
+// The reference's default surface-layer constants: von Kármán constant and gustiness
+// (similarity_theory_turbulent_fluxes.jl:174-199, :45-98), roughness lengths
+// (roughness_lengths.jl:93-146, :197-246), stop criterion, humidity and velocity difference
+// (component_interfaces.jl:353-358), surface-layer and boundary-layer heights and the
+// thermodynamic constants (Atmospheres/thermodynamic_parameters.jl:30-258). ne_params_default
+// sets them through this function; a flux-solve instance with them compiled in (so that the
+// fixed-point loop keeps no register for them) runs when a model's parameters equal them.
+__host__ __device__ inline void set_default_surface_constants(ne_params& q) {
+    q.von_karman_constant = 0.4;
+    q.convective_gustiness = 1;
+    q.mesoscale_gustiness = 0;
+    q.gustiness_parameter = 1.2;
+    q.minimum_gustiness = 0.01;
+    q.mesoscale_velocity = 0.0;
+    q.momentum_roughness = NE_ROUGHNESS_EDSON;
+    q.temperature_roughness = NE_ROUGHNESS_EDSON;
+    q.water_vapor_roughness = NE_ROUGHNESS_EDSON;
+    q.roughness_gravitational_acceleration = 9.80665;
+    q.air_kinematic_viscosity = 1.5e-5;
+    q.charnock_parameter = 0.02;
+    q.smooth_wall_parameter = 0.11;
+    q.maximum_momentum_roughness = 1.0;
+    q.reynolds_scaling_A = 5.85e-5;
+    q.reynolds_scaling_b = 0.72;
+    q.maximum_scalar_roughness = 1.6e-4;
+    q.zero_plane_displacement = 0.0;
+    q.stop_criteria = NE_STOP_CONVERGENCE;
+    q.maxiter = 100;
+    q.tolerance = 1e-8;
+    q.specific_humidity = NE_HUMIDITY_IMPURE_SATURATION;
+    q.velocity_difference = NE_VELOCITY_RELATIVE;
+    q.water_mole_fraction = 0.98;
+    q.surface_layer_height = 10.0;
+    q.boundary_layer_height = 512.0;
+    q.gravitational_acceleration = 9.80665;
+    q.gas_constant = 8.3144598;
+    q.dry_air_molar_mass = 0.02897;
+    q.water_molar_mass = 0.018015;
+    q.dry_air_adiabatic_exponent = 2.0 / 7.0;
+    q.water_vapor_heat_capacity = 1859.0;
+    q.liquid_water_heat_capacity = 4181.0;
+    q.reference_vaporization_enthalpy = 2500800.0;
+    q.reference_temperature = 273.16;
+    q.triple_point_temperature = 273.16;
+    q.triple_point_pressure = 611.657;
+}
+
+// True when every constant set_default_surface_constants sets already holds its default
+// (compared bit for bit, so −0.0 for a zero default counts as different).
+__host__ inline bool default_surface_constants(const ne_params& p) {
+    ne_params q;
+    memcpy(&q, &p, sizeof q);
+    set_default_surface_constants(q);
+    return memcmp(&q, &p, sizeof q) == 0;
+}
+
+// Flux-solve instance of a parameter set: 0 the general path, 1 the reference's default
+// Edson constants (shared ψ), 2 those and the default surface constants compiled in.
+__host__ inline int flux_path(const ne_params& p) {
+    if (!edson_default_constants(p)) return 0;
+    return default_surface_constants(p) ? 2 : 1;
+}
+
 // similarity_profile (similarity_theory_turbulent_fluxes.jl:242-253)
 __device__ __forceinline__ double similarity_profile(const ne_params& p, bool scalar, double h, double l, double L) {
     double zeta = h / L;
@@ -562,10 +626,17 @@ __device__ __forceinline__ T tdiv(T a, T b) { return a / b; }
 template <>
 __device__ __forceinline__ float tdiv<float>(float a, float b) { return __fdividef(a, b); }
 
-template <bool Fast, bool Mixed = false>
-__device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p, double ua, double va, double Ta,
+template <bool Fast, bool Mixed = false, bool Consts = false>
+__device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p_in, double ua, double va, double Ta,
                                                     double pa, double qa, double To_C, double uo, double vo,
                                                     bool inactive, double SW = 0.0, double LW = 0.0) {
+    // Consts: the default surface constants as literals (the host has checked that p holds them)
+    ne_params pc;
+    if constexpr (Consts) {
+        pc = p_in;
+        set_default_surface_constants(pc);
+    }
+    const ne_params& p = Consts ? pc : p_in;
     const double g = p.gravitational_acceleration;
     const double za = p.surface_layer_height;
     const double hbl = p.boundary_layer_height;

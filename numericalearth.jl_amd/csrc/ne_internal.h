@@ -201,7 +201,7 @@ struct BatchRec {
     int nat, nrt;
     int time_indexing;      // NE_TIME_CLAMPED or NE_TIME_CYCLICAL
     double time_period;
-    int fast;               // default Edson constants: the shared-ψ flux path
+    int fast;               // flux path (flux_path): 0 general, 1 default Edson constants, 2 also default constants
     double time;           // device clock of the member (advanced by the clock kernel)
 };
 
@@ -232,7 +232,7 @@ hipError_t launch_batch_stage(const BatchRec<Real>* recs, const WorkItem* work, 
                               int blocks, int stage, double beta_dt, bool two_color, hipStream_t s);
 template <typename Real>
 hipError_t launch_batch_surface_solve(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int block,
-                                      int blocks, bool fast, ne_counters* c, hipStream_t s);
+                                      int blocks, int path, ne_counters* c, hipStream_t s);
 template <typename Real>
 hipError_t launch_batch_surface_finalize(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int block,
                                          int blocks, double dt_frazil, hipStream_t s);

@@ -139,7 +139,7 @@ __device__ __forceinline__ void ocean_surface(const GridDev& g, const View3<cons
     vo = ((double)v(i, j, kN) + (double)v(i, jp, kN)) / 2.0;
 }
 
-template <bool Fast>
+template <bool Fast, bool Consts>
 __global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(NE_FLUX_WAVES_PER_EU)))
 flux_solve_exchange_kernel(GridDev g, ne_params p, View3<const double> ua, View3<const double> va,
                                            View3<const double> Ta, View3<const double> pa, View3<const double> qa,
@@ -160,7 +160,7 @@ flux_solve_exchange_kernel(GridDev g, ne_params p, View3<const double> ua, View3
             SW = sw.at2(i, j);
             LW = lw.at2(i, j);
         }
-        r = flux_column<Fast>(p, ua.at2(i, j), va.at2(i, j), Ta.at2(i, j), pa.at2(i, j), qa.at2(i, j), To, uo, vo,
+        r = flux_column<Fast, false, Consts>(p, ua.at2(i, j), va.at2(i, j), Ta.at2(i, j), pa.at2(i, j), qa.at2(i, j), To, uo, vo,
                               inactive, SW, LW);
         store_fluxes(out, i, j, r);
     }
@@ -172,7 +172,7 @@ flux_solve_exchange_kernel(GridDev g, ne_params p, View3<const double> ua, View3
 // One column of K1: interpolate the atmosphere at the column, solve, store the fluxes.
 // The skin-temperature balance also reads the downwelling radiation, interpolated here
 // exactly as K2 interpolates it.
-template <typename Real, bool Fast>
+template <typename Real, bool Fast, bool Consts>
 __device__ __forceinline__ ColumnFluxes surface_solve_column(const GridDev& g, const ne_params& p,
                                                              const AtmosDev<Real>& a, const double* __restrict__ fi,
                                                              const double* __restrict__ fj, const RadDev<Real>& rd,
@@ -199,7 +199,8 @@ __device__ __forceinline__ ColumnFluxes surface_solve_column(const GridDev& g, c
         LW = interp_field(rd.lw, rx, ry, rd.time);
     }
     // FP32 storage (the mixed form) also takes the FP32 pre-iteration of the fixed point
-    ColumnFluxes r = flux_column<Fast, NE_MIXED_PREITERATION && sizeof(Real) == 4>(p, ua, va, Ta, pa, qa, To, uo, vo,
+    ColumnFluxes r = flux_column<Fast, NE_MIXED_PREITERATION && sizeof(Real) == 4, Consts>(p, ua, va, Ta, pa, qa, To,
+                                                                                           uo, vo,
                                                                                    inactive, SW, LW);
     store_fluxes(out, i, j, r);
     return r;
@@ -209,7 +210,7 @@ __device__ __forceinline__ ColumnFluxes surface_solve_column(const GridDev& g, c
 // item is blockDim.x consecutive columns (i fastest) of one member. Members with the
 // default Edson constants are listed for the shared-ψ instance (Fast), the others for
 // the general one, as the single-model launcher chooses.
-template <typename Real, bool Fast>
+template <typename Real, bool Fast, bool Consts>
 __global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(NE_FLUX_WAVES_PER_EU)))
 batch_surface_solve_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __restrict__ work, int nwork,
                            ne_counters* counters) {
@@ -223,7 +224,7 @@ batch_surface_solve_kernel(const BatchRec<Real>* __restrict__ recs, const WorkIt
         ColumnFluxes r;
         bool inactive = true;
         if (in)
-            r = surface_solve_column<Real, Fast>(g, rc.p, rc.a, rc.afi, rc.afj, rc.r, rc.rfi, rc.rfj, cview(rc.T),
+            r = surface_solve_column<Real, Fast, Consts>(g, rc.p, rc.a, rc.afi, rc.afj, rc.r, rc.rfi, rc.rfj, cview(rc.T),
                                                  cview(rc.S), rc.u, rc.v, rc.f, i, j, inactive);
         bool solved = in && !inactive;
         accumulate_counters(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
@@ -425,7 +426,7 @@ __device__ __forceinline__ void surface_finalize_column(const GridDev& g, const 
 // then (with the solved values still in registers) the freeze clamp, the net fluxes other
 // than τ and the radiation, in the reference's phase order (fluxes from the unclamped SST,
 // clamp, assembly with the clamped SST, radiation).
-template <typename Real, bool Fast>
+template <typename Real, bool Fast, bool Consts>
 __global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(NE_FLUX_WAVES_PER_EU)))
 surface_update_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __restrict__ fi,
                       const double* __restrict__ fj, RadDev<Real> rd, const double* __restrict__ rfi,
@@ -438,7 +439,8 @@ surface_update_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __
     ColumnFluxes r;
     bool inactive = true;
     if (in) {
-        r = surface_solve_column<Real, Fast>(g, p, a, fi, fj, rd, rfi, rfj, cview(T), S, u, v, out, i, j, inactive);
+        r = surface_solve_column<Real, Fast, Consts>(g, p, a, fi, fj, rd, rfi, rfj, cview(T), S, u, v, out, i, j,
+                                                     inactive);
         surface_finalize_column<Real, false>(g, p, a, fi, fj, rd, rfi, rfj, T, S, out, n, ro, frazil, dt_frazil,
                                              freeze_flags, r.Qt, r.Qv, r.Jv, r.Ts_C, i, j);
     }
@@ -578,18 +580,13 @@ hipError_t launch_flux_solve_exchange(const GridDev& g, const ne_params& p, cons
                                       hipStream_t s) {
     View3<const double> sw = view<const double>(a.downwelling_shortwave);
     View3<const double> lw = view<const double>(a.downwelling_longwave);
-    if (edson_default_constants(p))
-        hipLaunchKernelGGL(flux_solve_exchange_kernel<true>, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p,
-                           view<const double>(a.u), view<const double>(a.v), view<const double>(a.T),
-                           view<const double>(a.p), view<const double>(a.q), view<const double>(o.T),
-                           view<const double>(o.S), view<const double>(o.u), view<const double>(o.v), sw, lw, out,
-                           c);
-    else
-        hipLaunchKernelGGL(flux_solve_exchange_kernel<false>, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p,
-                           view<const double>(a.u), view<const double>(a.v), view<const double>(a.T),
-                           view<const double>(a.p), view<const double>(a.q), view<const double>(o.T),
-                           view<const double>(o.S), view<const double>(o.u), view<const double>(o.v), sw, lw, out,
-                           c);
+    const int path = flux_path(p);
+    auto kernel = path == 2 ? flux_solve_exchange_kernel<true, true>
+                : path == 1 ? flux_solve_exchange_kernel<true, false> : flux_solve_exchange_kernel<false, false>;
+    hipLaunchKernelGGL(kernel, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, view<const double>(a.u),
+                       view<const double>(a.v), view<const double>(a.T), view<const double>(a.p),
+                       view<const double>(a.q), view<const double>(o.T), view<const double>(o.S),
+                       view<const double>(o.u), view<const double>(o.v), sw, lw, out, c);
     return hipGetLastError();
 }
 
@@ -619,12 +616,11 @@ hipError_t launch_surface_update(const GridDev& g, const ne_params& p, const Atm
                                  View3<Real> T, View3<const Real> S, View3<const Real> u, View3<const Real> v,
                                  const FluxOut& out, const NetOut& n, const RadOut& ro, View3<double> frazil,
                                  double dt_frazil, const uint8_t* freeze_flags, ne_counters* c, hipStream_t s) {
-    if (edson_default_constants(p))
-        hipLaunchKernelGGL((surface_update_kernel<Real, true>), grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, a, fi,
-                           fj, r, rfi, rfj, T, S, u, v, out, n, ro, frazil, dt_frazil, freeze_flags, c);
-    else
-        hipLaunchKernelGGL((surface_update_kernel<Real, false>), grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, a, fi,
-                           fj, r, rfi, rfj, T, S, u, v, out, n, ro, frazil, dt_frazil, freeze_flags, c);
+    const int path = flux_path(p);
+    auto kernel = path == 2 ? surface_update_kernel<Real, true, true>
+                : path == 1 ? surface_update_kernel<Real, true, false> : surface_update_kernel<Real, false, false>;
+    hipLaunchKernelGGL(kernel, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, a, fi, fj, r, rfi, rfj, T, S, u, v, out,
+                       n, ro, frazil, dt_frazil, freeze_flags, c);
     return hipGetLastError();
 }
 template hipError_t launch_surface_update<double>(const GridDev&, const ne_params&, const AtmosDev<double>&,
@@ -648,20 +644,17 @@ hipError_t launch_surface_tau(const GridDev& g, const ne_params& p, const FluxOu
 
 template <typename Real>
 hipError_t launch_batch_surface_solve(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int block,
-                                      int blocks, bool fast, ne_counters* c, hipStream_t s) {
+                                      int blocks, int path, ne_counters* c, hipStream_t s) {
     if (nwork <= 0) return hipSuccess;
     if (block <= 0 || block > BX * BY || (block & (block - 1)) != 0 || blocks <= 0) return hipErrorInvalidValue;
-    if (fast)
-        hipLaunchKernelGGL((batch_surface_solve_kernel<Real, true>), dim3(blocks), dim3(block), 0, s, recs, work,
-                           nwork, c);
-    else
-        hipLaunchKernelGGL((batch_surface_solve_kernel<Real, false>), dim3(blocks), dim3(block), 0, s, recs, work,
-                           nwork, c);
+    auto kernel = path == 2 ? batch_surface_solve_kernel<Real, true, true>
+                : path == 1 ? batch_surface_solve_kernel<Real, true, false> : batch_surface_solve_kernel<Real, false, false>;
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(block), 0, s, recs, work, nwork, c);
     return hipGetLastError();
 }
-template hipError_t launch_batch_surface_solve<double>(const BatchRec<double>*, const WorkItem*, int, int, int, bool,
+template hipError_t launch_batch_surface_solve<double>(const BatchRec<double>*, const WorkItem*, int, int, int, int,
                                                        ne_counters*, hipStream_t);
-template hipError_t launch_batch_surface_solve<float>(const BatchRec<float>*, const WorkItem*, int, int, int, bool,
+template hipError_t launch_batch_surface_solve<float>(const BatchRec<float>*, const WorkItem*, int, int, int, int,
                                                       ne_counters*, hipStream_t);
 
 template <typename Real>

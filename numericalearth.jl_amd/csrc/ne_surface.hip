@@ -27,6 +27,32 @@ constexpr int BY = 4;
 
 inline dim3 grid2d(int Nx, int Ny) { return dim3((Nx + BX - 1) / BX, (Ny + BY - 1) / BY); }
 
+// Counting sort of a block's BX·BY columns by a small key (0 … SORT_BINS−1): returns the
+// block-local column (0 … BX·BY−1) that work-item t computes, the columns taken in key order.
+// Ties are ordered by the LDS atomics, which is immaterial (columns are independent).
+constexpr int SORT_BINS = 64;
+__device__ __forceinline__ int sorted_block_column(int key, int t) {
+    __shared__ int s_hist[SORT_BINS];
+    __shared__ int s_col[BX * BY];
+    if (t < SORT_BINS) s_hist[t] = 0;
+    __syncthreads();
+    const int rank = atomicAdd(&s_hist[key], 1);
+    __syncthreads();
+    if (t < 64) {   // exclusive prefix over the bins: the first wavefront, a shuffle scan
+        const int v = t < SORT_BINS ? s_hist[t] : 0;
+        int x = v;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(x, d, 64);
+            if (t >= d) x += y;
+        }
+        if (t < SORT_BINS) s_hist[t] = x - v;
+    }
+    __syncthreads();
+    s_col[s_hist[key] + rank] = t;
+    __syncthreads();
+    return s_col[t];
+}
+
 // Block-wide reduction of the solver counters, then one atomic per block.
 __device__ void accumulate_counters(ne_counters* c, bool solved, int it, bool nonconv, bool nan) {
     __shared__ unsigned s_it[BX * BY], s_solved[BX * BY], s_nc[BX * BY], s_nan[BX * BY], s_max[BX * BY];
@@ -435,6 +461,18 @@ surface_update_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __
                       const uint8_t* __restrict__ freeze_flags, ne_counters* counters) {
     int i = blockIdx.x * BX + threadIdx.x;
     int j = blockIdx.y * BY + threadIdx.y;
+    if (out.iters.p) {
+        // Columns of the block in order of the iteration count the previous solve took: the
+        // fixed point runs until the slowest column of a wavefront converges, so wavefronts of
+        // columns with like counts leave fewer lanes idle. Each column's result does not depend
+        // on the lane that computes it.
+        const int t = threadIdx.x + threadIdx.y * BX;
+        const bool own = i < g.Nx && j < g.Ny;
+        const int key = own ? min(max(out.iters.at2(i, j), 0), SORT_BINS - 2) : SORT_BINS - 1;
+        const int c = sorted_block_column(key, t);
+        i = blockIdx.x * BX + c % BX;
+        j = blockIdx.y * BY + c / BX;
+    }
     bool in = i < g.Nx && j < g.Ny;
     ColumnFluxes r;
     bool inactive = true;

@@ -33,7 +33,7 @@ inline dim3 grid2d(int Nx, int Ny) { return dim3((Nx + BX - 1) / BX, (Ny + BY - 
 constexpr int SORT_BINS = 64;
 __device__ __forceinline__ int sorted_block_column(int key, int t) {
     __shared__ int s_hist[SORT_BINS];
-    __shared__ int s_col[BX * BY];
+    __shared__ int s_col[BX * BY];   // blocks of at most BX·BY work-items
     if (t < SORT_BINS) s_hist[t] = 0;
     __syncthreads();
     const int rank = atomicAdd(&s_hist[key], 1);
@@ -244,7 +244,13 @@ batch_surface_solve_kernel(const BatchRec<Real>* __restrict__ recs, const WorkIt
         const WorkItem it = work[wi];
         const BatchRec<Real>& rc = recs[it.member];
         const GridDev g = rc.g;
-        const int col = it.index + (int)threadIdx.x;
+        int col = it.index + (int)threadIdx.x;
+        if (rc.f.iters.p && blockDim.x > 64) {
+            // the item's columns in order of their previous iteration count (as the single model)
+            const int key = col < g.Nx * g.Ny ? min(max(rc.f.iters.at2(col % g.Nx, col / g.Nx), 0), SORT_BINS - 2)
+                                              : SORT_BINS - 1;
+            col = it.index + sorted_block_column(key, (int)threadIdx.x);
+        }
         const bool in = col < g.Nx * g.Ny;
         const int i = in ? col % g.Nx : 0, j = in ? col / g.Nx : 0;
         ColumnFluxes r;

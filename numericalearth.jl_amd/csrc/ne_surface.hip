@@ -174,6 +174,13 @@ flux_solve_exchange_kernel(GridDev g, ne_params p, View3<const double> ua, View3
                                            View3<const double> lw, FluxOut out, ne_counters* counters) {
     int i = blockIdx.x * BX + threadIdx.x;
     int j = blockIdx.y * BY + threadIdx.y;
+    if (out.iters.p) {   // columns in order of their previous iteration count (as in K1)
+        const bool own = i < g.Nx && j < g.Ny;
+        const int key = own ? min(max(out.iters.at2(i, j), 0), SORT_BINS - 2) : SORT_BINS - 1;
+        const int c = sorted_block_column(key, threadIdx.x + threadIdx.y * BX);
+        i = blockIdx.x * BX + c % BX;
+        j = blockIdx.y * BY + c / BX;
+    }
     bool in = i < g.Nx && j < g.Ny;
     ColumnFluxes r;
     bool inactive = true;

@@ -8,6 +8,8 @@ must match bit for bit, and so must the rows whose GPU arithmetic is the oracle'
 sequence of correctly rounded operations (interpolation, freeze clamp, net-flux
 assembly, radiation).
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 import torch
@@ -175,6 +177,35 @@ def test_flux_solve_matches_oracle(require_gpu):
     case, om, gm = _pair()
     _solve_both(case, om, gm)
     _compare_fluxes(gm, om)
+
+
+def test_flux_solve_column_order_changes_nothing(require_gpu):
+    """The solve takes each block's columns in order of the iteration counts in the iterations
+    view (so that wavefronts hold columns of like cost); without the view it keeps the natural
+    order. Scrambled counts, the real ones and none must give the same fluxes bit for bit."""
+    case, om, gm = _pair(seed=5)
+    ph = _solve_both(case, om, gm)
+    ref = {k: v.clone() for k, v in gm.fluxes.items()}
+    iters = gm.iterations.clone()
+    for trial in ("scrambled", "none"):
+        if trial == "scrambled":
+            gm.iterations.copy_(torch.randint(0, 40, gm.iterations.shape, device=gm.iterations.device,
+                                              dtype=gm.iterations.dtype, generator=torch.Generator(
+                                                  device=gm.iterations.device).manual_seed(3)))
+            fl = gm.desc.fluxes
+        else:
+            fl = abi.InterfaceFluxes.from_buffer_copy(gm.desc.fluxes)
+            fl.iterations = abi.View(None, 0, 0, 0)
+        for v in gm.fluxes.values():
+            v.zero_()
+        from numericalearth_hip._lib import check
+        check(gm.lib.ne_ao_flux_solve_f64(gm.ctx.handle, C.byref(gm.grid.struct), C.byref(gm.params),
+                                          C.byref(ph._ex()), C.byref(gm.desc.ocean), C.byref(fl), None),
+              "ne_ao_flux_solve_f64")
+        torch.cuda.synchronize()
+        for k in ref:
+            assert torch.equal(gm.fluxes[k], ref[k]), f"{k} changed with {trial} column order"
+    assert torch.equal(gm.iterations, iters)   # the scrambled counts were overwritten by the real ones
 
 
 def test_flux_solve_fixed_iterations(require_gpu):

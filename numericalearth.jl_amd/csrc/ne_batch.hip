@@ -126,12 +126,12 @@ hipError_t enqueue_step(ne_batch* b, double dt, hipStream_t s, ne_counters* coun
         int nw = b->solve_nwork[path][si];
         e = launch_batch_surface_solve<Real>(recs, b->solve_work[path][si], nw, t.surface_block,
                                              std::min(blocks_for(t.surface_blocks, nw), std::max(nw, 1)), path,
-                                             counters, s);
+                                             true, dt, counters, s);
         if (e != hipSuccess) return e;
     }
     int nsu = b->surface_nwork[si];
-    return launch_batch_surface_finalize<Real>(recs, b->surface_work[si], nsu, t.surface_block,
-                                               std::min(blocks_for(t.surface_blocks, nsu), std::max(nsu, 1)), dt, s);
+    return launch_batch_surface_tau<Real>(recs, b->surface_work[si], nsu, t.surface_block,
+                                          std::min(blocks_for(t.surface_blocks, nsu), std::max(nsu, 1)), s);
 }
 
 hipError_t enqueue(ne_batch* b, double dt, hipStream_t s, ne_counters* counters) {
@@ -385,7 +385,7 @@ int autotune(ne_batch* b, hipStream_t s, ne_batch_tuning* out) {
                     int nw = b->solve_nwork[path][c];
                     e = launch_batch_surface_solve<Real>(recs, b->solve_work[path][c], nw, SURFACE_BLOCKS[c],
                                                          std::min(blocks_for(req, nw), std::max(nw, 1)), path,
-                                                         nullptr, s);
+                                                         false, 0.0, nullptr, s);
                 }
                 return e;
             };

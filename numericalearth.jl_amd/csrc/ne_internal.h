@@ -232,10 +232,11 @@ hipError_t launch_batch_stage(const BatchRec<Real>* recs, const WorkItem* work, 
                               int blocks, int stage, double beta_dt, bool two_color, hipStream_t s);
 template <typename Real>
 hipError_t launch_batch_surface_solve(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int block,
-                                      int blocks, int path, ne_counters* c, hipStream_t s);
+                                      int blocks, int path, bool fused, double dt_frazil, ne_counters* c,
+                                      hipStream_t s);
 template <typename Real>
-hipError_t launch_batch_surface_finalize(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int block,
-                                         int blocks, double dt_frazil, hipStream_t s);
+hipError_t launch_batch_surface_tau(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int block,
+                                   int blocks, hipStream_t s);
 int batch_tiles_x(int Nx);
 int batch_tiles_y(int Ny, int tile_rows);
 

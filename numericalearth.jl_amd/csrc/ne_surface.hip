@@ -5,6 +5,8 @@
 // The flux solve is FP64-VALU bound (transcendental-heavy fixed point); lanes of a
 // wavefront retire independently and the wavefront leaves the loop when its last
 // lane has converged (the loop condition is the wavefront-wide vote).
+#include <type_traits>
+
 #include "ne_internal.h"
 
 namespace ne {
@@ -239,37 +241,6 @@ __device__ __forceinline__ ColumnFluxes surface_solve_column(const GridDev& g, c
     return r;
 }
 
-// Ensemble form of K1: a persistent grid walks the (member, first column) work list; an
-// item is blockDim.x consecutive columns (i fastest) of one member. Members with the
-// default Edson constants are listed for the shared-ψ instance (Fast), the others for
-// the general one, as the single-model launcher chooses.
-template <typename Real, bool Fast, bool Consts>
-__global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(NE_FLUX_WAVES_PER_EU)))
-batch_surface_solve_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __restrict__ work, int nwork,
-                           ne_counters* counters) {
-    for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
-        const WorkItem it = work[wi];
-        const BatchRec<Real>& rc = recs[it.member];
-        const GridDev g = rc.g;
-        int col = it.index + (int)threadIdx.x;
-        if (rc.f.iters.p && blockDim.x > 64) {
-            // the item's columns in order of their previous iteration count (as the single model)
-            const int key = col < g.Nx * g.Ny ? min(max(rc.f.iters.at2(col % g.Nx, col / g.Nx), 0), SORT_BINS - 2)
-                                              : SORT_BINS - 1;
-            col = it.index + sorted_block_column(key, (int)threadIdx.x);
-        }
-        const bool in = col < g.Nx * g.Ny;
-        const int i = in ? col % g.Nx : 0, j = in ? col / g.Nx : 0;
-        ColumnFluxes r;
-        bool inactive = true;
-        if (in)
-            r = surface_solve_column<Real, Fast, Consts>(g, rc.p, rc.a, rc.afi, rc.afj, rc.r, rc.rfi, rc.rfj, cview(rc.T),
-                                                 cview(rc.S), rc.u, rc.v, rc.f, i, j, inactive);
-        bool solved = in && !inactive;
-        accumulate_counters(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
-                            solved && !finite3(r.us, r.ts, r.qs));
-    }
-}
 
 // Freeze clamp of one column (freezing_limited_ocean_temperature.jl:95-118); returns frazil heat.
 template <typename Real>
@@ -513,11 +484,51 @@ __global__ void __launch_bounds__(BX* BY) surface_tau_kernel(GridDev g, ne_param
     n.tau_y.at2(i, j) = inactive ? 0.0 : tau_face(p, f.rty.at2(i, j), rty_s);
 }
 
-// Ensemble form of K2 over the same work list as the ensemble K1.
+// Ensemble form of K1: a persistent grid walks the (member, first column) work list; an
+// item is blockDim.x consecutive columns (i fastest) of one member. Members with the
+// default Edson constants are listed for the shared-ψ instance (Fast), the others for
+// the general one, as the single-model launcher chooses.
+// FUSED (the step): each column then goes on as in the single model's K1 (freeze clamp of the
+// flagged columns, net fluxes other than τ, radiation); without it (the autotuner's timing
+// runs) the kernel only solves, leaving the state as it is.
+template <typename Real, bool Fast, bool Consts, bool FUSED>
+__global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(NE_FLUX_WAVES_PER_EU)))
+batch_surface_solve_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __restrict__ work, int nwork,
+                           double dt_frazil, ne_counters* counters) {
+    for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+        const WorkItem it = work[wi];
+        const BatchRec<Real>& rc = recs[it.member];
+        const GridDev g = rc.g;
+        int col = it.index + (int)threadIdx.x;
+        if (rc.f.iters.p && blockDim.x > 64) {
+            // the item's columns in order of their previous iteration count (as the single model)
+            const int key = col < g.Nx * g.Ny ? min(max(rc.f.iters.at2(col % g.Nx, col / g.Nx), 0), SORT_BINS - 2)
+                                              : SORT_BINS - 1;
+            col = it.index + sorted_block_column(key, (int)threadIdx.x);
+        }
+        const bool in = col < g.Nx * g.Ny;
+        const int i = in ? col % g.Nx : 0, j = in ? col / g.Nx : 0;
+        ColumnFluxes r;
+        bool inactive = true;
+        if (in)
+        {
+            r = surface_solve_column<Real, Fast, Consts>(g, rc.p, rc.a, rc.afi, rc.afj, rc.r, rc.rfi, rc.rfj, cview(rc.T),
+                                                 cview(rc.S), rc.u, rc.v, rc.f, i, j, inactive);
+            if (FUSED)
+                surface_finalize_column<Real, false>(g, rc.p, rc.a, rc.afi, rc.afj, rc.r, rc.rfi, rc.rfj, rc.T,
+                                                     cview(rc.S), rc.f, rc.n, rc.ro, rc.frazil, dt_frazil,
+                                                     rc.freeze_flags, r.Qt, r.Qv, r.Jv, r.Ts_C, i, j);
+        }
+        bool solved = in && !inactive;
+        accumulate_counters(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
+                            solved && !finite3(r.us, r.ts, r.qs));
+    }
+}
+
+// Ensemble form of K2 over the same work list as the ensemble K1: τ of every column.
 template <typename Real>
 __global__ void __launch_bounds__(BX* BY)
-batch_surface_finalize_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __restrict__ work, int nwork,
-                              double dt_frazil) {
+batch_surface_tau_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __restrict__ work, int nwork) {
     for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
         const WorkItem it = work[wi];
         const BatchRec<Real>& rc = recs[it.member];
@@ -525,10 +536,11 @@ batch_surface_finalize_kernel(const BatchRec<Real>* __restrict__ recs, const Wor
         const int col = it.index + (int)threadIdx.x;
         if (col >= g.Nx * g.Ny) continue;
         const int i = col % g.Nx, j = col / g.Nx;
-        surface_finalize_column<Real, true>(g, rc.p, rc.a, rc.afi, rc.afj, rc.r, rc.rfi, rc.rfj, rc.T, cview(rc.S),
-                                            rc.f, rc.n, rc.ro, rc.frazil, dt_frazil, rc.freeze_flags,
-                                            rc.f.Qt.at2(i, j), rc.f.Qv.at2(i, j), rc.f.Jv.at2(i, j),
-                                            rc.f.Ts.at2(i, j), i, j);
+        double rtx_w, rty_s;
+        neighbour_momentum(g, rc.f.rtx, rc.f.rty, i, j, rtx_w, rty_s);
+        const bool inactive = !g.active(i, j, g.Nz - 1);
+        rc.n.tau_x.at2(i, j) = inactive ? 0.0 : tau_face(rc.p, rc.f.rtx.at2(i, j), rtx_w);
+        rc.n.tau_y.at2(i, j) = inactive ? 0.0 : tau_face(rc.p, rc.f.rty.at2(i, j), rty_s);
     }
 }
 
@@ -695,31 +707,35 @@ hipError_t launch_surface_tau(const GridDev& g, const ne_params& p, const FluxOu
 
 template <typename Real>
 hipError_t launch_batch_surface_solve(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int block,
-                                      int blocks, int path, ne_counters* c, hipStream_t s) {
+                                      int blocks, int path, bool fused, double dt_frazil, ne_counters* c,
+                                      hipStream_t s) {
     if (nwork <= 0) return hipSuccess;
     if (block <= 0 || block > BX * BY || (block & (block - 1)) != 0 || blocks <= 0) return hipErrorInvalidValue;
-    auto kernel = path == 2 ? batch_surface_solve_kernel<Real, true, true>
-                : path == 1 ? batch_surface_solve_kernel<Real, true, false> : batch_surface_solve_kernel<Real, false, false>;
-    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(block), 0, s, recs, work, nwork, c);
+    auto pick = [&](auto fused_tag) {
+        constexpr bool F = decltype(fused_tag)::value;
+        return path == 2 ? batch_surface_solve_kernel<Real, true, true, F>
+             : path == 1 ? batch_surface_solve_kernel<Real, true, false, F> : batch_surface_solve_kernel<Real, false, false, F>;
+    };
+    auto kernel = fused ? pick(std::true_type{}) : pick(std::false_type{});
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(block), 0, s, recs, work, nwork, dt_frazil, c);
     return hipGetLastError();
 }
 template hipError_t launch_batch_surface_solve<double>(const BatchRec<double>*, const WorkItem*, int, int, int, int,
-                                                       ne_counters*, hipStream_t);
+                                                       bool, double, ne_counters*, hipStream_t);
 template hipError_t launch_batch_surface_solve<float>(const BatchRec<float>*, const WorkItem*, int, int, int, int,
-                                                      ne_counters*, hipStream_t);
+                                                      bool, double, ne_counters*, hipStream_t);
 
 template <typename Real>
-hipError_t launch_batch_surface_finalize(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int block,
-                                         int blocks, double dt_frazil, hipStream_t s) {
+hipError_t launch_batch_surface_tau(const BatchRec<Real>* recs, const WorkItem* work, int nwork, int block,
+                                   int blocks, hipStream_t s) {
     if (nwork <= 0) return hipSuccess;
     if (block <= 0 || block > BX * BY || blocks <= 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(batch_surface_finalize_kernel<Real>, dim3(blocks), dim3(block), 0, s, recs, work, nwork,
-                       dt_frazil);
+    hipLaunchKernelGGL(batch_surface_tau_kernel<Real>, dim3(blocks), dim3(block), 0, s, recs, work, nwork);
     return hipGetLastError();
 }
-template hipError_t launch_batch_surface_finalize<double>(const BatchRec<double>*, const WorkItem*, int, int, int,
-                                                          double, hipStream_t);
-template hipError_t launch_batch_surface_finalize<float>(const BatchRec<float>*, const WorkItem*, int, int, int,
-                                                         double, hipStream_t);
+template hipError_t launch_batch_surface_tau<double>(const BatchRec<double>*, const WorkItem*, int, int, int,
+                                                     hipStream_t);
+template hipError_t launch_batch_surface_tau<float>(const BatchRec<float>*, const WorkItem*, int, int, int,
+                                                    hipStream_t);
 
 }  // namespace ne

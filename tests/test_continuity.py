@@ -109,3 +109,35 @@ def test_continuity_kernel_matches_oracle(require_gpu, precision):
         assert np.array_equal(w2.cpu().numpy(), ref)
     else:   # accumulated in FP64, each face rounded once to FP32
         assert np.array_equal(got, ref.astype(np.float32).astype(np.float64))
+
+
+@pytest.mark.gpu
+def test_model_steps_with_evolving_velocities_match_oracle(require_gpu):
+    """A caller that evolves u and v sets them before each step and recomputes w from continuity
+    (the model's update_vertical_velocity); four such steps equal the oracle doing the same, to
+    1e-12 in T and S, and w is the oracle's bit for bit after every update."""
+    import torch
+    from helpers import assert_rel
+    from numericalearth_hip.model import OceanOnlyModel
+    case = make_case(90, 40, 10, seed=34)
+    print("[case] 90x40x10 seed=34 evolving velocities", flush=True)
+    gm = OceanOnlyModel(case)
+    om = O.OracleModel(case)
+    om.reconcile()
+    for n in range(4):
+        f = 1.0 + 0.5 * np.sin(0.9 * n + 0.3)      # same factor for u and v: no barotropic transport
+        u, v = f * case.u, f * case.v
+        gm.u.copy_(torch.as_tensor(u, dtype=gm.u.dtype, device=gm.u.device))
+        gm.v.copy_(torch.as_tensor(v, dtype=gm.v.dtype, device=gm.v.device))
+        gm.update_vertical_velocity()
+        om.u[...] = u
+        om.v[...] = v
+        om.w[...] = _oracle_w(case, u, v)
+        torch.cuda.synchronize()
+        assert np.array_equal(gm.w.cpu().numpy(), om.w), f"w after update {n + 1}"
+        gm.time_step(1200.0)
+        om.time_step(1200.0)
+        torch.cuda.synchronize()
+        assert_rel(gm.T.cpu().numpy(), om.T, 1e-12, f"T after step {n + 1}")
+        assert_rel(gm.S.cpu().numpy(), om.S, 1e-12, f"S after step {n + 1}")
+    gm.close()

@@ -346,6 +346,20 @@ int ne_grid_immersed_levels(ne_ctx* ctx, const ne_grid* grid, ne_view bottom_hei
 int ne_fractional_indices_f64(ne_ctx* ctx, const ne_grid* grid, const ne_source_grid* source,
                               double* fi, double* fj, void* stream);
 
+/* Fractional indices in a source grid given by its cell-centre coordinates instead of a
+ * regular spacing: lambda_nodes[nx] (longitudes, degrees) and phi_nodes[ny] (latitudes), DEVICE
+ * arrays in ascending order. The grid the reference builds for JRA55 forcing has the files'
+ * own, unequally spaced (Gaussian) latitudes (DataWrangling/JRA55/JRA55_metadata.jl:27-40),
+ * whose fractional position in the stretched direction (FractionalIndices on a stretched
+ * LatitudeLongitudeGrid) is k + (x − x_k)/(x_{k+1} − x_k) in the node interval holding x.
+ * Periodic axes (period 360°) wrap from the last node to the first one a period on; bounded
+ * axes extend their first and last interval (the interpolation then holds the edge value).
+ * fi, fj as for ne_fractional_indices_f64; the source grid passed to the interpolation keeps
+ * its Nx, Ny and topologies (its spacing fields are then unused). */
+int ne_fractional_indices_nodes_f64(ne_ctx* ctx, const ne_grid* grid, const double* lambda_nodes, int32_t nx,
+                                    int32_t topology_x, const double* phi_nodes, int32_t ny, int32_t topology_y,
+                                    double* fi, double* fj, void* stream);
+
 /* _interpolate_primary_atmospheric_state! (interpolate_atmospheric_state.jl:91-137):
  * bilinear in space, linear in time, onto the exchange grid interior. */
 int ne_interp_atmos_f64(ne_ctx* ctx, const ne_grid* grid, const ne_prescribed_atmosphere* atmos,

@@ -369,6 +369,18 @@ int ne_fractional_indices_f64(ne_ctx* c, const ne_grid* g, const ne_source_grid*
     return dev_err(launch_fractional_indices(grid_dev(*g), *src, fi, fj, as_stream(stream)));
 }
 
+int ne_fractional_indices_nodes_f64(ne_ctx* c, const ne_grid* g, const double* lambda_nodes, int32_t nx,
+                                    int32_t topology_x, const double* phi_nodes, int32_t ny, int32_t topology_y,
+                                    double* fi, double* fj, void* stream) {
+    if (!c || !g || !lambda_nodes || !phi_nodes || !fi || !fj) return NE_ERR_NULL;
+    if (!g->lambda_c || !g->phi_c) return NE_ERR_NULL;
+    if (g->Nx <= 0 || g->Ny <= 0 || nx <= 0 || ny <= 0) return NE_ERR_SHAPE;
+    for (int32_t t : {topology_x, topology_y})
+        if (t != NE_PERIODIC && t != NE_BOUNDED && t != NE_FLAT) return NE_ERR_UNSUPPORTED;
+    return dev_err(launch_fractional_indices_nodes(grid_dev(*g), lambda_nodes, nx, topology_x, phi_nodes, ny,
+                                                   topology_y, fi, fj, as_stream(stream)));
+}
+
 int ne_interp_atmos_f64(ne_ctx* c, const ne_grid* g, const ne_prescribed_atmosphere* a, const double* fi,
                         const double* fj, const ne_exchange_state* out, void* stream) {
     if (!c || !out || !fi || !fj) return NE_ERR_NULL;

@@ -808,6 +808,34 @@ __device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p_in, doubl
     return r;
 }
 
+// Fractional index (0-based) of x in a source axis given by its cell-centre coordinates
+// nodes[0 … n−1] (ascending): k + (x − x_k)/(x_{k+1} − x_k) in the node interval that holds x.
+// Periodic axes (longitude, period 360°) wrap from the last node to the first one a period
+// on; bounded axes extend their first and last interval linearly (the interpolation then
+// holds the edge value, as it does for a regular grid). Flat axes and single nodes give 0.
+__host__ __device__ inline double fractional_index_nodes(double x, const double* nodes, int n, int topology) {
+    if (topology == NE_FLAT || n <= 1) return 0.0;
+    if (topology == NE_PERIODIC) {
+        const double x0 = nodes[0];
+        if (x < x0 || x >= x0 + 360.0) {   // only longitudes outside the record's turn are moved
+            double t = fmod(x - x0, 360.0);
+            if (t < 0.0) t += 360.0;
+            x = x0 + t;
+        }
+        if (x >= nodes[n - 1]) return (double)(n - 1) + (x - nodes[n - 1]) / (x0 + 360.0 - nodes[n - 1]);
+    } else {
+        if (x < nodes[0]) return (x - nodes[0]) / (nodes[1] - nodes[0]);
+        if (x >= nodes[n - 1]) return (double)(n - 1) + (x - nodes[n - 1]) / (nodes[n - 1] - nodes[n - 2]);
+    }
+    int lo = 0, hi = n - 1;   // nodes[lo] ≤ x < nodes[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (nodes[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return (double)lo + (x - nodes[lo]) / (nodes[lo + 1] - nodes[lo]);
+}
+
 // ---------------------------------------------------------------------------
 // Interpolation (interpolate_atmospheric_state.jl:91-182)
 // ---------------------------------------------------------------------------

@@ -122,6 +122,13 @@ __global__ void fractional_indices_kernel(GridDev g, ne_source_grid s, double* f
     if (n < g.Ny) fj[n] = s.topology_y == NE_FLAT ? 0.0 : (g.phi_c[n] - pc0) / s.dphi;
 }
 
+__global__ void fractional_indices_nodes_kernel(GridDev g, const double* xn, int nx, int tx, const double* yn, int ny,
+                                                int ty, double* fi, double* fj) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < g.Nx) fi[n] = fractional_index_nodes(g.lambda_c[n], xn, nx, tx);
+    if (n < g.Ny) fj[n] = fractional_index_nodes(g.phi_c[n], yn, ny, ty);
+}
+
 template <typename Real>
 __global__ void interp_atmos_kernel(GridDev g, AtmosDev<Real> a, const double* fi, const double* fj,
                                     View3<double> u, View3<double> v, View3<double> T, View3<double> p,
@@ -604,6 +611,14 @@ RadOut rad_out(const ne_radiative_fluxes& r) {
 
 hipError_t launch_immersed_levels(const GridDev& g, View3<const double> bottom, int32_t* kmin, hipStream_t s) {
     hipLaunchKernelGGL(immersed_levels_kernel, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, bottom, kmin);
+    return hipGetLastError();
+}
+
+hipError_t launch_fractional_indices_nodes(const GridDev& g, const double* xn, int nx, int tx, const double* yn,
+                                          int ny, int ty, double* fi, double* fj, hipStream_t s) {
+    int n = g.Nx > g.Ny ? g.Nx : g.Ny;
+    hipLaunchKernelGGL(fractional_indices_nodes_kernel, dim3((n + 255) / 256), dim3(256), 0, s, g, xn, nx, tx, yn, ny,
+                       ty, fi, fj);
     return hipGetLastError();
 }
 

@@ -76,6 +76,38 @@ class SourceSpec:
     dlambda: float
     phi0: float
     dphi: float
+    # Cell-centre coordinates of a source grid that is not equally spaced (ascending, degrees),
+    # e.g. the Gaussian latitudes of JRA55's grid; None: the regular spacing above.
+    lambda_nodes: np.ndarray | None = None
+    phi_nodes: np.ndarray | None = None
+
+    @property
+    def irregular(self) -> bool:
+        return self.lambda_nodes is not None or self.phi_nodes is not None
+
+    def centres(self) -> tuple[np.ndarray, np.ndarray]:
+        """Cell-centre longitudes and latitudes (the given nodes, or those of the regular spacing)."""
+        lam = (self.lambda_nodes if self.lambda_nodes is not None
+               else self.lambda0 + (np.arange(self.Nx) + 0.5) * self.dlambda)
+        phi = self.phi_nodes if self.phi_nodes is not None else self.phi0 + (np.arange(self.Ny) + 0.5) * self.dphi
+        return np.ascontiguousarray(lam, dtype=np.float64), np.ascontiguousarray(phi, dtype=np.float64)
+
+
+def gaussian_latitudes(n: int) -> np.ndarray:
+    """The n Gaussian latitudes (degrees, ascending): arcsin of the roots of the Legendre
+    polynomial P_n, the latitudes of a reduced-Gaussian / TL spectral model grid such as
+    JRA55's (n = 320)."""
+    x, _ = np.polynomial.legendre.leggauss(n)
+    return np.degrees(np.arcsin(x))
+
+
+def jra55_source_grid() -> SourceSpec:
+    """The JRA55-do atmosphere grid: 640 longitudes 0.5625° apart from 0°, periodic, on the
+    320 Gaussian latitudes (DataWrangling/JRA55/JRA55_metadata.jl:27-40 builds the grid from
+    the files' own coordinates)."""
+    lat = gaussian_latitudes(320)
+    return SourceSpec(640, 320, NE_PERIODIC, NE_BOUNDED, -0.28125, 0.5625, -90.0, 180.0 / 320,
+                      lambda_nodes=0.5625 * np.arange(640), phi_nodes=lat)
 
 
 @dataclass

@@ -103,6 +103,23 @@ class DeviceGrid:
                                               self.kmin.data_ptr(), stream_handle()), "ne_grid_immersed_levels")
 
 
+def fractional_indices(ctx, grid, spec, fi: torch.Tensor, fj: torch.Tensor) -> None:
+    """Fractional indices of the ocean grid's cell centres in a source grid
+    (prescribed_atmosphere_regridder.jl:51-71): from its regular spacing, or from its node
+    coordinates when it has them (an unequally spaced grid such as JRA55's Gaussian latitudes)."""
+    g = C.byref(grid.struct)
+    s = stream_handle()
+    if not spec.irregular:
+        check(ctx.lib.ne_fractional_indices_f64(ctx.handle, g, C.byref(_source(spec)), fi.data_ptr(), fj.data_ptr(),
+                                                s), "ne_fractional_indices_f64")
+        return
+    lam, phi = (torch.from_numpy(a).to(fi.device) for a in spec.centres())
+    check(ctx.lib.ne_fractional_indices_nodes_f64(ctx.handle, g, lam.data_ptr(), spec.Nx, spec.topology_x,
+                                                  phi.data_ptr(), spec.Ny, spec.topology_y, fi.data_ptr(),
+                                                  fj.data_ptr(), s), "ne_fractional_indices_nodes_f64")
+    torch.cuda.current_stream(fi.device).synchronize()     # the node arrays are freed on return
+
+
 def _source(spec) -> abi.SourceGrid:
     return abi.SourceGrid(spec.Nx, spec.Ny, spec.topology_x, spec.topology_y, spec.lambda0, spec.dlambda, spec.phi0,
                           spec.dphi)
@@ -177,12 +194,9 @@ class OceanOnlyModel:
         self.afj = torch.empty(Ny, dtype=f64, device=dev)
         self.rfi = torch.empty(Nx, dtype=f64, device=dev)
         self.rfj = torch.empty(Ny, dtype=f64, device=dev)
-        g = C.byref(self.grid.struct)
         s = stream_handle()
-        check(self.lib.ne_fractional_indices_f64(self.ctx.handle, g, C.byref(self.src_atmos), self.afi.data_ptr(),
-                                                 self.afj.data_ptr(), s), "ne_fractional_indices_f64")
-        check(self.lib.ne_fractional_indices_f64(self.ctx.handle, g, C.byref(self.src_rad), self.rfi.data_ptr(),
-                                                 self.rfj.data_ptr(), s), "ne_fractional_indices_f64")
+        fractional_indices(self.ctx, self.grid, case.atmos_grid, self.afi, self.afj)
+        fractional_indices(self.ctx, self.grid, case.rad_grid, self.rfi, self.rfj)
         z2 = lambda: torch.zeros((Ny, Nx), dtype=f64, device=dev)  # noqa: E731
         self.fluxes = {k: z2() for k in FLUX_NAMES}
         self.iterations = torch.zeros((Ny, Nx), dtype=torch.int32, device=dev)

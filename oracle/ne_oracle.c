@@ -560,6 +560,37 @@ void neo_fractional_indices(const ne_grid* g, const ne_source_grid* s, double* f
         fj[j] = s->topology_y == NE_FLAT ? 0.0 : (g->phi_c[j] - pc0) / s->dphi;
 }
 
+/* Fractional index of x among cell-centre nodes x_0 < … < x_{n-1} (the stretched-direction
+ * FractionalIndices of a LatitudeLongitudeGrid built from coordinate arrays, as for the JRA55
+ * grid, DataWrangling/JRA55/JRA55_metadata.jl:27-40): k + (x − x_k)/(x_{k+1} − x_k) in the
+ * interval holding x, found here by a linear scan. A periodic axis (360°) is the node list
+ * continued by x_0 + 360, x being brought into [x_0, x_0 + 360) first; a bounded axis
+ * continues its first and last interval. */
+static double fractional_index_on_nodes(double x, const double* xn, int n, int topo) {
+    if (topo == NE_FLAT || n <= 1) return 0.0;
+    if (topo == NE_PERIODIC) {
+        if (x < xn[0] || x >= xn[0] + 360.0) {
+            double t = fmod(x - xn[0], 360.0);
+            if (t < 0.0) t += 360.0;
+            x = xn[0] + t;
+        }
+        for (int k = 0; k < n; k++) {
+            double right = k + 1 < n ? xn[k + 1] : xn[0] + 360.0;
+            if (k + 1 == n || x < right) return (double)k + (x - xn[k]) / (right - xn[k]);
+        }
+    }
+    if (x < xn[0]) return (x - xn[0]) / (xn[1] - xn[0]);
+    for (int k = 0; k + 1 < n; k++)
+        if (x < xn[k + 1]) return (double)k + (x - xn[k]) / (xn[k + 1] - xn[k]);
+    return (double)(n - 1) + (x - xn[n - 1]) / (xn[n - 1] - xn[n - 2]);
+}
+
+void neo_fractional_indices_nodes(const ne_grid* g, const double* lambda_nodes, int nx, int topology_x,
+                                  const double* phi_nodes, int ny, int topology_y, double* fi, double* fj) {
+    for (int i = 0; i < g->Nx; i++) fi[i] = fractional_index_on_nodes(g->lambda_c[i], lambda_nodes, nx, topology_x);
+    for (int j = 0; j < g->Ny; j++) fj[j] = fractional_index_on_nodes(g->phi_c[j], phi_nodes, ny, topology_y);
+}
+
 /* Integer neighbours and weight of a fractional index: periodic wrap or a
  * zero-gradient (Neumann) halo at bounded edges. */
 static inline void neighbours(double f, int n, int topo, int* lo, int* hi, double* xi) {

@@ -49,6 +49,8 @@ int neo_flux_column(const ne_params* p, const double in[11], int inactive, doubl
 /* Phase restatements over a grid (host views). */
 void neo_immersed_levels(const ne_grid* g, ne_view bottom_height, int32_t* kmin);
 void neo_fractional_indices(const ne_grid* g, const ne_source_grid* s, double* fi, double* fj);
+void neo_fractional_indices_nodes(const ne_grid* g, const double* lambda_nodes, int nx, int topology_x,
+                                  const double* phi_nodes, int ny, int topology_y, double* fi, double* fj);
 void neo_interp_field(const ne_grid* g, const ne_source_grid* s, const double* fi, const double* fj,
                       ne_view src, ne_time_interp t, ne_view out);
 void neo_interp_atmos(const ne_grid* g, const ne_prescribed_atmosphere* a, const double* fi,

@@ -94,6 +94,8 @@ def lib():
         L.neo_interp_radiation.argtypes = [P(abi.Grid), P(abi.PrescribedRadiationDesc), P(C.c_double),
                                            P(C.c_double), abi.View, abi.View]
         L.neo_fractional_indices.argtypes = [P(abi.Grid), P(abi.SourceGrid), P(C.c_double), P(C.c_double)]
+        L.neo_fractional_indices_nodes.argtypes = [P(abi.Grid), P(C.c_double), C.c_int, C.c_int, P(C.c_double),
+                                                   C.c_int, C.c_int, P(C.c_double), P(C.c_double)]
         L.neo_freeze_clamp.argtypes = [P(abi.Grid), P(abi.Params), abi.View, abi.View, abi.View, C.c_double]
         L.neo_assemble_net_ocean_fluxes.argtypes = [P(abi.Grid), P(abi.Params), P(abi.InterfaceFluxes),
                                                     P(abi.ExchangeState), abi.View, P(abi.NetOceanFluxes)]
@@ -182,6 +184,20 @@ def snapshot_positions(times: np.ndarray, t: float, indexing: int = abi.NE_TIME_
     return cycle * len(times) + n1, (cycle + (1 if n2 < n1 else 0)) * len(times) + n2
 
 
+def fractional_indices(grid: abi.Grid, spec, fi: np.ndarray, fj: np.ndarray) -> None:
+    """Fractional indices of the grid's cell centres in a source grid: its regular spacing, or
+    its node coordinates when it has them (prescribed_atmosphere_regridder.jl:51-71)."""
+    L = lib()
+    if getattr(spec, "lambda_nodes", None) is None and getattr(spec, "phi_nodes", None) is None:
+        src = abi.SourceGrid(spec.Nx, spec.Ny, spec.topology_x, spec.topology_y, spec.lambda0, spec.dlambda,
+                             spec.phi0, spec.dphi)
+        L.neo_fractional_indices(C.byref(grid), C.byref(src), _dp(fi), _dp(fj))
+        return
+    lam, phi = spec.centres()
+    L.neo_fractional_indices_nodes(C.byref(grid), _dp(lam), spec.Nx, spec.topology_x, _dp(phi), spec.Ny,
+                                   spec.topology_y, _dp(fi), _dp(fj))
+
+
 class OracleModel:
     """CPU OceanOnlyModel on a SurfaceCase (copies of its arrays)."""
 
@@ -227,8 +243,8 @@ class OracleModel:
         self.afi, self.afj = K(np.zeros(Nx)), K(np.zeros(Ny))
         self.rfi, self.rfj = K(np.zeros(Nx)), K(np.zeros(Ny))
         L = lib()
-        L.neo_fractional_indices(C.byref(self.grid), C.byref(self.src_atmos), _dp(self.afi), _dp(self.afj))
-        L.neo_fractional_indices(C.byref(self.grid), C.byref(self.src_rad), _dp(self.rfi), _dp(self.rfj))
+        fractional_indices(self.grid, a, self.afi, self.afj)
+        fractional_indices(self.grid, r, self.rfi, self.rfj)
         z2 = lambda: K(np.zeros((Ny, Nx)))  # noqa: E731
         self.exchange_arrays = {k: z2() for k in EXCHANGE_NAMES}
         self.exchange = abi.ExchangeState(*(np_view(self.exchange_arrays[k]) for k in EXCHANGE_NAMES))

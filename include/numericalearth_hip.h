@@ -348,9 +348,10 @@ int ne_fractional_indices_f64(ne_ctx* ctx, const ne_grid* grid, const ne_source_
 
 /* Fractional indices in a source grid given by its cell-centre coordinates instead of a
  * regular spacing: lambda_nodes[nx] (longitudes, degrees) and phi_nodes[ny] (latitudes), DEVICE
- * arrays in ascending order. The grid the reference builds for JRA55 forcing has the files'
- * own, unequally spaced (Gaussian) latitudes (DataWrangling/JRA55/JRA55_metadata.jl:27-40),
- * whose fractional position in the stretched direction (FractionalIndices on a stretched
+ * arrays in ascending order. The reference builds the JRA55 forcing grid from the files'
+ * own cell bounds (DataWrangling/JRA55/JRA55_metadata.jl:38-48), so its latitude centres (the
+ * midpoints of bounds around 320 Gaussian latitudes) are unequally spaced; a fractional
+ * position in such a stretched direction (FractionalIndices on a stretched
  * LatitudeLongitudeGrid) is k + (x − x_k)/(x_{k+1} − x_k) in the node interval holding x.
  * Periodic axes (period 360°) wrap from the last node to the first one a period on; bounded
  * axes extend their first and last interval (the interpolation then holds the edge value).

@@ -93,21 +93,48 @@ class SourceSpec:
         return np.ascontiguousarray(lam, dtype=np.float64), np.ascontiguousarray(phi, dtype=np.float64)
 
 
+def source_grid_from_interfaces(lambda_faces, phi_faces, topology_x: int = NE_PERIODIC,
+                                topology_y: int = NE_BOUNDED) -> SourceSpec:
+    """A source grid given by its cell interfaces (ascending, degrees), as the reference builds
+    a LatitudeLongitudeGrid from a dataset's coordinate bounds: the cell centres are the
+    midpoints of consecutive interfaces."""
+    lf = np.asarray(lambda_faces, dtype=np.float64)
+    pf = np.asarray(phi_faces, dtype=np.float64)
+    nx, ny = len(lf) - 1, len(pf) - 1
+    if nx < 1 or ny < 1 or np.any(np.diff(lf) <= 0) or np.any(np.diff(pf) <= 0):
+        raise ValueError("interfaces must be ascending, at least two per direction")
+    return SourceSpec(nx, ny, topology_x, topology_y, float(lf[0]), float((lf[-1] - lf[0]) / nx), float(pf[0]),
+                      float((pf[-1] - pf[0]) / ny), lambda_nodes=0.5 * (lf[:-1] + lf[1:]),
+                      phi_nodes=0.5 * (pf[:-1] + pf[1:]))
+
+
 def gaussian_latitudes(n: int) -> np.ndarray:
     """The n Gaussian latitudes (degrees, ascending): arcsin of the roots of the Legendre
-    polynomial P_n, the latitudes of a reduced-Gaussian / TL spectral model grid such as
-    JRA55's (n = 320)."""
+    polynomial P_n, the latitudes of a TL spectral model's grid such as JRA55's (n = 320)."""
     x, _ = np.polynomial.legendre.leggauss(n)
     return np.degrees(np.arcsin(x))
 
 
-def jra55_source_grid() -> SourceSpec:
-    """The JRA55-do atmosphere grid: 640 longitudes 0.5625° apart from 0°, periodic, on the
-    320 Gaussian latitudes (DataWrangling/JRA55/JRA55_metadata.jl:27-40 builds the grid from
-    the files' own coordinates)."""
-    lat = gaussian_latitudes(320)
-    return SourceSpec(640, 320, NE_PERIODIC, NE_BOUNDED, -0.28125, 0.5625, -90.0, 180.0 / 320,
-                      lambda_nodes=0.5625 * np.arange(640), phi_nodes=lat)
+def gaussian_latitude_bounds(n: int) -> np.ndarray:
+    """n + 1 cell bounds (degrees) around the n Gaussian latitudes: each cell holds the area
+    its Gauss weight stands for, sin φ_{j+1/2} = −1 + Σ_{k≤j} w_k."""
+    _, w = np.polynomial.legendre.leggauss(n)
+    s = np.concatenate([[-1.0], -1.0 + np.cumsum(w)])
+    s[-1] = 1.0
+    return np.degrees(np.arcsin(np.clip(s, -1.0, 1.0)))
+
+
+def jra55_source_grid(lat_bounds=None, lon_bounds=None) -> SourceSpec:
+    """The JRA55-do atmosphere grid as the reference builds it
+    (DataWrangling/JRA55/JRA55_metadata.jl:38-48): longitude interfaces are the files' lower
+    `lon_bnds` closed by the first + 360, latitude interfaces their lower `lat_bnds` closed by
+    90°, and the cell centres are the interfaces' midpoints. Pass the files' lower bounds
+    (640 longitudes, 320 latitudes) when they are at hand; by default the longitudes are
+    0.5625° cells centred on 0°, 0.5625°, … and the latitude cells are the Gauss-weight
+    cells around the 320 Gaussian latitudes."""
+    lon = -0.28125 + 0.5625 * np.arange(640) if lon_bounds is None else np.asarray(lon_bounds, dtype=np.float64)
+    lat = gaussian_latitude_bounds(320)[:-1] if lat_bounds is None else np.asarray(lat_bounds, dtype=np.float64)
+    return source_grid_from_interfaces(np.append(lon, lon[0] + 360.0), np.append(lat, 90.0))
 
 
 @dataclass

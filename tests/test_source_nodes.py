@@ -1,5 +1,5 @@
 """Fractional indices in source grids given by node coordinates (unequally spaced latitudes
-such as JRA55's Gaussian ones), on the CPU.
+such as those of JRA55's grid), on the CPU.
 
 The oracle's node-based indices are pinned three ways: on equally spaced nodes they are the
 regular grid's indices (prescribed_atmosphere_regridder.jl:51-71); the bilinear value they
@@ -12,8 +12,8 @@ import pytest
 
 import oracle as O
 from numericalearth_hip import _ctypes_abi as abi
-from numericalearth_hip.generators import NE_BOUNDED, NE_FLAT, NE_PERIODIC, SourceSpec, gaussian_latitudes, \
-    jra55_source_grid
+from numericalearth_hip.generators import (NE_BOUNDED, NE_FLAT, NE_PERIODIC, SourceSpec, gaussian_latitude_bounds,
+                                           gaussian_latitudes, jra55_source_grid, source_grid_from_interfaces)
 
 
 def _grid(lam, phi):
@@ -113,15 +113,36 @@ def test_jra55_gaussian_grid():
     x, w = np.polynomial.legendre.leggauss(320)
     assert np.allclose(np.sin(np.radians(lat)), x, atol=1e-14)
     assert np.sum(w * x ** 2) == pytest.approx(2.0 / 3.0, rel=1e-13)
-    # not equally spaced
-    d = np.diff(lat)
-    assert d.max() - d.min() > 1e-4
+    # the Gauss-weight cells: each brackets its latitude and holds its weight's area
+    b = gaussian_latitude_bounds(320)
+    assert b[0] == -90.0 and b[-1] == 90.0 and np.all((b[:-1] < lat) & (lat < b[1:]))
+    assert np.allclose(np.diff(np.sin(np.radians(b))), w, atol=1e-13)
+    # the grid the reference builds: interfaces closed by +360 and 90°, centres their midpoints
     spec = jra55_source_grid()
-    assert spec.irregular and spec.centres()[0][1] == 0.5625
+    lam_c, phi_c = spec.centres()
+    assert spec.irregular and (spec.Nx, spec.Ny) == (640, 320)
+    assert np.allclose(lam_c, 0.5625 * np.arange(640), atol=1e-12)
+    assert np.allclose(phi_c, 0.5 * (b[:-1] + b[1:]), atol=1e-13)
+    assert np.abs(phi_c - lat).max() < 0.1 and np.ptp(np.diff(phi_c)) > 1e-4    # near, not on, the Gaussian latitudes
     rng = np.random.default_rng(8)
     lam = np.arange(1440) * 0.25 + 0.125 - 180.0                # a quarter-degree ocean grid from 180°W
     phi = np.linspace(-79.9, 79.9, 560)
     fi, fj = _indices(spec, lam, phi)
     f = rng.standard_normal(320)
-    assert np.allclose(_value(f, fj, NE_BOUNDED), np.interp(phi, lat, f), atol=1e-12)
+    assert np.allclose(_value(f, fj, NE_BOUNDED), np.interp(phi, phi_c, f), atol=1e-12)
     assert np.allclose(fi % 640, ((lam % 360.0) / 0.5625), atol=1e-10)
+
+
+def test_source_grid_from_interfaces():
+    lf = np.array([10.0, 11.0, 13.0, 16.0])
+    pf = np.array([-30.0, -10.0, 0.0, 5.0, 50.0])
+    s = source_grid_from_interfaces(lf, pf, NE_BOUNDED, NE_BOUNDED)
+    assert (s.Nx, s.Ny) == (3, 4)
+    lam_c, phi_c = s.centres()
+    assert np.array_equal(lam_c, [10.5, 12.0, 14.5]) and np.array_equal(phi_c, [-20.0, -5.0, 2.5, 27.5])
+    # the user's own lower bounds go through jra55_source_grid's closure
+    g = jra55_source_grid(lat_bounds=np.linspace(-90.0, 89.4375, 320), lon_bounds=np.arange(640) * 0.5625)
+    assert g.centres()[0][-1] == 359.71875 and g.centres()[1][-1] == pytest.approx(89.71875)
+    for bad in ([1.0], [2.0, 1.0]):
+        with pytest.raises(ValueError):
+            source_grid_from_interfaces(bad, pf)

@@ -162,24 +162,33 @@ class OceanOnlyModel:
         self._rtimes = np.ascontiguousarray(case.rad_times, dtype=np.float64)
         self.time_indexing = time_indexing
         self.time_indexing_kind, self.time_period = resolve_time_indexing(time_indexing, self._atimes)
+        given = {"atmosphere": atmosphere_source, "radiation": radiation_source}
         sources = {"atmosphere": atmosphere_source or ArraySnapshots(case.atmos, self._atimes),
                    "radiation": radiation_source or ArraySnapshots(case.rad, self._rtimes)}
         windows = {"atmosphere": atmosphere_window, "radiation": radiation_window if radiation else None}
         resident = {}
         for series, arrays, times in (("atmosphere", case.atmos, self._atimes), ("radiation", case.rad, self._rtimes)):
+            src = sources[series]
+            if given[series] is not None and (src.n_times != len(times) or not np.array_equal(src.times, times)):
+                raise ValueError(f"the {series} source's times differ from the case's")
             N = windows[series]
             if N is None:
                 continue
             if int(N) < 2:
                 raise ValueError(f"{series}_window must hold at least two snapshots")
-            src = sources[series]
-            if src.n_times != len(times) or not np.array_equal(src.times, times):
-                raise ValueError(f"the {series} source's times differ from the case's")
             resident[series] = initial_window(times, int(N), time_indexing)
         self.atm, self.rad = {}, {}
         for series, arrays, out in (("atmosphere", case.atmos, self.atm), ("radiation", case.rad, self.rad)):
-            if series not in resident:
+            if series not in resident and given[series] is None:
                 out.update({k: _dev(v, dev, R) for k, v in arrays.items()})
+                continue
+            if series not in resident:           # a source without a window: the whole series, read once
+                src = sources[series]
+                host = np.empty(src.shape, dtype=np.float64 if R == torch.float64 else np.float32)
+                for k in arrays:
+                    for n in range(src.n_times):
+                        src.read(n, k, host[n])
+                    out[k] = torch.from_numpy(host).to(dev)
                 continue
             src, N = sources[series], int(windows[series])
             stage = np.empty(src.shape[1:], dtype=np.float64 if R == torch.float64 else np.float32)

@@ -8,7 +8,8 @@ import torch
 
 from numericalearth_hip import _ctypes_abi as abi
 from numericalearth_hip._lib import NEError
-from numericalearth_hip.ensemble import Ensemble, build_members, draw_ensemble, member_case, member_params
+from numericalearth_hip.ensemble import (Ensemble, MemberSpec, build_members, draw_ensemble, member_case,
+                                         member_params)
 from numericalearth_hip.model import OceanOnlyModel
 
 pytestmark = pytest.mark.gpu
@@ -92,6 +93,32 @@ def test_batch_equals_members_stepped_alone(require_gpu, precision):
     for m in alone:
         m.time_step(DT)
     _assert_same(alone, batched, f"{precision} after release")
+
+
+def _edge_specs():
+    """Members of edge shapes: a single column, one level, one row, two levels, ragged sizes."""
+    shapes = [(1, 1, 12), (7, 5, 1), (70, 1, 6), (9, 6, 2), (33, 17, 3), (64, 4, 5)]
+    out = []
+    for k, (nx, ny, nz) in enumerate(shapes):
+        lon0, lat0 = 20.0 + 40.0 * k, -30.0 + 5.0 * k
+        out.append(MemberSpec(nx, ny, nz, 100 + k, (lon0, lon0 + 0.25 * nx), (lat0, lat0 + 0.25 * ny),
+                              0.011 + 0.001 * k, 1.0))
+    print("[batch] edge shapes", shapes, flush=True)
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_with_edge_members_equals_members_alone(require_gpu, precision):
+    specs = _edge_specs()
+    alone = _build(specs, precision, two_color=(2,), skin=(4,))
+    batched = _build(specs, precision, two_color=(2,), skin=(4,))
+    ens = Ensemble(batched)
+    for _ in range(3):
+        ens.time_step(DT)
+        for m in alone:
+            m.time_step(DT)
+    _assert_same(alone, batched, f"{precision} edge members")
+    ens.close()
 
 
 def test_graph_replays_equal_eager_steps(require_gpu):

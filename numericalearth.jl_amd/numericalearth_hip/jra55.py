@@ -19,7 +19,7 @@ from pathlib import Path
 
 import numpy as np
 
-from .generators import SurfaceCase, source_grid_from_interfaces
+from .generators import NE_BOUNDED, SurfaceCase, source_grid_from_interfaces
 
 # JRA55_dataset_variable_names (JRA55_metadata.jl:135-147), by the model's field names
 ATMOSPHERE_VARIABLES = {"u": "uas", "v": "vas", "T": "tas", "q": "huss", "p": "psl", "rain": "prra",
@@ -78,7 +78,12 @@ class JRA55Snapshots:
     the first, from the first file's ``time`` variable. Every file holds the same number of
     snapshots; the grid comes from the first file's bounds."""
 
-    def __init__(self, directory, fields: dict, filename=repeat_year_filename, interval: float | None = 10800.0):
+    def __init__(self, directory, fields: dict, filename=repeat_year_filename, interval: float | None = 10800.0,
+                 region=None, pad: int = 2):
+        if region is not None:
+            (west, east), (south, north) = region
+            if not (east > west and north > south):
+                raise ValueError(f"region {region} is empty")
         self.directory = Path(directory)
         self.short = dict(fields)
         self._ds = {k: open_dataset(self.directory / filename(s)) for k, s in self.short.items()}
@@ -98,9 +103,62 @@ class JRA55Snapshots:
                 raise ValueError(f"{self.short[k]}: dimensions {self._var[k].dimensions}, expected (time, lat, lon)")
         lat = np.asarray(first.variables["lat"][:], dtype=np.float64)
         self._flip = len(lat) > 1 and lat[0] > lat[-1]         # stored north to south: served south to north
-        self.shape = (len(self.times), len(lat), len(first.variables["lon"][:]))
         self._scale = {k: (float(_attr(v, "scale_factor", 1.0)), float(_attr(v, "add_offset", 0.0)))
                        for k, v in self._var.items()}
+        self._lf, self._pf = self._interfaces(first)
+        nx, ny = len(self._lf) - 1, len(self._pf) - 1
+        self._cols, self._rows = np.arange(nx), slice(0, ny)
+        self._lon_shift, self._periodic = 0.0, True
+        if region is not None:
+            self._restrict(region, int(pad))
+        self.shape = (len(self.times), self._rows.stop - self._rows.start, len(self._cols))
+
+    def _interfaces(self, ds):
+        """Full-record interfaces as the reference builds them: lower lon/lat bounds closed by
+        +360 and 90° (JRA55_metadata.jl:38-48); files without bounds give interfaces halfway
+        between their coordinates."""
+        v = ds.variables
+        lon = np.asarray(v["lon"][:], dtype=np.float64)
+        lat = np.asarray(v["lat"][:], dtype=np.float64)
+        if "lon_bnds" in v and "lat_bnds" in v:
+            lf = np.asarray(v["lon_bnds"][:], dtype=np.float64).min(axis=1)
+            pf = np.asarray(v["lat_bnds"][:], dtype=np.float64).min(axis=1)
+            if self._flip:
+                pf = pf[::-1]
+            return np.append(lf, lf[0] + 360.0), np.append(pf, 90.0)
+        lat = np.sort(lat)
+        lf = np.concatenate([[lon[0] - 0.5 * (lon[1] - lon[0])], 0.5 * (lon[1:] + lon[:-1])])
+        pf = np.concatenate([[max(-90.0, lat[0] - 0.5 * (lat[1] - lat[0]))], 0.5 * (lat[1:] + lat[:-1]),
+                             [min(90.0, lat[-1] + 0.5 * (lat[-1] - lat[-2]))]])
+        return np.append(lf, lf[0] + 360.0), pf
+
+    def _restrict(self, region, pad):
+        """Keep the cells of a bounding box ((west, east), (south, north)) plus `pad` cells on each
+        side (the reference's `region = BoundingBox(...)` of a JRA55 series): rows and columns are
+        read only there, and the grid is the sub-grid, bounded in longitude unless it closes the
+        circle, its longitudes in the box's own convention (a box across 0° given as −20…30
+        stays −20…30 whatever the files' 0…360)."""
+        (west, east), (south, north) = region
+        lf, pf = self._lf, self._pf
+        nx, ny = len(lf) - 1, len(pf) - 1
+        j0 = max(0, int(np.searchsorted(pf, south, side="right")) - 1 - pad)
+        j1 = min(ny, int(np.searchsorted(pf, north, side="left")) + pad)
+        self._rows = slice(j0, j1)
+        if east - west >= 360.0:
+            return
+        turn = np.floor((west - lf[0]) / 360.0)
+        w = west - 360.0 * turn                                 # the box's west edge within the record's turn
+        i0 = int(np.searchsorted(lf, w, side="right")) - 1 - pad
+        ext = lambda i: lf[i % nx] + 360.0 * (i // nx)          # noqa: E731  (interfaces continued round the circle)
+        i1 = i0
+        while ext(i1) < w + (east - west) and i1 - i0 < nx:
+            i1 += 1
+        i1 += pad
+        if i1 - i0 >= nx:                                       # the box and its margins close the circle
+            return
+        self._cols = np.arange(i0, i1) % nx
+        self._lon_faces = np.array([ext(i) for i in range(i0, i1 + 1)]) + 360.0 * turn
+        self._periodic = False
 
     @property
     def n_times(self) -> int:
@@ -111,33 +169,21 @@ class JRA55Snapshots:
         a = np.asarray(self._var[name][n])
         if self._flip:
             a = a[::-1]
+        a = a[self._rows]
+        if len(self._cols) != a.shape[1]:
+            a = a[:, self._cols]
         scale, offset = self._scale[name]
         if scale != 1.0 or offset != 0.0:
             a = a * scale + offset
         np.copyto(out, a, casting="unsafe")
 
     def grid(self):
-        """The source grid as the reference builds it: lower lon/lat bounds closed by +360 and
-        90°, centres at the interfaces' midpoints (JRA55_metadata.jl:38-48); files without
-        bounds give interfaces halfway between their coordinates."""
-        ds = next(iter(self._ds.values()))
-        v = ds.variables
-        lon = np.asarray(v["lon"][:], dtype=np.float64)
-        lat = np.asarray(v["lat"][:], dtype=np.float64)
-        if "lon_bnds" in v and "lat_bnds" in v:
-            lf = np.asarray(v["lon_bnds"][:], dtype=np.float64).min(axis=1)
-            pf = np.asarray(v["lat_bnds"][:], dtype=np.float64).min(axis=1)
-            if self._flip:
-                pf = pf[::-1]
-            lf = np.append(lf, lf[0] + 360.0)
-            pf = np.append(pf, 90.0)
-        else:
-            lat = np.sort(lat)
-            lf = np.concatenate([[lon[0] - 0.5 * (lon[1] - lon[0])], 0.5 * (lon[1:] + lon[:-1])])
-            lf = np.append(lf, lf[0] + 360.0)
-            pf = np.concatenate([[max(-90.0, lat[0] - 0.5 * (lat[1] - lat[0]))], 0.5 * (lat[1:] + lat[:-1]),
-                                 [min(90.0, lat[-1] + 0.5 * (lat[-1] - lat[-2]))]])
-        return source_grid_from_interfaces(lf, pf)
+        """The source grid: the record's interfaces (centres at their midpoints), or the region's
+        sub-grid."""
+        pf = self._pf[self._rows.start:self._rows.stop + 1]
+        if self._periodic:
+            return source_grid_from_interfaces(self._lf, pf)
+        return source_grid_from_interfaces(self._lon_faces, pf, topology_x=NE_BOUNDED)
 
     def close(self) -> None:
         self._var.clear()                        # scipy's reader closes only once no variable refers to its map

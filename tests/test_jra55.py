@@ -1,12 +1,13 @@
 """JRA55-do files as snapshot sources, on the CPU (NetCDF-3 files in the repeat-year layout,
 written with scipy): snapshot slices in model order, repeat-year times, the grid from the
-files' bounds, north-to-south storage, packed variables, and a clear refusal of NetCDF-4
-when no NetCDF-4 reader is installed."""
+files' bounds, north-to-south storage, packed variables, bounding-box regions read as sub-blocks,
+and a clear refusal of NetCDF-4 when no NetCDF-4 reader is installed."""
 import numpy as np
 import pytest
 
 from jra55_files import write_jra55_files
-from numericalearth_hip.generators import gaussian_latitude_bounds, jra55_source_grid, make_case
+from numericalearth_hip.generators import (NE_BOUNDED, NE_PERIODIC, gaussian_latitude_bounds, jra55_source_grid,
+                                           make_case)
 from numericalearth_hip.jra55 import (ATMOSPHERE_VARIABLES, RADIATION_VARIABLES, JRA55Snapshots, open_dataset,
                                       with_forcing)
 
@@ -105,3 +106,34 @@ def test_with_forcing_builds_the_case(tmp_path):
     assert not only_atm.rad_grid.irregular                     # radiation left as the case had it
     atm.close()
     rad.close()
+
+
+def test_region_reads_a_sub_block_across_the_date_line(tmp_path):
+    rng = np.random.default_rng(5)
+    data = _fields(rng, ["tas"])
+    write_jra55_files(tmp_path, data, NLON, NLAT)
+    full = JRA55Snapshots(tmp_path, {"T": "tas"})
+    box = ((-30.0, 40.0), (-20.0, 30.0))                         # across 0°, files in 0…360
+    sub = JRA55Snapshots(tmp_path, {"T": "tas"}, region=box, pad=1)
+    g, gf = sub.grid(), full.grid()
+    lam, phi = g.centres()
+    lam_f, phi_f = gf.centres()
+    assert g.topology_x == NE_BOUNDED and sub.shape == (NT, g.Ny, g.Nx) and g.Nx < NLON and g.Ny < NLAT
+    # the box and a cell of margin inside the sub-grid, its longitudes in the box's convention
+    assert lam[0] < -30.0 < 40.0 < lam[-1] and phi[0] < -20.0 < 30.0 < phi[-1]
+    assert np.all(np.diff(lam) > 0)
+    cols = np.round((lam % 360.0) / 22.5).astype(int) % NLON
+    rows = np.searchsorted(phi_f, phi)
+    assert np.allclose(phi_f[rows], phi, atol=1e-12) and np.allclose(lam_f[cols] % 360.0, lam % 360.0, atol=1e-12)
+    a, b = np.empty((NLAT, NLON)), np.empty((g.Ny, g.Nx))
+    for n in range(NT):
+        full.read(n, "T", a)
+        sub.read(n, "T", b)
+        assert np.array_equal(b, a[np.ix_(rows, cols)])
+    # a box that closes the circle keeps the whole record, periodic
+    ring = JRA55Snapshots(tmp_path, {"T": "tas"}, region=((0.0, 360.0), (-20.0, 30.0)))
+    assert ring.grid().topology_x == NE_PERIODIC and ring.grid().Nx == NLON
+    with pytest.raises(ValueError):
+        JRA55Snapshots(tmp_path, {"T": "tas"}, region=((10.0, 10.0), (0.0, 5.0)))
+    for s in (full, sub, ring):
+        s.close()

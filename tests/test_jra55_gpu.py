@@ -10,7 +10,8 @@ import torch
 
 import oracle as O
 from jra55_files import write_jra55_files
-from numericalearth_hip.generators import make_case
+from helpers import rel_err
+from numericalearth_hip.generators import NE_BOUNDED, make_case
 from numericalearth_hip.jra55 import ATMOSPHERE_VARIABLES, RADIATION_VARIABLES, JRA55Snapshots, with_forcing
 from numericalearth_hip.model import OceanOnlyModel
 from numericalearth_hip.time_series import Cyclical
@@ -53,3 +54,42 @@ def test_models_forced_from_files_step_alike(require_gpu, tmp_path, precision):
         m.close()
     atm.close()
     rad.close()
+
+
+def test_regional_model_reads_only_its_box(require_gpu, tmp_path):
+    """A regional model across 0° forced from a bounding-box region of the files steps like the
+    same model forced from the whole record. The sub-grid's fractional indices differ from the
+    whole grid's by an integer offset, so the interpolation weights agree only to rounding: each
+    field must agree to 1e-12 of its largest value, and elementwise to 1e-9 (a near-zero stress
+    amplifies the rounding's relative size)."""
+    glob = make_case(64, 32, 2, seed=19, n_times=4, dt_times=10800.0)
+    files = {s: glob.atmos[k] for k, s in ATMOSPHERE_VARIABLES.items()}
+    files.update({s: glob.rad[k] for k, s in RADIATION_VARIABLES.items()})
+    write_jra55_files(tmp_path, files, 32, 16)
+    ocean = make_case(40, 24, 4, seed=31, longitude=(-30.0, 40.0), latitude=(-20.0, 30.0), topology_x=NE_BOUNDED)
+    box = ((-30.0, 40.0), (-20.0, 30.0))
+    print(f"[jra55] regional ocean (40, 24, 4) in box {box}", flush=True)
+    srcs = {r: (JRA55Snapshots(tmp_path, ATMOSPHERE_VARIABLES, region=r),
+                JRA55Snapshots(tmp_path, RADIATION_VARIABLES, region=r)) for r in (None, box)}
+    assert srcs[box][0].shape[1:] < srcs[None][0].shape[1:]
+    models = {r: OceanOnlyModel(with_forcing(ocean, a, b), atmosphere_source=a, radiation_source=b,
+                                time_indexing=Cyclical()) for r, (a, b) in srcs.items()}
+    full, sub = models[None], models[box]
+    for step in range(4):
+        torch.cuda.synchronize()
+        same = (full.iterations == sub.iterations).float().mean().item()
+        assert same >= 0.99, same
+        ok = (full.iterations == sub.iterations).cpu().numpy()
+        pairs = [(sub.fluxes[k].cpu().numpy()[ok], full.fluxes[k].cpu().numpy()[ok], k) for k in full.fluxes]
+        pairs += [(sub.T.cpu().numpy(), full.T.cpu().numpy(), "T"), (sub.S.cpu().numpy(), full.S.cpu().numpy(), "S")]
+        for a, b, k in pairs:
+            elem, field = rel_err(a, b)
+            assert field <= 1e-12 and elem <= 1e-9, (k, step, elem, field)
+        full.time_step(3600.0)
+        sub.time_step(3600.0)
+        print(f"[jra55] step {step + 1}", flush=True)
+    for m in models.values():
+        m.close()
+    for a, b in srcs.values():
+        a.close()
+        b.close()

@@ -369,6 +369,13 @@ int ne_fractional_indices_f64(ne_ctx* c, const ne_grid* g, const ne_source_grid*
     return dev_err(launch_fractional_indices(grid_dev(*g), *src, fi, fj, as_stream(stream)));
 }
 
+int ne_log_f64(ne_ctx* c, const double* x, double* y, int64_t n, void* stream) {
+    if (!c || !x || !y) return NE_ERR_NULL;
+    if (n < 0 || n > ((int64_t)1 << 38)) return NE_ERR_SHAPE;
+    if (n == 0) return NE_OK;
+    return dev_err(launch_log(x, y, n, as_stream(stream)));
+}
+
 int ne_fractional_indices_nodes_f64(ne_ctx* c, const ne_grid* g, const double* lambda_nodes, int32_t nx,
                                     int32_t topology_x, const double* phi_nodes, int32_t ny, int32_t topology_y,
                                     double* fi, double* fj, void* stream) {

@@ -122,6 +122,11 @@ __global__ void fractional_indices_kernel(GridDev g, ne_source_grid s, double* f
     if (n < g.Ny) fj[n] = s.topology_y == NE_FLAT ? 0.0 : (g.phi_c[n] - pc0) / s.dphi;
 }
 
+__global__ void log_kernel(const double* x, double* y, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) y[i] = ne_log(x[i]);
+}
+
 __global__ void fractional_indices_nodes_kernel(GridDev g, const double* xn, int nx, int tx, const double* yn, int ny,
                                                 int ty, double* fi, double* fj) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -611,6 +616,11 @@ RadOut rad_out(const ne_radiative_fluxes& r) {
 
 hipError_t launch_immersed_levels(const GridDev& g, View3<const double> bottom, int32_t* kmin, hipStream_t s) {
     hipLaunchKernelGGL(immersed_levels_kernel, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, bottom, kmin);
+    return hipGetLastError();
+}
+
+hipError_t launch_log(const double* x, double* y, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(log_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, n);
     return hipGetLastError();
 }
 

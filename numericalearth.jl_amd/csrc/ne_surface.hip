@@ -325,8 +325,9 @@ __device__ __forceinline__ double radiation_column(const ne_params& p, double Ts
     double cinv = 1.0 / p.ocean_heat_capacity;
     double aice = 0.0;
     double Ts = Ts_C + NE_CELSIUS_TO_KELVIN;
-    double T2 = Ts * Ts;
-    Iup = p.stefan_boltzmann_constant * p.ocean_emissivity * (T2 * T2);
+    // T^4 as Julia evaluates Float64^Integer (radiation_kernels.jl:3): the compensated power, as in
+    // the skin-temperature balance
+    Iup = p.stefan_boltzmann_constant * p.ocean_emissivity * pow_int(Ts, 4);
     Ila = -p.ocean_emissivity * LW;
     Itsw = -(1.0 - p.ocean_albedo) * SW;
     Ila = Ila * (1.0 - aice);

@@ -768,8 +768,9 @@ void neo_apply_air_sea_radiation(const ne_grid* g, const ne_params* p, ne_view s
             double aice = 0.0;
             double Ts = V2(Tsv, i, j) + CELSIUS_TO_KELVIN;
             double SW = V2(sw, i, j), LW = V2(lw, i, j);
-            double T2 = Ts * Ts;
-            double Iup = sigma * eps * (T2 * T2);
+            /* emitted_longwave_radiation σϵT^4 (radiation_kernels.jl:3); T^4 as Julia evaluates
+               Float64^Integer, the compensated power (as in the skin-temperature balance) */
+            double Iup = sigma * eps * neo_pow_int(Ts, 4);
             double Ila = -eps * LW;
             double Itsw = -(1.0 - alpha) * SW;
             Ila = Ila * (1.0 - aice);

@@ -7,7 +7,10 @@ import numpy as np
 import pytest
 
 REPO = Path(__file__).resolve().parent.parent
-for p in (REPO / "numericalearth.jl_amd", REPO / "oracle", REPO / "tests"):
+# NE_ORACLE_DIR: a directory holding a modified copy of oracle.py, which the mutation test
+# (test_oracle_mutants.py) runs the pins against in a child process.
+ORACLE_DIR = Path(os.environ.get("NE_ORACLE_DIR", REPO / "oracle"))
+for p in (REPO / "numericalearth.jl_amd", ORACLE_DIR, REPO / "tests"):
     if str(p) not in sys.path:
         sys.path.insert(0, str(p))
 

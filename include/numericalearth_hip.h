@@ -442,6 +442,18 @@ int ne_budget_f64(ne_ctx* ctx, const ne_grid* grid, ne_view c, double weight,
 int ne_budget_f32(ne_ctx* ctx, const ne_grid* grid, ne_view c, double weight,
                   double* out, void* stream);
 
+/* ne_budget_f64 without any allocation, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It
+ * has no counterpart in the reference. Same kernels and result as ne_budget_f64, bit for
+ * bit; the block partials live in a fixed array of the library on each device (1 MiB, held
+ * until the process ends) instead of the context's workspace. Grids that need more than
+ * NE_BUDGET_NOALLOC_MAX_BLOCKS blocks, ceil(Nx/64)·ceil(Ny/4), get NE_ERR_CAPACITY. Every
+ * call on one device shares that array: calls that overlap on one device (other streams,
+ * other threads) race, so the caller orders them. */
+#define NE_BUDGET_NOALLOC_MAX_BLOCKS 65536
+int ne_budget_f64_noalloc(ne_ctx* ctx, const ne_grid* grid, ne_view c, double weight,
+                          double* out, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Model: OceanOnlyModel(ocean; atmosphere, radiation) + time_step!
  * (earth_system_model.jl:399-402; time_step_earth_system_model.jl:14-83).

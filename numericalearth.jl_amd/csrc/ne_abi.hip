@@ -318,7 +318,8 @@ int ne_ctx_create(int device, ne_ctx** out) {
     c->scratch = nullptr;
     c->scratch_bytes = 0;
     if (hipMalloc((void**)&c->counters, sizeof(ne_counters)) != hipSuccess) { delete c; return NE_ERR_DEVICE; }
-    if (hipMemset(c->counters, 0, sizeof(ne_counters)) != hipSuccess) {
+    if (hipMemset(c->counters, 0, sizeof(ne_counters)) != hipSuccess ||
+        budget_noalloc_init(device) != hipSuccess) {   // once per device, with `device` current
         (void)hipFree(c->counters);
         delete c;
         return NE_ERR_DEVICE;
@@ -509,6 +510,17 @@ int ne_budget_f32(ne_ctx* c, const ne_grid* g, ne_view cv, double weight, double
     int r;
     if ((r = check_grid(g)) || (r = check_view3(cv))) return r;
     return dev_err(launch_budget<float>(c, grid_dev(*g), view<const float>(cv), weight, out, as_stream(stream)));
+}
+
+int ne_budget_f64_noalloc(ne_ctx* c, const ne_grid* g, ne_view cv, double weight, double* out, void* stream) {
+    if (!c || !out) return NE_ERR_NULL;
+    int r;
+    if ((r = check_grid(g)) || (r = check_view3(cv))) return r;
+    const GridDev gd = grid_dev(*g);
+    if (budget_blocks(gd) > NE_BUDGET_NOALLOC_MAX_BLOCKS) return NE_ERR_CAPACITY;
+    double* partial = budget_noalloc_scratch(c->device);   // looked up by ne_ctx_create
+    if (!partial) return NE_ERR_DEVICE;
+    return dev_err(launch_budget_noalloc(partial, gd, view<const double>(cv), weight, out, as_stream(stream)));
 }
 
 }  // extern "C"

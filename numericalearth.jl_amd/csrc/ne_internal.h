@@ -172,6 +172,13 @@ hipError_t launch_w_continuity(const GridDev& g, View3<const Real> u, View3<cons
 template <typename Real>
 hipError_t launch_budget(ne_ctx* ctx, const GridDev& g, View3<const Real> c, double weight, double* out,
                          hipStream_t s);
+// Allocation-free budget: the address of the library's fixed partials array on `device`
+// (looked up once per device by budget_noalloc_init, with that device current).
+hipError_t budget_noalloc_init(int device);
+double* budget_noalloc_scratch(int device);
+int64_t budget_blocks(const GridDev& g);
+hipError_t launch_budget_noalloc(double* partial, const GridDev& g, View3<const double> c, double weight,
+                                 double* out, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Batched ensemble: one device record per member model, read by persistent kernels

@@ -16,6 +16,15 @@
 // forcing carry none of its registers.
 #include "ne_internal.h"
 
+#include <mutex>
+
+// Block partials of the allocation-free budget (ne_budget_f64_noalloc): one (sum,
+// compensation) pair per 64×4 block of columns, NE_BUDGET_NOALLOC_MAX_BLOCKS pairs (1 MiB).
+// Each device holds its own copy for the life of the process. In host code the name is a
+// stand-in for the symbol, not a device address: kernels are given the address that
+// hipGetSymbolAddress returns for the current device (ne::budget_noalloc_scratch).
+static __device__ __align__(256) double scratch[2 * NE_BUDGET_NOALLOC_MAX_BLOCKS];
+
 namespace ne {
 
 namespace {
@@ -668,6 +677,48 @@ hipError_t launch_budget(ne_ctx* ctx, const GridDev& g, View3<const Real> c, dou
 }
 template hipError_t launch_budget<double>(ne_ctx*, const GridDev&, View3<const double>, double, double*,
                                           hipStream_t);
+
+namespace {
+std::mutex noalloc_mu;
+std::vector<double*> noalloc_ptr;   // per device: the address of `scratch` (nullptr: not looked up yet)
+}  // namespace
+
+hipError_t budget_noalloc_init(int device) {
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if ((int)noalloc_ptr.size() <= device) noalloc_ptr.resize(device + 1, nullptr);
+    if (noalloc_ptr[device]) return hipSuccess;
+    int current = -1;
+    hipError_t e = hipGetDevice(&current);
+    if (e != hipSuccess) return e;
+    if (current != device) return hipErrorInvalidDevice;   // the caller makes `device` current
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    noalloc_ptr[device] = static_cast<double*>(p);
+    return hipSuccess;
+}
+
+double* budget_noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    return device >= 0 && device < (int)noalloc_ptr.size() ? noalloc_ptr[device] : nullptr;
+}
+
+int64_t budget_blocks(const GridDev& g) {
+    const dim3 gr = grid2d(g.Nx, g.Ny);
+    return (int64_t)gr.x * gr.y;
+}
+
+// The kernels of launch_budget with the partials in the device's `scratch` array; allocates
+// nothing. Every block overwrites its own pair, so nothing needs clearing between calls.
+hipError_t launch_budget_noalloc(double* partial, const GridDev& g, View3<const double> c, double weight,
+                                 double* out, hipStream_t s) {
+    const dim3 gr = grid2d(g.Nx, g.Ny);
+    const int64_t nb = (int64_t)gr.x * gr.y;
+    hipLaunchKernelGGL(budget_partial_kernel<double>, gr, dim3(BX, BY), 0, s, g, c, weight, partial);
+    hipLaunchKernelGGL(budget_final_kernel, dim3(1), dim3(256), 0, s, partial, nb, out);
+    return hipGetLastError();
+}
 template hipError_t launch_budget<float>(ne_ctx*, const GridDev&, View3<const float>, double, double*,
                                          hipStream_t);
 

@@ -41,6 +41,7 @@ NE_STOP_CONVERGENCE, NE_STOP_FIXED_ITERATIONS = 0, 1
 NE_HUMIDITY_IMPURE_SATURATION, NE_HUMIDITY_FIXED = 0, 1
 NE_VELOCITY_RELATIVE, NE_VELOCITY_WIND = 0, 1
 NE_TIME_CLAMPED, NE_TIME_CYCLICAL = 0, 1
+NE_BUDGET_NOALLOC_MAX_BLOCKS = 65536   # blocks of 64×4 columns the allocation-free budget holds partials for
 NO_POSITION = -(2 ** 63)   # ne_model_snapshots_at: a series with no snapshots
 
 _d = C.c_double

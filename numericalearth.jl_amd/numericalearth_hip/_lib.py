@@ -55,6 +55,7 @@ SIGNATURES = {
                                                                                  C.c_double, vp],
     "ne_budget_f64": [vp, P(abi.Grid), abi.View, C.c_double, vp, vp],
     "ne_budget_f32": [vp, P(abi.Grid), abi.View, C.c_double, vp, vp],
+    "ne_budget_f64_noalloc": [vp, P(abi.Grid), abi.View, C.c_double, vp, vp],
     "ne_model_create": [vp, P(abi.ModelDesc), P(vp), vp],
     "ne_model_destroy": [vp],
     "ne_model_time_step": [vp, C.c_double, vp],

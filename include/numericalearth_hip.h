@@ -84,6 +84,13 @@ typedef struct ne_grid {
                                     of each column; Nz marks a land column          */
     const double* z_face;        /* [Nz+1] cell-face height (m); face k is the bottom of
                                     cell k, face Nz the surface                      */
+    int32_t halo;                /* halo width of the caller's 2-D surface views (0: the
+                                    views cover the interior only). With halo ≥ 1 the
+                                    interpolations and the flux solve also write the ring
+                                    i = −1, Nx and j = −1, Ny, as the reference launches
+                                    them over 0:N+1 in every non-Flat direction
+                                    (InterfaceComputations.jl:99-116); single-column grids
+                                    have no ring                                      */
 } ne_grid;
 
 /* Regular longitude-latitude source grid of a prescribed atmosphere or radiation

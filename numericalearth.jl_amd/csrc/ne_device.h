@@ -85,6 +85,7 @@ struct GridDev {
     const double* __restrict__ phi_c;
     const int32_t* __restrict__ kmin;
     const double* __restrict__ zf;
+    int ring;   // 1: the per-phase surface entry points also write the ring i = -1, Nx, j = -1, Ny
 
     // Column index after periodic wrapping; -1 outside a bounded direction.
     __device__ __forceinline__ int64_t column(int i, int j) const {
@@ -111,6 +112,7 @@ __host__ inline GridDev grid_dev(const ne_grid& g) {
     d.advection = g.advection;
     d.dz = g.dz; d.zc = g.z_center; d.dy_xface = g.dy_xface; d.dx_yface = g.dx_yface;
     d.Az = g.Az; d.lambda_c = g.lambda_c; d.phi_c = g.phi_c; d.kmin = g.kmin; d.zf = g.z_face;
+    d.ring = (g.halo >= 1 && !(g.Nx == 1 && g.Ny == 1)) ? 1 : 0;
     return d;
 }
 
@@ -949,84 +951,98 @@ __host__ __device__ inline ne_time_interp time_indices(const double* times, int 
 
 // ---------------------------------------------------------------------------
 // WENO reconstructions: Jiang & Shu (1996) candidate stencils and smoothness
-// indicators with Borges et al. (2008) WENO-Z weights, ε = 1e-8. Values are ordered
-// along the upwind direction (the face lies after the third of five values, or the
-// second of three). The weights are normalised with a single division: multiplying
-// every α_r = d_r (1 + (τ/(β_r+ε))²) by Π_s (β_s+ε)² leaves α_r/Σα unchanged, and the
-// 1/6 of the candidate polynomials moves into the same denominator. In FP32 the
+// indicators with Borges et al. (2008) WENO-Z weights (exponent 2), ε = 1e-8, written
+// in terms of the differences δ_n = c_{n+1} − c_n of the values along the upwind
+// direction (the face lies after the third of five values, or the second of three):
+//
+//   β_r = 13/12 s_r² + 1/4 e_r², s = (δ₁−δ₀, δ₂−δ₁, δ₃−δ₂), e = (3δ₁−δ₀, δ₁+δ₂, 3δ₂−δ₃)
+//   q_r − c₂ = (5δ₁−2δ₀, δ₁+2δ₂, 4δ₂−δ₃)/6
+//
+// The face value is c₂ + Σ_r ω_r (q_r − c₂): a uniform field gives c₂ exactly, and the
+// reconstruction of the mirrored values is the negated reconstruction of the negated
+// differences, so the upwind side is chosen by selecting differences. Scaling every β_r
+// and ε by the same factor leaves the weights unchanged; the FP64 form uses 4β_r =
+// (13/3)s_r² + e_r² with 4ε. The FP64 weights are normalised with one division:
+// multiplying every α_r = d_r (1 + (τ/(β_r+ε))²) by Π_s (β_s+ε)² leaves α_r/Σα unchanged;
+// the division is a reciprocal refined by two Newton steps (within 2 ulp). In FP32 the
 // Π(β+ε)² products could underflow, so FP32 keeps per-stencil reciprocals instead.
 // Fused multiply-adds are allowed here (contract(fast)).
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ double weno5z(double c0, double c1, double c2, double c3, double c4) {
+// a/b for b > 0 normal: hardware reciprocal, two Newton steps, one product.
+__device__ __forceinline__ double quotient(double a, double b) {
 #pragma clang fp contract(fast)
-    const double eps = NE_WENO_EPS;
-    double d0 = c0 - 2.0 * c1 + c2, e0 = c0 - 4.0 * c1 + 3.0 * c2;
-    double d1 = c1 - 2.0 * c2 + c3, e1 = c1 - c3;
-    double d2 = c2 - 2.0 * c3 + c4, e2 = 3.0 * c2 - 4.0 * c3 + c4;
-    double b0 = (13.0 / 12.0) * d0 * d0 + 0.25 * e0 * e0;
-    double b1 = (13.0 / 12.0) * d1 * d1 + 0.25 * e1 * e1;
-    double b2 = (13.0 / 12.0) * d2 * d2 + 0.25 * e2 * e2;
-    double tau = fabs(b0 - b2);
-    double t2 = tau * tau;
-    double s0 = (b0 + eps) * (b0 + eps), s1 = (b1 + eps) * (b1 + eps), s2 = (b2 + eps) * (b2 + eps);
-    double a0 = 0.1 * (s0 + t2) * (s1 * s2);
-    double a1 = 0.6 * (s1 + t2) * (s0 * s2);
-    double a2 = 0.3 * (s2 + t2) * (s0 * s1);
-    double q0 = 2.0 * c0 - 7.0 * c1 + 11.0 * c2;
-    double q1 = -c1 + 5.0 * c2 + 2.0 * c3;
-    double q2 = 2.0 * c2 + 5.0 * c3 - c4;
-    return (a0 * q0 + a1 * q1 + a2 * q2) / (6.0 * (a0 + a1 + a2));
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    return a * r;
 }
 
-__device__ __forceinline__ float weno5z(float c0, float c1, float c2, float c3, float c4) {
+// Σ_r ω_r (q_r − c₂) of the 5-point WENO-Z reconstruction from the four differences.
+__device__ __forceinline__ double weno5z_core(double d0, double d1, double d2, double d3) {
+#pragma clang fp contract(fast)
+    const double eps4 = 4.0 * NE_WENO_EPS;
+    const double k = 2.0816659994661326;   // √(13/3)
+    const double s0 = k * (d1 - d0), s1 = k * (d2 - d1), s2 = k * (d3 - d2);
+    const double e0 = fma(3.0, d1, -d0), e1 = d1 + d2, e2 = fma(3.0, d2, -d3);
+    const double b0 = fma(s0, s0, e0 * e0);
+    const double b1 = fma(s1, s1, e1 * e1);
+    const double b2 = fma(s2, s2, e2 * e2);
+    const double tau = fabs(b0 - b2);
+    const double t2 = tau * tau;
+    const double p0 = (b0 + eps4) * (b0 + eps4), p1 = (b1 + eps4) * (b1 + eps4), p2 = (b2 + eps4) * (b2 + eps4);
+    const double a0 = (0.1 * (p0 + t2)) * (p1 * p2);
+    const double a1 = (0.6 * (p1 + t2)) * (p0 * p2);
+    const double a2 = (0.3 * (p2 + t2)) * (p0 * p1);
+    const double r0 = fma(5.0, d1, -2.0 * d0), r1 = fma(2.0, d2, d1), r2 = fma(4.0, d2, -d3);
+    return quotient(fma(a0, r0, fma(a1, r1, a2 * r2)), 6.0 * (a0 + a1 + a2));
+}
+
+__device__ __forceinline__ float weno5z_core(float d0, float d1, float d2, float d3) {
 #pragma clang fp contract(fast)
     const float eps = (float)NE_WENO_EPS;
-    float d0 = c0 - 2.0f * c1 + c2, e0 = c0 - 4.0f * c1 + 3.0f * c2;
-    float d1 = c1 - 2.0f * c2 + c3, e1 = c1 - c3;
-    float d2 = c2 - 2.0f * c3 + c4, e2 = 3.0f * c2 - 4.0f * c3 + c4;
-    float b0 = (13.0f / 12.0f) * d0 * d0 + 0.25f * e0 * e0;
-    float b1 = (13.0f / 12.0f) * d1 * d1 + 0.25f * e1 * e1;
-    float b2 = (13.0f / 12.0f) * d2 * d2 + 0.25f * e2 * e2;
-    float tau = fabsf(b0 - b2);
-    float r0 = tau * __builtin_amdgcn_rcpf(b0 + eps);
-    float r1 = tau * __builtin_amdgcn_rcpf(b1 + eps);
-    float r2 = tau * __builtin_amdgcn_rcpf(b2 + eps);
-    float a0 = 0.1f * (1.0f + r0 * r0);
-    float a1 = 0.6f * (1.0f + r1 * r1);
-    float a2 = 0.3f * (1.0f + r2 * r2);
-    float q0 = 2.0f * c0 - 7.0f * c1 + 11.0f * c2;
-    float q1 = -c1 + 5.0f * c2 + 2.0f * c3;
-    float q2 = 2.0f * c2 + 5.0f * c3 - c4;
-    return (a0 * q0 + a1 * q1 + a2 * q2) * __builtin_amdgcn_rcpf(6.0f * (a0 + a1 + a2));
+    const float s0 = d1 - d0, s1 = d2 - d1, s2 = d3 - d2;
+    const float e0 = fmaf(3.0f, d1, -d0), e1 = d1 + d2, e2 = fmaf(3.0f, d2, -d3);
+    const float b0 = (13.0f / 12.0f) * s0 * s0 + 0.25f * e0 * e0;
+    const float b1 = (13.0f / 12.0f) * s1 * s1 + 0.25f * e1 * e1;
+    const float b2 = (13.0f / 12.0f) * s2 * s2 + 0.25f * e2 * e2;
+    const float tau = fabsf(b0 - b2);
+    const float g0 = tau * __builtin_amdgcn_rcpf(b0 + eps);
+    const float g1 = tau * __builtin_amdgcn_rcpf(b1 + eps);
+    const float g2 = tau * __builtin_amdgcn_rcpf(b2 + eps);
+    const float a0 = 0.1f * (1.0f + g0 * g0);
+    const float a1 = 0.6f * (1.0f + g1 * g1);
+    const float a2 = 0.3f * (1.0f + g2 * g2);
+    const float r0 = fmaf(5.0f, d1, -2.0f * d0), r1 = fmaf(2.0f, d2, d1), r2 = fmaf(4.0f, d2, -d3);
+    return (a0 * r0 + a1 * r1 + a2 * r2) * __builtin_amdgcn_rcpf(6.0f * (a0 + a1 + a2));
 }
 
-__device__ __forceinline__ double weno3z(double c0, double c1, double c2) {
+// Σ_r ω_r (q_r − c₁) of the 3-point WENO-Z reconstruction (values c₀, c₁, c₂; face after
+// c₁): β_r = δ_r², q_r − c₁ = δ_r/2, linear weights 1/3, 2/3.
+__device__ __forceinline__ double weno3z_core(double d0, double d1) {
 #pragma clang fp contract(fast)
     const double eps = NE_WENO_EPS;
-    double d0 = c1 - c0, d1 = c2 - c1;
-    double b0 = d0 * d0, b1 = d1 * d1;
-    double tau = fabs(b0 - b1);
-    double t2 = tau * tau;
-    double s0 = (b0 + eps) * (b0 + eps), s1 = (b1 + eps) * (b1 + eps);
-    double a0 = (1.0 / 3.0) * (s0 + t2) * s1;
-    double a1 = (2.0 / 3.0) * (s1 + t2) * s0;
-    double q0 = -c0 + 3.0 * c1;
-    double q1 = c1 + c2;
-    return (a0 * q0 + a1 * q1) / (2.0 * (a0 + a1));
+    const double b0 = d0 * d0, b1 = d1 * d1;
+    const double tau = fabs(b0 - b1);
+    const double t2 = tau * tau;
+    const double p0 = (b0 + eps) * (b0 + eps), p1 = (b1 + eps) * (b1 + eps);
+    const double a0 = (1.0 / 3.0) * (p0 + t2) * p1;
+    const double a1 = (2.0 / 3.0) * (p1 + t2) * p0;
+    return quotient(fma(a0, d0, a1 * d1), 2.0 * (a0 + a1));
 }
 
-__device__ __forceinline__ float weno3z(float c0, float c1, float c2) {
+__device__ __forceinline__ float weno3z_core(float d0, float d1) {
 #pragma clang fp contract(fast)
     const float eps = (float)NE_WENO_EPS;
-    float d0 = c1 - c0, d1 = c2 - c1;
-    float b0 = d0 * d0, b1 = d1 * d1;
-    float tau = fabsf(b0 - b1);
-    float r0 = tau * __builtin_amdgcn_rcpf(b0 + eps);
-    float r1 = tau * __builtin_amdgcn_rcpf(b1 + eps);
-    float a0 = (1.0f / 3.0f) * (1.0f + r0 * r0);
-    float a1 = (2.0f / 3.0f) * (1.0f + r1 * r1);
-    return (a0 * (-c0 + 3.0f * c1) + a1 * (c1 + c2)) * __builtin_amdgcn_rcpf(2.0f * (a0 + a1));
+    const float b0 = d0 * d0, b1 = d1 * d1;
+    const float tau = fabsf(b0 - b1);
+    const float g0 = tau * __builtin_amdgcn_rcpf(b0 + eps);
+    const float g1 = tau * __builtin_amdgcn_rcpf(b1 + eps);
+    const float a0 = (1.0f / 3.0f) * (1.0f + g0 * g0);
+    const float a1 = (2.0f / 3.0f) * (1.0f + g1 * g1);
+    return (a0 * d0 + a1 * d1) * __builtin_amdgcn_rcpf(2.0f * (a0 + a1));
 }
 
 // Reduced order at a face between cells m-1 and m from the activity bits of cells
@@ -1038,16 +1054,25 @@ __device__ __forceinline__ int face_order(unsigned act) {
     return 3;
 }
 
-// Upwind-biased face value from the six cells m-3 … m+2 around the face. The upwind
-// side is chosen by selecting the (mirrored) inputs, so a wavefront with mixed flow
-// directions evaluates one reconstruction per lane.
+// Upwind-biased face value from the six cells m-3 … m+2 around the face (order 3: WENO5,
+// 2: WENO3, else the upwind cell). With the flow towards +: c₂ + core(δ₀, δ₁, δ₂, δ₃);
+// towards −, the mirrored values c₅ … c₁: c₃ − core(δ₄, δ₃, δ₂, δ₁) (δ_n = c_{n+1} − c_n).
+// Only the cells the order uses are read, and one reconstruction is evaluated per lane.
 template <typename R>
 __device__ __forceinline__ R face_value(R c0, R c1, R c2, R c3, R c4, R c5, int order, R vel) {
-    bool left = vel >= (R)0;
-    R v0 = left ? c0 : c5, v1 = left ? c1 : c4, v2 = left ? c2 : c3, v3 = left ? c3 : c2, v4 = left ? c4 : c1;
-    if (order == 3) return weno5z(v0, v1, v2, v3, v4);
-    if (order == 2) return weno3z(v1, v2, v3);
-    return v2;
+    const bool left = vel >= (R)0;
+    const R base = left ? c2 : c3;
+    R core = (R)0;
+    if (order >= 2) {
+        const R D1 = c2 - c1, D2 = c3 - c2, D3 = c4 - c3;
+        if (order == 3) {
+            const R D0 = c1 - c0, D4 = c5 - c4;
+            core = weno5z_core(left ? D0 : D4, left ? D1 : D3, D2, left ? D3 : D1);
+        } else {
+            core = weno3z_core(left ? D1 : D3, D2);
+        }
+    }
+    return left ? base + core : base - core;
 }
 
 }  // namespace ne

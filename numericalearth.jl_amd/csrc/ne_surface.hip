@@ -134,15 +134,40 @@ __global__ void fractional_indices_nodes_kernel(GridDev g, const double* xn, int
     if (n < g.Ny) fj[n] = fractional_index_nodes(g.phi_c[n], yn, ny, ty);
 }
 
+// The ring of the per-phase surface entry points (grids with GridDev::ring): positions
+// i = -1 … Nx on rows j = -1 and Ny, then i = -1 and Nx on rows 0 … Ny-1, in that order.
+// The reference launches its interpolation and flux kernels over 0:N+1 in every non-Flat
+// direction (InterfaceComputations.jl:99-116).
+__host__ __device__ inline int ring_size(const GridDev& g) { return 2 * (g.Nx + 2) + 2 * g.Ny; }
+__device__ __forceinline__ bool ring_position(const GridDev& g, int r, int& i, int& j) {
+    const int W = g.Nx + 2;
+    if (r < W) { i = r - 1; j = -1; return true; }
+    r -= W;
+    if (r < W) { i = r - 1; j = g.Ny; return true; }
+    r -= W;
+    if (r < g.Ny) { i = -1; j = r; return true; }
+    r -= g.Ny;
+    if (r < g.Ny) { i = g.Nx; j = r; return true; }
+    return false;
+}
+
+// Fractional index of exchange node i along a direction of N nodes: the interior index, or for
+// the ring nodes i = -1 and N, whose coordinates lie one spacing beyond the edge nodes of the
+// equally spaced exchange grid, the index extended from the two edge nodes. In a periodic
+// direction it is the wrapped node's index shifted by the source's period (the interpolation
+// wraps it back); in a bounded one it lies beyond the source's edge node (the interpolation
+// holds the edge value).
+__device__ __forceinline__ double ring_index(const double* f, int i, int N) {
+    if (i >= 0 && i < N) return f[i];
+    if (N == 1) return f[0];
+    return i < 0 ? f[0] - (f[1] - f[0]) : f[N - 1] + (f[N - 1] - f[N - 2]);
+}
+
 template <typename Real>
-__global__ void interp_atmos_kernel(GridDev g, AtmosDev<Real> a, const double* fi, const double* fj,
-                                    View3<double> u, View3<double> v, View3<double> T, View3<double> p,
-                                    View3<double> q, View3<double> rain, View3<double> snow) {
-    int i = blockIdx.x * BX + threadIdx.x;
-    int j = blockIdx.y * BY + threadIdx.y;
-    if (i >= g.Nx || j >= g.Ny) return;
-    Neighbours x = neighbours(fi[i], a.Nx, a.tx);
-    Neighbours y = neighbours(fj[j], a.Ny, a.ty);
+__device__ __forceinline__ void interp_atmos_at(const AtmosDev<Real>& a, const Neighbours& x, const Neighbours& y,
+                                                int i, int j, View3<double>& u, View3<double>& v, View3<double>& T,
+                                                View3<double>& p, View3<double>& q, View3<double>& rain,
+                                                View3<double>& snow) {
     u.at2(i, j) = interp_field(a.u, x, y, a.time);
     v.at2(i, j) = interp_field(a.v, x, y, a.time);
     T.at2(i, j) = interp_field(a.T, x, y, a.time);
@@ -153,6 +178,29 @@ __global__ void interp_atmos_kernel(GridDev g, AtmosDev<Real> a, const double* f
 }
 
 template <typename Real>
+__global__ void interp_atmos_kernel(GridDev g, AtmosDev<Real> a, const double* fi, const double* fj,
+                                    View3<double> u, View3<double> v, View3<double> T, View3<double> p,
+                                    View3<double> q, View3<double> rain, View3<double> snow) {
+    int i = blockIdx.x * BX + threadIdx.x;
+    int j = blockIdx.y * BY + threadIdx.y;
+    if (i >= g.Nx || j >= g.Ny) return;
+    Neighbours x = neighbours(fi[i], a.Nx, a.tx);
+    Neighbours y = neighbours(fj[j], a.Ny, a.ty);
+    interp_atmos_at(a, x, y, i, j, u, v, T, p, q, rain, snow);
+}
+
+template <typename Real>
+__global__ void interp_atmos_ring_kernel(GridDev g, AtmosDev<Real> a, const double* fi, const double* fj,
+                                         View3<double> u, View3<double> v, View3<double> T, View3<double> p,
+                                         View3<double> q, View3<double> rain, View3<double> snow) {
+    int i, j;
+    if (!ring_position(g, (int)(blockIdx.x * blockDim.x + threadIdx.x), i, j)) return;
+    Neighbours x = neighbours(ring_index(fi, i, g.Nx), a.Nx, a.tx);
+    Neighbours y = neighbours(ring_index(fj, j, g.Ny), a.Ny, a.ty);
+    interp_atmos_at(a, x, y, i, j, u, v, T, p, q, rain, snow);
+}
+
+template <typename Real>
 __global__ void interp_radiation_kernel(GridDev g, RadDev<Real> r, const double* fi, const double* fj,
                                         View3<double> sw, View3<double> lw) {
     int i = blockIdx.x * BX + threadIdx.x;
@@ -160,6 +208,17 @@ __global__ void interp_radiation_kernel(GridDev g, RadDev<Real> r, const double*
     if (i >= g.Nx || j >= g.Ny) return;
     Neighbours x = neighbours(fi[i], r.Nx, r.tx);
     Neighbours y = neighbours(fj[j], r.Ny, r.ty);
+    sw.at2(i, j) = interp_field(r.sw, x, y, r.time);
+    lw.at2(i, j) = interp_field(r.lw, x, y, r.time);
+}
+
+template <typename Real>
+__global__ void interp_radiation_ring_kernel(GridDev g, RadDev<Real> r, const double* fi, const double* fj,
+                                             View3<double> sw, View3<double> lw) {
+    int i, j;
+    if (!ring_position(g, (int)(blockIdx.x * blockDim.x + threadIdx.x), i, j)) return;
+    Neighbours x = neighbours(ring_index(fi, i, g.Nx), r.Nx, r.tx);
+    Neighbours y = neighbours(ring_index(fj, j, g.Ny), r.Ny, r.ty);
     sw.at2(i, j) = interp_field(r.sw, x, y, r.time);
     lw.at2(i, j) = interp_field(r.lw, x, y, r.time);
 }
@@ -214,6 +273,36 @@ flux_solve_exchange_kernel(GridDev g, ne_params p, View3<const double> ua, View3
     bool solved = in && !inactive;
     accumulate_counters(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
                         solved && !finite3(r.us, r.ts, r.qs));
+}
+
+// The flux solve on the ring: the exchange state is read at the ring position (the
+// interpolation's ring), the ocean surface of a ring node in a periodic direction is that of
+// the wrapped interior column (the values the reference's filled halo holds), and a ring node
+// beyond a bounded edge is inactive (zero interface state). The solve runs as in the interior
+// (same instance of the column solve); the ring is not counted in the solver counters.
+template <bool Fast, bool Consts>
+__global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(NE_FLUX_WAVES_PER_EU)))
+flux_solve_ring_kernel(GridDev g, ne_params p, View3<const double> ua, View3<const double> va,
+                       View3<const double> Ta, View3<const double> pa, View3<const double> qa,
+                       View3<const double> T, View3<const double> S, View3<const double> u,
+                       View3<const double> v, View3<const double> sw, View3<const double> lw, FluxOut out) {
+    int i, j;
+    if (!ring_position(g, (int)(blockIdx.x * blockDim.x + threadIdx.x), i, j)) return;
+    double To = 0.0, So = 0.0, uo = 0.0, vo = 0.0;
+    bool inactive = true;
+    if (g.column(i, j) >= 0) {
+        const int wi = g.wx(i), wj = g.wy(j);
+        ocean_surface<double>(g, T, S, u, v, wi, wj, To, So, uo, vo);
+        inactive = !g.active(wi, wj, g.Nz - 1);
+    }
+    double SW = 0.0, LW = 0.0;
+    if (!Fast && p.interface_temperature == NE_TEMPERATURE_SKIN && p.radiation) {
+        SW = sw.at2(i, j);
+        LW = lw.at2(i, j);
+    }
+    ColumnFluxes r = flux_column<Fast, false, Consts>(p, ua.at2(i, j), va.at2(i, j), Ta.at2(i, j), pa.at2(i, j),
+                                                      qa.at2(i, j), To, uo, vo, inactive, SW, LW);
+    store_fluxes(out, i, j, r);
 }
 
 // One column of K1: interpolate the atmosphere at the column, solve, store the fluxes.
@@ -646,6 +735,10 @@ hipError_t launch_interp_atmos(const GridDev& g, const AtmosDev<Real>& a, const 
     hipLaunchKernelGGL(interp_atmos_kernel<Real>, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, a, fi, fj,
                        view<double>(o.u), view<double>(o.v), view<double>(o.T), view<double>(o.p),
                        view<double>(o.q), view<double>(o.rain), view<double>(o.snow));
+    if (g.ring)
+        hipLaunchKernelGGL(interp_atmos_ring_kernel<Real>, dim3((ring_size(g) + 255) / 256), dim3(256), 0, s, g, a,
+                           fi, fj, view<double>(o.u), view<double>(o.v), view<double>(o.T), view<double>(o.p),
+                           view<double>(o.q), view<double>(o.rain), view<double>(o.snow));
     return hipGetLastError();
 }
 template hipError_t launch_interp_atmos<double>(const GridDev&, const AtmosDev<double>&, const double*,
@@ -657,6 +750,9 @@ template <typename Real>
 hipError_t launch_interp_radiation(const GridDev& g, const RadDev<Real>& r, const double* fi, const double* fj,
                                    View3<double> sw, View3<double> lw, hipStream_t s) {
     hipLaunchKernelGGL(interp_radiation_kernel<Real>, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, r, fi, fj, sw, lw);
+    if (g.ring)
+        hipLaunchKernelGGL(interp_radiation_ring_kernel<Real>, dim3((ring_size(g) + 255) / 256), dim3(256), 0, s, g,
+                           r, fi, fj, sw, lw);
     return hipGetLastError();
 }
 template hipError_t launch_interp_radiation<double>(const GridDev&, const RadDev<double>&, const double*,
@@ -676,6 +772,14 @@ hipError_t launch_flux_solve_exchange(const GridDev& g, const ne_params& p, cons
                        view<const double>(a.v), view<const double>(a.T), view<const double>(a.p),
                        view<const double>(a.q), view<const double>(o.T), view<const double>(o.S),
                        view<const double>(o.u), view<const double>(o.v), sw, lw, out, c);
+    if (g.ring) {
+        auto ring = path == 2 ? flux_solve_ring_kernel<true, true>
+                  : path == 1 ? flux_solve_ring_kernel<true, false> : flux_solve_ring_kernel<false, false>;
+        hipLaunchKernelGGL(ring, dim3((ring_size(g) + BX * BY - 1) / (BX * BY)), dim3(BX * BY), 0, s, g, p,
+                           view<const double>(a.u), view<const double>(a.v), view<const double>(a.T),
+                           view<const double>(a.p), view<const double>(a.q), view<const double>(o.T),
+                           view<const double>(o.S), view<const double>(o.u), view<const double>(o.v), sw, lw, out);
+    }
     return hipGetLastError();
 }
 

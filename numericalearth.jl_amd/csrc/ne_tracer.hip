@@ -11,9 +11,11 @@
 // in a register window. Row-dependent values are wavefront-uniform (one wavefront per row)
 // and kept in scalar registers. Tiles are ordered so that each XCD (blocks are dealt
 // round-robin to the 8 XCDs) receives a contiguous band of tiles, keeping halo re-reads
-// inside one L2. Register pressure sets the occupancy (LDS does not): the FP64 kernel runs 5
-// wavefronts per SIMD, the FP32-storage kernel 6 or more; instances without the two-colour
-// forcing carry none of its registers.
+// inside one L2. The level loop is software-pipelined: the inputs of level k+1 are loaded
+// while level k is computed, so memory requests stay in flight through the arithmetic.
+// Register pressure sets the occupancy (LDS does not): the FP64 kernel runs 4 wavefronts per
+// SIMD, the FP32-storage kernel 7 (5 with the two-colour forcing); instances without the
+// two-colour forcing carry none of its registers.
 #include "ne_internal.h"
 
 #include <mutex>
@@ -34,14 +36,26 @@ constexpr int TC = 63;            // cells per tile row
 #ifndef NE_PLAIN_TR
 #define NE_PLAIN_TR 4
 #endif
-// Minimum wavefronts per SIMD requested for the stage kernels, per storage type (measured:
-// FP64 5 — 95 VGPRs with a 20-byte spill beats 4 without; FP32 6).
+// Minimum wavefronts per SIMD requested for the stage kernels. The software pipeline holds
+// the next level's inputs in registers, and a spilled register defeats it (a reload from
+// scratch waits for every load in flight), so the targets are the largest occupancies whose
+// register budget fits without spills, measured on 1440×560×10: FP64 4 (≈116–125 VGPRs;
+// 5 spills and runs 30 % slower), FP32 7 (≤ 72 VGPRs, a 12-byte spill, faster than 5 or 6)
+// and 5 for the FP32 instances with the two-colour forcing (≈ 93 VGPRs).
 #ifndef NE_STAGE_WAVES_PER_EU_F32
-#define NE_STAGE_WAVES_PER_EU_F32 6
+#define NE_STAGE_WAVES_PER_EU_F32 7
+#endif
+#ifndef NE_STAGE_WAVES_PER_EU_F32_TWO_COLOR
+#define NE_STAGE_WAVES_PER_EU_F32_TWO_COLOR 5
 #endif
 #ifndef NE_STAGE_WAVES_PER_EU_F64
-#define NE_STAGE_WAVES_PER_EU_F64 5
+#define NE_STAGE_WAVES_PER_EU_F64 4
 #endif
+template <typename Real, bool TWO_COLOR>
+constexpr int stage_waves() {
+    return sizeof(Real) == 8 ? NE_STAGE_WAVES_PER_EU_F64
+                             : (TWO_COLOR ? NE_STAGE_WAVES_PER_EU_F32_TWO_COLOR : NE_STAGE_WAVES_PER_EU_F32);
+}
 constexpr int PLAIN_TR = NE_PLAIN_TR;  // cell rows per tile of the single-model kernel
 constexpr int NXCD = 8;
 constexpr int KMIN_OUTSIDE = 1 << 30;  // kmin of a column outside a bounded domain
@@ -237,15 +251,18 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
         return;
     }
 
-    // Vertical register window of the owned column: levels k-2 … k+3
+    // Vertical register window of the owned column: levels k-2 … k+3 while level k is
+    // computed; level k+3 arrives with the level's other inputs (below).
     R zT[6], zS[6];
 #pragma unroll
-    for (int n = 0; n < 6; n++) {
+    for (int n = 0; n < 5; n++) {
         int kk = kstart - 2 + n;
         bool a = cell && kk >= mykmin && kk < Nz;
         zT[n] = a ? ldb(Tin, coff + (uint32_t)kk * L.c_sz) : (R)0;
         zS[n] = a ? ldb(Sin, coff + (uint32_t)kk * L.c_sz) : (R)0;
     }
+    zT[5] = (R)0;
+    zS[5] = (R)0;
     R FzT = (R)0, FzS = (R)0;  // flux through the bottom face of the current level
 
     // TwoColorRadiation: J₀ of the column and the surface height η of the Beer's-law factors
@@ -253,40 +270,63 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
     const double J0 = (tc && cell) ? ldb(F.J0, joff()) : 0.0;
     const double eta = tc ? g.zf[Nz] : 0.0;
 
-    // Inputs of one level, loaded at its start so that the first barrier's wait covers their
-    // latency: its plane (staged in LDS below), the velocities of the faces this work-item
-    // reconstructs and, for FP32 storage, c⁰ of its cell (FP64 loads c⁰ at the update: its two
-    // extra registers would cost a wavefront per SIMD). Measured alternatives, all slower:
-    // loading level k+1 during level k (more live registers, lower occupancy), loading the
-    // velocities at their point of use (exposed latency), the S vertical reconstruction after
-    // the second barrier.
-    constexpr bool EARLY_C0 = sizeof(R) == 4;
+    // Inputs of one level, loaded one level ahead (a software pipeline): each input of level
+    // k+1 is loaded right after its last use at level k — the plane (staged in LDS at the start
+    // of the level) once it is in LDS, the x-face velocity after the x faces, the y-face ones
+    // after the y faces, w after the z face, c⁰ after the update — so every wavefront keeps
+    // memory requests in flight through its arithmetic while holding one copy of each input
+    // (the plane and the window's top are the only values carried twice). The loads carry no
+    // condition: a value a work-item does not use (inactive or outside cells, faces of order 0,
+    // levels past the top) is read from an address that is always inside the view and never
+    // enters arithmetic, because face orders only admit active cells. Conditional loads would
+    // make the compiler wait for every load in flight wherever their paths merge.
     const bool ynorth = row == TR - 1;
+    const uint32_t uoff_l = Xk != NEVER_FACE ? uoff : 0u;
+    const uint32_t voff_l = Sk != NEVER_FACE ? voff_s : 0u;
+    const uint32_t vnoff_l = Nk != NEVER_FACE ? voff_s + dvn : 0u;
+    const uint32_t woff_l = cell ? coff + dw : 0u;
     R nT[NPL], nS[NPL];
-    R nu = (R)0, nv = (R)0, nvn = (R)0, nw = (R)0, nT0 = (R)0, nS0 = (R)0;
-    auto load_level = [&](int k) {
+    R nu, nv, nvn, nw, nT0 = (R)0, nS0 = (R)0, nzT, nzS;
+    double ndz, nzb = 0.0, nzt = 0.0;   // Δz of the level, heights of its faces (two-colour)
+    auto load_plane = [&](int k) {
         const uint32_t kc = (uint32_t)k * L.c_sz;
 #pragma unroll
         for (int q = 0; q < NPL; q++) {
-            const bool a = tid + q * NTHR < PN && k >= pkm[q];
-            nT[q] = a ? ldb(Tin, poff[q] + kc) : (R)0;
-            nS[q] = a ? ldb(Sin, poff[q] + kc) : (R)0;
+            nT[q] = ldb(Tin, poff[q] + kc);
+            nS[q] = ldb(Sin, poff[q] + kc);
         }
-        nu = order_at(k, Xk) > 0 ? ldb(u, uoff + (uint32_t)k * L.u_sz) : (R)0;
-        nv = order_at(k, Sk) > 0 ? ldb(v, voff_s + (uint32_t)k * L.v_sz) : (R)0;
-        nvn = order_at(k, Nk) > 0 ? ldb(v, voff_s + dvn + (uint32_t)k * L.v_sz) : (R)0;
-        const bool zf = cell && adv && k >= mykmin && k <= Nz - 2;
-        nw = zf ? ldb(w, coff + dw + (uint32_t)(k + 1) * L.w_sz) : (R)0;
-        if (EARLY_C0 && MODE == 0) {
-            const bool upd = cell && k >= mykmin;
-            nT0 = upd ? ldb(T0, coff + kc) : (R)0;
-            nS0 = upd ? ldb(S0, coff + kc) : (R)0;
+        const uint32_t kz = (uint32_t)min(k + 3, Nz - 1) * L.c_sz;   // the window's top at level k
+        nzT = ldb(Tin, coff + kz);
+        nzS = ldb(Sin, coff + kz);
+        // the grid's per-level values travel with the level's other loads: read at their point of
+        // use, each would wait for every load in flight (memory counters complete in order)
+        ndz = g.dz[k];
+        if (tc) {
+            nzb = g.zf[k];
+            nzt = g.zf[k + 1];
         }
     };
+    auto load_u = [&](int k) { nu = ldb(u, uoff_l + (uint32_t)k * L.u_sz); };
+    auto load_v = [&](int k) {
+        nv = ldb(v, voff_l + (uint32_t)k * L.v_sz);
+        nvn = ldb(v, vnoff_l + (uint32_t)k * L.v_sz);
+    };
+    auto load_w = [&](int k) { nw = ldb(w, woff_l + (uint32_t)(k + 1) * L.w_sz); };
+    auto load_c0 = [&](int k) {
+        if (MODE == 0) {
+            nT0 = ldb(T0, coff + (uint32_t)k * L.c_sz);
+            nS0 = ldb(S0, coff + (uint32_t)k * L.c_sz);
+        }
+    };
+    load_plane(kstart);
+    load_u(kstart);
+    load_v(kstart);
+    load_w(kstart);
+    load_c0(kstart);
     for (int k = kstart; k < Nz; k++) {
+        const int kn = min(k + 1, Nz - 1);   // the level loaded ahead (the last one again at the top)
         const uint32_t kc = (uint32_t)k * L.c_sz;
-        // 1. stage the plane of level k and take its velocities
-        load_level(k);
+        // 1. stage the plane of level k; take its velocities, c⁰ and the window's top
 #pragma unroll
         for (int q = 0; q < NPL; q++) {
             const int n = tid + q * NTHR;
@@ -295,8 +335,12 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                 sh.cS[n] = nS[q];
             }
         }
+        const double dzd = wave_uniform(ndz), zfb = wave_uniform(nzb), zft = wave_uniform(nzt);
+        zT[5] = nzT;
+        zS[5] = nzS;
         __syncthreads();
-        const R dzk = (R)g.dz[k];
+        load_plane(kn);
+        const R dzk = (R)dzd;
 
         // 2a. x face owned by this lane: cells i0+lane-3 … i0+lane+2 of row j0+row; T, then S
         // (one tracer at a time keeps a single reconstruction's temporaries live)
@@ -314,6 +358,7 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                 (tr ? sh.fxS : sh.fxT)[row * TW + lane] = f;
             }
         }
+        load_u(kn);
         // 2b. y faces: the south face of row j0+row, and for the last wavefront the north face of the tile
         if (ycol) {
 #pragma unroll 1
@@ -332,6 +377,7 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                 (tr ? sh.fyS : sh.fyT)[r0 * TC + lane] = f;
             }
         }
+        load_v(kn);
         // 2c. z face at the top of level k (cells k-2 … k+3 of the owned column)
         const int zorder = (cell && adv) ? (k >= mykmin && k <= Nz - 2 ? 1 : 0) +
                                                (k >= mykmin + 1 && k <= Nz - 3 ? 1 : 0) +
@@ -342,6 +388,7 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
             FztT = (R)Azc * nw * face_value(zT[0], zT[1], zT[2], zT[3], zT[4], zT[5], zorder, nw);
             FztS = (R)Azc * nw * face_value(zS[0], zS[1], zS[2], zS[3], zS[4], zS[5], zorder, nw);
         }
+        load_w(kn);
         __syncthreads();
 
         // 3. difference the face fluxes, add the top boundary flux, update
@@ -351,18 +398,18 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
             div_T = div_T + (FztT - FzT);
             R div_S = (sh.fxS[xo + 1] - sh.fxS[xo]) + (sh.fyS[yo + TC] - sh.fyS[yo]);
             div_S = div_S + (FztS - FzS);
-            const R V = (R)(Azc * g.dz[k]);
+            const R V = (R)(Azc * dzd);
             R GT = -((R)1 / V * div_T);
             R GS = -((R)1 / V * div_S);
             if (tc) {
                 // ε₁ ∂z J₁ + (1 − ε₁) ∂z J₂; the face below the lowest active cell carries no flux
                 // (the level's two faces; uniform over the tile, recomputed rather than carried)
-                const double E1t = exp(F.kappa1 * (g.zf[k + 1] - eta)), E2t = exp(F.kappa2 * (g.zf[k + 1] - eta));
-                const double E1b = exp(F.kappa1 * (g.zf[k] - eta)), E2b = exp(F.kappa2 * (g.zf[k] - eta));
+                const double E1t = exp(F.kappa1 * (zft - eta)), E2t = exp(F.kappa2 * (zft - eta));
+                const double E1b = exp(F.kappa1 * (zfb - eta)), E2b = exp(F.kappa2 * (zfb - eta));
                 const bool below = k - 1 >= mykmin;
                 const double a1 = J0 * E1t, b1 = below ? J0 * E1b : 0.0;
                 const double a2 = J0 * E2t, b2 = below ? J0 * E2b : 0.0;
-                const double dz = g.dz[k];
+                const double dz = dzd;
                 const double dJ1dz = (a1 - b1) / dz;
                 const double dJ2dz = (a2 - b2) / dz;
                 GT = GT + (R)(F.first_fraction * dJ1dz + (1.0 - F.first_fraction) * dJ2dz);
@@ -381,10 +428,8 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                 // The update is formed in FP64 and rounded once to the storage type (for FP32
                 // storage this keeps the per-stage rounding of the state at half an ulp).
                 const uint32_t key = 2u * ((uint32_t)ci + (uint32_t)g.Nx * ((uint32_t)cj + (uint32_t)g.Ny * k));
-                const R c0T = EARLY_C0 ? nT0 : ldb(T0, coff + kc);
-                const R c0S = EARLY_C0 ? nS0 : ldb(S0, coff + kc);
-                const R Tn = round_state<R>((double)c0T + beta_dt * (double)GT, key, seed, stochastic);
-                const R Sn = round_state<R>((double)c0S + beta_dt * (double)GS, key + 1u, seed, stochastic);
+                const R Tn = round_state<R>((double)nT0 + beta_dt * (double)GT, key, seed, stochastic);
+                const R Sn = round_state<R>((double)nS0 + beta_dt * (double)GS, key + 1u, seed, stochastic);
                 stb(Tout, coff + kc, Tn);
                 stb(Sout, coff + kc, Sn);
                 // the freeze clamp's test on the stored values (freeze_column, ne_surface.hip)
@@ -394,17 +439,12 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                 stb(Sout, coff + kc, GS);
             }
         }
+        load_c0(kn);
         FzT = FztT;
         FzS = FztS;
-        // 4. slide the vertical window up one level
-        if (cell) {
+        // 4. slide the vertical window up one level (its new top arrives with level k+1)
 #pragma unroll
-            for (int n = 0; n < 5; n++) { zT[n] = zT[n + 1]; zS[n] = zS[n + 1]; }
-            int kk = k + 4;
-            bool a = kk >= mykmin && kk < Nz;
-            zT[5] = a ? ldb(Tin, coff + (uint32_t)kk * L.c_sz) : (R)0;
-            zS[5] = a ? ldb(Sin, coff + (uint32_t)kk * L.c_sz) : (R)0;
-        }
+        for (int n = 0; n < 5; n++) { zT[n] = zT[n + 1]; zS[n] = zS[n + 1]; }
     }
     if (watch && cell) F.freeze_flags[(int64_t)ci + (int64_t)cj * g.Nx] = frz ? 1 : 0;
 }
@@ -413,7 +453,7 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
 // round-robin to the 8 XCDs) receives a contiguous band of tiles.
 template <typename Real, int MODE, bool TWO_COLOR, bool WATCH = false>
 __global__ void __launch_bounds__(TW * PLAIN_TR)
-__attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? NE_STAGE_WAVES_PER_EU_F32 : NE_STAGE_WAVES_PER_EU_F64)))
+__attribute__((amdgpu_waves_per_eu(stage_waves<Real, TWO_COLOR>())))
 tracer_tile_kernel(GridDev g, TracerLayout L, const Real* u, const Real* v, const Real* w, const Real* T0,
                    const Real* S0, const Real* Tin, const Real* Sin, Real* Tout, Real* Sout, const double* JT,
                    const double* JS, const double* Jw, TracerForcing F, double beta_dt, int stochastic,
@@ -435,7 +475,7 @@ tracer_tile_kernel(GridDev g, TracerLayout L, const Real* u, const Real* v, cons
 // stage 1: T¹ = T + β₁Δt G(T); stage 2: T² = T + β₂Δt G(T¹); stage 3: T = T + β₃Δt G(T²).
 template <typename Real, int TR, bool TWO_COLOR, bool WATCH>
 __global__ void __launch_bounds__(TW * TR)
-__attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? NE_STAGE_WAVES_PER_EU_F32 : NE_STAGE_WAVES_PER_EU_F64)))
+__attribute__((amdgpu_waves_per_eu(stage_waves<Real, TWO_COLOR>())))
 batch_stage_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __restrict__ work, int nwork,
                    int stage, double beta_dt) {
     __shared__ TileShared<Real, TR> sh;

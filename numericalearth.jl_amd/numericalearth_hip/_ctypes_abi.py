@@ -62,6 +62,7 @@ class Grid(C.Structure):
         ("topology_x", _i32), ("topology_y", _i32), ("advection", _i32),
         ("dz", _p), ("z_center", _p), ("dy_xface", _p), ("dx_yface", _p), ("Az", _p),
         ("lambda_c", _p), ("phi_c", _p), ("kmin", _p), ("z_face", _p),
+        ("halo", _i32),
     ]
 
 

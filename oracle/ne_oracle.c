@@ -615,16 +615,47 @@ static inline double bilinear(ne_view d, int n, int i0, int i1, int j0, int j1, 
     return w00 * V3(d, i0, j0, n) + w01 * V3(d, i0, j1, n) + w10 * V3(d, i1, j0, n) + w11 * V3(d, i1, j1, n);
 }
 
+/* Width of the ring the interface kernels also cover: the reference launches them over
+ * 0:N+1 in every non-Flat direction, except on single-column grids
+ * (InterfaceComputations.jl:99-116); here when the caller's views have a halo. */
+static inline int ring_width(const ne_grid* g) {
+    return (g->halo >= 1 && !(g->Nx == 1 && g->Ny == 1)) ? 1 : 0;
+}
+
+/* Coordinate of exchange node i of n (centres x[0 … n−1]): interior nodes, and the halo nodes
+ * −1 and n of the equally spaced grid one spacing beyond the edge nodes. */
+static inline double halo_node(const double* x, int n, int i) {
+    if (i >= 0 && i < n) return x[i];
+    if (n == 1) return x[0];
+    return i < 0 ? x[0] - (x[1] - x[0]) : x[n - 1] + (x[n - 1] - x[n - 2]);
+}
+
+/* Fractional index of exchange node i: the given interior index, or at a ring node the index of
+ * its coordinate in a regular source grid (FractionalIndices of the halo node,
+ * prescribed_atmosphere_regridder.jl:55-58); a source given by node coordinates (no regular
+ * spacing) extends the edge indices linearly. */
+static inline double node_index(const double* f, const double* x, int n, int i, int topology, double origin,
+                                double spacing) {
+    if (i >= 0 && i < n) return f[i];
+    if (topology == NE_FLAT) return 0.0;
+    if (spacing > 0.0) return (halo_node(x, n, i) - (origin + 0.5 * spacing)) / spacing;
+    if (n == 1) return f[0];
+    return i < 0 ? f[0] - (f[1] - f[0]) : f[n - 1] + (f[n - 1] - f[n - 2]);
+}
+
 void neo_interp_field(const ne_grid* g, const ne_source_grid* s, const double* fi, const double* fj,
                       ne_view src, ne_time_interp t, ne_view out) {
+    const int R = ring_width(g);
 #pragma omp parallel for NEO_THREADS
-    for (int j = 0; j < g->Ny; j++) {
+    for (int j = -R; j < g->Ny + R; j++) {
         int j0, j1; double eta;
-        neighbours(fj[j], s->Ny, s->topology_y, &j0, &j1, &eta);
-        for (int i = 0; i < g->Nx; i++) {
+        neighbours(node_index(fj, g->phi_c, g->Ny, j, s->topology_y, s->phi0, s->dphi), s->Ny, s->topology_y, &j0,
+                   &j1, &eta);
+        for (int i = -R; i < g->Nx + R; i++) {
             if (src.data == NULL) { V2(out, i, j) = 0.0; continue; }
             int i0, i1; double xi;
-            neighbours(fi[i], s->Nx, s->topology_x, &i0, &i1, &xi);
+            neighbours(node_index(fi, g->lambda_c, g->Nx, i, s->topology_x, s->lambda0, s->dlambda), s->Nx,
+                       s->topology_x, &i0, &i1, &xi);
             double a = bilinear(src, t.n1, i0, i1, j0, j1, xi, eta);
             double b = bilinear(src, t.n2, i0, i1, j0, j1, xi, eta);
             V2(out, i, j) = (1.0 - t.tilde_n) * a + t.tilde_n * b;
@@ -653,12 +684,16 @@ void neo_interp_radiation(const ne_grid* g, const ne_prescribed_radiation* r, co
 /* Flux solve over the grid                                                  */
 /* ------------------------------------------------------------------------- */
 
+/* On the ring (ring_width) the ocean surface is read from the caller's halo, which holds
+ * what Oceananigans' fill_halo_regions! leaves there, and a ring cell beyond a bounded edge
+ * is inactive (inactive_node outside the domain). */
 void neo_ao_flux_solve(const ne_grid* g, const ne_params* p, const ne_exchange_state* a,
                        const ne_ocean_state* o, const ne_interface_fluxes* f) {
     int kN = g->Nz - 1;
+    const int R = ring_width(g);
 #pragma omp parallel for schedule(dynamic, 4) NEO_THREADS
-    for (int j = 0; j < g->Ny; j++) {
-        for (int i = 0; i < g->Nx; i++) {
+    for (int j = -R; j < g->Ny + R; j++) {
+        for (int i = -R; i < g->Nx + R; i++) {
             double in[11], out[9];
             in[0] = V2(a->u, i, j); in[1] = V2(a->v, i, j); in[2] = V2(a->T, i, j);
             in[3] = V2(a->p, i, j); in[4] = V2(a->q, i, j);

@@ -487,7 +487,14 @@ def test_freezing_steps_clamp_every_flagged_column(require_gpu, precision):
             om.time_step(1200.0)
             assert_rel(T, om.T, RTOL, "T")
             assert_rel(S, om.S, RTOL, "S")
-            assert_rel(fz, om.frazil, RTOL, "frazil heat")
+            # The frazil heat is a column sum of ρc(T_m − T)Δz/Δt over cells a few mK below
+            # T_m: each term is the difference of two values of size |T_m|, so it is compared
+            # against the size of those terms (a 1-ulp difference in T, which the T comparison
+            # allows, moves it by ulp(T)/(T_m − T) relative to itself).
+            rho_c = p.ocean_reference_density * p.ocean_heat_capacity
+            dz = case.grid.dz[:, None, None]
+            terms = rho_c / 1200.0 * np.sum(np.where(active, 2.0 * np.abs(Tm) * dz, 0.0), axis=0)
+            assert_rel_terms(fz, om.frazil, terms, RTOL, "frazil heat")
             assert np.array_equal(fz == 0.0, om.frazil == 0.0)
     assert frozen > 100, "the case froze too few columns to exercise the clamp"
     gm.close()

@@ -50,20 +50,20 @@ __host__ inline View3<T> view(const ne_view& v) {
     return r;
 }
 
-// Julia's min/max on floats: NaN-propagating, -0.0 < +0.0.
+// Julia's min/max on floats: NaN-propagating, -0.0 < +0.0. Written as selects without early
+// returns: branches here become divergent blocks that assign a literal operand, for which the
+// compiler holds 64-bit constants in (spilled) vector registers inside the flux iteration.
 template <typename T>
 __device__ __forceinline__ T jmin(T a, T b) {
-    if (a != a || b != b) return a + b;
-    if (a < b) return a;
-    if (b < a) return b;
-    return __builtin_signbit(a) ? a : b;
+    T r = a < b ? a : b;                                  // b when b < a
+    r = a == b ? (__builtin_signbit(a) ? a : b) : r;      // ±0: the negative one
+    return (a != a || b != b) ? a + b : r;               // NaN
 }
 template <typename T>
 __device__ __forceinline__ T jmax(T a, T b) {
-    if (a != a || b != b) return a + b;
-    if (a > b) return a;
-    if (b > a) return b;
-    return __builtin_signbit(a) ? b : a;
+    T r = a > b ? a : b;
+    r = a == b ? (__builtin_signbit(a) ? b : a) : r;
+    return (a != a || b != b) ? a + b : r;
 }
 
 __host__ __device__ __forceinline__ int wrapi(int i, int n) {

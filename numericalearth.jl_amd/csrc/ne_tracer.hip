@@ -148,6 +148,11 @@ __device__ __forceinline__ double wave_uniform(double x) {
     return __hiloint2double(hi, lo);
 }
 
+// 1/x for a positive normal x: the FP64 reciprocal refined by Newton steps (within 2 ulp), the
+// hardware reciprocal in FP32 (1 ulp).
+__device__ __forceinline__ double reciprocal(double x) { return quotient(1.0, x); }
+__device__ __forceinline__ float reciprocal(float x) { return __builtin_amdgcn_rcpf(x); }
+
 template <typename T>
 __device__ __forceinline__ T ldb(const T* base, uint32_t off) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
@@ -218,8 +223,11 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
     // north y face of the tile (last wavefront)
     const uint32_t Xk = (xrow && adv) ? thresholds(&sh.km[(row + 3) * PXW + lane], 1) : NEVER_FACE;
     const uint32_t Sk = (ycol && adv) ? thresholds(&sh.km[row * PXW + lane + 3], PXW) : NEVER_FACE;
-    const uint32_t Nk =
-        (ycol && adv && row == TR - 1) ? thresholds(&sh.km[(row + 1) * PXW + lane + 3], PXW) : NEVER_FACE;
+    // The tile's north face row is shared out: wavefront 0 reconstructs it for T, wavefront 1
+    // for S (a single-row tile's wavefront does both), so no wavefront carries more than one
+    // extra reconstruction into the level's second barrier.
+    const bool north = TR >= 2 ? row < 2 : true;
+    const uint32_t Nk = (ycol && adv && north) ? thresholds(&sh.km[TR * PXW + lane + 3], PXW) : NEVER_FACE;
     const int fiw = g.wx(i0 + lane);
     const int fjs = g.wy(j0 + row);
     const int fjn = g.wy(j0 + TR);
@@ -280,7 +288,6 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
     // levels past the top) is read from an address that is always inside the view and never
     // enters arithmetic, because face orders only admit active cells. Conditional loads would
     // make the compiler wait for every load in flight wherever their paths merge.
-    const bool ynorth = row == TR - 1;
     const uint32_t uoff_l = Xk != NEVER_FACE ? uoff : 0u;
     const uint32_t voff_l = Sk != NEVER_FACE ? voff_s : 0u;
     const uint32_t vnoff_l = Nk != NEVER_FACE ? voff_s + dvn : 0u;
@@ -342,13 +349,16 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
         load_plane(kn);
         const R dzk = (R)dzd;
 
-        // 2a. x face owned by this lane: cells i0+lane-3 … i0+lane+2 of row j0+row; T, then S
-        // (one tracer at a time keeps a single reconstruction's temporaries live)
+        // 2a. x face owned by this lane: cells i0+lane-3 … i0+lane+2 of row j0+row; T, then S.
+        // FP64 interleaves the two tracers' reconstructions (independent chains to issue from:
+        // 8 µs of 158 per stage, measured); FP32 takes one at a time, whose temporaries alone fit
+        // its 7-wavefront register budget.
+        constexpr int FACE_UNROLL = sizeof(R) == 8 ? 2 : 1;
         {
             const int order = order_at(k, Xk);
             const R A = (R)Axf * dzk;
             const int base = (row + 3) * PXW + lane;
-#pragma unroll 1
+#pragma unroll FACE_UNROLL
             for (int tr = 0; tr < 2; tr++) {
                 const R* t = (tr ? sh.cS : sh.cT) + base;
                 R f = (R)0;
@@ -361,10 +371,11 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
         load_u(kn);
         // 2b. y faces: the south face of row j0+row, and for the last wavefront the north face of the tile
         if (ycol) {
-#pragma unroll 1
-            for (int pass = 0; pass < (ynorth ? 4 : 2); pass++) {
-                const int side = pass >> 1, tr = pass & 1;
-                const int r0 = row + side;                // plane row of the first stencil cell
+#pragma unroll FACE_UNROLL
+            for (int pass = 0; pass < (TR >= 2 ? (north ? 3 : 2) : 4); pass++) {
+                const int side = pass >> 1;               // 0: south face of the row, 1: north face of the tile
+                const int tr = (TR >= 2 && side) ? row : (pass & 1);
+                const int r0 = side ? TR : row;           // plane row of the first stencil cell
                 const double Ay = side ? Ayn : Ays;
                 const int order = order_at(k, side ? Nk : Sk);
                 R f = (R)0;
@@ -399,8 +410,9 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
             R div_S = (sh.fxS[xo + 1] - sh.fxS[xo]) + (sh.fyS[yo + TC] - sh.fyS[yo]);
             div_S = div_S + (FztS - FzS);
             const R V = (R)(Azc * dzd);
-            R GT = -((R)1 / V * div_T);
-            R GS = -((R)1 / V * div_S);
+            const R rV = reciprocal(V);
+            R GT = -(rV * div_T);
+            R GS = -(rV * div_S);
             if (tc) {
                 // ε₁ ∂z J₁ + (1 − ε₁) ∂z J₂; the face below the lowest active cell carries no flux
                 // (the level's two faces; uniform over the tile, recomputed rather than carried)
@@ -421,8 +433,8 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                 const uint32_t jo = joff();
                 double FT = ldb(JT, jo) + (0.0 + 0.0);
                 double FS = ldb(JS, jo) + (((double)zS[2] * ldb(Jw, jo) - 0.0) + 0.0);
-                GT = GT - (R)(FT * Azc) / V;
-                GS = GS - (R)(FS * Azc) / V;
+                GT = GT - (R)(FT * Azc) * rV;
+                GS = GS - (R)(FS * Azc) * rV;
             }
             if (MODE == 0) {
                 // The update is formed in FP64 and rounded once to the storage type (for FP32

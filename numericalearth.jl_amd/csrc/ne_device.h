@@ -1,8 +1,15 @@
 // ne_device.h — device-side building blocks shared by the HIP kernels.
 //
-// Every formula keeps the evaluation order of the reference expression it restates
-// (cited per function); the library is compiled with -ffp-contract=off so that the
-// FP64 path is the same sequence of IEEE operations as the reference arithmetic.
+// Formulas follow the reference expression they restate (cited per function), and the library
+// is compiled with -ffp-contract=off. The general FP64 flux instance keeps the reference's
+// evaluation order with the math library's elementary functions. The instance for the default
+// Edson constants ("Fast", and "Consts" with the default surface constants as literals) departs
+// from it at the ulp level: its own logarithm ne_log (within 2 ulp), A·exp(−b·log R★) for
+// A/R★^b, x·√x for (1 + x)^{3/2}, shared subexpressions of ψ_m and ψ_s, and a degree-7 Taylor
+// polynomial of ψ at |ζ| ≤ 5·10⁻⁴. Those differences stay far inside the 1e-12 comparison with
+// the oracle, but near a stop threshold they can move a column's fixed point by one iteration
+// (its fluxes then differ by that iteration's drift, below the stop tolerance). The WENO
+// reconstructions use fused multiply-adds and a reciprocal-based division (stated there).
 #pragma once
 #include <string.h>
 
@@ -220,8 +227,9 @@ __device__ __forceinline__ double psi_scalar(const ne_params& p, double z) {
 #include "ne_psi_series.h"
 #define NE_PSI_SERIES_RADIUS 5e-4
 
-// Elementary functions by precision: FP64 uses the accurate device library; the FP32
-// pre-iteration of the mixed form uses the hardware log₂/exp₂-based approximations.
+// Elementary functions by precision: FP64 uses the math library (the Fast flux instance its
+// own ne_log below); the FP32 pre-iteration of the mixed form uses the hardware
+// log₂/exp₂-based approximations.
 // Natural logarithm of a positive, finite, normal double in about 30 vector instructions (the
 // library's log takes about 100): x = 2^e·m with m in [√½, √2), log x = e·ln 2 + 2·atanh(s),
 // s = (m − 1)/(m + 1), |s| ≤ 0.1716 — the argument reduction of Cody & Waite (Software Manual

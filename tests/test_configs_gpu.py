@@ -1,0 +1,181 @@
+"""The benchmark configurations at their own workloads, on the GPU.
+
+* Config 1: 100 coupled FP64 steps of the 90×40×10 case against the oracle, step by step.
+* Config 3: the mixed form at 1440×560×10 over 1000 steps of 20 min against the FP64 path (which
+  equals the oracle to 1e-12 per step). The workload's own sensitivity is measured beside it: the
+  FP64 path started from the FP32-rounded initial state. The mixed form adds no error beyond what
+  rounding its inputs to FP32 already causes, and its SST differs by ≤ 1e-5 in the L2 norm.
+* Config 4: the 64-member ensemble (sizes log-uniform between 1×1×50 and 360×140×20, each with its
+  own bathymetry, atmosphere, Charnock parameter and roughness) stepped as one batch equals the
+  members stepped alone bit for bit, FP64 and mixed; its smallest, median and largest FP64 members
+  equal the oracle to 1e-12 per step.
+* Config 5: requests of 1×1×50 … 720×280×10, FP64 and mixed 50/50, arriving as a Poisson stream at
+  50 req/s, served with continuous batching: every request completes, and the smallest, a median
+  and the largest give bit for bit the result they give served alone.
+"""
+import copy
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from helpers import assert_rel
+from numericalearth_hip.ensemble import Ensemble, build_members, draw_ensemble, member_case, member_params
+from numericalearth_hip.generators import make_case
+from numericalearth_hip.model import OceanOnlyModel
+
+pytestmark = pytest.mark.gpu
+DT = 1200.0
+RTOL = 1e-12
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def test_config1_hundred_fp64_steps_match_oracle(require_gpu):
+    case = make_case(90, 40, 10, seed=0)
+    om = O.OracleModel(case).reconcile()
+    gm = OceanOnlyModel(case)
+    worst = 0.0
+    for step in range(100):
+        om.time_step(DT)
+        gm.time_step(DT)
+        torch.cuda.synchronize()
+        eT, fT = assert_rel(_np(gm.T), om.T, RTOL, f"T step {step + 1}")
+        eS, fS = assert_rel(_np(gm.S), om.S, RTOL, f"S step {step + 1}")
+        worst = max(worst, eT, eS)
+        if (step + 1) % 25 == 0:
+            print(f"[config1] step {step + 1}: worst relative T/S error so far {worst:.2e}", flush=True)
+    for k in ("sensible_heat", "latent_heat", "friction_velocity"):
+        # the last step's fluxes come from states equal to ~1e-14; a column whose stop test flipped
+        # differs by one iteration's drift (below the 1e-8 stop tolerance on u★ + θ★ + q★)
+        a, b = _np(gm.fluxes[k]), om.flux_arrays[k]
+        assert np.max(np.abs(a - b)) <= 1e-7 * np.max(np.abs(b)), k
+    assert gm.clock == (100 * DT, 100)
+    gm.close()
+
+
+def _sst_errors(sst, ref, wet):
+    d = (sst - ref)[wet]
+    r = ref[wet]
+    return {"max": float(np.max(np.abs(d)) / np.max(np.abs(r))),
+            "l2": float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(r * r)))}
+
+
+def test_config3_mixed_form_at_quarter_degree(require_gpu):
+    steps = 1000
+    case = make_case(1440, 560, 10, seed=0)
+    wet = case.grid.kmin < case.shape[2]
+    rounded = copy.copy(case)
+    rounded.T = case.T.astype(np.float32).astype(np.float64)
+    rounded.S = case.S.astype(np.float32).astype(np.float64)
+    m64 = OceanOnlyModel(case, precision="fp64")
+    m32 = OceanOnlyModel(case, precision="mixed")
+    r64 = OceanOnlyModel(rounded, precision="fp64")
+    H0 = {"fp64": m64.heat_content(), "mixed": m32.heat_content()}
+    for n in range(steps):
+        for m in (m64, m32, r64):
+            m.time_step(DT)
+        if (n + 1) % 250 == 0:
+            torch.cuda.synchronize()
+            print(f"[config3] step {n + 1}", flush=True)
+    torch.cuda.synchronize()
+    sst64 = _np(m64.T[-1])
+    mixed = _sst_errors(_np(m32.T[-1]).astype(np.float64), sst64, wet)
+    sensitivity = _sst_errors(_np(r64.T[-1]), sst64, wet)
+    flux = {}
+    for k in ("sensible_heat", "latent_heat"):
+        a, b = _np(m32.fluxes[k]), _np(m64.fluxes[k])
+        flux[k] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    dH64 = m64.heat_content() - H0["fp64"]
+    dH32 = m32.heat_content() - H0["mixed"]
+    print(f"[config3] after {steps} steps: mixed SST {mixed}, FP32-rounded start {sensitivity}, "
+          f"fluxes {flux}, heat gained FP64 {dH64:.6e} J mixed {dH32:.6e} J", flush=True)
+    assert mixed["l2"] <= 1e-5
+    # the max norm is the workload's sensitivity to rounding its initial state to FP32: the mixed
+    # form stays within twice what that rounding alone causes
+    assert mixed["max"] <= 2.0 * sensitivity["max"]
+    assert max(flux.values()) <= 1e-3
+    assert abs(dH32 - dH64) <= 1e-3 * abs(dH64)
+    for m in (m64, m32, r64):
+        m.close()
+
+
+def _state(m):
+    d = {"T": m.T, "S": m.S}
+    d.update({f"flux.{k}": v for k, v in m.fluxes.items()})
+    d.update({f"net.{k}": v for k, v in m.net.items()})
+    return {k: _np(v) for k, v in d.items()}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_config4_ensemble_of_64(require_gpu, precision):
+    specs = draw_ensemble(64, 0)
+    shapes = [(s.Nx, s.Ny, s.Nz) for s in specs]
+    cells = [s.Nx * s.Ny * s.Nz for s in specs]
+    order = np.argsort(cells)
+    picks = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
+    print(f"[config4] {precision}: 64 members, {sum(cells)} cells, smallest {shapes[picks[0]]}, "
+          f"median {shapes[picks[1]]}, largest {shapes[picks[2]]}", flush=True)
+    alone = build_members(specs, precision)
+    batched = build_members(specs, precision)
+    ens = Ensemble(batched)
+    ens.autotune()
+    ens.use_graph(True)
+    oracles = {}
+    if precision == "fp64":
+        for k in picks:
+            oracles[k] = O.OracleModel(member_case(specs[k]), params=member_params(specs[k])).reconcile()
+    for step in range(3):
+        ens.time_step(DT)
+        for m in alone:
+            m.time_step(DT)
+        torch.cuda.synchronize()
+        for n, (a, b) in enumerate(zip(alone, batched)):
+            sa, sb = _state(a), _state(b)
+            for k in sa:
+                assert np.array_equal(sa[k], sb[k], equal_nan=True), (precision, step, n, shapes[n], k)
+        for k, om in oracles.items():
+            om.time_step(DT)
+            assert_rel(_np(batched[k].T), om.T, RTOL, f"member {k} {shapes[k]} T step {step + 1}")
+            assert_rel(_np(batched[k].S), om.S, RTOL, f"member {k} {shapes[k]} S step {step + 1}")
+    ens.close()
+    for m in alone + batched:
+        m.close()
+
+
+def test_config5_serving_at_50_requests_per_second(require_gpu):
+    from numericalearth_hip.serving import Request, Server, latency_summary, poisson_arrivals
+    rng = np.random.default_rng(7)
+    pool = draw_ensemble(24, 7, smallest=(1, 1, 50), largest=(720, 280, 10), resolution=0.5)
+    cases = [(member_case(s), member_params(s)) for s in pool]
+    arrivals = poisson_arrivals(50.0, 1.0, rng)
+    reqs = []
+    for k, t in enumerate(arrivals):
+        c, p = cases[int(rng.integers(len(cases)))]
+        reqs.append(Request(id=k, case=c, params=p, precision="fp64" if rng.uniform() < 0.5 else "mixed",
+                            steps=100, dt=DT, arrival=float(t)))
+    sizes = [int(np.prod(r.case.shape)) for r in reqs]
+    print(f"[config5] {len(reqs)} requests, sizes {min(sizes)} … {max(sizes)} cells, "
+          f"{sum(r.precision == 'fp64' for r in reqs)} FP64", flush=True)
+    srv = Server(budget_bytes=200e9)
+    done = srv.run([copy.copy(r) for r in reqs])
+    srv.close()
+    assert len(done) == len(reqs)
+    summary = latency_summary(done)
+    print(f"[config5] served: {summary}", flush=True)
+    order = np.argsort(sizes)
+    for idx in (int(order[0]), int(order[len(order) // 2]), int(order[-1])):
+        target = reqs[idx]
+        fresh = copy.copy(target)
+        fresh.result, fresh.arrival = {}, 0.0
+        s2 = Server(budget_bytes=200e9)
+        alone = s2.run([fresh])[0]
+        s2.close()
+        mixed = next(r for r in done if r.id == target.id)
+        for k in alone.result:
+            assert np.array_equal(alone.result[k], mixed.result[k], equal_nan=True), (target.id, k)
+        print(f"[config5] request {target.id} {target.case.shape} {target.precision}: identical alone and "
+              f"in the stream", flush=True)

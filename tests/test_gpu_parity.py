@@ -101,6 +101,14 @@ def _compare_fluxes(gm, om, rtol=RTOL, max_iteration_mismatch_frac=1e-4, envelop
     frac = mismatch.mean()
     print(f"[flux] iteration mismatches: {mismatch.sum()} of {mismatch.size}", flush=True)
     assert frac <= max_iteration_mismatch_frac
+    if mismatch.any():
+        # A column whose stop test flipped (its drift met the tolerance within the last bits) stopped
+        # one iteration before or after the oracle's: the two results are consecutive iterates, which
+        # differ by that iteration's drift |Δu★| + |Δθ★| + |Δq★|, below the stop tolerance.
+        assert np.all(np.abs(it_g[mismatch] - it_o[mismatch]) == 1), "a stop iteration differs by more than one"
+        drift = sum(np.abs(_np(gm.fluxes[k])[mismatch] - om.flux_arrays[k][mismatch])
+                    for k in ("friction_velocity", "temperature_scale", "water_vapor_scale"))
+        assert np.all(drift <= om.params.tolerance * (1.0 + 1e-6)), f"flipped columns differ by {drift.max():.3e}"
     ok = ~mismatch
     worst = {}
     for k in O.FLUX_NAMES:

@@ -56,6 +56,9 @@ def parse():
     ap.add_argument("--stream-every", type=int, default=9, help="steps between snapshots (9: 3-hourly at 20 min)")
     ap.add_argument("--stream-window", type=int, default=3, help="snapshot slots per streamed series")
     ap.add_argument("--stream-snapshots", type=int, default=16, help="snapshots in the (cyclical) series")
+    ap.add_argument("--extras", choices=("on", "off"), default="on",
+                    help="also measure the other precision form, the 64-member ensemble and one serving rate")
+    ap.add_argument("--extras-rate", type=float, default=200.0, help="request rate of the serving measurement")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="only run a device-to-device copy of this many GiB (--steps times) and report its bandwidth")
     return ap.parse_args()
@@ -106,18 +109,18 @@ def copy_bandwidth(gib: float, reps: int) -> dict:
     return {"bytes_per_copy": 2 * n * 8, "ms_per_copy": ms, "gbs": 2 * n * 8 / (ms * 1e-3) / 1e9}
 
 
-def ensemble_report(args):
+def ensemble_measure(n: int, seed: int, precision: str, steps: int, warmup: int, dt: float) -> dict:
     """Throughput of an ensemble stepped as one batch against its members stepped one after another."""
     import torch
     from numericalearth_hip.ensemble import Ensemble, build_members, draw_ensemble
 
-    specs = draw_ensemble(args.ensemble, args.seed)
-    members = build_members(specs, precision=args.precision)
+    specs = draw_ensemble(n, seed)
+    members = build_members(specs, precision=precision)
     cells = sum(s.cells for s in specs)
     torch.cuda.synchronize()
 
     def timed(step, k):
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             step()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -128,21 +131,23 @@ def ensemble_report(args):
 
     def sequential():
         for m in members:
-            m.time_step(args.dt)
+            m.time_step(dt)
 
-    t_seq = timed(sequential, args.steps)
+    t_seq = timed(sequential, steps)
     ens = Ensemble(members)
     default = ens.tuning
-    t_default = timed(lambda: ens.time_step(args.dt), args.steps)
+    t_default = timed(lambda: ens.time_step(dt), steps)
     chosen = ens.autotune()
-    t_tuned = timed(lambda: ens.time_step(args.dt), args.steps)
+    t_tuned = timed(lambda: ens.time_step(dt), steps)
     ens.use_graph(True)
-    t_graph = timed(lambda: ens.time_step(args.dt), args.steps)
+    t_graph = timed(lambda: ens.time_step(dt), steps)
     info = ens.info()
     ens.close()
+    for m in members:
+        m.close()
     shapes = [(s.Nx, s.Ny, s.Nz) for s in specs]
-    out = {"report": "ensemble", "members": len(specs), "cells": cells, "precision": args.precision,
-           "steps": args.steps, "dt": args.dt,
+    return {"report": "ensemble", "members": len(specs), "cells": cells, "precision": precision,
+           "steps": steps, "dt": dt,
            "smallest": min(shapes, key=lambda s: s[0] * s[1] * s[2]),
            "largest": max(shapes, key=lambda s: s[0] * s[1] * s[2]),
            "sequential_ms_per_step": 1e3 * t_seq, "sequential_cell_updates_per_s": cells / t_seq,
@@ -150,10 +155,14 @@ def ensemble_report(args):
            "batched_graph_ms_per_step": 1e3 * t_graph, "batched_graph_cell_updates_per_s": cells / t_graph,
            "speedup_graph_over_sequential": t_seq / t_graph, "default_tuning": default, "autotuned": chosen,
            "work_items": info}
-    print(json.dumps(out), flush=True)
 
 
-def serving_report(args):
+def ensemble_report(args):
+    print(json.dumps(ensemble_measure(args.ensemble, args.seed, args.precision, args.steps, args.warmup, args.dt)),
+          flush=True)
+
+
+def serving_measure(args, rates) -> dict:
     """Continuous-batching server against the plain per-request path under Poisson arrivals.
 
     Requests are 100-step solves whose grids are drawn log-uniform between 1×1×50 and
@@ -168,7 +177,6 @@ def serving_report(args):
     rng = np.random.default_rng(args.seed)
     pool = draw_ensemble(args.serve_pool, args.seed, smallest=(1, 1, 50), largest=(720, 280, 10), resolution=0.5)
     cases = [(member_case(s), member_params(s)) for s in pool]
-    rates = [float(x) for x in args.serve.split(",")]
     out = {"report": "serving", "steps_per_request": args.serve_steps, "dt": args.dt,
            "budget_bytes": args.serve_budget, "cell_budget": args.serve_cells, "duration_s": args.serve_seconds, "pool": len(pool), "rates": {}}
 
@@ -205,7 +213,35 @@ def serving_report(args):
         print(f"# rate {rate}: plain {plain}", file=sys.stderr, flush=True)
         out["rates"][str(rate)] = {"offered_requests": len(reqs), "batched": batched, "plain": plain,
                                    "throughput_ratio": batched["cell_updates_per_s"] / plain["cell_updates_per_s"]}
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def serving_report(args):
+    print(json.dumps(serving_measure(args, [float(x) for x in args.serve.split(",")])), flush=True)
+
+
+def form_measure(workload: str, precision: str, seed: int, steps: int, warmup: int, dt: float) -> dict:
+    """ms per step of one precision form on the workload (inputs resident in HBM)."""
+    import torch
+    from numericalearth_hip.generators import make_case
+    from numericalearth_hip.model import OceanOnlyModel
+
+    Nx, Ny, Nz = WORKLOADS[workload]
+    m = OceanOnlyModel(make_case(Nx, Ny, Nz, seed=seed), precision=precision)
+    for _ in range(warmup):
+        m.time_step(dt)
+    m.timing_begin(steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m.time_step(dt)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    kernel_ms, nrec = m.timing_end()
+    m.close()
+    return {"precision": precision, "steps": steps, "ms_per_step": 1e3 * el / steps,
+            "cell_updates_per_s": Nx * Ny * Nz * steps / el,
+            "kernels_ms_per_step": {k: v / nrec for k, v in kernel_ms.items()}}
 
 
 def mixed_report(args):
@@ -285,7 +321,7 @@ def cpu_baseline(case, dt: float, target_s: float) -> dict:
 
     threads = min(16, os.cpu_count() or 1)
     t_all, n_all = sample(threads, 3, 10, target_s)
-    t_one, n_one = sample(1, 1, 2, 0.0)
+    t_one, n_one = sample(1, 1, 10, 0.0)
     cpu_model = "unknown"
     try:
         for line in open("/proc/cpuinfo"):
@@ -448,6 +484,25 @@ def main():
     if valu and "valu_busy_rk_stage" in valu:
         out["roofline"]["valu_busy"] = valu["valu_busy_rk_stage"]
     model.close()
+    if args.extras == "on" and args.workload == "quarter_degree_1440x560x10":
+        # the metric's other forms on the same run: the mixed (FP32 storage) step, the 64-member
+        # ensemble against its members stepped one after another, and the serving loop's latency
+        other = "mixed" if args.precision == "fp64" else "fp64"
+        form = form_measure(args.workload, other, args.seed, 100, 10, args.dt)
+        form["speedup_mixed_over_fp64"] = (ms_per_step / form["ms_per_step"] if other == "mixed"
+                                           else form["ms_per_step"] / ms_per_step)
+        out["forms"] = {other: form}
+        ens = ensemble_measure(64, args.seed, "fp64", 20, 3, args.dt)
+        out["ensemble_64"] = {k: ens[k] for k in ("members", "cells", "smallest", "largest", "sequential_ms_per_step",
+                                                  "batched_graph_ms_per_step", "batched_graph_cell_updates_per_s",
+                                                  "speedup_graph_over_sequential")}
+        srv = serving_measure(args, [args.extras_rate])["rates"][str(args.extras_rate)]
+        out["serving"] = {"rate_req_per_s": args.extras_rate, "window_s": args.serve_seconds,
+                          "steps_per_request": args.serve_steps, "requests": srv["offered_requests"],
+                          "p50_ms": srv["batched"]["p50_ms"], "p99_ms": srv["batched"]["p99_ms"],
+                          "cell_updates_per_s": srv["batched"]["cell_updates_per_s"],
+                          "plain_p99_ms": srv["plain"]["p99_ms"],
+                          "throughput_over_plain": srv["throughput_ratio"]}
     copy = copy_bandwidth(2.0, 20)
     out["roofline"]["measured_copy_gbs"] = copy["gbs"]
     out["roofline"]["frac_of_measured_copy"] = achieved / copy["gbs"]

@@ -67,6 +67,24 @@ def _seconds_per_unit(units: str) -> float:
     return _UNIT_SECONDS[m.group(1).lower()]
 
 
+def restrict_interfaces(lo: float, hi: float, interfaces: np.ndarray) -> tuple[int, int]:
+    """Interfaces i0 … i1 (0-based, inclusive) of the cells that keep [lo, hi] of an ascending
+    interface vector: the cell whose centre is at or below lo, the one whose centre is at or above
+    hi, and those between, so the box lies inside the hull of the kept centres; a box edge in a
+    cell's first half, or on a face, takes one more cell, an edge past the centre none; clamped at
+    the vector's ends, at least one cell (metadata_field.jl:47-56)."""
+    f = np.asarray(interfaces, dtype=np.float64)
+    n = len(f)
+    k = min(max(int(np.searchsorted(f, lo, side="right")) - 1, 0), n - 2)
+    i0 = k if 0.5 * (f[k] + f[k + 1]) <= lo else max(k - 1, 0)
+    m = min(max(int(np.searchsorted(f, hi, side="left")), 1), n - 1)
+    i1 = m if 0.5 * (f[m - 1] + f[m]) >= hi else min(m + 1, n - 1)
+    if i1 - i0 < 1:
+        i1 = min(i0 + 1, n - 1)
+        i0 = i1 - 1
+    return i0, i1
+
+
 class JRA55Snapshots:
     """Snapshots of JRA55 variables, one NetCDF file per variable, read one 2-D slice at a time.
 
@@ -79,7 +97,7 @@ class JRA55Snapshots:
     snapshots; the grid comes from the first file's bounds."""
 
     def __init__(self, directory, fields: dict, filename=repeat_year_filename, interval: float | None = 10800.0,
-                 region=None, pad: int = 2):
+                 region=None):
         if region is not None:
             (west, east), (south, north) = region
             if not (east > west and north > south):
@@ -110,7 +128,7 @@ class JRA55Snapshots:
         self._cols, self._rows = np.arange(nx), slice(0, ny)
         self._lon_shift, self._periodic = 0.0, True
         if region is not None:
-            self._restrict(region, int(pad))
+            self._restrict(region)
         self.shape = (len(self.times), self._rows.stop - self._rows.start, len(self._cols))
 
     def _interfaces(self, ds):
@@ -132,32 +150,34 @@ class JRA55Snapshots:
                              [min(90.0, lat[-1] + 0.5 * (lat[-1] - lat[-2]))]])
         return np.append(lf, lf[0] + 360.0), pf
 
-    def _restrict(self, region, pad):
-        """Keep the cells of a bounding box ((west, east), (south, north)) plus `pad` cells on each
-        side (the reference's `region = BoundingBox(...)` of a JRA55 series): rows and columns are
-        read only there, and the grid is the sub-grid, bounded in longitude unless it closes the
-        circle, its longitudes in the box's own convention (a box across 0° given as −20…30
-        stays −20…30 whatever the files' 0…360)."""
+    def _restrict(self, region):
+        """Keep the cells of a bounding box ((west, east), (south, north)) as the reference restricts
+        a series given by interface vectors (restrict(bbox, interfaces::AbstractVector, N),
+        metadata_field.jl:47-56, through restrict_longitude, :66-67): the cells whose centres
+        bracket the box (restrict_interfaces). Rows and columns are read only there, and the grid
+        is the sub-grid, bounded in longitude unless it closes the circle, its longitudes in the
+        box's own convention (a box across 0° given as −20…30 stays −20…30 whatever the files'
+        0…360). A box that crosses the record's seam (the files' first longitude) is continued
+        round the circle: the reference's vector restriction clamps at the record's end there and
+        would leave the part of the box beyond the seam outside its grid."""
         (west, east), (south, north) = region
         lf, pf = self._lf, self._pf
-        nx, ny = len(lf) - 1, len(pf) - 1
-        j0 = max(0, int(np.searchsorted(pf, south, side="right")) - 1 - pad)
-        j1 = min(ny, int(np.searchsorted(pf, north, side="left")) + pad)
+        nx = len(lf) - 1
+        j0, j1 = restrict_interfaces(south, north, pf)
         self._rows = slice(j0, j1)
         if east - west >= 360.0:
             return
         turn = np.floor((west - lf[0]) / 360.0)
         w = west - 360.0 * turn                                 # the box's west edge within the record's turn
-        i0 = int(np.searchsorted(lf, w, side="right")) - 1 - pad
-        ext = lambda i: lf[i % nx] + 360.0 * (i // nx)          # noqa: E731  (interfaces continued round the circle)
-        i1 = i0
-        while ext(i1) < w + (east - west) and i1 - i0 < nx:
-            i1 += 1
-        i1 += pad
-        if i1 - i0 >= nx:                                       # the box and its margins close the circle
+        e = w + (east - west)
+        # the interfaces continued one turn either side: the same cells as the record's own vector
+        # wherever the bracketing stays inside the record, and the neighbours across the seam where not
+        faces = np.concatenate([lf[:-1] - 360.0, lf[:-1], lf + 360.0])
+        i0, i1 = restrict_interfaces(w, e, faces)
+        if i1 - i0 >= nx:                                       # the box closes the circle
             return
         self._cols = np.arange(i0, i1) % nx
-        self._lon_faces = np.array([ext(i) for i in range(i0, i1 + 1)]) + 360.0 * turn
+        self._lon_faces = faces[i0:i1 + 1] + 360.0 * turn
         self._periodic = False
 
     @property

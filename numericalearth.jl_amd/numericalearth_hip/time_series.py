@@ -221,9 +221,24 @@ class SnapshotStream:
         check(load().ne_model_snapshot_resident(self.model.handle, self.which, p), "ne_model_snapshot_resident")
         self.uploads += 1
 
+    def _drop_stale(self, first: int, last: int) -> None:
+        """Give back the staging buffers of read-ahead positions the clock will not read: those
+        before `first` (a step longer than the snapshot interval, or a clock set back) or beyond
+        the read-ahead window after `last`. Their reads are waited for, since the workers are
+        still writing into the buffer."""
+        for p in [q for q in self.reads if q < first or q > last + self.read_ahead]:
+            buf, futs = self.reads.pop(p)
+            for f in futs:
+                f.cancel()
+            for f in futs:
+                if not f.cancelled():
+                    f.exception()               # wait; a failed read of a dropped position is moot
+            self.free.append((buf, None))
+
     def prepare(self, positions, stream: torch.cuda.Stream) -> None:
         """Make the positions read next resident and ordered before `stream`'s next work."""
         need = sorted({int(p) for p in positions})
+        self._drop_stale(need[0], need[-1])
         for p in need:
             if self.slot_position.get(p % self.slots) != p:
                 self._request(p)

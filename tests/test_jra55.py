@@ -114,13 +114,13 @@ def test_region_reads_a_sub_block_across_the_date_line(tmp_path):
     write_jra55_files(tmp_path, data, NLON, NLAT)
     full = JRA55Snapshots(tmp_path, {"T": "tas"})
     box = ((-30.0, 40.0), (-20.0, 30.0))                         # across 0°, files in 0…360
-    sub = JRA55Snapshots(tmp_path, {"T": "tas"}, region=box, pad=1)
+    sub = JRA55Snapshots(tmp_path, {"T": "tas"}, region=box)
     g, gf = sub.grid(), full.grid()
     lam, phi = g.centres()
     lam_f, phi_f = gf.centres()
     assert g.topology_x == NE_BOUNDED and sub.shape == (NT, g.Ny, g.Nx) and g.Nx < NLON and g.Ny < NLAT
-    # the box and a cell of margin inside the sub-grid, its longitudes in the box's convention
-    assert lam[0] < -30.0 < 40.0 < lam[-1] and phi[0] < -20.0 < 30.0 < phi[-1]
+    # the box inside the hull of the kept centres, its longitudes in the box's convention
+    assert lam[0] <= -30.0 < 40.0 <= lam[-1] and phi[0] <= -20.0 < 30.0 <= phi[-1]
     assert np.all(np.diff(lam) > 0)
     cols = np.round((lam % 360.0) / 22.5).astype(int) % NLON
     rows = np.searchsorted(phi_f, phi)
@@ -137,3 +137,44 @@ def test_region_reads_a_sub_block_across_the_date_line(tmp_path):
         JRA55Snapshots(tmp_path, {"T": "tas"}, region=((10.0, 10.0), (0.0, 5.0)))
     for s in (full, sub, ring):
         s.close()
+
+
+def test_region_cells_bracket_the_box_with_centres():
+    """restrict_interfaces keeps the cells whose centres bracket [lo, hi] and no more: the first kept
+    centre is at or below lo and the second above it, the last at or above hi and the one before below
+    it; an edge on a face or in a cell's first half takes that neighbour, an edge past the centre does
+    not; the ends of the vector clamp (metadata_field.jl:28-33, :47-56)."""
+    from numericalearth_hip.jra55 import restrict_interfaces
+    rng = np.random.default_rng(3)
+    faces = np.cumsum(np.concatenate([[-80.0], rng.uniform(0.5, 2.0, 120)]))   # stretched, like JRA55's latitudes
+    centres = 0.5 * (faces[1:] + faces[:-1])
+    for _ in range(500):
+        lo, hi = np.sort(rng.uniform(faces[2], faces[-3], 2))
+        i0, i1 = restrict_interfaces(lo, hi, faces)
+        c = centres[i0:i1]
+        assert c[0] <= lo and c[-1] >= hi                       # the kept centres bracket the box
+        assert len(c) == 1 or (c[1] > lo and c[-2] < hi)        # and no cell more than that needs
+    k = 40
+    mid = centres[k]
+    assert restrict_interfaces(faces[k], mid + 0.1, faces)[0] == k - 1          # edge on a face: one more
+    assert restrict_interfaces(0.5 * (faces[k] + mid), mid + 0.1, faces)[0] == k - 1   # first half: one more
+    assert restrict_interfaces(mid + 1e-9, mid + 0.1, faces)[0] == k              # past the centre: none
+    assert restrict_interfaces(mid, mid + 0.1, faces)[0] == k                     # on the centre: none
+    assert restrict_interfaces(faces[0] - 5.0, faces[3], faces)[0] == 0           # clamped at the ends
+    assert restrict_interfaces(faces[-4], faces[-1] + 5.0, faces)[1] == len(faces) - 1
+    assert restrict_interfaces(-200.0, -150.0, faces) == (0, 1)                   # outside: one cell
+
+
+def test_region_across_the_seam_continues_round_the_circle(tmp_path):
+    """A box that crosses the files' first longitude keeps the cells on both sides of it (the
+    reference's vector restriction would clamp at the record's end), bracketed the same way."""
+    rng = np.random.default_rng(6)
+    data = _fields(rng, ["tas"])
+    write_jra55_files(tmp_path, data, NLON, NLAT)
+    full = JRA55Snapshots(tmp_path, {"T": "tas"})
+    sub = JRA55Snapshots(tmp_path, {"T": "tas"}, region=((350.0, 380.0), (-10.0, 10.0)))
+    lam, _ = sub.grid().centres()
+    assert lam[0] <= 350.0 and lam[-1] >= 380.0 and np.all(np.diff(lam) > 0)
+    assert np.allclose(np.diff(lam), 360.0 / NLON)
+    full.close()
+    sub.close()

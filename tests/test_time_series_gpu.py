@@ -74,6 +74,40 @@ def test_streamed_window_steps_like_resident_series(require_gpu, precision, inde
     win.close()
 
 
+class _SlowSource:
+    """A snapshot source whose reads take a while (a file system under load)."""
+
+    def __init__(self, inner, delay=0.002):
+        self.inner, self.delay = inner, delay
+        self.n_times, self.times, self.shape = inner.n_times, inner.times, inner.shape
+
+    def read(self, n, name, out):
+        import time
+        time.sleep(self.delay)
+        return self.inner.read(n, name, out)
+
+
+def test_stream_survives_skipped_positions(require_gpu):
+    """Steps three snapshot intervals long skip positions the read-ahead had already requested:
+    their staging buffers are given back (before, once a handful of positions had been skipped no
+    buffer was left free and the next request failed)."""
+    from numericalearth_hip.time_series import ArraySnapshots
+    case = _case(9, seed=13)
+    src = _SlowSource(ArraySnapshots(case.atmos, case.atmos_times))
+    full = OceanOnlyModel(case, time_indexing=Cyclical())
+    win = OceanOnlyModel(case, time_indexing=Cyclical(), atmosphere_window=3, atmosphere_source=src)
+    st = win.streams["atmosphere"]
+    print(f"[window] dt = 3 snapshot intervals, staging buffers {st.n_staging}", flush=True)
+    for n in range(24):
+        full.time_step(3 * HOUR)
+        win.time_step(3 * HOUR)
+        _assert_same(win, full, f"after step {n + 1}")
+        assert len(st.free) + len(st.reads) == st.n_staging
+        assert len(st.reads) <= st.read_ahead, f"stale reads held after step {n + 1}: {sorted(st.reads)}"
+    full.close()
+    win.close()
+
+
 def test_window_streams_from_an_npy_store(require_gpu, tmp_path):
     case = _case(7, seed=12)
     atm = NpySnapshotStore(NpySnapshotStore.write(tmp_path / "atm", case.atmos, case.atmos_times), tuple(case.atmos))

@@ -432,6 +432,31 @@ int ne_tracer_rk_stage_f64(ne_ctx* ctx, const ne_grid* grid, const ne_params* pa
                            ne_view T_out, ne_view S_out,
                            const ne_net_ocean_fluxes* net, double beta_dt, void* stream);
 
+/* FP32 forms for models on Float32 storage (the mixed form's convention, the precision form of
+ * the tracer stencil and the interpolation): prescribed series and 3-D fields (u, v, w, T, S, G,
+ * stage buffers) are float; fractional indices, the exchange state and every 2-D flux are double.
+ * The interpolation is formed in FP64 from the float series; the WENO5 reconstructions run in FP32;
+ * the stage update c0 + beta_dt·G is formed in FP64 and rounded to float to nearest.
+ * ne_interp_atmos_f32 / ne_interp_radiation_f32 replace the kernels of
+ * interpolate_atmospheric_state.jl:91-137 and interpolate_radiation_state.jl:43-69 on Float32
+ * series; ne_tracer_tendency_weno5_f32 / ne_tracer_rk_stage_f32 the tendency and stage of
+ * ocean_simulation.jl:65-94, :347 (Oceananigans) on a Float32 grid. */
+int ne_interp_atmos_f32(ne_ctx* ctx, const ne_grid* grid, const ne_prescribed_atmosphere* atmos,
+                        const double* fi, const double* fj,
+                        const ne_exchange_state* out, void* stream);
+int ne_interp_radiation_f32(ne_ctx* ctx, const ne_grid* grid, const ne_prescribed_radiation* rad,
+                            const double* fi, const double* fj,
+                            ne_view downwelling_shortwave, ne_view downwelling_longwave,
+                            void* stream);
+int ne_tracer_tendency_weno5_f32(ne_ctx* ctx, const ne_grid* grid, const ne_params* params,
+                                 const ne_ocean_state* ocean, const ne_net_ocean_fluxes* net,
+                                 ne_view G_T, ne_view G_S, void* stream);
+int ne_tracer_rk_stage_f32(ne_ctx* ctx, const ne_grid* grid, const ne_params* params,
+                           ne_view u, ne_view v, ne_view w,
+                           ne_view T0, ne_view S0, ne_view T_in, ne_view S_in,
+                           ne_view T_out, ne_view S_out,
+                           const ne_net_ocean_fluxes* net, double beta_dt, void* stream);
+
 /* Vertical velocity from continuity, for callers that evolve u and v: replaces Oceananigans'
  * compute_w_from_continuity! of the HydrostaticFreeSurfaceModel that ocean_simulation builds
  * (ocean_simulation.jl:487-499; Oceananigans not vendored). Per column, w = 0 at the bottom

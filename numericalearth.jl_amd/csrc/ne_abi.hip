@@ -462,7 +462,7 @@ int ne_tracer_tendency_weno5_f64(ne_ctx* c, const ne_grid* g, const ne_params* p
         return r;
     const ne_view* tr[] = {&o->T, &o->S, &GT, &GS};
     if ((r = check_tracer_views(g, tr, 4, o->u, o->v, o->w, net, sizeof(double)))) return r;
-    return dev_err(launch_tendency(grid_dev(*g), view<const double>(o->u), view<const double>(o->v),
+    return dev_err(launch_tendency<double>(grid_dev(*g), view<const double>(o->u), view<const double>(o->v),
                                    view<const double>(o->w), view<const double>(o->T), view<const double>(o->S),
                                    view<const double>(net->J_T), view<const double>(net->J_S),
                                    view<const double>(net->J_w), tracer_forcing(p, *net), view<double>(GT),
@@ -487,6 +487,64 @@ int ne_tracer_rk_stage_f64(ne_ctx* c, const ne_grid* g, const ne_params* p, ne_v
                                            view<double>(Sout), view<const double>(net->J_T),
                                            view<const double>(net->J_S), view<const double>(net->J_w),
                                            tracer_forcing(p, *net), beta_dt, false, 0.0, as_stream(stream)));
+}
+
+// FP32 forms of the per-phase entry points (the mixed form's convention): FP32 prescribed series
+// and 3-D fields, FP64 fractional indices, exchange state and 2-D fluxes; arithmetic as in the
+// mixed model (FP32 WENO5, FP64-formed stage update rounded once to FP32).
+int ne_interp_atmos_f32(ne_ctx* c, const ne_grid* g, const ne_prescribed_atmosphere* a, const double* fi,
+                        const double* fj, const ne_exchange_state* out, void* stream) {
+    if (!c || !out || !fi || !fj) return NE_ERR_NULL;
+    int r;
+    if ((r = check_grid(g)) || (r = check_atmos(a))) return r;
+    if (!out->u.data || !out->v.data || !out->T.data || !out->p.data || !out->q.data) return NE_ERR_NULL;
+    return dev_err(launch_interp_atmos<float>(grid_dev(*g), atmos_dev<float>(*a), fi, fj, *out, as_stream(stream)));
+}
+
+int ne_interp_radiation_f32(ne_ctx* c, const ne_grid* g, const ne_prescribed_radiation* rad, const double* fi,
+                            const double* fj, ne_view sw, ne_view lw, void* stream) {
+    if (!c || !fi || !fj) return NE_ERR_NULL;
+    int r;
+    if ((r = check_grid(g)) || (r = check_rad(rad))) return r;
+    if (!sw.data || !lw.data) return NE_ERR_NULL;
+    return dev_err(launch_interp_radiation<float>(grid_dev(*g), rad_dev<float>(*rad), fi, fj, view<double>(sw),
+                                                  view<double>(lw), as_stream(stream)));
+}
+
+int ne_tracer_tendency_weno5_f32(ne_ctx* c, const ne_grid* g, const ne_params* p, const ne_ocean_state* o,
+                                 const ne_net_ocean_fluxes* net, ne_view GT, ne_view GS, void* stream) {
+    if (!c) return NE_ERR_NULL;
+    int r;
+    if ((r = check_grid(g)) || (r = check_ocean(o)) || (r = check_net(net)) || (r = check_view3(GT)) ||
+        (r = check_view3(GS)) || (r = check_forcing(g, p, net)))
+        return r;
+    const ne_view* tr[] = {&o->T, &o->S, &GT, &GS};
+    if ((r = check_tracer_views(g, tr, 4, o->u, o->v, o->w, net, sizeof(float)))) return r;
+    return dev_err(launch_tendency<float>(grid_dev(*g), view<const float>(o->u), view<const float>(o->v),
+                                          view<const float>(o->w), view<const float>(o->T), view<const float>(o->S),
+                                          view<const double>(net->J_T), view<const double>(net->J_S),
+                                          view<const double>(net->J_w), tracer_forcing(p, *net), view<float>(GT),
+                                          view<float>(GS), as_stream(stream)));
+}
+
+int ne_tracer_rk_stage_f32(ne_ctx* c, const ne_grid* g, const ne_params* p, ne_view u, ne_view v, ne_view w,
+                           ne_view T0, ne_view S0, ne_view Tin, ne_view Sin, ne_view Tout, ne_view Sout,
+                           const ne_net_ocean_fluxes* net, double beta_dt, void* stream) {
+    if (!c) return NE_ERR_NULL;
+    int r;
+    if ((r = check_grid(g)) || (r = check_net(net)) || (r = check_forcing(g, p, net))) return r;
+    const ne_view* vs[] = {&u, &v, &w, &T0, &S0, &Tin, &Sin, &Tout, &Sout};
+    for (const ne_view* x : vs)
+        if ((r = check_view3(*x))) return r;
+    if (Tin.data == Tout.data || Sin.data == Sout.data) return NE_ERR_SHAPE;
+    const ne_view* tr[] = {&T0, &S0, &Tin, &Sin, &Tout, &Sout};
+    if ((r = check_tracer_views(g, tr, 6, u, v, w, net, sizeof(float)))) return r;
+    return dev_err(launch_rk_stage<float>(grid_dev(*g), view<const float>(u), view<const float>(v),
+                                          view<const float>(w), view<const float>(T0), view<const float>(S0),
+                                          view<const float>(Tin), view<const float>(Sin), view<float>(Tout),
+                                          view<float>(Sout), view<const double>(net->J_T),
+                                          view<const double>(net->J_S), view<const double>(net->J_w),
+                                          tracer_forcing(p, *net), beta_dt, false, 0.0, as_stream(stream)));
 }
 
 int ne_compute_w_from_continuity_f64(ne_ctx* c, const ne_grid* g, ne_view u, ne_view v, ne_view w, void* stream) {

@@ -161,10 +161,11 @@ __host__ __device__ inline uint32_t step_seed(double time) {
     x ^= x >> 16;
     return x;
 }
-hipError_t launch_tendency(const GridDev& g, View3<const double> u, View3<const double> v, View3<const double> w,
-                           View3<const double> T, View3<const double> S, View3<const double> JT,
-                           View3<const double> JS, View3<const double> Jw, TracerForcing forcing, View3<double> GT,
-                           View3<double> GS, hipStream_t s);
+template <typename Real>
+hipError_t launch_tendency(const GridDev& g, View3<const Real> u, View3<const Real> v, View3<const Real> w,
+                           View3<const Real> T, View3<const Real> S, View3<const double> JT,
+                           View3<const double> JS, View3<const double> Jw, TracerForcing forcing, View3<Real> GT,
+                           View3<Real> GS, hipStream_t s);
 // Vertical velocity from continuity (all Nz+1 faces of w).
 template <typename Real>
 hipError_t launch_w_continuity(const GridDev& g, View3<const Real> u, View3<const Real> v, View3<Real> w,

@@ -690,21 +690,30 @@ template hipError_t launch_rk_stage<float>(const GridDev&, View3<const float>, V
                                            View3<const double>, View3<const double>, View3<const double>,
                                            TracerForcing, double, bool, double, hipStream_t);
 
-hipError_t launch_tendency(const GridDev& g, View3<const double> u, View3<const double> v, View3<const double> w,
-                           View3<const double> T, View3<const double> S, View3<const double> JT,
-                           View3<const double> JS, View3<const double> Jw, TracerForcing forcing, View3<double> GT,
-                           View3<double> GS, hipStream_t s) {
-    TracerLayout L = host_layout<double>(T, u, v, w, JT);
+template <typename Real>
+hipError_t launch_tendency(const GridDev& g, View3<const Real> u, View3<const Real> v, View3<const Real> w,
+                           View3<const Real> T, View3<const Real> S, View3<const double> JT,
+                           View3<const double> JS, View3<const double> Jw, TracerForcing forcing, View3<Real> GT,
+                           View3<Real> GS, hipStream_t s) {
+    TracerLayout L = host_layout<Real>(T, u, v, w, JT);
     if (forcing.two_color)
-        hipLaunchKernelGGL((tracer_tile_kernel<double, 1, true>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, PLAIN_TR), 0,
+        hipLaunchKernelGGL((tracer_tile_kernel<Real, 1, true>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, PLAIN_TR), 0,
                            s, g, L, u.p, v.p, w.p, T.p, S.p, T.p, S.p, GT.p, GS.p, JT.p, JS.p, Jw.p, forcing, 0.0, 0,
                            0u);
     else
-        hipLaunchKernelGGL((tracer_tile_kernel<double, 1, false>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, PLAIN_TR),
+        hipLaunchKernelGGL((tracer_tile_kernel<Real, 1, false>), dim3(grid_blocks(g.Nx, g.Ny)), dim3(TW, PLAIN_TR),
                            0, s, g, L, u.p, v.p, w.p, T.p, S.p, T.p, S.p, GT.p, GS.p, JT.p, JS.p, Jw.p, forcing, 0.0,
                            0, 0u);
     return hipGetLastError();
 }
+template hipError_t launch_tendency<double>(const GridDev&, View3<const double>, View3<const double>,
+                                            View3<const double>, View3<const double>, View3<const double>,
+                                            View3<const double>, View3<const double>, View3<const double>,
+                                            TracerForcing, View3<double>, View3<double>, hipStream_t);
+template hipError_t launch_tendency<float>(const GridDev&, View3<const float>, View3<const float>,
+                                           View3<const float>, View3<const float>, View3<const float>,
+                                           View3<const double>, View3<const double>, View3<const double>,
+                                           TracerForcing, View3<float>, View3<float>, hipStream_t);
 
 template <typename Real>
 hipError_t launch_budget(ne_ctx* ctx, const GridDev& g, View3<const Real> c, double weight, double* out,

@@ -70,6 +70,12 @@ SIGNATURES = {
     "ne_model_snapshots_at": [vp, C.c_double, P(C.c_int64)],
     "ne_model_snapshot_resident": [vp, C.c_int32, C.c_int64],
     "ne_compute_w_from_continuity_f64": [vp, P(abi.Grid), abi.View, abi.View, abi.View, vp],
+    "ne_interp_atmos_f32": [vp, P(abi.Grid), P(abi.PrescribedAtmosphereDesc), vp, vp, P(abi.ExchangeState), vp],
+    "ne_interp_radiation_f32": [vp, P(abi.Grid), P(abi.PrescribedRadiationDesc), vp, vp, abi.View, abi.View, vp],
+    "ne_tracer_tendency_weno5_f32": [vp, P(abi.Grid), P(abi.Params), P(abi.OceanState), P(abi.NetOceanFluxes),
+                                     abi.View, abi.View, vp],
+    "ne_tracer_rk_stage_f32": [vp, P(abi.Grid), P(abi.Params)] + [abi.View] * 9 + [P(abi.NetOceanFluxes),
+                                                                                 C.c_double, vp],
     "ne_batch_create": [vp, C.c_int32, C.c_int, P(vp), P(vp), vp],
     "ne_batch_destroy": [vp],
     "ne_batch_add": [vp, vp],

@@ -515,7 +515,9 @@ typedef struct ne_model_desc {
     int32_t n_radiation_times;
     const double* radiation_times;        /* HOST */
     void*   workspace;                    /* optional caller-owned device memory for the RK stage
-                                             buffers and freeze flags (ne_model_workspace_bytes);
+                                             buffers, freeze flags and rounding residual
+                                             (ne_model_workspace_bytes), 256-byte aligned
+                                             (NE_ERR_SHAPE otherwise);
                                              NULL: the model allocates them itself */
     int64_t workspace_bytes;
     /* Snapshot time indexing (FieldTimeSeries time_indexing; JRA55 atmospheres are Cyclical,
@@ -538,9 +540,10 @@ typedef struct ne_model_desc {
  * two are interpolated across the wrap. */
 enum { NE_TIME_CLAMPED = 0, NE_TIME_CYCLICAL = 1 };
 
-/* Bytes of device workspace a model of this descriptor needs: four RK stage buffers of T, S
- * and one byte per column of freeze flags (the step's last stage marks the columns holding a
- * cell below its freezing point, and the freeze clamp after the step visits only those). */
+/* Bytes of device workspace a model of this descriptor needs: four RK stage buffers of T, S,
+ * one byte per column of freeze flags (the step's last stage marks the columns holding a
+ * cell below its freezing point, and the freeze clamp after the step visits only those), and
+ * the rounding residual (ne_model_rounding_residual: four doubles per 63×4 tile). */
 int ne_model_workspace_bytes(const ne_model_desc* desc, int64_t* bytes);
 
 typedef struct ne_model ne_model;
@@ -602,6 +605,20 @@ int ne_model_snapshot_resident(ne_model* model, int32_t which, int64_t position)
 
 /* Clock (seconds) and iteration count of the model. Host only. */
 int ne_model_clock(const ne_model* model, double* time, int64_t* iteration);
+
+/* Rounding residual of a NE_MIXED_FP32 model's state: since creation, Σ V·(x − stored) over the
+ * active cells of every step's last RK stage, for T (out[0] + out[1], K·m³) and S (out[2] +
+ * out[3], psu·m³), each a compensated (sum, compensation) pair, where x is the stage's FP64
+ * update and `stored` the float written (the last stage differences its FP32 face fluxes and forms
+ * x in FP64, so Σ V·(x − c⁰) telescopes to the boundary fluxes). The budget of the FP32 state
+ * plus this residual is conserved to FP64 round-off: ρ₀c₀·(Σ V·T + out[0] + out[1]) changes by
+ * −Δt·ρ₀c₀·Σ Az·Jᵀ per step, as the FP64 state's heat content does
+ * (test/test_tracer_budget.jl:66-76 checks that closure for the reference). Zero for an FP64
+ * model, whose update is stored exactly. The last stage adds each tile's share to the tile's
+ * running totals, so stepping carries no reduction; this call sums the tiles (one kernel, in a
+ * fixed order) into `out`, device memory of 4 doubles, in stream order on `stream`. Steps taken
+ * in a batch (ne_batch_time_step) add to the same totals. */
+int ne_model_rounding_residual(const ne_model* model, double* out, void* stream);
 
 /* Per-kernel timing of ne_model_time_step with HIP events recorded on the step's own
  * stream (no counterpart in the reference, whose examples time whole steps in callbacks,

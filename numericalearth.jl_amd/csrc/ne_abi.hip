@@ -236,8 +236,15 @@ size_t freeze_flag_bytes(const ne_model_desc& d) {
     return ((size_t)d.grid.Nx * (size_t)d.grid.Ny + 255) & ~(size_t)255;
 }
 
-// Device workspace of a model: the four RK stage buffers, then the freeze flags.
-size_t model_workspace_bytes(const ne_model_desc& d) { return 4 * stage_buffer_bytes(d) + freeze_flag_bytes(d); }
+// The model's rounding residual (totals and tile partials), rounded up to 256 bytes.
+size_t residual_bytes(const ne_model_desc& d) {
+    return ((size_t)residual_slots(grid_dev(d.grid)) * RESIDUAL_SLOT * sizeof(double) + 255) & ~(size_t)255;
+}
+
+// Device workspace of a model: the four RK stage buffers, the freeze flags, the rounding residual.
+size_t model_workspace_bytes(const ne_model_desc& d) {
+    return 4 * stage_buffer_bytes(d) + freeze_flag_bytes(d) + residual_bytes(d);
+}
 
 }  // namespace
 
@@ -651,6 +658,7 @@ static int model_ocean_step(ne_model* m, double dt, hipStream_t s, hipEvent_t* e
     F3.freeze_flags = m->freeze_flags;
     F3.melt0 = d.params.freshwater_melting_temperature;
     F3.liquidus_slope = d.params.liquidus_slope;
+    if (sr) F3.residual = m->residual;   // and an FP32 state's rounding residual
     e = launch_rk_stage<Real>(g, u, v, w, T0, S0, view<const Real>(m->vT2), view<const Real>(m->vS2),
                               view<Real>(d.ocean.T), view<Real>(d.ocean.S), JT, JS, Jw, F3, beta[2] * dt, sr, m->time,
                               s);
@@ -711,6 +719,10 @@ int ne_model_create(ne_ctx* c, const ne_model_desc* desc, ne_model** out, void* 
     void** bufs[4] = {&m->T1, &m->S1, &m->T2, &m->S2};
     ne_view* views[4] = {&m->vT1, &m->vS1, &m->vT2, &m->vS2};
     if (desc->workspace) {
+        if ((reinterpret_cast<uintptr_t>(desc->workspace) & 255) != 0) {   // its buffers start 256-aligned
+            delete m;
+            return NE_ERR_SHAPE;
+        }
         if (desc->workspace_bytes < (int64_t)model_workspace_bytes(*desc)) {
             delete m;
             return NE_ERR_CAPACITY;
@@ -718,13 +730,17 @@ int ne_model_create(ne_ctx* c, const ne_model_desc* desc, ne_model** out, void* 
         m->owns_buffers = false;
         for (int b = 0; b < 4; b++) *bufs[b] = static_cast<char*>(desc->workspace) + b * n;
         m->freeze_flags = reinterpret_cast<uint8_t*>(static_cast<char*>(desc->workspace) + 4 * n);
-    } else if (hipMalloc(reinterpret_cast<void**>(&m->freeze_flags), freeze_flag_bytes(*desc)) != hipSuccess) {
+        m->residual = reinterpret_cast<double*>(static_cast<char*>(desc->workspace) + 4 * n +
+                                                freeze_flag_bytes(*desc));
+    } else if (hipMalloc(reinterpret_cast<void**>(&m->freeze_flags), freeze_flag_bytes(*desc)) != hipSuccess ||
+               hipMalloc(reinterpret_cast<void**>(&m->residual), residual_bytes(*desc)) != hipSuccess) {
         ne_model_destroy(m);
         return NE_ERR_DEVICE;
     }
-    // every column is visited until a step's last stage has flagged them (creation is never
-    // inside a captured graph, so the fill may be a memset)
-    if (hipMemsetAsync(m->freeze_flags, 1, freeze_flag_bytes(*desc), as_stream(stream)) != hipSuccess) {
+    // every column is visited until a step's last stage has flagged them; the residual starts
+    // at zero (creation is never inside a captured graph, so the fills may be memsets)
+    if (hipMemsetAsync(m->freeze_flags, 1, freeze_flag_bytes(*desc), as_stream(stream)) != hipSuccess ||
+        hipMemsetAsync(m->residual, 0, residual_bytes(*desc), as_stream(stream)) != hipSuccess) {
         ne_model_destroy(m);
         return NE_ERR_DEVICE;
     }
@@ -766,6 +782,7 @@ int ne_model_destroy(ne_model* m) {
         for (void* b : bufs)
             if (b && hipFree(b) != hipSuccess) r = NE_ERR_DEVICE;
         if (m->freeze_flags && hipFree(m->freeze_flags) != hipSuccess) r = NE_ERR_DEVICE;
+        if (m->residual && hipFree(m->residual) != hipSuccess) r = NE_ERR_DEVICE;
     }
     for (hipEvent_t e : m->events) (void)hipEventDestroy(e);
     delete m;
@@ -879,6 +896,11 @@ int ne_model_snapshot_resident(ne_model* m, int32_t which, int64_t position) {
     if (m->d.time_indexing != NE_TIME_CYCLICAL && (position < 0 || position >= nt)) return NE_ERR_SHAPE;
     slot[(size_t)floor_mod(position, (int64_t)slot.size())] = position;
     return NE_OK;
+}
+
+int ne_model_rounding_residual(const ne_model* m, double* out, void* stream) {
+    if (!m || !out) return NE_ERR_NULL;
+    return dev_err(launch_residual_reduce(m->residual, grid_dev(m->d.grid), out, as_stream(stream)));
 }
 
 int ne_model_clock(const ne_model* m, double* time, int64_t* iteration) {

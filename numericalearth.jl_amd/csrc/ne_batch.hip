@@ -172,6 +172,7 @@ BatchRec<Real> member_record(const ne_model* m, const double* atimes_dev, const 
     r.ro = rad_out(d.radiative);
     r.frazil = view<double>(d.frazil_heat);
     r.freeze_flags = m->freeze_flags;
+    r.residual = m->residual;
     r.atimes = atimes_dev;
     r.rtimes = rtimes_dev;
     r.nat = (int)m->atimes.size();

@@ -26,6 +26,9 @@ struct ne_model {
     // one byte per column, written by the step's last RK stage: 1 where a cell of the column is
     // below its freezing point (or not known), so the freeze clamp that follows visits only those
     uint8_t* freeze_flags = nullptr;
+    // rounding residual of the FP32 state (residual_bytes): per tile, the running compensated
+    // totals Σ V·(x − stored) of T and S (sum, compensation, sum, compensation)
+    double* residual = nullptr;
     double time = 0.0;
     double last_dt = 0.0;
     int64_t iteration = 0;
@@ -74,10 +77,14 @@ struct TracerForcing {
     // cell of the column is below T_m = melt0 − liquidus_slope·S (the freeze clamp's test)
     uint8_t* freeze_flags;
     double melt0, liquidus_slope;
+    // rounding residual of an FP32 state's last stage (nullptr: none): the stage forms its
+    // update conservatively in FP64 and adds Σ V·(x − stored) of T and S over the tile's cells
+    // to the tile's slot (the kernel offsets the pointer to it)
+    double* residual;
 };
 
 inline TracerForcing tracer_forcing(const ne_params* p, const ne_net_ocean_fluxes& n) {
-    TracerForcing f = {0, 0.0, 0.0, 0.0, nullptr, nullptr, 0.0, 0.0};
+    TracerForcing f = {0, 0.0, 0.0, 0.0, nullptr, nullptr, 0.0, 0.0, nullptr};
     if (p && p->radiative_forcing == NE_RADIATIVE_FORCING_TWO_COLOR) {
         f.two_color = 1;
         f.first_fraction = p->first_color_fraction;
@@ -180,6 +187,12 @@ double* budget_noalloc_scratch(int device);
 int64_t budget_blocks(const GridDev& g);
 hipError_t launch_budget_noalloc(double* partial, const GridDev& g, View3<const double> c, double weight,
                                  double* out, hipStream_t s);
+// Rounding residual of an FP32 state: one slot per tile (T sum, T compensation, S sum,
+// S compensation), into which the block of the step's last stage adds the tile's Σ V·(x −
+// stored); the number of slots a model keeps, and their fixed-order sum into out[4].
+constexpr int RESIDUAL_SLOT = 4;
+int64_t residual_slots(const GridDev& g);
+hipError_t launch_residual_reduce(const double* residual, const GridDev& g, double* out, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Batched ensemble: one device record per member model, read by persistent kernels
@@ -207,6 +220,7 @@ struct BatchRec {
     RadOut ro;
     View3<double> frazil;
     uint8_t* freeze_flags;  // the member's freeze flags (written by stage 3, read by K2)
+    double* residual;       // the member's rounding residual (running totals per tile)
     const double* atimes;   // device copies of the snapshot times
     const double* rtimes;
     int nat, nrt;

@@ -18,6 +18,8 @@
 // two-colour forcing carry none of its registers.
 #include "ne_internal.h"
 
+#include <type_traits>
+
 #include <mutex>
 
 // Block partials of the allocation-free budget (ne_budget_f64_noalloc): one (sum,
@@ -41,9 +43,13 @@ constexpr int TC = 63;            // cells per tile row
 // scratch waits for every load in flight), so the targets are the largest occupancies whose
 // register budget fits without spills, measured on 1440×560×10: FP64 4 (≈116–125 VGPRs;
 // 5 spills and runs 30 % slower), FP32 7 (≤ 72 VGPRs, a 12-byte spill, faster than 5 or 6)
-// and 5 for the FP32 instances with the two-colour forcing (≈ 93 VGPRs).
+// and 5 for the FP32 instances with the two-colour forcing (≈ 93 VGPRs). The FP32 state's last
+// stage, which forms its update in FP64, takes 6 (≤ 80 VGPRs).
 #ifndef NE_STAGE_WAVES_PER_EU_F32
 #define NE_STAGE_WAVES_PER_EU_F32 7
+#endif
+#ifndef NE_STAGE_WAVES_PER_EU_F32_LAST
+#define NE_STAGE_WAVES_PER_EU_F32_LAST 6
 #endif
 #ifndef NE_STAGE_WAVES_PER_EU_F32_TWO_COLOR
 #define NE_STAGE_WAVES_PER_EU_F32_TWO_COLOR 5
@@ -51,10 +57,12 @@ constexpr int TC = 63;            // cells per tile row
 #ifndef NE_STAGE_WAVES_PER_EU_F64
 #define NE_STAGE_WAVES_PER_EU_F64 4
 #endif
-template <typename Real, bool TWO_COLOR>
+template <typename Real, bool TWO_COLOR, bool LAST>
 constexpr int stage_waves() {
     return sizeof(Real) == 8 ? NE_STAGE_WAVES_PER_EU_F64
-                             : (TWO_COLOR ? NE_STAGE_WAVES_PER_EU_F32_TWO_COLOR : NE_STAGE_WAVES_PER_EU_F32);
+           : TWO_COLOR       ? NE_STAGE_WAVES_PER_EU_F32_TWO_COLOR
+           : LAST            ? NE_STAGE_WAVES_PER_EU_F32_LAST
+                             : NE_STAGE_WAVES_PER_EU_F32;
 }
 constexpr int PLAIN_TR = NE_PLAIN_TR;  // cell rows per tile of the single-model kernel
 constexpr int NXCD = 8;
@@ -72,6 +80,7 @@ struct TileShared {
     R fyT[(TR + 1) * TC], fyS[(TR + 1) * TC];
     int km[PN];
     int kstart;
+    double resid[2 * TR];   // the rows' rounding residuals of T and S (FP32 state's last stage)
 };
 
 __host__ __device__ inline int tiles_x(int Nx) { return (Nx + TC - 1) / TC; }
@@ -110,33 +119,26 @@ __device__ __forceinline__ uint32_t hash32(uint32_t x) {
     return x;
 }
 
-// The float next to r towards +∞ (up) or −∞, for finite r.
-__device__ __forceinline__ float next_float(float r, bool up) {
-    if (r == 0.0f) return __uint_as_float(up ? 1u : 0x80000001u);
-    uint32_t b = __float_as_uint(r);
-    const bool positive = (b >> 31) == 0;
-    return __uint_as_float(positive == up ? b + 1u : b - 1u);
-}
-
 // Rounding of the new FP64 state value x to the storage type. FP32 storage of the model
-// state (the mixed form) rounds stochastically: to one of the two neighbouring floats with
+// state (the mixed form) rounds stochastically: to one of the two floats around x with
 // probabilities that make the expected stored value x, drawn from a hash of the cell, the
 // tracer and the step (`seed`), so results are reproducible and identical on every path
 // that takes the same step. Round-to-nearest would drop the same fraction of an ulp every
 // step wherever the increment is steady, a bias that grows linearly with the step count.
+// The draw adds 29 random bits h (uniform on [0, 2²⁹)) to the 29 bits of x's significand
+// that a float does not hold and clears them: |x| rounds up to the next float in magnitude
+// exactly when h ≥ 2²⁹ − f, f the cleared bits, i.e. with probability f·2⁻²⁹, the distance
+// to the float below over the gap (a carry into the exponent is the step to the next binade),
+// and the result converts to float exactly (for |x| within the normal floats).
 template <typename R>
 __device__ __forceinline__ R round_state(double x, uint32_t key, uint32_t seed, bool stochastic) {
-    R r = (R)x;
     if (sizeof(R) == 4 && stochastic) {
-        const double d = x - (double)r;           // exact: |d| ≤ ½ ulp of r
-        if (d != 0.0) {
-            const float other = next_float((float)r, d > 0.0);
-            const double p = fabs(d) / fabs((double)other - (double)r);
-            const double u = (double)(hash32(hash32(key) ^ seed) >> 8) * (1.0 / 16777216.0);
-            if (u < p) r = (R)other;
-        }
+        constexpr uint64_t LOW = (1ull << 29) - 1;
+        const uint64_t h = (uint64_t)(hash32(hash32(key) ^ seed) >> 3);
+        const uint64_t b = ((uint64_t)__double_as_longlong(x) + h) & ~LOW;
+        return (R)__longlong_as_double((long long)b);
     }
-    return r;
+    return (R)x;
 }
 
 // A wave-uniform value as a scalar (the tile kernels' blocks are 64 wide, so every value that
@@ -160,6 +162,22 @@ __device__ __forceinline__ T ldb(const T* base, uint32_t off) {
 template <typename T>
 __device__ __forceinline__ void stb(T* base, uint32_t off, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
+}
+
+// Double-double accumulation (sum, compensation): the budgets and the rounding residual.
+struct DD {
+    double s, c;
+};
+
+__device__ __forceinline__ DD dd_add(DD a, DD b) {
+    double t = a.s + b.s;
+    double e;
+    if (fabs(a.s) >= fabs(b.s)) e = (a.s - t) + b.s;
+    else e = (b.s - t) + a.s;
+    DD r;
+    r.s = t;
+    r.c = (a.c + b.c) + e;
+    return r;
 }
 
 // One tile (columns i0 … i0+62, rows j0 … j0+TR-1) marched over its active levels.
@@ -254,6 +272,13 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
     // freeze watch (the step's last stage; instances without it carry none of its code)
     const bool watch = WATCH && MODE == 0 && F.freeze_flags != nullptr;   // uniform
     bool frz = false;
+    // Conservative last stage of an FP32 state: the face fluxes (FP32) are differenced and the
+    // update formed in FP64, so Σ V·(x − c⁰) telescopes to the boundary fluxes to FP64
+    // round-off, and Σ V·(x − stored) — exact per cell, as x and the stored float are within
+    // an ulp — is summed for the model's rounding residual.
+    constexpr bool CONS = WATCH && MODE == 0 && sizeof(R) == 4;
+    const bool resid = CONS && F.residual != nullptr;   // uniform
+    double resT = 0.0, resS = 0.0;
     if (kstart >= Nz) {  // uniform: the whole tile is land
         if (watch && cell) F.freeze_flags[(int64_t)ci + (int64_t)cj * g.Nx] = 0;
         return;
@@ -404,15 +429,16 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
 
         // 3. difference the face fluxes, add the top boundary flux, update
         if (cell && k >= mykmin) {
+            typedef typename std::conditional<CONS, double, R>::type D;   // arithmetic of the update
             const int xo = row * TW + lane, yo = row * TC + lane;
-            R div_T = (sh.fxT[xo + 1] - sh.fxT[xo]) + (sh.fyT[yo + TC] - sh.fyT[yo]);
-            div_T = div_T + (FztT - FzT);
-            R div_S = (sh.fxS[xo + 1] - sh.fxS[xo]) + (sh.fyS[yo + TC] - sh.fyS[yo]);
-            div_S = div_S + (FztS - FzS);
-            const R V = (R)(Azc * dzd);
-            const R rV = reciprocal(V);
-            R GT = -(rV * div_T);
-            R GS = -(rV * div_S);
+            D div_T = ((D)sh.fxT[xo + 1] - (D)sh.fxT[xo]) + ((D)sh.fyT[yo + TC] - (D)sh.fyT[yo]);
+            div_T = div_T + ((D)FztT - (D)FzT);
+            D div_S = ((D)sh.fxS[xo + 1] - (D)sh.fxS[xo]) + ((D)sh.fyS[yo + TC] - (D)sh.fyS[yo]);
+            div_S = div_S + ((D)FztS - (D)FzS);
+            const D V = (D)(Azc * dzd);
+            const D rV = reciprocal(V);
+            D GT = -(rV * div_T);
+            D GS = -(rV * div_S);
             if (tc) {
                 // ε₁ ∂z J₁ + (1 − ε₁) ∂z J₂; the face below the lowest active cell carries no flux
                 // (the level's two faces; uniform over the tile, recomputed rather than carried)
@@ -424,7 +450,7 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                 const double dz = dzd;
                 const double dJ1dz = (a1 - b1) / dz;
                 const double dJ2dz = (a2 - b2) / dz;
-                GT = GT + (R)(F.first_fraction * dJ1dz + (1.0 - F.first_fraction) * dJ2dz);
+                GT = GT + (D)(F.first_fraction * dJ1dz + (1.0 - F.first_fraction) * dJ2dz);
             }
             if (k == Nz - 1) {
                 // MultipleFluxes(Jᵀ, FreshwaterExchange{:T}) = Jᵀ + 0 on a static grid;
@@ -433,22 +459,28 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
                 const uint32_t jo = joff();
                 double FT = ldb(JT, jo) + (0.0 + 0.0);
                 double FS = ldb(JS, jo) + (((double)zS[2] * ldb(Jw, jo) - 0.0) + 0.0);
-                GT = GT - (R)(FT * Azc) * rV;
-                GS = GS - (R)(FS * Azc) * rV;
+                GT = GT - (D)(FT * Azc) * rV;
+                GS = GS - (D)(FS * Azc) * rV;
             }
             if (MODE == 0) {
                 // The update is formed in FP64 and rounded once to the storage type (for FP32
                 // storage this keeps the per-stage rounding of the state at half an ulp).
                 const uint32_t key = 2u * ((uint32_t)ci + (uint32_t)g.Nx * ((uint32_t)cj + (uint32_t)g.Ny * k));
-                const R Tn = round_state<R>((double)nT0 + beta_dt * (double)GT, key, seed, stochastic);
-                const R Sn = round_state<R>((double)nS0 + beta_dt * (double)GS, key + 1u, seed, stochastic);
+                const double xT = (double)nT0 + beta_dt * (double)GT;
+                const double xS = (double)nS0 + beta_dt * (double)GS;
+                const R Tn = round_state<R>(xT, key, seed, stochastic);
+                const R Sn = round_state<R>(xS, key + 1u, seed, stochastic);
                 stb(Tout, coff + kc, Tn);
                 stb(Sout, coff + kc, Sn);
+                if (resid) {
+                    resT += (double)V * (xT - (double)Tn);
+                    resS += (double)V * (xS - (double)Sn);
+                }
                 // the freeze clamp's test on the stored values (freeze_column, ne_surface.hip)
                 if (watch) frz = frz || (double)Tn < F.melt0 - F.liquidus_slope * (double)Sn;
             } else {
-                stb(Tout, coff + kc, GT);
-                stb(Sout, coff + kc, GS);
+                stb(Tout, coff + kc, (R)GT);
+                stb(Sout, coff + kc, (R)GS);
             }
         }
         load_c0(kn);
@@ -459,13 +491,38 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
         for (int n = 0; n < 5; n++) { zT[n] = zT[n + 1]; zS[n] = zS[n + 1]; }
     }
     if (watch && cell) F.freeze_flags[(int64_t)ci + (int64_t)cj * g.Nx] = frz ? 1 : 0;
+    if (resid) {   // the tile's residuals: each row's by a fixed butterfly, then the rows in order
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            resT += __shfl_xor(resT, off);
+            resS += __shfl_xor(resS, off);
+        }
+        if (lane == 0) {
+            sh.resid[2 * row] = resT;
+            sh.resid[2 * row + 1] = resS;
+        }
+        __syncthreads();
+        if (tid == 0) {   // into the tile's running double-double totals
+            double sT = 0.0, sS = 0.0;
+            for (int r = 0; r < TR; r++) {
+                sT += sh.resid[2 * r];
+                sS += sh.resid[2 * r + 1];
+            }
+            const DD T = dd_add(DD{F.residual[0], F.residual[1]}, DD{sT, 0.0});
+            const DD S = dd_add(DD{F.residual[2], F.residual[3]}, DD{sS, 0.0});
+            F.residual[0] = T.s;
+            F.residual[1] = T.c;
+            F.residual[2] = S.s;
+            F.residual[3] = S.c;
+        }
+    }
 }
 
 // Single model: one block per tile, tiles ordered so that each XCD (blocks are dealt
 // round-robin to the 8 XCDs) receives a contiguous band of tiles.
 template <typename Real, int MODE, bool TWO_COLOR, bool WATCH = false>
 __global__ void __launch_bounds__(TW * PLAIN_TR)
-__attribute__((amdgpu_waves_per_eu(stage_waves<Real, TWO_COLOR>())))
+__attribute__((amdgpu_waves_per_eu(stage_waves<Real, TWO_COLOR, WATCH && MODE == 0>())))
 tracer_tile_kernel(GridDev g, TracerLayout L, const Real* u, const Real* v, const Real* w, const Real* T0,
                    const Real* S0, const Real* Tin, const Real* Sin, Real* Tout, Real* Sout, const double* JT,
                    const double* JS, const double* Jw, TracerForcing F, double beta_dt, int stochastic,
@@ -476,6 +533,7 @@ tracer_tile_kernel(GridDev g, TracerLayout L, const Real* u, const Real* v, cons
     const int per = (ntiles + NXCD - 1) / NXCD;
     const int tile = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
     if (tile >= ntiles) return;
+    if (F.residual) F.residual += RESIDUAL_SLOT * tile;   // this tile's totals
     tile_body<Real, MODE, PLAIN_TR, TWO_COLOR, WATCH>(sh, g, L, u, v, w, T0, S0, Tin, Sin, Tout, Sout, JT, JS, Jw, F,
                                                      beta_dt, (tile % ntx) * TC, (tile / ntx) * PLAIN_TR,
                                                      stochastic != 0, seed);
@@ -487,7 +545,7 @@ tracer_tile_kernel(GridDev g, TracerLayout L, const Real* u, const Real* v, cons
 // stage 1: T¹ = T + β₁Δt G(T); stage 2: T² = T + β₂Δt G(T¹); stage 3: T = T + β₃Δt G(T²).
 template <typename Real, int TR, bool TWO_COLOR, bool WATCH>
 __global__ void __launch_bounds__(TW * TR)
-__attribute__((amdgpu_waves_per_eu(stage_waves<Real, TWO_COLOR>())))
+__attribute__((amdgpu_waves_per_eu(stage_waves<Real, TWO_COLOR, WATCH>())))
 batch_stage_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __restrict__ work, int nwork,
                    int stage, double beta_dt) {
     __shared__ TileShared<Real, TR> sh;
@@ -505,7 +563,7 @@ batch_stage_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __re
         const Real* Sin = stage == 1 ? rc.S.p : stage == 2 ? rc.S1.p : rc.S2.p;
         Real* Tout = stage == 1 ? rc.T1.p : stage == 2 ? rc.T2.p : rc.T.p;
         Real* Sout = stage == 1 ? rc.S1.p : stage == 2 ? rc.S2.p : rc.S.p;
-        TracerForcing F = {0, 0.0, 0.0, 0.0, nullptr, nullptr, 0.0, 0.0};
+        TracerForcing F = {0, 0.0, 0.0, 0.0, nullptr, nullptr, 0.0, 0.0, nullptr};
         if (rc.p.radiative_forcing == NE_RADIATIVE_FORCING_TWO_COLOR) {
             F.two_color = 1;
             F.first_fraction = rc.p.first_color_fraction;
@@ -517,6 +575,7 @@ batch_stage_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __re
             F.freeze_flags = rc.freeze_flags;
             F.melt0 = rc.p.freshwater_melting_temperature;
             F.liquidus_slope = rc.p.liquidus_slope;
+            if (sizeof(Real) == 4 && rc.residual) F.residual = rc.residual + RESIDUAL_SLOT * it.index;
         }
         __syncthreads();   // the previous item's reads of the LDS are complete
         tile_body<Real, 0, TR, TWO_COLOR, WATCH>(sh, g, L, rc.u.p, rc.v.p, rc.w.p, rc.T.p, rc.S.p, Tin, Sin, Tout, Sout, rc.n.J_T.p,
@@ -560,21 +619,6 @@ constexpr int BX = 64;
 constexpr int BY = 4;
 
 inline dim3 grid2d(int Nx, int Ny) { return dim3((Nx + BX - 1) / BX, (Ny + BY - 1) / BY); }
-
-struct DD {
-    double s, c;
-};
-
-__device__ __forceinline__ DD dd_add(DD a, DD b) {
-    double t = a.s + b.s;
-    double e;
-    if (fabs(a.s) >= fabs(b.s)) e = (a.s - t) + b.s;
-    else e = (b.s - t) + a.s;
-    DD r;
-    r.s = t;
-    r.c = (a.c + b.c) + e;
-    return r;
-}
 
 template <typename Real>
 __global__ void budget_partial_kernel(GridDev g, View3<const Real> c, double weight, double* partial) {
@@ -633,6 +677,44 @@ __global__ void budget_final_kernel(const double* partial, int64_t n, double* ou
     if (t == 0) {
         out[0] = ss[0];
         out[1] = sc[0];
+    }
+}
+
+// Rounding residual of an FP32 state: the tiles' running totals summed in a fixed order
+// (strided over the work-items, then a tree) into out (T sum, T compensation, S sum,
+// S compensation).
+constexpr int RB = 1024;   // work-items of the reduction
+__global__ void __launch_bounds__(RB) residual_reduce_kernel(const double* residual, int nslots, double* out) {
+    __shared__ double ts[RB], tc[RB], ss[RB], sc[RB];
+    const int t = threadIdx.x;
+    const double4* slot = reinterpret_cast<const double4*>(residual);
+    DD aT = {0.0, 0.0}, aS = {0.0, 0.0};
+    for (int b = t; b < nslots; b += RB) {
+        const double4 v = slot[b];
+        aT = dd_add(aT, DD{v.x, v.y});
+        aS = dd_add(aS, DD{v.z, v.w});
+    }
+    ts[t] = aT.s;
+    tc[t] = aT.c;
+    ss[t] = aS.s;
+    sc[t] = aS.c;
+    __syncthreads();
+    for (int w = RB / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            const DD rT = dd_add(DD{ts[t], tc[t]}, DD{ts[t + w], tc[t + w]});
+            const DD rS = dd_add(DD{ss[t], sc[t]}, DD{ss[t + w], sc[t + w]});
+            ts[t] = rT.s;
+            tc[t] = rT.c;
+            ss[t] = rS.s;
+            sc[t] = rS.c;
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = ts[0];
+        out[1] = tc[0];
+        out[2] = ss[0];
+        out[3] = sc[0];
     }
 }
 
@@ -772,6 +854,17 @@ int64_t budget_blocks(const GridDev& g) {
 
 // The kernels of launch_budget with the partials in the device's `scratch` array; allocates
 // nothing. Every block overwrites its own pair, so nothing needs clearing between calls.
+// One slot of 4 doubles per tile of the tiling with the most tiles (the fewest rows: the
+// single-model tiling or the batch's 4-row one); a batch of taller tiles uses the first slots.
+int64_t residual_slots(const GridDev& g) {
+    return (int64_t)tiles_x(g.Nx) * tiles_y(g.Ny, PLAIN_TR < 4 ? PLAIN_TR : 4);
+}
+
+hipError_t launch_residual_reduce(const double* residual, const GridDev& g, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(residual_reduce_kernel, dim3(1), dim3(RB), 0, s, residual, (int)residual_slots(g), out);
+    return hipGetLastError();
+}
+
 hipError_t launch_budget_noalloc(double* partial, const GridDev& g, View3<const double> c, double weight,
                                  double* out, hipStream_t s) {
     const dim3 gr = grid2d(g.Nx, g.Ny);

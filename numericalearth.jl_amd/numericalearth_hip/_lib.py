@@ -61,6 +61,7 @@ SIGNATURES = {
     "ne_model_time_step": [vp, C.c_double, vp],
     "ne_model_update_state": [vp, vp],
     "ne_model_clock": [vp, P(C.c_double), P(C.c_int64)],
+    "ne_model_rounding_residual": [vp, vp, vp],
     "ne_model_timing_begin": [vp, C.c_int],
     "ne_model_timing_end": [vp, P(C.c_double), P(C.c_int)],
     "ne_model_workspace_bytes": [P(abi.ModelDesc), P(C.c_int64)],

@@ -315,6 +315,20 @@ class OceanOnlyModel:
     def heat_content(self) -> float:
         return self.budget(self.T, self.params.ocean_reference_density * self.params.ocean_heat_capacity)
 
+    def rounding_residual(self, stream=None) -> tuple[float, float]:
+        """Σ V·(x − stored) of T and S over the steps so far (zero in FP64): what the FP32 state of
+        the mixed form has lost to rounding, so Σ V·T + this is conserved to FP64 round-off."""
+        out = torch.empty(4, dtype=torch.float64, device=self.T.device)
+        check(self.lib.ne_model_rounding_residual(self.handle, out.data_ptr(), stream_handle(stream)),
+              "ne_model_rounding_residual")
+        t, tc, s, sc = out.tolist()
+        return t + tc, s + sc
+
+    def conserved_heat(self) -> float:
+        """Heat content of the state plus ρ₀c₀ times the rounding residual of T."""
+        rho_c = self.params.ocean_reference_density * self.params.ocean_heat_capacity
+        return self.heat_content() + rho_c * self.rounding_residual()[0]
+
     def close(self) -> None:
         for st in getattr(self, "streams", {}).values():
             st.close()

@@ -4,7 +4,8 @@
 * Config 3: the mixed form at 1440×560×10 over 1000 steps of 20 min against the FP64 path (which
   equals the oracle to 1e-12 per step). The workload's own sensitivity is measured beside it: the
   FP64 path started from the FP32-rounded initial state. The mixed form adds no error beyond what
-  rounding its inputs to FP32 already causes, and its SST differs by ≤ 1e-5 in the L2 norm.
+  rounding its inputs to FP32 already causes, its SST differs by ≤ 1e-5 in the L2 norm, and its
+  heat budget (FP32 state plus rounding residual) closes to 1e-11 of the surface heat flux.
 * Config 4: the 64-member ensemble (sizes log-uniform between 1×1×50 and 360×140×20, each with its
   own bathymetry, atmosphere, Charnock parameter and roughness) stepped as one batch equals the
   members stepped alone bit for bit, FP64 and mixed; its smallest, median and largest FP64 members
@@ -75,7 +76,15 @@ def test_config3_mixed_form_at_quarter_degree(require_gpu):
     m32 = OceanOnlyModel(case, precision="mixed")
     r64 = OceanOnlyModel(rounded, precision="fp64")
     H0 = {"fp64": m64.heat_content(), "mixed": m32.heat_content()}
+    C0 = m32.conserved_heat()
+    rho_c = m32.params.ocean_reference_density * m32.params.ocean_heat_capacity
+    Az = torch.from_numpy(case.grid.Az).to("cuda")[:, None]
+    expected = torch.zeros((), dtype=torch.float64, device="cuda")   # Σ −Δt ρ₀c₀ Az Jᵀ of the mixed form
+    scale = torch.zeros_like(expected)                               # Σ Δt ρ₀c₀ Az |Jᵀ|
     for n in range(steps):
+        JT = m32.net["J_T"]
+        expected -= DT * rho_c * (JT * Az).sum()
+        scale += DT * rho_c * (JT.abs() * Az).sum()
         for m in (m64, m32, r64):
             m.time_step(DT)
         if (n + 1) % 250 == 0:
@@ -91,8 +100,15 @@ def test_config3_mixed_form_at_quarter_degree(require_gpu):
         flux[k] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
     dH64 = m64.heat_content() - H0["fp64"]
     dH32 = m32.heat_content() - H0["mixed"]
+    C1 = m32.conserved_heat()
+    closure = abs((C1 - C0) - float(expected)) / float(scale)
     print(f"[config3] after {steps} steps: mixed SST {mixed}, FP32-rounded start {sensitivity}, "
-          f"fluxes {flux}, heat gained FP64 {dH64:.6e} J mixed {dH32:.6e} J", flush=True)
+          f"fluxes {flux}, heat gained FP64 {dH64:.6e} J mixed {dH32:.6e} J; mixed heat budget (state + "
+          f"rounding residual) closes to {closure:.2e} of Σ|surface heat flux| (one FP64 ulp of the heat "
+          f"content is {np.spacing(abs(C1)) / float(scale):.2e} of it)", flush=True)
+    # conservation to FP64 round-off: the reference's budget test asks rtol 1e-11
+    # (test_tracer_budget.jl:129)
+    assert closure <= 1e-11
     assert mixed["l2"] <= 1e-5
     # the max norm is the workload's sensitivity to rounding its initial state to FP32: the mixed
     # form stays within twice what that rounding alone causes
@@ -137,6 +153,11 @@ def test_config4_ensemble_of_64(require_gpu, precision):
             sa, sb = _state(a), _state(b)
             for k in sa:
                 assert np.array_equal(sa[k], sb[k], equal_nan=True), (precision, step, n, shapes[n], k)
+            # the same stored values leave the same rounding residual, summed over the batch's tiles
+            ra, rb = a.rounding_residual(), b.rounding_residual()
+            assert np.allclose(ra, rb, rtol=1e-12, atol=0.0), (precision, step, n, ra, rb)
+            if precision == "fp64":
+                assert ra == (0.0, 0.0), (n, ra)      # an FP64 state stores its update exactly
         for k, om in oracles.items():
             om.time_step(DT)
             assert_rel(_np(batched[k].T), om.T, RTOL, f"member {k} {shapes[k]} T step {step + 1}")

@@ -353,13 +353,6 @@ int ne_grid_immersed_levels(ne_ctx* ctx, const ne_grid* grid, ne_view bottom_hei
 int ne_fractional_indices_f64(ne_ctx* ctx, const ne_grid* grid, const ne_source_grid* source,
                               double* fi, double* fj, void* stream);
 
-/* The flux solve's FP64 natural logarithm applied elementwise to x[n] (DEVICE arrays):
- * y[i] = log(x[i]) within 2 ulp for positive, finite, normal x (other arguments are not
- * defined). It has no counterpart in the reference; it exposes the library's own logarithm
- * (argument reduction to [√½, √2) and the atanh series, in place of the math library's) so
- * its accuracy can be checked against a correctly rounded one. */
-int ne_log_f64(ne_ctx* ctx, const double* x, double* y, int64_t n, void* stream);
-
 /* Fractional indices in a source grid given by its cell-centre coordinates instead of a
  * regular spacing: lambda_nodes[nx] (longitudes, degrees) and phi_nodes[ny] (latitudes), DEVICE
  * arrays in ascending order. The reference builds the JRA55 forcing grid from the files'

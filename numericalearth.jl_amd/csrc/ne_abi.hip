@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "ne_internal.h"
+#include "numericalearth_hip_testing.h"
 
 using namespace ne;
 

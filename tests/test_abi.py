@@ -13,6 +13,7 @@ from numericalearth_hip import _lib
 from numericalearth_hip.build import library_path
 
 HEADER = REPO / "include" / "numericalearth_hip.h"
+HEADERS = sorted((REPO / "include").glob("*.h"))   # the product header and the test-only one
 
 STRUCTS = {
     "ne_view": abi.View, "ne_grid": abi.Grid, "ne_source_grid": abi.SourceGrid, "ne_time_interp": abi.TimeInterp,
@@ -24,10 +25,17 @@ STRUCTS = {
 }
 
 
-def declared_functions():
-    text = HEADER.read_text()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*int\s+(ne_\w+)\s*\(", text, flags=re.M)))
+def declared_functions(headers=HEADERS):
+    names = set()
+    for h in headers:
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        names |= set(re.findall(r"^\s*int\s+(ne_\w+)\s*\(", text, flags=re.M))
+    return sorted(names)
+
+
+def test_test_hooks_stay_out_of_the_product_header():
+    assert "ne_log_f64" not in declared_functions([HEADER])
+    assert "ne_log_f64" in declared_functions()
 
 
 def test_library_exports_every_declared_entry_point():

@@ -59,14 +59,37 @@ def parse():
     ap.add_argument("--extras", choices=("on", "off"), default="on",
                     help="also measure the other precision form, the 64-member ensemble and one serving rate")
     ap.add_argument("--extras-rate", type=float, default=200.0, help="request rate of the serving measurement")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="before the timed steps, launch a kernel of known traffic (5 arrays read, 2 written, "
+                         "each the size of one stage array) three times, for memory counters to be read against")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="only run a device-to-device copy of this many GiB (--steps times) and report its bandwidth")
     return ap.parse_args()
 
 
+def traffic_probe(model, array_bytes: int, launches: int = 3):
+    """ne_stream_probe_f64 with 5 inputs and 2 outputs of `array_bytes` each: a launch moves exactly
+    7·array_bytes of HBM (5 read, 2 written), the reference for reading memory counters."""
+    import ctypes as C
+    import torch
+    n = array_bytes // 8
+    arrays = [torch.rand(n, dtype=torch.float64, device="cuda") for _ in range(7)]
+    ins = (C.c_void_p * 5)(*[a.data_ptr() for a in arrays[:5]])
+    outs = (C.c_void_p * 2)(*[a.data_ptr() for a in arrays[5:]])
+    for _ in range(launches):
+        r = model.lib.ne_stream_probe_f64(model.ctx.handle, ins, 5, outs, 2, n, None)
+        if r:
+            raise RuntimeError(f"ne_stream_probe_f64 failed with ne_status {r}")
+    torch.cuda.synchronize()
+    expect = sum(arrays[:5])
+    assert torch.equal(arrays[5], expect) and torch.equal(arrays[6], expect + 1.0)
+    del arrays
+
+
 def measured_traffic(workload: str, precision: str):
-    """HBM bytes per launch of the RK stage kernel from the committed FETCH_SIZE/WRITE_SIZE
-    summary (calibrated against a device copy of known size), or None."""
+    """HBM bytes per launch of the RK stage kernel from the committed memory-counter summary: the
+    measured read requests at the 128 bytes per request that ne_stream_probe_f64 (a kernel of known
+    traffic, profiled in the same runs) establishes, plus the kernel's stores; or None."""
     import re
     f = REPO / "profiles" / "tracer_rk_stage_traffic_1440x560x10.md"
     if workload != "quarter_degree_1440x560x10" or not f.exists():
@@ -419,6 +442,8 @@ def main():
     for _ in range(args.warmup):
         model.time_step(args.dt)
     torch.cuda.synchronize()
+    if args.traffic_probe:
+        traffic_probe(model, Nx * Ny * Nz * (8 if args.precision == "fp64" else 4))
     model.ctx.counters(reset=True)
     model.timing_begin(args.steps)
     torch.cuda.synchronize()

@@ -1,7 +1,8 @@
 /*
  * numericalearth_hip_testing.h — entry points of libnumericalearth_hip that exist for tests
- * of the library's own numerics and have no counterpart in the reference. They are exported by
- * the same library; a caller of the coupled step never needs them.
+ * of the library's own numerics and for measurements of its kernels, and have no counterpart in
+ * the reference. They are exported by the same library; a caller of the coupled step never needs
+ * them.
  */
 #ifndef NUMERICALEARTH_HIP_TESTING_H
 #define NUMERICALEARTH_HIP_TESTING_H
@@ -18,6 +19,16 @@ extern "C" {
  * atanh series, in place of the math library's) so its accuracy can be checked against a
  * correctly rounded one. */
 int ne_log_f64(ne_ctx* ctx, const double* x, double* y, int64_t n, void* stream);
+
+/* A kernel of known HBM traffic, to read hardware memory counters against: out_j[i] =
+ * Σ_k in_k[i] + j for i < n, coalesced and i fastest, so one launch reads n_in·n·8 bytes and
+ * writes n_out·n·8 bytes. `in` and `out` are HOST arrays of n_in ≤ NE_STREAM_PROBE_MAX_IN and
+ * n_out ≤ NE_STREAM_PROBE_MAX_OUT DEVICE pointers to n doubles each (distinct, not
+ * overlapping). */
+#define NE_STREAM_PROBE_MAX_IN 8
+#define NE_STREAM_PROBE_MAX_OUT 4
+int ne_stream_probe_f64(ne_ctx* ctx, const double* const* in, int32_t n_in, double* const* out, int32_t n_out,
+                        int64_t n, void* stream);
 
 #ifdef __cplusplus
 }

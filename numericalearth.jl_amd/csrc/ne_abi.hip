@@ -385,6 +385,19 @@ int ne_log_f64(ne_ctx* c, const double* x, double* y, int64_t n, void* stream) {
     return dev_err(launch_log(x, y, n, as_stream(stream)));
 }
 
+int ne_stream_probe_f64(ne_ctx* c, const double* const* in, int32_t n_in, double* const* out, int32_t n_out,
+                        int64_t n, void* stream) {
+    if (!c || !in || !out) return NE_ERR_NULL;
+    if (n_in < 0 || n_in > NE_STREAM_PROBE_MAX_IN || n_out < 1 || n_out > NE_STREAM_PROBE_MAX_OUT || n < 0)
+        return NE_ERR_SHAPE;
+    for (int k = 0; k < n_in; k++)
+        if (!in[k]) return NE_ERR_NULL;
+    for (int j = 0; j < n_out; j++)
+        if (!out[j]) return NE_ERR_NULL;
+    if (n == 0) return NE_OK;
+    return dev_err(launch_stream_probe(in, n_in, out, n_out, n, as_stream(stream)));
+}
+
 int ne_fractional_indices_nodes_f64(ne_ctx* c, const ne_grid* g, const double* lambda_nodes, int32_t nx,
                                     int32_t topology_x, const double* phi_nodes, int32_t ny, int32_t topology_y,
                                     double* fi, double* fj, void* stream) {

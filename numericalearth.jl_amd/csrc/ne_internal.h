@@ -191,6 +191,9 @@ hipError_t launch_budget_noalloc(double* partial, const GridDev& g, View3<const 
 // S compensation), into which the block of the step's last stage adds the tile's Σ V·(x −
 // stored); the number of slots a model keeps, and their fixed-order sum into out[4].
 constexpr int RESIDUAL_SLOT = 4;
+// Traffic probe (ne_stream_probe_f64): n_in ≤ NE_STREAM_PROBE_MAX_IN arrays read, n_out written.
+hipError_t launch_stream_probe(const double* const* in, int n_in, double* const* out, int n_out, int64_t n,
+                               hipStream_t s);
 int64_t residual_slots(const GridDev& g);
 hipError_t launch_residual_reduce(const double* residual, const GridDev& g, double* out, hipStream_t s);
 

@@ -17,7 +17,9 @@
 // SIMD, the FP32-storage kernel 7 (5 with the two-colour forcing); instances without the
 // two-colour forcing carry none of its registers.
 #include "ne_internal.h"
+#include "numericalearth_hip_testing.h"
 
+#include <algorithm>
 #include <type_traits>
 
 #include <mutex>
@@ -718,7 +720,37 @@ __global__ void __launch_bounds__(RB) residual_reduce_kernel(const double* resid
     }
 }
 
+// Traffic probe: out_j[i] = Σ_k in_k[i] + j over n doubles, i fastest (one double per
+// work-item, grid-stride), so a launch moves exactly (n_in + n_out)·n·8 bytes of HBM.
+struct StreamProbeArgs {
+    const double* in[NE_STREAM_PROBE_MAX_IN];
+    double* out[NE_STREAM_PROBE_MAX_OUT];
+    int n_in, n_out;
+    int64_t n;
+};
+
+__global__ void __launch_bounds__(256) stream_probe_kernel(StreamProbeArgs a) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * 256) {
+        double s = 0.0;
+        for (int k = 0; k < a.n_in; k++) s += a.in[k][i];
+        for (int j = 0; j < a.n_out; j++) a.out[j][i] = s + (double)j;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_stream_probe(const double* const* in, int n_in, double* const* out, int n_out, int64_t n,
+                               hipStream_t s) {
+    StreamProbeArgs a = {};
+    for (int k = 0; k < n_in; k++) a.in[k] = in[k];
+    for (int j = 0; j < n_out; j++) a.out[j] = out[j];
+    a.n_in = n_in;
+    a.n_out = n_out;
+    a.n = n;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(stream_probe_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
 
 template <typename Real>
 TracerLayout host_layout(const View3<const Real>& c, const View3<const Real>& u, const View3<const Real>& v,

@@ -38,6 +38,7 @@ SIGNATURES = {
     "ne_grid_immersed_levels": [vp, P(abi.Grid), abi.View, vp, vp],
     "ne_fractional_indices_f64": [vp, P(abi.Grid), P(abi.SourceGrid), vp, vp, vp],
     "ne_log_f64": [vp, vp, vp, C.c_int64, vp],
+    "ne_stream_probe_f64": [vp, vp, C.c_int32, vp, C.c_int32, C.c_int64, vp],
     "ne_fractional_indices_nodes_f64": [vp, P(abi.Grid), vp, C.c_int32, C.c_int32, vp, C.c_int32, C.c_int32, vp, vp,
                                         vp],
     "ne_interp_atmos_f64": [vp, P(abi.Grid), P(abi.PrescribedAtmosphereDesc), vp, vp, P(abi.ExchangeState), vp],

@@ -56,3 +56,23 @@ def test_log_exact_values(require_gpu):
     assert np.max(_ulps(y[k != 0], ref[k != 0])) <= 1.0
     assert y[k == 0][0] == 0.0 and _device_log(np.array([1.0]))[0] == 0.0
     assert np.isnan(_device_log(np.array([np.nan]))[0])
+
+
+@pytest.mark.parametrize("n_in,n_out,n", [(5, 2, 1000003), (0, 1, 77), (8, 4, 4096)])
+def test_stream_probe_writes_the_sums(require_gpu, n_in, n_out, n):
+    """ne_stream_probe_f64, the kernel of known traffic the memory counters are read against:
+    out_j = Σ_k in_k + j, summed in order from 0 (so equal to torch's same-order sum)."""
+    import ctypes as C
+    ctx = Context.get(0)
+    g = torch.Generator(device="cuda").manual_seed(n)
+    ins = [torch.rand(n, dtype=torch.float64, device="cuda", generator=g) for _ in range(n_in)]
+    outs = [torch.full((n,), -1.0, dtype=torch.float64, device="cuda") for _ in range(n_out)]
+    pin = (C.c_void_p * max(n_in, 1))(*[a.data_ptr() for a in ins])
+    pout = (C.c_void_p * n_out)(*[a.data_ptr() for a in outs])
+    check(ctx.lib.ne_stream_probe_f64(ctx.handle, pin, n_in, pout, n_out, n, stream_handle()), "ne_stream_probe_f64")
+    torch.cuda.synchronize()
+    s = torch.zeros(n, dtype=torch.float64, device="cuda")
+    for a in ins:
+        s = s + a
+    for j, o in enumerate(outs):
+        assert torch.equal(o, s + float(j)), j

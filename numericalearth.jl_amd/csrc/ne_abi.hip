@@ -242,9 +242,16 @@ size_t residual_bytes(const ne_model_desc& d) {
     return ((size_t)residual_slots(grid_dev(d.grid)) * RESIDUAL_SLOT * sizeof(double) + 255) & ~(size_t)255;
 }
 
-// Device workspace of a model: the four RK stage buffers, the freeze flags, the rounding residual.
+// The deferred-column list of the FP32 flux solve: a count and one int per column, rounded up to
+// 256 bytes.
+size_t deferred_bytes(const ne_model_desc& d) {
+    return ((1 + (size_t)d.grid.Nx * (size_t)d.grid.Ny) * sizeof(int) + 255) & ~(size_t)255;
+}
+
+// Device workspace of a model: the four RK stage buffers, the freeze flags, the rounding residual,
+// the deferred-column list.
 size_t model_workspace_bytes(const ne_model_desc& d) {
-    return 4 * stage_buffer_bytes(d) + freeze_flag_bytes(d) + residual_bytes(d);
+    return 4 * stage_buffer_bytes(d) + freeze_flag_bytes(d) + residual_bytes(d) + deferred_bytes(d);
 }
 
 }  // namespace
@@ -629,18 +636,18 @@ static int model_update_state(ne_model* m, double dt_frazil, hipStream_t s, hipE
         e = launch_surface_update<double>(g, d.params, a, d.atmos_fi, d.atmos_fj, r, d.radiation_fi, d.radiation_fj,
                                           view<double>(d.ocean.T), view<const double>(d.ocean.S),
                                           view<const double>(d.ocean.u), view<const double>(d.ocean.v), f, n, ro,
-                                          frazil, dt_frazil, flags, m->ctx->counters, s);
+                                          frazil, dt_frazil, flags, m->ctx->counters, nullptr, s);
     } else {
         AtmosDev<float> a = atmos_dev<float>(d.atmosphere);
         RadDev<float> r = rad_dev<float>(d.radiation);
         e = launch_surface_update<float>(g, d.params, a, d.atmos_fi, d.atmos_fj, r, d.radiation_fi, d.radiation_fj,
                                          view<float>(d.ocean.T), view<const float>(d.ocean.S),
                                          view<const float>(d.ocean.u), view<const float>(d.ocean.v), f, n, ro, frazil,
-                                         dt_frazil, flags, m->ctx->counters, s);
+                                         dt_frazil, flags, m->ctx->counters, m->deferred, s);
     }
     if (e != hipSuccess) return NE_ERR_DEVICE;
     if (ev && hipEventRecord(ev[4], s) != hipSuccess) return NE_ERR_DEVICE;
-    e = launch_surface_tau(g, d.params, f, n, s);
+    e = launch_surface_tau(g, d.params, f, n, d.precision == NE_FP64 ? nullptr : m->deferred, s);
     if (e == hipSuccess && ev && hipEventRecord(ev[5], s) != hipSuccess) return NE_ERR_DEVICE;
     return dev_err(e);
 }
@@ -746,15 +753,19 @@ int ne_model_create(ne_ctx* c, const ne_model_desc* desc, ne_model** out, void* 
         m->freeze_flags = reinterpret_cast<uint8_t*>(static_cast<char*>(desc->workspace) + 4 * n);
         m->residual = reinterpret_cast<double*>(static_cast<char*>(desc->workspace) + 4 * n +
                                                 freeze_flag_bytes(*desc));
+        m->deferred = reinterpret_cast<int*>(static_cast<char*>(desc->workspace) + 4 * n + freeze_flag_bytes(*desc) +
+                                             residual_bytes(*desc));
     } else if (hipMalloc(reinterpret_cast<void**>(&m->freeze_flags), freeze_flag_bytes(*desc)) != hipSuccess ||
-               hipMalloc(reinterpret_cast<void**>(&m->residual), residual_bytes(*desc)) != hipSuccess) {
+               hipMalloc(reinterpret_cast<void**>(&m->residual), residual_bytes(*desc)) != hipSuccess ||
+               hipMalloc(reinterpret_cast<void**>(&m->deferred), deferred_bytes(*desc)) != hipSuccess) {
         ne_model_destroy(m);
         return NE_ERR_DEVICE;
     }
     // every column is visited until a step's last stage has flagged them; the residual starts
     // at zero (creation is never inside a captured graph, so the fills may be memsets)
     if (hipMemsetAsync(m->freeze_flags, 1, freeze_flag_bytes(*desc), as_stream(stream)) != hipSuccess ||
-        hipMemsetAsync(m->residual, 0, residual_bytes(*desc), as_stream(stream)) != hipSuccess) {
+        hipMemsetAsync(m->residual, 0, residual_bytes(*desc), as_stream(stream)) != hipSuccess ||
+        hipMemsetAsync(m->deferred, 0, sizeof(int), as_stream(stream)) != hipSuccess) {
         ne_model_destroy(m);
         return NE_ERR_DEVICE;
     }
@@ -797,6 +808,7 @@ int ne_model_destroy(ne_model* m) {
             if (b && hipFree(b) != hipSuccess) r = NE_ERR_DEVICE;
         if (m->freeze_flags && hipFree(m->freeze_flags) != hipSuccess) r = NE_ERR_DEVICE;
         if (m->residual && hipFree(m->residual) != hipSuccess) r = NE_ERR_DEVICE;
+        if (m->deferred && hipFree(m->deferred) != hipSuccess) r = NE_ERR_DEVICE;
     }
     for (hipEvent_t e : m->events) (void)hipEventDestroy(e);
     delete m;

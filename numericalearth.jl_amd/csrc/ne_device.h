@@ -460,6 +460,7 @@ struct ColumnFluxes {
     double Qv, Qt, Jv, rtx, rty, us, ts, qs, Ts_C;
     int iterations;
     bool converged;
+    bool deferred;   // Defer: the FP32 iteration did not finish the column (nothing else is set)
 };
 
 // ---------------------------------------------------------------------------
@@ -659,7 +660,11 @@ __device__ __noinline__ int flux_iterate_general(const ne_params& p, double ua, 
 // until its drift falls below NE_MIXED_DRIFT (a few times the FP32 rounding noise of
 // u★ + θ★ + q★) and takes that iterate; only a column whose FP32 iteration does not get there
 // within NE_MIXED_MAXITER iterations (or leaves the finite range) continues in FP64 from it
-// with the reference's stop criterion. Finishing in FP64 as well (≈ 3 more FP64 iterations)
+// with the reference's stop criterion. Defer = true (for kernels whose parameters take the FP32
+// iteration) returns such a column as `deferred` instead of continuing: its FP64 iteration needs
+// twice the registers of the FP32 one, so the kernel that defers runs at 6 wavefronts per SIMD
+// instead of 4 and a second kernel finishes the few deferred columns with Defer = false, which
+// gives them the same values. Finishing in FP64 as well (≈ 3 more FP64 iterations)
 // changed neither the 200-step SST error against the FP64 model (9.8e-6 of the largest SST)
 // nor the flux errors (2.4e-5 of their scale) at 1440×560×10, and cost 13 % of the step.
 #ifndef NE_MIXED_DRIFT
@@ -674,7 +679,7 @@ __device__ __forceinline__ T tdiv(T a, T b) { return a / b; }
 template <>
 __device__ __forceinline__ float tdiv<float>(float a, float b) { return __fdividef(a, b); }
 
-template <bool Fast, bool Mixed = false, bool Consts = false>
+template <bool Fast, bool Mixed = false, bool Consts = false, bool Defer = false>
 __device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p_in, double ua, double va, double Ta,
                                                     double pa, double qa, double To_C, double uo, double vo,
                                                     bool inactive, double SW = 0.0, double LW = 0.0) {
@@ -774,6 +779,13 @@ __device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p_in, doubl
             }
         }
 
+        if constexpr (Defer && Mixed) {
+            if (!done32) {
+                ColumnFluxes d = {};
+                d.deferred = true;
+                return d;
+            }
+        }
         while (!done32) {
             // (iteration 0 always iterates; ConvergenceStopCriteria / FixedIterations tests below)
             double bstar = goTv * (ts * onedq + dTv * qs);
@@ -846,6 +858,7 @@ __device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p_in, doubl
     r.Ts_C = sT - NE_CELSIUS_TO_KELVIN;
     r.iterations = it;
     r.converged = !(converge && reached);
+    r.deferred = false;
     return r;
 }
 

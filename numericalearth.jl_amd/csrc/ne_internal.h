@@ -29,6 +29,8 @@ struct ne_model {
     // rounding residual of the FP32 state (residual_bytes): per tile, the running compensated
     // totals Σ V·(x − stored) of T and S (sum, compensation, sum, compensation)
     double* residual = nullptr;
+    // columns the FP32 flux solve leaves to its second kernel (count, then one int per column)
+    int* deferred = nullptr;
     double time = 0.0;
     double last_dt = 0.0;
     int64_t iteration = 0;
@@ -138,9 +140,12 @@ hipError_t launch_surface_update(const GridDev& g, const ne_params& p, const Atm
                                  const double* fj, const RadDev<Real>& r, const double* rfi, const double* rfj,
                                  View3<Real> T, View3<const Real> S, View3<const Real> u, View3<const Real> v,
                                  const FluxOut& out, const NetOut& n, const RadOut& ro, View3<double> frazil,
-                                 double dt_frazil, const uint8_t* freeze_flags, ne_counters* c, hipStream_t s);
+                                 double dt_frazil, const uint8_t* freeze_flags, ne_counters* c, int* deferred,
+                                 hipStream_t s);
+// deferred (FP32 storage; nullptr: none): count + one int per column for the columns K1 leaves to
+// its second kernel; the τ kernel, which runs after both, clears the count.
 hipError_t launch_surface_tau(const GridDev& g, const ne_params& p, const FluxOut& f, const NetOut& n,
-                              hipStream_t s);
+                              int* deferred, hipStream_t s);
 
 // Tracer launchers (ne_tracer.hip).
 // stochastic: round the new state stochastically (FP32 storage only; the stage that writes

@@ -308,7 +308,7 @@ flux_solve_ring_kernel(GridDev g, ne_params p, View3<const double> ua, View3<con
 // One column of K1: interpolate the atmosphere at the column, solve, store the fluxes.
 // The skin-temperature balance also reads the downwelling radiation, interpolated here
 // exactly as K2 interpolates it.
-template <typename Real, bool Fast, bool Consts>
+template <typename Real, bool Fast, bool Consts, bool Defer = false>
 __device__ __forceinline__ ColumnFluxes surface_solve_column(const GridDev& g, const ne_params& p,
                                                              const AtmosDev<Real>& a, const double* __restrict__ fi,
                                                              const double* __restrict__ fj, const RadDev<Real>& rd,
@@ -335,10 +335,9 @@ __device__ __forceinline__ ColumnFluxes surface_solve_column(const GridDev& g, c
         LW = interp_field(rd.lw, rx, ry, rd.time);
     }
     // FP32 storage (the mixed form) also takes the FP32 pre-iteration of the fixed point
-    ColumnFluxes r = flux_column<Fast, NE_MIXED_PREITERATION && sizeof(Real) == 4, Consts>(p, ua, va, Ta, pa, qa, To,
-                                                                                           uo, vo,
-                                                                                   inactive, SW, LW);
-    store_fluxes(out, i, j, r);
+    ColumnFluxes r = flux_column<Fast, NE_MIXED_PREITERATION && sizeof(Real) == 4, Consts, Defer>(
+        p, ua, va, Ta, pa, qa, To, uo, vo, inactive, SW, LW);
+    if (!Defer || !r.deferred) store_fluxes(out, i, j, r);
     return r;
 }
 
@@ -538,13 +537,16 @@ __device__ __forceinline__ void surface_finalize_column(const GridDev& g, const 
 // then (with the solved values still in registers) the freeze clamp, the net fluxes other
 // than τ and the radiation, in the reference's phase order (fluxes from the unclamped SST,
 // clamp, assembly with the clamped SST, radiation).
-template <typename Real, bool Fast, bool Consts>
+// Defer (FP32 storage with parameters that take the FP32 iteration): a column whose FP32
+// iteration does not finish it is appended to `deferred` (count, then flattened columns) and left
+// to surface_deferred_kernel.
+template <typename Real, bool Fast, bool Consts, bool Defer = false>
 __global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(NE_FLUX_WAVES_PER_EU)))
 surface_update_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __restrict__ fi,
                       const double* __restrict__ fj, RadDev<Real> rd, const double* __restrict__ rfi,
                       const double* __restrict__ rfj, View3<Real> T, View3<const Real> S, View3<const Real> u,
                       View3<const Real> v, FluxOut out, NetOut n, RadOut ro, View3<double> frazil, double dt_frazil,
-                      const uint8_t* __restrict__ freeze_flags, ne_counters* counters) {
+                      const uint8_t* __restrict__ freeze_flags, ne_counters* counters, int* deferred) {
     int i = blockIdx.x * BX + threadIdx.x;
     int j = blockIdx.y * BY + threadIdx.y;
     if (out.iters.p) {
@@ -561,23 +563,60 @@ surface_update_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __
     }
     bool in = i < g.Nx && j < g.Ny;
     ColumnFluxes r;
+    r.deferred = false;
     bool inactive = true;
     if (in) {
-        r = surface_solve_column<Real, Fast, Consts>(g, p, a, fi, fj, rd, rfi, rfj, cview(T), S, u, v, out, i, j,
-                                                     inactive);
-        surface_finalize_column<Real, false>(g, p, a, fi, fj, rd, rfi, rfj, T, S, out, n, ro, frazil, dt_frazil,
-                                             freeze_flags, r.Qt, r.Qv, r.Jv, r.Ts_C, i, j);
+        r = surface_solve_column<Real, Fast, Consts, Defer>(g, p, a, fi, fj, rd, rfi, rfj, cview(T), S, u, v, out, i,
+                                                            j, inactive);
+        if (Defer && r.deferred)
+            deferred[1 + atomicAdd(&deferred[0], 1)] = i + j * g.Nx;
+        else
+            surface_finalize_column<Real, false>(g, p, a, fi, fj, rd, rfi, rfj, T, S, out, n, ro, frazil, dt_frazil,
+                                                 freeze_flags, r.Qt, r.Qv, r.Jv, r.Ts_C, i, j);
     }
-    bool solved = in && !inactive;
+    bool solved = in && !inactive && !(Defer && r.deferred);
     accumulate_counters(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
                         solved && !finite3(r.us, r.ts, r.qs));
 }
 
+// The columns surface_update_kernel deferred, finished as the kernel without deferral finishes
+// them (the FP32 iteration again, then the FP64 one): a grid-stride loop over the list, which is
+// empty in most steps. surface_tau_kernel, which runs after it, clears the count.
+template <typename Real, bool Fast, bool Consts>
+__global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(NE_FLUX_WAVES_PER_EU)))
+surface_deferred_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __restrict__ fi,
+                        const double* __restrict__ fj, RadDev<Real> rd, const double* __restrict__ rfi,
+                        const double* __restrict__ rfj, View3<Real> T, View3<const Real> S, View3<const Real> u,
+                        View3<const Real> v, FluxOut out, NetOut n, RadOut ro, View3<double> frazil,
+                        double dt_frazil, const uint8_t* __restrict__ freeze_flags, ne_counters* counters,
+                        const int* __restrict__ deferred) {
+    const int count = deferred[0];   // the same for every work-item: nothing writes it meanwhile
+    // rounds of one column per work-item, uniform over the block (the counters synchronise it)
+    for (int base = blockIdx.x * BX * BY; base < count; base += gridDim.x * BX * BY) {
+        const int q = base + (int)(threadIdx.x + threadIdx.y * BX);
+        ColumnFluxes r;
+        bool inactive = true;
+        if (q < count) {
+            const int col = deferred[1 + q];
+            const int i = col % g.Nx, j = col / g.Nx;
+            r = surface_solve_column<Real, Fast, Consts>(g, p, a, fi, fj, rd, rfi, rfj, cview(T), S, u, v, out, i, j,
+                                                         inactive);
+            surface_finalize_column<Real, false>(g, p, a, fi, fj, rd, rfi, rfj, T, S, out, n, ro, frazil,
+                                                 dt_frazil, freeze_flags, r.Qt, r.Qv, r.Jv, r.Ts_C, i, j);
+        }
+        const bool solved = q < count && !inactive;
+        accumulate_counters(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
+                            solved && !finite3(r.us, r.ts, r.qs));
+    }
+}
+
 // After the fused K1: τ of every column from its own and its west / south neighbours'
 // momentum fluxes, which K1 wrote (the one part of the update that reads other columns).
-__global__ void __launch_bounds__(BX* BY) surface_tau_kernel(GridDev g, ne_params p, FluxOut f, NetOut n) {
+__global__ void __launch_bounds__(BX* BY) surface_tau_kernel(GridDev g, ne_params p, FluxOut f, NetOut n,
+                                                            int* deferred) {
     int i = blockIdx.x * BX + threadIdx.x;
     int j = blockIdx.y * BY + threadIdx.y;
+    if (deferred && i == 0 && j == 0) deferred[0] = 0;   // the deferred columns have been finished
     if (i >= g.Nx || j >= g.Ny) return;
     double rtx_w, rty_s;
     neighbour_momentum(g, f.rtx, f.rty, i, j, rtx_w, rty_s);
@@ -803,17 +842,40 @@ hipError_t launch_radiation(const GridDev& g, const ne_params& p, View3<const do
     return hipGetLastError();
 }
 
+// The parameters for which the FP32 form's flux solve runs its FP32 iteration (flux_column,
+// Mixed && Fast): every column then either finishes in FP32 or is deferred.
+bool mixed_iteration_applies(const ne_params& p) {
+    const bool shared_scalar = p.temperature_roughness == p.water_vapor_roughness &&
+                               (p.temperature_roughness == NE_ROUGHNESS_EDSON ||
+                                p.temperature_roughness_length == p.water_vapor_roughness_length);
+    return NE_MIXED_PREITERATION && p.stop_criteria == NE_STOP_CONVERGENCE && shared_scalar &&
+           p.momentum_roughness == NE_ROUGHNESS_EDSON && p.water_vapor_roughness == NE_ROUGHNESS_EDSON;
+}
+
 template <typename Real>
 hipError_t launch_surface_update(const GridDev& g, const ne_params& p, const AtmosDev<Real>& a, const double* fi,
                                  const double* fj, const RadDev<Real>& r, const double* rfi, const double* rfj,
                                  View3<Real> T, View3<const Real> S, View3<const Real> u, View3<const Real> v,
                                  const FluxOut& out, const NetOut& n, const RadOut& ro, View3<double> frazil,
-                                 double dt_frazil, const uint8_t* freeze_flags, ne_counters* c, hipStream_t s) {
+                                 double dt_frazil, const uint8_t* freeze_flags, ne_counters* c, int* deferred,
+                                 hipStream_t s) {
     const int path = flux_path(p);
+    const dim3 grid = grid2d(g.Nx, g.Ny);
+    if (sizeof(Real) == 4 && path >= 1 && deferred && mixed_iteration_applies(p)) {
+        // the FP32 iteration alone at 6 wavefronts per SIMD, then the deferred columns
+        auto k1 = path == 2 ? surface_update_kernel<Real, true, true, true> : surface_update_kernel<Real, true, false, true>;
+        auto k2 = path == 2 ? surface_deferred_kernel<Real, true, true> : surface_deferred_kernel<Real, true, false>;
+        hipLaunchKernelGGL(k1, grid, dim3(BX, BY), 0, s, g, p, a, fi, fj, r, rfi, rfj, T, S, u, v, out, n, ro, frazil,
+                           dt_frazil, freeze_flags, c, deferred);
+        const unsigned blocks = std::min<unsigned>(grid.x * grid.y, 256u);
+        hipLaunchKernelGGL(k2, dim3(blocks), dim3(BX, BY), 0, s, g, p, a, fi, fj, r, rfi, rfj, T, S, u, v, out, n, ro,
+                           frazil, dt_frazil, freeze_flags, c, deferred);
+        return hipGetLastError();
+    }
     auto kernel = path == 2 ? surface_update_kernel<Real, true, true>
                 : path == 1 ? surface_update_kernel<Real, true, false> : surface_update_kernel<Real, false, false>;
-    hipLaunchKernelGGL(kernel, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, a, fi, fj, r, rfi, rfj, T, S, u, v, out,
-                       n, ro, frazil, dt_frazil, freeze_flags, c);
+    hipLaunchKernelGGL(kernel, grid, dim3(BX, BY), 0, s, g, p, a, fi, fj, r, rfi, rfj, T, S, u, v, out, n, ro, frazil,
+                       dt_frazil, freeze_flags, c, (int*)nullptr);
     return hipGetLastError();
 }
 template hipError_t launch_surface_update<double>(const GridDev&, const ne_params&, const AtmosDev<double>&,
@@ -821,17 +883,17 @@ template hipError_t launch_surface_update<double>(const GridDev&, const ne_param
                                                   const double*, View3<double>, View3<const double>,
                                                   View3<const double>, View3<const double>, const FluxOut&,
                                                   const NetOut&, const RadOut&, View3<double>, double,
-                                                  const uint8_t*, ne_counters*, hipStream_t);
+                                                  const uint8_t*, ne_counters*, int*, hipStream_t);
 template hipError_t launch_surface_update<float>(const GridDev&, const ne_params&, const AtmosDev<float>&,
                                                  const double*, const double*, const RadDev<float>&, const double*,
                                                  const double*, View3<float>, View3<const float>,
                                                  View3<const float>, View3<const float>, const FluxOut&,
                                                  const NetOut&, const RadOut&, View3<double>, double,
-                                                 const uint8_t*, ne_counters*, hipStream_t);
+                                                 const uint8_t*, ne_counters*, int*, hipStream_t);
 
 hipError_t launch_surface_tau(const GridDev& g, const ne_params& p, const FluxOut& f, const NetOut& n,
-                              hipStream_t s) {
-    hipLaunchKernelGGL(surface_tau_kernel, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, f, n);
+                              int* deferred, hipStream_t s) {
+    hipLaunchKernelGGL(surface_tau_kernel, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, p, f, n, deferred);
     return hipGetLastError();
 }
 

@@ -236,3 +236,28 @@ def test_mixed_precisions_are_refused(require_gpu):
         ens.add(b)
     assert e.value.status == abi.NE_ERR_UNSUPPORTED
     ens.close()
+
+
+def test_mixed_deferred_columns_equal_the_inline_finish(require_gpu):
+    """The mixed form's single model runs the FP32 flux iteration in a kernel of its own and finishes
+    the columns it leaves (drift still above the FP32 threshold) in a second kernel; the batch
+    finishes them inline in one kernel. With maxiter = 6 the FP32 iteration stops after 5 and most
+    columns are left to the FP64 iteration: both paths must give the same bits."""
+    specs = _specs(n=4, seed=11)
+    alone, batched = [], []
+    for s in specs:
+        p = member_params(s)
+        p.maxiter = 6
+        alone.append(OceanOnlyModel(member_case(s), params=p, precision="mixed"))
+        batched.append(OceanOnlyModel(member_case(s), params=p, precision="mixed"))
+    ens = Ensemble(batched)
+    for _ in range(2):
+        ens.time_step(DT)
+        for m in alone:
+            m.time_step(DT)
+    _assert_same(alone, batched, "mixed, maxiter 6")
+    its = np.concatenate([m.iterations.cpu().numpy().ravel() for m in alone])
+    deferred = int(np.sum(its == 6))
+    print(f"[batch] columns finished by the FP64 iteration: {deferred} of {its.size}", flush=True)
+    assert deferred > 0
+    ens.close()

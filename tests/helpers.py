@@ -57,4 +57,6 @@ def assert_rel_envelope(a, b, envelope, rtol, name=""):
     assert field <= rtol, f"{name}: field-scaled error {field:.3e} > {rtol:.1e}"
     with np.errstate(invalid="ignore", divide="ignore"):
         elem = float(np.max(np.where(np.abs(b) > 0, diff / np.abs(b), 0.0), initial=0.0))
-    return elem, field
+        # how much of the envelope the elements beyond rtol·|b| use
+        used = float(np.max(np.where((diff > rtol * np.abs(b)) & (env > 0), diff / env, 0.0), initial=0.0))
+    return elem, field, used

@@ -88,8 +88,10 @@ def _solve_both(case, om, gm):
     return ph
 
 
-# Amplification allowed over the oracle's own one-ulp response (see _flux_envelope).
-ENVELOPE_FACTOR = 64.0
+# Amplification allowed over the oracle's own one-ulp response (see _flux_envelope). The
+# elements beyond 1e-12 relative use at most 0.92 of that response in these tests (printed by
+# _compare_fluxes); 4 leaves room for a different box's transcendental rounding, not for error.
+ENVELOPE_FACTOR = 4.0
 
 
 def _compare_fluxes(gm, om, rtol=RTOL, max_iteration_mismatch_frac=1e-4, envelope=None):
@@ -119,6 +121,9 @@ def _compare_fluxes(gm, om, rtol=RTOL, max_iteration_mismatch_frac=1e-4, envelop
         else:
             worst[k] = assert_rel_envelope(a, b, ENVELOPE_FACTOR * envelope[k][ok], rtol, k)
     print("[flux] worst relative errors:", {k: f"{v[0]:.2e}" for k, v in worst.items()}, flush=True)
+    if envelope is not None:
+        print("[flux] largest use of the envelope (|Δ| / one-ulp response, beyond 1e-12 relative):",
+              {k: f"{v[2] * ENVELOPE_FACTOR:.2f}" for k, v in worst.items()}, flush=True)
 
 
 def _oracle_on_state(om, T, S, time=None, nudge=None):

@@ -450,6 +450,19 @@ int ne_tracer_rk_stage_f32(ne_ctx* ctx, const ne_grid* grid, const ne_params* pa
                            ne_view T_out, ne_view S_out,
                            const ne_net_ocean_fluxes* net, double beta_dt, void* stream);
 
+/* fill_halo_regions! of a cell-centred 3-D field (T or S) after a step, as Oceananigans leaves
+ * the tracers of the HydrostaticFreeSurfaceModel that ocean_simulation builds
+ * (ocean_simulation.jl:487-499; Oceananigans not vendored): the halo of widths Hx, Hy, Hz
+ * (0 … 64) around the view's interior takes, at each point, the value of the interior point it
+ * maps to — wrapped in a periodic direction, the nearest interior point in a bounded one and
+ * vertically (the zero-gradient halo of the flux and no-flux boundary conditions of the tracers).
+ * The caller's memory must hold the halo. The step kernels read neighbours by index and need no
+ * halo; this is for consumers of the caller's halo-padded arrays. */
+int ne_fill_halo_f64(ne_ctx* ctx, const ne_grid* grid, ne_view c, int32_t Hx, int32_t Hy, int32_t Hz,
+                     void* stream);
+int ne_fill_halo_f32(ne_ctx* ctx, const ne_grid* grid, ne_view c, int32_t Hx, int32_t Hy, int32_t Hz,
+                     void* stream);
+
 /* Vertical velocity from continuity, for callers that evolve u and v: replaces Oceananigans'
  * compute_w_from_continuity! of the HydrostaticFreeSurfaceModel that ocean_simulation builds
  * (ocean_simulation.jl:487-499; Oceananigans not vendored). Per column, w = 0 at the bottom

@@ -584,6 +584,28 @@ int ne_compute_w_from_continuity_f64(ne_ctx* c, const ne_grid* g, ne_view u, ne_
                                                view<double>(w), as_stream(stream)));
 }
 
+}  // extern "C"
+
+template <typename Real>
+static int fill_halo(ne_ctx* c, const ne_grid* g, ne_view cv, int32_t Hx, int32_t Hy, int32_t Hz, void* stream) {
+    if (!c) return NE_ERR_NULL;
+    int r;
+    if ((r = check_grid(g)) || (r = check_view3(cv))) return r;
+    if (Hx < 0 || Hy < 0 || Hz < 0 || Hx > 64 || Hy > 64 || Hz > 64) return NE_ERR_SHAPE;
+    if (Hx + Hy + Hz == 0) return NE_OK;
+    return dev_err(launch_halo_fill<Real>(grid_dev(*g), view<Real>(cv), Hx, Hy, Hz, as_stream(stream)));
+}
+
+extern "C" {
+
+int ne_fill_halo_f64(ne_ctx* c, const ne_grid* g, ne_view cv, int32_t Hx, int32_t Hy, int32_t Hz, void* stream) {
+    return fill_halo<double>(c, g, cv, Hx, Hy, Hz, stream);
+}
+
+int ne_fill_halo_f32(ne_ctx* c, const ne_grid* g, ne_view cv, int32_t Hx, int32_t Hy, int32_t Hz, void* stream) {
+    return fill_halo<float>(c, g, cv, Hx, Hy, Hz, stream);
+}
+
 int ne_budget_f64(ne_ctx* c, const ne_grid* g, ne_view cv, double weight, double* out, void* stream) {
     if (!c || !out) return NE_ERR_NULL;
     int r;

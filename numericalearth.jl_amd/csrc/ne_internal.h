@@ -196,6 +196,10 @@ hipError_t launch_budget_noalloc(double* partial, const GridDev& g, View3<const 
 // S compensation), into which the block of the step's last stage adds the tile's Σ V·(x −
 // stored); the number of slots a model keeps, and their fixed-order sum into out[4].
 constexpr int RESIDUAL_SLOT = 4;
+// Halo of a cell-centred field (ne_fill_halo_*): wrapped in periodic directions, the nearest
+// interior value in bounded ones and vertically.
+template <typename Real>
+hipError_t launch_halo_fill(const GridDev& g, View3<Real> c, int Hx, int Hy, int Hz, hipStream_t s);
 // Traffic probe (ne_stream_probe_f64): n_in ≤ NE_STREAM_PROBE_MAX_IN arrays read, n_out written.
 hipError_t launch_stream_probe(const double* const* in, int n_in, double* const* out, int n_out, int64_t n,
                                hipStream_t s);

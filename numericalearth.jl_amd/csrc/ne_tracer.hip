@@ -737,7 +737,38 @@ __global__ void __launch_bounds__(256) stream_probe_kernel(StreamProbeArgs a) {
     }
 }
 
+// Halo of a cell-centred field: every halo point takes the value of the interior point it maps
+// to — wrapped in a periodic direction, the nearest interior point in a bounded one and
+// vertically (the zero-gradient halo of a flux boundary condition). Interior points are only
+// read and each halo point is written once, so the work-items are independent.
+template <typename Real>
+__global__ void __launch_bounds__(256) halo_fill_kernel(View3<Real> c, int Nx, int Ny, int Nz, int Hx, int Hy,
+                                                        int Hz, int periodic_x, int periodic_y) {
+    const int64_t W = Nx + 2 * Hx, H = Ny + 2 * Hy, n = W * H * (int64_t)(Nz + 2 * Hz);
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+        const int i = (int)(t % W) - Hx;
+        const int j = (int)((t / W) % H) - Hy;
+        const int k = (int)(t / (W * H)) - Hz;
+        if (i >= 0 && i < Nx && j >= 0 && j < Ny && k >= 0 && k < Nz) continue;
+        const int si = periodic_x ? ((i % Nx) + Nx) % Nx : min(max(i, 0), Nx - 1);
+        const int sj = periodic_y ? ((j % Ny) + Ny) % Ny : min(max(j, 0), Ny - 1);
+        const int sk = min(max(k, 0), Nz - 1);
+        c(i, j, k) = c(si, sj, sk);
+    }
+}
+
 }  // namespace
+
+template <typename Real>
+hipError_t launch_halo_fill(const GridDev& g, View3<Real> c, int Hx, int Hy, int Hz, hipStream_t s) {
+    const int64_t n = (int64_t)(g.Nx + 2 * Hx) * (g.Ny + 2 * Hy) * (g.Nz + 2 * Hz);
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(halo_fill_kernel<Real>, dim3((unsigned)blocks), dim3(256), 0, s, c, g.Nx, g.Ny, g.Nz, Hx, Hy, Hz,
+                       g.tx == NE_PERIODIC ? 1 : 0, g.ty == NE_PERIODIC ? 1 : 0);
+    return hipGetLastError();
+}
+template hipError_t launch_halo_fill<double>(const GridDev&, View3<double>, int, int, int, hipStream_t);
+template hipError_t launch_halo_fill<float>(const GridDev&, View3<float>, int, int, int, hipStream_t);
 
 hipError_t launch_stream_probe(const double* const* in, int n_in, double* const* out, int n_out, int64_t n,
                                hipStream_t s) {

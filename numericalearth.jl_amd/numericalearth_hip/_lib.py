@@ -39,6 +39,8 @@ SIGNATURES = {
     "ne_fractional_indices_f64": [vp, P(abi.Grid), P(abi.SourceGrid), vp, vp, vp],
     "ne_log_f64": [vp, vp, vp, C.c_int64, vp],
     "ne_stream_probe_f64": [vp, vp, C.c_int32, vp, C.c_int32, C.c_int64, vp],
+    "ne_fill_halo_f64": [vp, vp, abi.View, C.c_int32, C.c_int32, C.c_int32, vp],
+    "ne_fill_halo_f32": [vp, vp, abi.View, C.c_int32, C.c_int32, C.c_int32, vp],
     "ne_fractional_indices_nodes_f64": [vp, P(abi.Grid), vp, C.c_int32, C.c_int32, vp, C.c_int32, C.c_int32, vp, vp,
                                         vp],
     "ne_interp_atmos_f64": [vp, P(abi.Grid), P(abi.PrescribedAtmosphereDesc), vp, vp, P(abi.ExchangeState), vp],

@@ -76,6 +76,7 @@ def test_null_arguments_are_rejected_without_a_device():
     assert L.ne_batch_info(None, None, None, None, None) == abi.NE_ERR_NULL
     assert L.ne_model_rounding_residual(None, None, None) == abi.NE_ERR_NULL
     assert L.ne_stream_probe_f64(None, None, 5, None, 2, 10, None) == abi.NE_ERR_NULL
+    assert L.ne_fill_halo_f64(None, None, abi.View(), 1, 1, 1, None) == abi.NE_ERR_NULL
 
 
 def test_julia_mirror_has_the_header_params_size():

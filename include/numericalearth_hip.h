@@ -549,9 +549,9 @@ enum { NE_TIME_CLAMPED = 0, NE_TIME_CYCLICAL = 1 };
 /* Bytes of device workspace a model of this descriptor needs: four RK stage buffers of T, S,
  * one byte per column of freeze flags (the step's last stage marks the columns holding a
  * cell below its freezing point, and the freeze clamp after the step visits only those), and
- * the rounding residual (ne_model_rounding_residual: four doubles per 63×4 tile) and the list of
+ * the rounding residual (ne_model_rounding_residual: four doubles per 63×4 tile), the list of
  * columns whose FP32 flux iteration the mixed form finishes in a second kernel (one int per
- * column). */
+ * column) and the two-colour forcing's Beer's-law factors at the Nz+1 faces. */
 int ne_model_workspace_bytes(const ne_model_desc* desc, int64_t* bytes);
 
 typedef struct ne_model ne_model;

@@ -248,10 +248,16 @@ size_t deferred_bytes(const ne_model_desc& d) {
     return ((1 + (size_t)d.grid.Nx * (size_t)d.grid.Ny) * sizeof(int) + 255) & ~(size_t)255;
 }
 
+// The Beer's-law factors of the two-colour forcing at the Nz+1 faces, rounded up to 256 bytes.
+size_t beer_bytes(const ne_model_desc& d) {
+    return (2 * ((size_t)d.grid.Nz + 1) * sizeof(double) + 255) & ~(size_t)255;
+}
+
 // Device workspace of a model: the four RK stage buffers, the freeze flags, the rounding residual,
-// the deferred-column list.
+// the deferred-column list, the Beer's-law factors.
 size_t model_workspace_bytes(const ne_model_desc& d) {
-    return 4 * stage_buffer_bytes(d) + freeze_flag_bytes(d) + residual_bytes(d) + deferred_bytes(d);
+    return 4 * stage_buffer_bytes(d) + freeze_flag_bytes(d) + residual_bytes(d) + deferred_bytes(d) +
+           beer_bytes(d);
 }
 
 }  // namespace
@@ -687,7 +693,8 @@ static int model_ocean_step(ne_model* m, double dt, hipStream_t s, hipEvent_t* e
     hipError_t e;
     if (ev && hipEventRecord(ev[0], s) != hipSuccess) return NE_ERR_DEVICE;
     const bool sr = sizeof(Real) == 4;   // FP32 state: stochastic rounding of the stage that writes it
-    const TracerForcing F = tracer_forcing(&d.params, d.net);
+    TracerForcing F = tracer_forcing(&d.params, d.net);
+    if (F.two_color) F.beer = m->beer;   // the factors filled at creation
     e = launch_rk_stage<Real>(g, u, v, w, T0, S0, T0, S0, view<Real>(m->vT1), view<Real>(m->vS1), JT, JS, Jw, F,
                               beta[0] * dt, false, m->time, s);
     if (e != hipSuccess) return NE_ERR_DEVICE;
@@ -777,9 +784,12 @@ int ne_model_create(ne_ctx* c, const ne_model_desc* desc, ne_model** out, void* 
                                                 freeze_flag_bytes(*desc));
         m->deferred = reinterpret_cast<int*>(static_cast<char*>(desc->workspace) + 4 * n + freeze_flag_bytes(*desc) +
                                              residual_bytes(*desc));
+        m->beer = reinterpret_cast<double*>(static_cast<char*>(desc->workspace) + 4 * n + freeze_flag_bytes(*desc) +
+                                            residual_bytes(*desc) + deferred_bytes(*desc));
     } else if (hipMalloc(reinterpret_cast<void**>(&m->freeze_flags), freeze_flag_bytes(*desc)) != hipSuccess ||
                hipMalloc(reinterpret_cast<void**>(&m->residual), residual_bytes(*desc)) != hipSuccess ||
-               hipMalloc(reinterpret_cast<void**>(&m->deferred), deferred_bytes(*desc)) != hipSuccess) {
+               hipMalloc(reinterpret_cast<void**>(&m->deferred), deferred_bytes(*desc)) != hipSuccess ||
+               hipMalloc(reinterpret_cast<void**>(&m->beer), beer_bytes(*desc)) != hipSuccess) {
         ne_model_destroy(m);
         return NE_ERR_DEVICE;
     }
@@ -787,7 +797,9 @@ int ne_model_create(ne_ctx* c, const ne_model_desc* desc, ne_model** out, void* 
     // at zero (creation is never inside a captured graph, so the fills may be memsets)
     if (hipMemsetAsync(m->freeze_flags, 1, freeze_flag_bytes(*desc), as_stream(stream)) != hipSuccess ||
         hipMemsetAsync(m->residual, 0, residual_bytes(*desc), as_stream(stream)) != hipSuccess ||
-        hipMemsetAsync(m->deferred, 0, sizeof(int), as_stream(stream)) != hipSuccess) {
+        hipMemsetAsync(m->deferred, 0, sizeof(int), as_stream(stream)) != hipSuccess ||
+        launch_beer_factors(grid_dev(desc->grid), desc->params.first_absorption_coefficient,
+                            desc->params.second_absorption_coefficient, m->beer, as_stream(stream)) != hipSuccess) {
         ne_model_destroy(m);
         return NE_ERR_DEVICE;
     }
@@ -831,6 +843,7 @@ int ne_model_destroy(ne_model* m) {
         if (m->freeze_flags && hipFree(m->freeze_flags) != hipSuccess) r = NE_ERR_DEVICE;
         if (m->residual && hipFree(m->residual) != hipSuccess) r = NE_ERR_DEVICE;
         if (m->deferred && hipFree(m->deferred) != hipSuccess) r = NE_ERR_DEVICE;
+        if (m->beer && hipFree(m->beer) != hipSuccess) r = NE_ERR_DEVICE;
     }
     for (hipEvent_t e : m->events) (void)hipEventDestroy(e);
     delete m;

@@ -173,6 +173,7 @@ BatchRec<Real> member_record(const ne_model* m, const double* atimes_dev, const 
     r.frazil = view<double>(d.frazil_heat);
     r.freeze_flags = m->freeze_flags;
     r.residual = m->residual;
+    r.beer = m->beer;
     r.atimes = atimes_dev;
     r.rtimes = rtimes_dev;
     r.nat = (int)m->atimes.size();

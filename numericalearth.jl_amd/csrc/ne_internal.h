@@ -31,6 +31,8 @@ struct ne_model {
     double* residual = nullptr;
     // columns the FP32 flux solve leaves to its second kernel (count, then one int per column)
     int* deferred = nullptr;
+    // Beer's-law factors of TwoColorRadiation at every face (TracerForcing::beer), filled at creation
+    double* beer = nullptr;
     double time = 0.0;
     double last_dt = 0.0;
     int64_t iteration = 0;
@@ -83,10 +85,13 @@ struct TracerForcing {
     // update conservatively in FP64 and adds Σ V·(x − stored) of T and S over the tile's cells
     // to the tile's slot (the kernel offsets the pointer to it)
     double* residual;
+    // Beer's-law factors of the two colours at every face (nullptr: computed in place):
+    // beer[2k] = exp(κ₁(z_f[k] − η)), beer[2k+1] = exp(κ₂(z_f[k] − η)), k = 0 … Nz
+    const double* beer;
 };
 
 inline TracerForcing tracer_forcing(const ne_params* p, const ne_net_ocean_fluxes& n) {
-    TracerForcing f = {0, 0.0, 0.0, 0.0, nullptr, nullptr, 0.0, 0.0, nullptr};
+    TracerForcing f = {0, 0.0, 0.0, 0.0, nullptr, nullptr, 0.0, 0.0, nullptr, nullptr};
     if (p && p->radiative_forcing == NE_RADIATIVE_FORCING_TWO_COLOR) {
         f.two_color = 1;
         f.first_fraction = p->first_color_fraction;
@@ -196,6 +201,9 @@ hipError_t launch_budget_noalloc(double* partial, const GridDev& g, View3<const 
 // S compensation), into which the block of the step's last stage adds the tile's Σ V·(x −
 // stored); the number of slots a model keeps, and their fixed-order sum into out[4].
 constexpr int RESIDUAL_SLOT = 4;
+// Beer's-law factors of the two colours at the Nz+1 faces (TracerForcing::beer), as the stage
+// kernel would compute them in place.
+hipError_t launch_beer_factors(const GridDev& g, double kappa1, double kappa2, double* beer, hipStream_t s);
 // Halo of a cell-centred field (ne_fill_halo_*): wrapped in periodic directions, the nearest
 // interior value in bounded ones and vertically.
 template <typename Real>
@@ -233,6 +241,7 @@ struct BatchRec {
     View3<double> frazil;
     uint8_t* freeze_flags;  // the member's freeze flags (written by stage 3, read by K2)
     double* residual;       // the member's rounding residual (running totals per tile)
+    const double* beer;     // the member's Beer's-law factors (two-colour forcing)
     const double* atimes;   // device copies of the snapshot times
     const double* rtimes;
     int nat, nrt;

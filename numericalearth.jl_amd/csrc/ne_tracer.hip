@@ -187,7 +187,8 @@ __device__ __forceinline__ DD dd_add(DD a, DD b) {
 // cells). Every work-item of the block calls this with the same tile. Every address is a
 // 32-bit byte offset from a uniform base (the host checks that each view spans < 4 GiB),
 // computed once per tile; a level adds its uniform k-stride.
-template <typename Real, int MODE, int TR, bool TWO_COLOR, bool WATCH = false>
+// BEER (with TWO_COLOR): the Beer's-law factors come from F.beer instead of being computed.
+template <typename Real, int MODE, int TR, bool TWO_COLOR, bool WATCH = false, bool BEER = false>
 __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDev& g, const TracerLayout& L,
                                           const Real* __restrict__ u, const Real* __restrict__ v,
                                           const Real* __restrict__ w, const Real* T0, const Real* S0,
@@ -335,7 +336,7 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
         // the grid's per-level values travel with the level's other loads: read at their point of
         // use, each would wait for every load in flight (memory counters complete in order)
         ndz = g.dz[k];
-        if (tc) {
+        if (tc && !BEER) {
             nzb = g.zf[k];
             nzt = g.zf[k + 1];
         }
@@ -444,8 +445,21 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
             if (tc) {
                 // ε₁ ∂z J₁ + (1 − ε₁) ∂z J₂; the face below the lowest active cell carries no flux
                 // (the level's two faces; uniform over the tile, recomputed rather than carried)
-                const double E1t = exp(F.kappa1 * (zft - eta)), E2t = exp(F.kappa2 * (zft - eta));
-                const double E1b = exp(F.kappa1 * (zfb - eta)), E2b = exp(F.kappa2 * (zfb - eta));
+                double E1t, E2t, E1b, E2b;
+                if constexpr (BEER) {
+                    // the level's two faces from the table, at a uniform index: scalar loads, which
+                    // wait on their own counter and leave the vector loads in flight alone
+                    const int ku = wave_uniform(k);
+                    E1b = F.beer[2 * ku];
+                    E2b = F.beer[2 * ku + 1];
+                    E1t = F.beer[2 * ku + 2];
+                    E2t = F.beer[2 * ku + 3];
+                } else {
+                    E1t = exp(F.kappa1 * (zft - eta));
+                    E2t = exp(F.kappa2 * (zft - eta));
+                    E1b = exp(F.kappa1 * (zfb - eta));
+                    E2b = exp(F.kappa2 * (zfb - eta));
+                }
                 const bool below = k - 1 >= mykmin;
                 const double a1 = J0 * E1t, b1 = below ? J0 * E1b : 0.0;
                 const double a2 = J0 * E2t, b2 = below ? J0 * E2b : 0.0;
@@ -522,7 +536,7 @@ __device__ __forceinline__ void tile_body(TileShared<Real, TR>& sh, const GridDe
 
 // Single model: one block per tile, tiles ordered so that each XCD (blocks are dealt
 // round-robin to the 8 XCDs) receives a contiguous band of tiles.
-template <typename Real, int MODE, bool TWO_COLOR, bool WATCH = false>
+template <typename Real, int MODE, bool TWO_COLOR, bool WATCH = false, bool BEER = false>
 __global__ void __launch_bounds__(TW * PLAIN_TR)
 __attribute__((amdgpu_waves_per_eu(stage_waves<Real, TWO_COLOR, WATCH && MODE == 0>())))
 tracer_tile_kernel(GridDev g, TracerLayout L, const Real* u, const Real* v, const Real* w, const Real* T0,
@@ -536,7 +550,7 @@ tracer_tile_kernel(GridDev g, TracerLayout L, const Real* u, const Real* v, cons
     const int tile = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
     if (tile >= ntiles) return;
     if (F.residual) F.residual += RESIDUAL_SLOT * tile;   // this tile's totals
-    tile_body<Real, MODE, PLAIN_TR, TWO_COLOR, WATCH>(sh, g, L, u, v, w, T0, S0, Tin, Sin, Tout, Sout, JT, JS, Jw, F,
+    tile_body<Real, MODE, PLAIN_TR, TWO_COLOR, WATCH, BEER>(sh, g, L, u, v, w, T0, S0, Tin, Sin, Tout, Sout, JT, JS, Jw, F,
                                                      beta_dt, (tile % ntx) * TC, (tile / ntx) * PLAIN_TR,
                                                      stochastic != 0, seed);
 }
@@ -565,13 +579,14 @@ batch_stage_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __re
         const Real* Sin = stage == 1 ? rc.S.p : stage == 2 ? rc.S1.p : rc.S2.p;
         Real* Tout = stage == 1 ? rc.T1.p : stage == 2 ? rc.T2.p : rc.T.p;
         Real* Sout = stage == 1 ? rc.S1.p : stage == 2 ? rc.S2.p : rc.S.p;
-        TracerForcing F = {0, 0.0, 0.0, 0.0, nullptr, nullptr, 0.0, 0.0, nullptr};
+        TracerForcing F = {0, 0.0, 0.0, 0.0, nullptr, nullptr, 0.0, 0.0, nullptr, nullptr};
         if (rc.p.radiative_forcing == NE_RADIATIVE_FORCING_TWO_COLOR) {
             F.two_color = 1;
             F.first_fraction = rc.p.first_color_fraction;
             F.kappa1 = rc.p.first_absorption_coefficient;
             F.kappa2 = rc.p.second_absorption_coefficient;
             F.J0 = rc.n.J_sw.p;
+            F.beer = rc.beer;
         }
         if (WATCH) {   // the step's last stage flags the columns the freeze clamp visits
             F.freeze_flags = rc.freeze_flags;
@@ -580,7 +595,7 @@ batch_stage_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __re
             if (sizeof(Real) == 4 && rc.residual) F.residual = rc.residual + RESIDUAL_SLOT * it.index;
         }
         __syncthreads();   // the previous item's reads of the LDS are complete
-        tile_body<Real, 0, TR, TWO_COLOR, WATCH>(sh, g, L, rc.u.p, rc.v.p, rc.w.p, rc.T.p, rc.S.p, Tin, Sin, Tout, Sout, rc.n.J_T.p,
+        tile_body<Real, 0, TR, TWO_COLOR, WATCH, TWO_COLOR>(sh, g, L, rc.u.p, rc.v.p, rc.w.p, rc.T.p, rc.S.p, Tin, Sin, Tout, Sout, rc.n.J_T.p,
                                rc.n.J_S.p, rc.n.J_w.p, F, beta_dt, (it.index % ntx) * TC, (it.index / ntx) * TR,
                                stage == 3, step_seed(rc.time));
     }
@@ -757,7 +772,23 @@ __global__ void __launch_bounds__(256) halo_fill_kernel(View3<Real> c, int Nx, i
     }
 }
 
+// Beer's-law factors at the faces, with the expressions the stage kernel evaluates in place.
+__global__ void beer_factors_kernel(const double* __restrict__ zf, int Nz, double kappa1, double kappa2,
+                                    double* __restrict__ beer) {
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k > Nz) return;
+    const double eta = zf[Nz];
+    beer[2 * k] = exp(kappa1 * (zf[k] - eta));
+    beer[2 * k + 1] = exp(kappa2 * (zf[k] - eta));
+}
+
 }  // namespace
+
+hipError_t launch_beer_factors(const GridDev& g, double kappa1, double kappa2, double* beer, hipStream_t s) {
+    hipLaunchKernelGGL(beer_factors_kernel, dim3((g.Nz + 1 + 63) / 64), dim3(64), 0, s, g.zf, g.Nz, kappa1, kappa2,
+                       beer);
+    return hipGetLastError();
+}
 
 template <typename Real>
 hipError_t launch_halo_fill(const GridDev& g, View3<Real> c, int Hx, int Hy, int Hz, hipStream_t s) {
@@ -810,7 +841,14 @@ hipError_t launch_rk_stage(const GridDev& g, View3<const Real> u, View3<const Re
     const int sr = stochastic ? 1 : 0;
     const uint32_t seed = step_seed(time);
     const bool watch = forcing.freeze_flags != nullptr;
-    if (forcing.two_color && watch)
+    const bool beer = forcing.beer != nullptr;   // a model's factors (per-phase calls compute them)
+    if (forcing.two_color && watch && beer)
+        hipLaunchKernelGGL((tracer_tile_kernel<Real, 0, true, true, true>), grid, block, 0, s, g, L, u.p, v.p, w.p,
+                           T0.p, S0.p, Tin.p, Sin.p, Tout.p, Sout.p, JT.p, JS.p, Jw.p, forcing, beta_dt, sr, seed);
+    else if (forcing.two_color && beer)
+        hipLaunchKernelGGL((tracer_tile_kernel<Real, 0, true, false, true>), grid, block, 0, s, g, L, u.p, v.p, w.p,
+                           T0.p, S0.p, Tin.p, Sin.p, Tout.p, Sout.p, JT.p, JS.p, Jw.p, forcing, beta_dt, sr, seed);
+    else if (forcing.two_color && watch)
         hipLaunchKernelGGL((tracer_tile_kernel<Real, 0, true, true>), grid, block, 0, s, g, L, u.p, v.p, w.p, T0.p,
                            S0.p, Tin.p, Sin.p, Tout.p, Sout.p, JT.p, JS.p, Jw.p, forcing, beta_dt, sr, seed);
     else if (forcing.two_color)

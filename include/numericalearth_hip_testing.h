@@ -20,6 +20,13 @@ extern "C" {
  * correctly rounded one. */
 int ne_log_f64(ne_ctx* ctx, const double* x, double* y, int64_t n, void* stream);
 
+/* The flux solve's FP64 arctangent for arguments x ≥ 1 applied elementwise to x[n] (DEVICE
+ * arrays): y[i] = atan(x[i]) within 2 ulp for x ≥ 1 (+∞ included; other arguments are not
+ * defined). It exposes the library's own arctangent (one quotient of magnitude ≤ tan(π/8) and a
+ * degree-10 polynomial in its square, in place of the math library's) so its accuracy can be
+ * checked against a correctly rounded one. */
+int ne_atan_ge1_f64(ne_ctx* ctx, const double* x, double* y, int64_t n, void* stream);
+
 /* A kernel of known HBM traffic, to read hardware memory counters against: out_j[i] =
  * Σ_k in_k[i] + j for i < n, coalesced and i fastest, so one launch reads n_in·n·8 bytes and
  * writes n_out·n·8 bytes. `in` and `out` are HOST arrays of n_in ≤ NE_STREAM_PROBE_MAX_IN and

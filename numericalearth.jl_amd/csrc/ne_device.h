@@ -4,9 +4,11 @@
 // is compiled with -ffp-contract=off. The general FP64 flux instance keeps the reference's
 // evaluation order with the math library's elementary functions. The instance for the default
 // Edson constants ("Fast", and "Consts" with the default surface constants as literals) departs
-// from it at the ulp level: its own logarithm ne_log (within 2 ulp), A·exp(−b·log R★) for
-// A/R★^b, x·√x for (1 + x)^{3/2}, shared subexpressions of ψ_m and ψ_s, and a degree-7 Taylor
-// polynomial of ψ at |ζ| ≤ 5·10⁻⁴. Those differences stay far inside the 1e-12 comparison with
+// from it at the ulp level: its own logarithm ne_log and arctangent ne_atan_ge1 (each within
+// 2 ulp), A·exp(−b·log R★) for A/R★^b, x·√x for (1 + x)^{3/2}, shared subexpressions of ψ_m and
+// ψ_s, and a degree-7 Taylor polynomial of ψ at |ζ| ≤ 5·10⁻⁴; with the default surface constants
+// also logarithms of quotients as differences of logarithms and ζ as products with 1/L.
+// Those differences stay far inside the 1e-12 comparison with
 // the oracle, but near a stop threshold they can move a column's fixed point by one iteration
 // (its fluxes then differ by that iteration's drift, below the stop tolerance). The WENO
 // reconstructions use fused multiply-adds and a reciprocal-based division (stated there).
@@ -225,11 +227,12 @@ __device__ __forceinline__ double psi_scalar(const ne_params& p, double z) {
 // ---------------------------------------------------------------------------
 
 #include "ne_psi_series.h"
+#include "ne_atan_poly.h"
 #define NE_PSI_SERIES_RADIUS 5e-4
 
 // Elementary functions by precision: FP64 uses the math library (the Fast flux instance its
-// own ne_log below); the FP32 pre-iteration of the mixed form uses the hardware
-// log₂/exp₂-based approximations.
+// own ne_log and ne_atan_ge1 below); the FP32 pre-iteration of the mixed form uses the
+// hardware log₂/exp₂-based approximations.
 // Natural logarithm of a positive, finite, normal double in about 30 vector instructions (the
 // library's log takes about 100): x = 2^e·m with m in [√½, √2), log x = e·ln 2 + 2·atanh(s),
 // s = (m − 1)/(m + 1), |s| ≤ 0.1716 — the argument reduction of Cody & Waite (Software Manual
@@ -261,12 +264,29 @@ __device__ __forceinline__ double ne_log(double x) {
     return fma(de, 0.6931471805599453, fma(de, 2.3190468138462996e-17, lm));
 }
 
+// Arctangent of x ≥ 1 in about 35 vector instructions with one division (the library's atan
+// takes about 80 with two). The stability functions' arctangents have such arguments:
+// (1 − 15ζ)^¼ and (1 + 2(1 − cζ)^⅓)/√3 with ζ < 0. atan x = π/2 − atan(1/x), and for
+// x < 1 + √2 also = π/4 − atan((1 − x)/(1 + x)); the branch taken keeps the quotient u within
+// |u| ≤ tan(π/8), where atan u = u + u·w·P(w), w = u² (ne_atan_poly.h: 0.07 ulp). 1 − x is
+// exact for 1 ≤ x ≤ 1 + √2, the quotient rounds once, and π/4 and π/2 are added in two parts,
+// so the result is within 2 ulp (test_elementary_gpu.py: against numpy's arctan). x = +∞
+// gives π/2; arguments below 1 are outside its range.
+__device__ __forceinline__ double ne_atan_ge1(double x) {
+    const bool near = x < 2.4142135623730951;
+    const double u = (near ? 1.0 - x : 1.0) / (near ? 1.0 + x : x);
+    const double w = u * u;
+    const double a = fma(u * w, ne_atan_poly(w), u);
+    return near ? 0.78539816339744828 + (3.0616169978683830e-17 - a)
+                : 1.5707963267948966 + (6.1232339957367660e-17 - a);
+}
+
 __device__ __forceinline__ double tlog(double x) { return ne_log(x); }
 __device__ __forceinline__ float tlog(float x) { return __logf(x); }
 __device__ __forceinline__ double texp(double x) { return exp(x); }
 __device__ __forceinline__ float texp(float x) { return __expf(x); }
-__device__ __forceinline__ double tatan(double x) { return atan(x); }
-__device__ __forceinline__ float tatan(float x) { return atanf(x); }
+__device__ __forceinline__ double tatan_ge1(double x) { return ne_atan_ge1(x); }
+__device__ __forceinline__ float tatan_ge1(float x) { return atanf(x); }
 __device__ __forceinline__ double tcbrt(double x) { return cbrt(x); }
 __device__ __forceinline__ float tcbrt(float x) { return cbrtf(x); }
 __device__ __forceinline__ double tsqrt(double x) { return sqrt(x); }
@@ -278,9 +298,9 @@ __device__ __forceinline__ T psi_m_default(T z) {
     const T pi = (T)3.14159265358979323846, s3 = (T)1.7320508075688772935;
     if (z < (T)0) {
         T xs = tsqrt((T)1 - (T)15 * z), xm = tsqrt(xs);
-        T psi1 = (T)2 * tlog(((T)1 + xm) * (T)0.5) + tlog(((T)1 + xs) * (T)0.5) - (T)2 * tatan(xm) + pi / (T)2;
+        T psi1 = (T)2 * tlog(((T)1 + xm) * (T)0.5) + tlog(((T)1 + xs) * (T)0.5) - (T)2 * tatan_ge1(xm) + pi / (T)2;
         T f2 = tcbrt((T)1 - (T)10.15 * z);
-        T psi2 = (T)1.5 * tlog(((T)1 + f2 + f2 * f2) * (T)(1.0 / 3.0)) - s3 * tatan(((T)1 + (T)2 * f2) * ((T)1 / s3)) + pi / s3;
+        T psi2 = (T)1.5 * tlog(((T)1 + f2 + f2 * f2) * (T)(1.0 / 3.0)) - s3 * tatan_ge1(((T)1 + (T)2 * f2) * ((T)1 / s3)) + pi / s3;
         T f = z * z / ((T)1 + z * z);
         return ((T)1 - f) * psi1 + f * psi2;
     }
@@ -296,7 +316,7 @@ __device__ __forceinline__ T psi_s_default(T z) {
         T xs = tsqrt((T)1 - (T)15 * z);
         T psi1 = (T)2 * tlog(((T)1 + xs) * (T)0.5);
         T f2 = tcbrt((T)1 - (T)34.15 * z);
-        T psi2 = (T)1.5 * tlog(((T)1 + f2 + f2 * f2) * (T)(1.0 / 3.0)) - s3 * tatan(((T)1 + (T)2 * f2) * ((T)1 / s3)) + pi / s3;
+        T psi2 = (T)1.5 * tlog(((T)1 + f2 + f2 * f2) * (T)(1.0 / 3.0)) - s3 * tatan_ge1(((T)1 + (T)2 * f2) * ((T)1 / s3)) + pi / s3;
         T f = z * z / ((T)1 + z * z);
         return ((T)1 - f) * psi1 + f * psi2;
     }
@@ -318,11 +338,11 @@ __device__ __forceinline__ void psi_pair_default(T z, T& pm, T& ps) {
     if (z < (T)0) {
         T xs = tsqrt((T)1 - (T)15 * z), xm = tsqrt(xs);
         T l2 = tlog(((T)1 + xs) * (T)0.5);
-        T psi1m = (T)2 * tlog(((T)1 + xm) * (T)0.5) + l2 - (T)2 * tatan(xm) + pi / (T)2;
+        T psi1m = (T)2 * tlog(((T)1 + xm) * (T)0.5) + l2 - (T)2 * tatan_ge1(xm) + pi / (T)2;
         T psi1s = (T)2 * l2;
         T fm = tcbrt((T)1 - (T)10.15 * z), fs = tcbrt((T)1 - (T)34.15 * z);
-        T psi2m = (T)1.5 * tlog(((T)1 + fm + fm * fm) * (T)(1.0 / 3.0)) - s3 * tatan(((T)1 + (T)2 * fm) * ((T)1 / s3)) + pi / s3;
-        T psi2s = (T)1.5 * tlog(((T)1 + fs + fs * fs) * (T)(1.0 / 3.0)) - s3 * tatan(((T)1 + (T)2 * fs) * ((T)1 / s3)) + pi / s3;
+        T psi2m = (T)1.5 * tlog(((T)1 + fm + fm * fm) * (T)(1.0 / 3.0)) - s3 * tatan_ge1(((T)1 + (T)2 * fm) * ((T)1 / s3)) + pi / s3;
+        T psi2s = (T)1.5 * tlog(((T)1 + fs + fs * fs) * (T)(1.0 / 3.0)) - s3 * tatan_ge1(((T)1 + (T)2 * fs) * ((T)1 / s3)) + pi / s3;
         T f = z * z / ((T)1 + z * z);
         pm = ((T)1 - f) * psi1m + f * psi2m;
         ps = ((T)1 - f) * psi1s + f * psi2s;
@@ -727,6 +747,20 @@ __device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p_in, doubl
                                     p.temperature_roughness_length == p.water_vapor_roughness_length);
         const int maxit = p.maxiter;
         const double tol = p.tolerance;
+        // LogForm (the default constants compiled in): the logarithms of the profiles' quotients
+        // as differences of logarithms, log(h/ℓ) = log h − log ℓ, with log R★ = log ℓᵤ + log u★ −
+        // log ν and log ℓq = min(log A − b·log R★, log ℓmax), so an iteration takes two
+        // logarithms (ℓᵤ, u★) and no quotients where it took three of each, and the three
+        // stability arguments as products with 1/L. The terms are of size ≤ 12 and the results
+        // agree with the quotient forms to a few ulp of those terms. u★ > 0 (the iteration's first
+        // guess, and κU over a positive profile with U ≥ the minimum gustiness).
+        constexpr bool LogForm = Fast && Consts;
+        const double zd = za - p.zero_plane_displacement;
+        const double ln_nu = LogForm ? ne_log(p.air_kinematic_viscosity) : 0.0;
+        const double ln_A = LogForm ? ne_log(p.reynolds_scaling_A) : 0.0;
+        const double ln_lqmax = LogForm ? ne_log(p.maximum_scalar_roughness) : 0.0;
+        const double ln_zd = LogForm ? ne_log(zd) : 0.0;
+        const double ln2 = 0.6931471805599453;
 
         bool done32 = false;   // the FP32 fixed point converged: no FP64 iteration
         if (Mixed && Fast && converge && shared_scalar && p.momentum_roughness == NE_ROUGHNESS_EDSON &&
@@ -792,8 +826,13 @@ __device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p_in, doubl
             double Usg2 = gustiness_squared(p, us, bstar, hbl);
             double U = sqrt(du2dv2 + Usg2);
             double lu = momentum_roughness(p, us);
-            double lq, lt;
-            if (Fast && p.water_vapor_roughness == NE_ROUGHNESS_EDSON) {
+            double lq, lt, lnlu = 0.0, lnlq = 0.0;
+            if constexpr (LogForm) {
+                lnlu = ne_log(lu);
+                const double lnRs = lnlu + ne_log(us) - ln_nu;
+                lq = jmin(p.reynolds_scaling_A * exp(-p.reynolds_scaling_b * lnRs), p.maximum_scalar_roughness);
+                lnlq = jmin(ln_A - p.reynolds_scaling_b * lnRs, ln_lqmax);
+            } else if (Fast && p.water_vapor_roughness == NE_ROUGHNESS_EDSON) {
                 double Rstar = lu * us / p.air_kinematic_viscosity;
                 double ls = Rstar == 0.0 ? 0.0 : p.reynolds_scaling_A * exp(-p.reynolds_scaling_b * ne_log(Rstar));
                 lq = jmin(ls, p.maximum_scalar_roughness);
@@ -803,15 +842,24 @@ __device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p_in, doubl
             lt = shared_scalar ? lq
                                : scalar_roughness(p, p.temperature_roughness, p.temperature_roughness_length, lu, us);
             double dh = jmax(za - p.zero_plane_displacement, 2.0 * lu);
-            double L = bstar == 0.0 ? (double)INFINITY : us * us / (kappa * bstar);
             double chiu, chiq, chit;
-            if (Fast) {
+            if constexpr (LogForm) {
+                const double invL = kappa * bstar / (us * us);
+                const double lndh = 2.0 * lu > zd ? ln2 + lnlu : ln_zd;
+                double pmh, psh;
+                psi_pair_default(dh * invL, pmh, psh);
+                chiu = kappa / ((lndh - lnlu) - pmh + psi_m_default(lu * invL));
+                chiq = kappa / ((lndh - lnlq) - psh + psi_s_default(lq * invL));
+                chit = chiq;
+            } else if (Fast) {
+                double L = bstar == 0.0 ? (double)INFINITY : us * us / (kappa * bstar);
                 double pmh, psh;
                 psi_pair_default(dh / L, pmh, psh);
                 chiu = kappa / (ne_log(dh / lu) - pmh + psi_m_default(lu / L));
                 chiq = kappa / (ne_log(dh / lq) - psh + psi_s_default(lq / L));
                 chit = shared_scalar ? chiq : kappa / (ne_log(dh / lt) - psh + psi_s_default(lt / L));
             } else {
+                double L = bstar == 0.0 ? (double)INFINITY : us * us / (kappa * bstar);
                 chiu = kappa / similarity_profile(p, false, dh, lu, L);
                 chiq = kappa / similarity_profile(p, true, dh, lq, L);
                 chit = shared_scalar ? chiq : kappa / similarity_profile(p, true, dh, lt, L);

@@ -55,6 +55,16 @@ __device__ __forceinline__ int sorted_block_column(int key, int t) {
     return s_col[t];
 }
 
+// Sort key of a column for the flux solve: the iteration count its previous solve took, and
+// the columns that ended stable (buoyancy flux into the ocean, ζ > 0: θ★ + 0.61·T·q★ > 0 with
+// T ≈ 290 K) after the unstable ones, so that fewer wavefronts take both branches of the
+// stability functions.
+__device__ __forceinline__ int solve_order_key(const FluxOut& out, int i, int j) {
+    const int it = min(max(out.iters.at2(i, j), 0), SORT_BINS / 2 - 2);
+    const bool stable = out.ts.at2(i, j) + 176.0 * out.qs.at2(i, j) > 0.0;
+    return stable ? SORT_BINS / 2 - 1 + it : it;
+}
+
 // Block-wide reduction of the solver counters, then one atomic per block.
 __device__ void accumulate_counters(ne_counters* c, bool solved, int it, bool nonconv, bool nan) {
     __shared__ unsigned s_it[BX * BY], s_solved[BX * BY], s_nc[BX * BY], s_nan[BX * BY], s_max[BX * BY];
@@ -125,6 +135,11 @@ __global__ void fractional_indices_kernel(GridDev g, ne_source_grid s, double* f
 __global__ void log_kernel(const double* x, double* y, int64_t n) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i < n) y[i] = ne_log(x[i]);
+}
+
+__global__ void atan_ge1_kernel(const double* x, double* y, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) y[i] = ne_atan_ge1(x[i]);
 }
 
 __global__ void fractional_indices_nodes_kernel(GridDev g, const double* xn, int nx, int tx, const double* yn, int ny,
@@ -247,9 +262,9 @@ flux_solve_exchange_kernel(GridDev g, ne_params p, View3<const double> ua, View3
                                            View3<const double> lw, FluxOut out, ne_counters* counters) {
     int i = blockIdx.x * BX + threadIdx.x;
     int j = blockIdx.y * BY + threadIdx.y;
-    if (out.iters.p) {   // columns in order of their previous iteration count (as in K1)
+    if (out.iters.p) {   // columns in the order K1 takes them (previous iteration count, stability)
         const bool own = i < g.Nx && j < g.Ny;
-        const int key = own ? min(max(out.iters.at2(i, j), 0), SORT_BINS - 2) : SORT_BINS - 1;
+        const int key = own ? solve_order_key(out, i, j) : SORT_BINS - 1;
         const int c = sorted_block_column(key, threadIdx.x + threadIdx.y * BX);
         i = blockIdx.x * BX + c % BX;
         j = blockIdx.y * BY + c / BX;
@@ -550,13 +565,14 @@ surface_update_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __
     int i = blockIdx.x * BX + threadIdx.x;
     int j = blockIdx.y * BY + threadIdx.y;
     if (out.iters.p) {
-        // Columns of the block in order of the iteration count the previous solve took: the
-        // fixed point runs until the slowest column of a wavefront converges, so wavefronts of
-        // columns with like counts leave fewer lanes idle. Each column's result does not depend
-        // on the lane that computes it.
+        // Columns of the block in order of the iteration count the previous solve took, the
+        // stable ones last (solve_order_key): the fixed point runs until the slowest column of a
+        // wavefront converges, so wavefronts of columns with like counts leave fewer lanes idle,
+        // and a wavefront without stable columns skips that branch. Each column's result does
+        // not depend on the lane that computes it.
         const int t = threadIdx.x + threadIdx.y * BX;
         const bool own = i < g.Nx && j < g.Ny;
-        const int key = own ? min(max(out.iters.at2(i, j), 0), SORT_BINS - 2) : SORT_BINS - 1;
+        const int key = own ? solve_order_key(out, i, j) : SORT_BINS - 1;
         const int c = sorted_block_column(key, t);
         i = blockIdx.x * BX + c % BX;
         j = blockIdx.y * BY + c / BX;
@@ -642,9 +658,8 @@ batch_surface_solve_kernel(const BatchRec<Real>* __restrict__ recs, const WorkIt
         const GridDev g = rc.g;
         int col = it.index + (int)threadIdx.x;
         if (rc.f.iters.p && blockDim.x > 64) {
-            // the item's columns in order of their previous iteration count (as the single model)
-            const int key = col < g.Nx * g.Ny ? min(max(rc.f.iters.at2(col % g.Nx, col / g.Nx), 0), SORT_BINS - 2)
-                                              : SORT_BINS - 1;
+            // the item's columns in the single model's order (previous iteration count, stability)
+            const int key = col < g.Nx * g.Ny ? solve_order_key(rc.f, col % g.Nx, col / g.Nx) : SORT_BINS - 1;
             col = it.index + sorted_block_column(key, (int)threadIdx.x);
         }
         const bool in = col < g.Nx * g.Ny;
@@ -745,6 +760,11 @@ RadOut rad_out(const ne_radiative_fluxes& r) {
 
 hipError_t launch_immersed_levels(const GridDev& g, View3<const double> bottom, int32_t* kmin, hipStream_t s) {
     hipLaunchKernelGGL(immersed_levels_kernel, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, bottom, kmin);
+    return hipGetLastError();
+}
+
+hipError_t launch_atan_ge1(const double* x, double* y, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(atan_ge1_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, n);
     return hipGetLastError();
 }
 

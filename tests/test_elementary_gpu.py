@@ -1,7 +1,9 @@
 """The flux solve's FP64 logarithm (ne_log_f64) against numpy's, on the GPU: within 2 ulp over
 the ranges the solve passes it (arguments between ½ and 2 after reduction, ratios of heights to
 roughness lengths, Reynolds numbers) and over the whole normal range, and exact where the value
-is: log 1 = 0, and log 2^k = k·ln 2 rounded once."""
+is: log 1 = 0, and log 2^k = k·ln 2 rounded once. Its arctangent for arguments ≥ 1
+(ne_atan_ge1_f64) against numpy's: within 2 ulp on both sides of the switch at 1 + √2 and up to
+the largest doubles, π/4 at 1 and π/2 at +∞."""
 from decimal import Decimal, getcontext
 
 import numpy as np
@@ -20,6 +22,16 @@ def _device_log(x):
     xd = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).cuda()
     yd = torch.empty_like(xd)
     check(ctx.lib.ne_log_f64(ctx.handle, xd.data_ptr(), yd.data_ptr(), xd.numel(), stream_handle()), "ne_log_f64")
+    torch.cuda.synchronize()
+    return yd.cpu().numpy()
+
+
+def _device_atan_ge1(x):
+    ctx = Context.get(0)
+    xd = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).cuda()
+    yd = torch.empty_like(xd)
+    check(ctx.lib.ne_atan_ge1_f64(ctx.handle, xd.data_ptr(), yd.data_ptr(), xd.numel(), stream_handle()),
+          "ne_atan_ge1_f64")
     torch.cuda.synchronize()
     return yd.cpu().numpy()
 
@@ -56,6 +68,27 @@ def test_log_exact_values(require_gpu):
     assert np.max(_ulps(y[k != 0], ref[k != 0])) <= 1.0
     assert y[k == 0][0] == 0.0 and _device_log(np.array([1.0]))[0] == 0.0
     assert np.isnan(_device_log(np.array([np.nan]))[0])
+
+
+def test_atan_ge1_within_two_ulp(require_gpu):
+    rng = np.random.default_rng(43)
+    s = 1.0 + np.sqrt(2.0)
+    samples = {
+        "[1, 1 + √2)": rng.uniform(1.0, s, 1 << 20),
+        "around 1 + √2": s + rng.uniform(-1e-6, 1e-6, 1 << 16),
+        "stability arguments (1 − 15ζ)^¼, (1 + 2(1 − cζ)^⅓)/√3": np.concatenate(
+            [(1.0 + 15.0 * 10.0 ** rng.uniform(-4, 3, 1 << 18)) ** 0.25,
+             (1.0 + 2.0 * np.cbrt(1.0 + 34.15 * 10.0 ** rng.uniform(-4, 3, 1 << 18))) / np.sqrt(3.0)]),
+        "[1 + √2, 1e308]": 10.0 ** rng.uniform(np.log10(s), 308.0, 1 << 20),
+    }
+    for name, x in samples.items():
+        y = _device_atan_ge1(x)
+        worst = float(np.max(_ulps(y, np.arctan(x))))
+        print(f"[atan] {name}: worst {worst:.2f} ulp over {x.size} arguments", flush=True)
+        assert worst <= 2.0, name
+    y = _device_atan_ge1(np.array([1.0, np.inf, np.nan, np.nextafter(1.0, 2.0)]))
+    assert y[0] == np.pi / 4 and y[1] == np.pi / 2 and np.isnan(y[2])
+    assert _ulps(y[3:], np.arctan(np.array([np.nextafter(1.0, 2.0)])))[0] <= 1.0
 
 
 @pytest.mark.parametrize("n_in,n_out,n", [(5, 2, 1000003), (0, 1, 77), (8, 4, 4096)])

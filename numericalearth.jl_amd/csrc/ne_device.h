@@ -4,8 +4,8 @@
 // is compiled with -ffp-contract=off. The general FP64 flux instance keeps the reference's
 // evaluation order with the math library's elementary functions. The instance for the default
 // Edson constants ("Fast", and "Consts" with the default surface constants as literals) departs
-// from it at the ulp level: its own logarithm ne_log and arctangent ne_atan_ge1 (each within
-// 2 ulp), A·exp(−b·log R★) for A/R★^b, x·√x for (1 + x)^{3/2}, shared subexpressions of ψ_m and
+// from it at the ulp level: its own logarithm ne_log, arctangent ne_atan_ge1 and cube root
+// ne_cbrt_ge1 (each within 2 ulp), A·exp(−b·log R★) for A/R★^b, x·√x for (1 + x)^{3/2}, shared subexpressions of ψ_m and
 // ψ_s, and a degree-7 Taylor polynomial of ψ at |ζ| ≤ 5·10⁻⁴; with the default surface constants
 // also logarithms of quotients as differences of logarithms and ζ as products with 1/L.
 // Those differences stay far inside the 1e-12 comparison with
@@ -231,7 +231,7 @@ __device__ __forceinline__ double psi_scalar(const ne_params& p, double z) {
 #define NE_PSI_SERIES_RADIUS 5e-4
 
 // Elementary functions by precision: FP64 uses the math library (the Fast flux instance its
-// own ne_log and ne_atan_ge1 below); the FP32 pre-iteration of the mixed form uses the
+// own ne_log, ne_atan_ge1 and ne_cbrt_ge1 below); the FP32 pre-iteration of the mixed form uses the
 // hardware log₂/exp₂-based approximations.
 // Natural logarithm of a positive, finite, normal double in about 30 vector instructions (the
 // library's log takes about 100): x = 2^e·m with m in [√½, √2), log x = e·ln 2 + 2·atanh(s),
@@ -281,14 +281,34 @@ __device__ __forceinline__ double ne_atan_ge1(double x) {
                 : 1.5707963267948966 + (6.1232339957367660e-17 - a);
 }
 
+// Cube root of x ≥ 1 in 14 vector instructions (the library's cbrt takes about 31). The
+// stability functions' cube roots have such arguments: 1 − cζ with ζ < 0. A first value from the
+// hardware's FP32 log₂ and exp₂, y₀ = 2^(log₂x / 3) (relative error near 10⁻⁷), then one step of
+// Halley's iteration for y³ = x in FP64, y = y₀ − y₀(y₀³ − x)/(2y₀³ + x), which cubes the
+// relative error: the correction is of size 10⁻⁷·y₀, so its reciprocal needs 10⁻⁹ only — the
+// hardware's approximation (about 2⁻²⁶ alone, which left 32 ulp) with one Newton step — and
+// y₀³ − x is one fused operation; the result is within 1 ulp
+// (test_elementary_gpu.py: against numpy's cbrt). Arguments below 1 or from 2¹²⁸ (beyond FP32)
+// are outside its range.
+__device__ __forceinline__ double ne_cbrt_ge1(double x) {
+    const float l2 = __builtin_amdgcn_logf((float)x);
+    const double y0 = (double)__builtin_amdgcn_exp2f(l2 * (1.0f / 3.0f));
+    const double y2 = y0 * y0;
+    const double t = fma(y2, y0, -x);
+    const double d = fma(y2 + y2, y0, x);
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return fma(-(y0 * t), r, y0);
+}
+
 __device__ __forceinline__ double tlog(double x) { return ne_log(x); }
 __device__ __forceinline__ float tlog(float x) { return __logf(x); }
 __device__ __forceinline__ double texp(double x) { return exp(x); }
 __device__ __forceinline__ float texp(float x) { return __expf(x); }
 __device__ __forceinline__ double tatan_ge1(double x) { return ne_atan_ge1(x); }
 __device__ __forceinline__ float tatan_ge1(float x) { return atanf(x); }
-__device__ __forceinline__ double tcbrt(double x) { return cbrt(x); }
-__device__ __forceinline__ float tcbrt(float x) { return cbrtf(x); }
+__device__ __forceinline__ double tcbrt_ge1(double x) { return ne_cbrt_ge1(x); }
+__device__ __forceinline__ float tcbrt_ge1(float x) { return cbrtf(x); }
 __device__ __forceinline__ double tsqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float tsqrt(float x) { return sqrtf(x); }
 
@@ -299,7 +319,7 @@ __device__ __forceinline__ T psi_m_default(T z) {
     if (z < (T)0) {
         T xs = tsqrt((T)1 - (T)15 * z), xm = tsqrt(xs);
         T psi1 = (T)2 * tlog(((T)1 + xm) * (T)0.5) + tlog(((T)1 + xs) * (T)0.5) - (T)2 * tatan_ge1(xm) + pi / (T)2;
-        T f2 = tcbrt((T)1 - (T)10.15 * z);
+        T f2 = tcbrt_ge1((T)1 - (T)10.15 * z);
         T psi2 = (T)1.5 * tlog(((T)1 + f2 + f2 * f2) * (T)(1.0 / 3.0)) - s3 * tatan_ge1(((T)1 + (T)2 * f2) * ((T)1 / s3)) + pi / s3;
         T f = z * z / ((T)1 + z * z);
         return ((T)1 - f) * psi1 + f * psi2;
@@ -315,7 +335,7 @@ __device__ __forceinline__ T psi_s_default(T z) {
     if (z < (T)0) {
         T xs = tsqrt((T)1 - (T)15 * z);
         T psi1 = (T)2 * tlog(((T)1 + xs) * (T)0.5);
-        T f2 = tcbrt((T)1 - (T)34.15 * z);
+        T f2 = tcbrt_ge1((T)1 - (T)34.15 * z);
         T psi2 = (T)1.5 * tlog(((T)1 + f2 + f2 * f2) * (T)(1.0 / 3.0)) - s3 * tatan_ge1(((T)1 + (T)2 * f2) * ((T)1 / s3)) + pi / s3;
         T f = z * z / ((T)1 + z * z);
         return ((T)1 - f) * psi1 + f * psi2;
@@ -340,7 +360,7 @@ __device__ __forceinline__ void psi_pair_default(T z, T& pm, T& ps) {
         T l2 = tlog(((T)1 + xs) * (T)0.5);
         T psi1m = (T)2 * tlog(((T)1 + xm) * (T)0.5) + l2 - (T)2 * tatan_ge1(xm) + pi / (T)2;
         T psi1s = (T)2 * l2;
-        T fm = tcbrt((T)1 - (T)10.15 * z), fs = tcbrt((T)1 - (T)34.15 * z);
+        T fm = tcbrt_ge1((T)1 - (T)10.15 * z), fs = tcbrt_ge1((T)1 - (T)34.15 * z);
         T psi2m = (T)1.5 * tlog(((T)1 + fm + fm * fm) * (T)(1.0 / 3.0)) - s3 * tatan_ge1(((T)1 + (T)2 * fm) * ((T)1 / s3)) + pi / s3;
         T psi2s = (T)1.5 * tlog(((T)1 + fs + fs * fs) * (T)(1.0 / 3.0)) - s3 * tatan_ge1(((T)1 + (T)2 * fs) * ((T)1 / s3)) + pi / s3;
         T f = z * z / ((T)1 + z * z);

@@ -142,6 +142,11 @@ __global__ void atan_ge1_kernel(const double* x, double* y, int64_t n) {
     if (i < n) y[i] = ne_atan_ge1(x[i]);
 }
 
+__global__ void cbrt_ge1_kernel(const double* x, double* y, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) y[i] = ne_cbrt_ge1(x[i]);
+}
+
 __global__ void fractional_indices_nodes_kernel(GridDev g, const double* xn, int nx, int tx, const double* yn, int ny,
                                                 int ty, double* fi, double* fj) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -760,6 +765,11 @@ RadOut rad_out(const ne_radiative_fluxes& r) {
 
 hipError_t launch_immersed_levels(const GridDev& g, View3<const double> bottom, int32_t* kmin, hipStream_t s) {
     hipLaunchKernelGGL(immersed_levels_kernel, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, bottom, kmin);
+    return hipGetLastError();
+}
+
+hipError_t launch_cbrt_ge1(const double* x, double* y, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(cbrt_ge1_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, n);
     return hipGetLastError();
 }
 

@@ -3,7 +3,8 @@ the ranges the solve passes it (arguments between ½ and 2 after reduction, rati
 roughness lengths, Reynolds numbers) and over the whole normal range, and exact where the value
 is: log 1 = 0, and log 2^k = k·ln 2 rounded once. Its arctangent for arguments ≥ 1
 (ne_atan_ge1_f64) against numpy's: within 2 ulp on both sides of the switch at 1 + √2 and up to
-the largest doubles, π/4 at 1 and π/2 at +∞."""
+the largest doubles, π/4 at 1 and π/2 at +∞. Its cube root for arguments ≥ 1 (ne_cbrt_ge1_f64)
+against numpy's: within 1 ulp up to 1e38, exact at small cubes and powers of eight."""
 from decimal import Decimal, getcontext
 
 import numpy as np
@@ -32,6 +33,16 @@ def _device_atan_ge1(x):
     yd = torch.empty_like(xd)
     check(ctx.lib.ne_atan_ge1_f64(ctx.handle, xd.data_ptr(), yd.data_ptr(), xd.numel(), stream_handle()),
           "ne_atan_ge1_f64")
+    torch.cuda.synchronize()
+    return yd.cpu().numpy()
+
+
+def _device_cbrt_ge1(x):
+    ctx = Context.get(0)
+    xd = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).cuda()
+    yd = torch.empty_like(xd)
+    check(ctx.lib.ne_cbrt_ge1_f64(ctx.handle, xd.data_ptr(), yd.data_ptr(), xd.numel(), stream_handle()),
+          "ne_cbrt_ge1_f64")
     torch.cuda.synchronize()
     return yd.cpu().numpy()
 
@@ -89,6 +100,23 @@ def test_atan_ge1_within_two_ulp(require_gpu):
     y = _device_atan_ge1(np.array([1.0, np.inf, np.nan, np.nextafter(1.0, 2.0)]))
     assert y[0] == np.pi / 4 and y[1] == np.pi / 2 and np.isnan(y[2])
     assert _ulps(y[3:], np.arctan(np.array([np.nextafter(1.0, 2.0)])))[0] <= 1.0
+
+
+def test_cbrt_ge1_within_one_ulp(require_gpu):
+    rng = np.random.default_rng(47)
+    samples = {
+        "[1, 8]": rng.uniform(1.0, 8.0, 1 << 20),
+        "stability arguments 1 − cζ": 1.0 + 34.15 * 10.0 ** rng.uniform(-4, 4, 1 << 19),
+        "[8, 1e38]": 10.0 ** rng.uniform(np.log10(8.0), 38.0, 1 << 20),
+        "cubes": np.arange(1.0, 1 << 17) ** 3,
+    }
+    for name, x in samples.items():
+        y = _device_cbrt_ge1(x)
+        worst = float(np.max(_ulps(y, np.cbrt(x))))
+        print(f"[cbrt] {name}: worst {worst:.2f} ulp over {x.size} arguments", flush=True)
+        assert worst <= 1.0, name
+    y = _device_cbrt_ge1(np.array([1.0, 8.0, 27.0, 2.0 ** 120]))
+    assert y[0] == 1.0 and y[1] == 2.0 and y[2] == 3.0 and y[3] == 2.0 ** 40
 
 
 @pytest.mark.parametrize("n_in,n_out,n", [(5, 2, 1000003), (0, 1, 77), (8, 4, 4096)])

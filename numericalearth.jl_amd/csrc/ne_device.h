@@ -80,6 +80,15 @@ __host__ __device__ __forceinline__ int wrapi(int i, int n) {
     return r < 0 ? r + n : r;
 }
 
+// wrapi for the indices grids and source fields use, −n ≤ i < 2n: one conditional add or
+// subtract in place of a remainder by a run-time n (about 30 instructions); any other i takes
+// the remainder, in a branch of its own.
+__host__ __device__ __forceinline__ int wrap_near(int i, int n) {
+    const int r = i < 0 ? i + n : (i >= n ? i - n : i);
+    if (r >= 0 && r < n) return r;
+    return wrapi(i, n);
+}
+
 // Grid geometry as seen by kernels (device pointers, passed by value).
 struct GridDev {
     int Nx, Ny, Nz;
@@ -98,9 +107,9 @@ struct GridDev {
 
     // Column index after periodic wrapping; -1 outside a bounded direction.
     __device__ __forceinline__ int64_t column(int i, int j) const {
-        if (tx == NE_PERIODIC) i = wrapi(i, Nx);
+        if (tx == NE_PERIODIC) i = wrap_near(i, Nx);
         else if (i < 0 || i >= Nx) return -1;
-        if (ty == NE_PERIODIC) j = wrapi(j, Ny);
+        if (ty == NE_PERIODIC) j = wrap_near(j, Ny);
         else if (j < 0 || j >= Ny) return -1;
         return (int64_t)i + (int64_t)j * Nx;
     }
@@ -110,8 +119,8 @@ struct GridDev {
         int64_t c = column(i, j);
         return c >= 0 && k >= kmin[c];
     }
-    __device__ __forceinline__ int wx(int i) const { return tx == NE_PERIODIC ? wrapi(i, Nx) : i; }
-    __device__ __forceinline__ int wy(int j) const { return ty == NE_PERIODIC ? wrapi(j, Ny) : j; }
+    __device__ __forceinline__ int wx(int i) const { return tx == NE_PERIODIC ? wrap_near(i, Nx) : i; }
+    __device__ __forceinline__ int wy(int j) const { return ty == NE_PERIODIC ? wrap_near(j, Ny) : j; }
 };
 
 __host__ inline GridDev grid_dev(const ne_grid& g) {
@@ -301,16 +310,28 @@ __device__ __forceinline__ double ne_cbrt_ge1(double x) {
     return fma(-(y0 * t), r, y0);
 }
 
+// FP32 functions of the mixed form's flux iteration: the hardware's approximations (log₂, exp₂,
+// square root and reciprocal, each within about 1 FP32 ulp for normal arguments) without the
+// library's handling of denormal and special arguments, which takes 10–17 instructions per call
+// where these take 1–3. The iteration passes positive, normal arguments (a cube root of zero
+// gives zero); its iterate is accurate to its stop criterion, a few FP32 ulp of u★ + θ★ + q★.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fcbrt(float x) { return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.0f / 3.0f)); }
+
 __device__ __forceinline__ double tlog(double x) { return ne_log(x); }
-__device__ __forceinline__ float tlog(float x) { return __logf(x); }
+__device__ __forceinline__ float tlog(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
 __device__ __forceinline__ double texp(double x) { return exp(x); }
 __device__ __forceinline__ float texp(float x) { return __expf(x); }
 __device__ __forceinline__ double tatan_ge1(double x) { return ne_atan_ge1(x); }
 __device__ __forceinline__ float tatan_ge1(float x) { return atanf(x); }
 __device__ __forceinline__ double tcbrt_ge1(double x) { return ne_cbrt_ge1(x); }
-__device__ __forceinline__ float tcbrt_ge1(float x) { return cbrtf(x); }
+__device__ __forceinline__ float tcbrt_ge1(float x) { return fcbrt(x); }
 __device__ __forceinline__ double tsqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float tsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ float tsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+template <typename T>
+__device__ __forceinline__ T tdiv(T a, T b) { return a / b; }
+template <>
+__device__ __forceinline__ float tdiv<float>(float a, float b) { return a * frcp(b); }
 
 template <typename T>
 __device__ __forceinline__ T psi_m_default(T z) {
@@ -321,7 +342,7 @@ __device__ __forceinline__ T psi_m_default(T z) {
         T psi1 = (T)2 * tlog(((T)1 + xm) * (T)0.5) + tlog(((T)1 + xs) * (T)0.5) - (T)2 * tatan_ge1(xm) + pi / (T)2;
         T f2 = tcbrt_ge1((T)1 - (T)10.15 * z);
         T psi2 = (T)1.5 * tlog(((T)1 + f2 + f2 * f2) * (T)(1.0 / 3.0)) - s3 * tatan_ge1(((T)1 + (T)2 * f2) * ((T)1 / s3)) + pi / s3;
-        T f = z * z / ((T)1 + z * z);
+        T f = tdiv(z * z, (T)1 + z * z);
         return ((T)1 - f) * psi1 + f * psi2;
     }
     T e = texp(-fmin((T)50, (T)0.35 * z));
@@ -337,7 +358,7 @@ __device__ __forceinline__ T psi_s_default(T z) {
         T psi1 = (T)2 * tlog(((T)1 + xs) * (T)0.5);
         T f2 = tcbrt_ge1((T)1 - (T)34.15 * z);
         T psi2 = (T)1.5 * tlog(((T)1 + f2 + f2 * f2) * (T)(1.0 / 3.0)) - s3 * tatan_ge1(((T)1 + (T)2 * f2) * ((T)1 / s3)) + pi / s3;
-        T f = z * z / ((T)1 + z * z);
+        T f = tdiv(z * z, (T)1 + z * z);
         return ((T)1 - f) * psi1 + f * psi2;
     }
     T e = texp(-fmin((T)50, (T)0.35 * z));
@@ -363,7 +384,7 @@ __device__ __forceinline__ void psi_pair_default(T z, T& pm, T& ps) {
         T fm = tcbrt_ge1((T)1 - (T)10.15 * z), fs = tcbrt_ge1((T)1 - (T)34.15 * z);
         T psi2m = (T)1.5 * tlog(((T)1 + fm + fm * fm) * (T)(1.0 / 3.0)) - s3 * tatan_ge1(((T)1 + (T)2 * fm) * ((T)1 / s3)) + pi / s3;
         T psi2s = (T)1.5 * tlog(((T)1 + fs + fs * fs) * (T)(1.0 / 3.0)) - s3 * tatan_ge1(((T)1 + (T)2 * fs) * ((T)1 / s3)) + pi / s3;
-        T f = z * z / ((T)1 + z * z);
+        T f = tdiv(z * z, (T)1 + z * z);
         pm = ((T)1 - f) * psi1m + f * psi2m;
         ps = ((T)1 - f) * psi1s + f * psi2s;
         return;
@@ -714,11 +735,6 @@ __device__ __noinline__ int flux_iterate_general(const ne_params& p, double ua, 
 #define NE_MIXED_MAXITER 40
 #endif
 
-template <typename T>
-__device__ __forceinline__ T tdiv(T a, T b) { return a / b; }
-template <>
-__device__ __forceinline__ float tdiv<float>(float a, float b) { return __fdividef(a, b); }
-
 template <bool Fast, bool Mixed = false, bool Consts = false, bool Defer = false>
 __device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p_in, double ua, double va, double Ta,
                                                     double pa, double qa, double To_C, double uo, double vo,
@@ -803,19 +819,19 @@ __device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p_in, doubl
                 float bstar = fgoTv * (fts * fonedq + fdTv * fqs);
                 float Usg2 = fmeso2;
                 if (p.convective_gustiness) {
-                    float Ug = fmaxf(fgmin, fbeta * cbrtf(fmaxf(0.0f, -fus * bstar) * fhbl));
+                    float Ug = fmaxf(fgmin, fbeta * fcbrt(fmaxf(0.0f, -fus * bstar) * fhbl));
                     Usg2 = Usg2 + Ug * Ug;
                 }
-                float U = sqrtf(fdu2dv2 + Usg2);
-                float lu = fminf(fCg_g * fus * fus + __fdividef(fCnu_nu, fus), flmax);
+                float U = tsqrt(fdu2dv2 + Usg2);
+                float lu = fminf(fCg_g * fus * fus + tdiv(fCnu_nu, fus), flmax);
                 float Rs = lu * fus * finv_nu;
-                float lq = fminf(Rs == 0.0f ? 0.0f : fA * __expf(-fb * __logf(Rs)), fsmax);
+                float lq = fminf(Rs == 0.0f ? 0.0f : fA * texp(-fb * tlog(Rs)), fsmax);
                 float dh = fmaxf(fzd, 2.0f * lu);
-                float L = bstar == 0.0f ? (float)INFINITY : __fdividef(fus * fus, fkappa * bstar);
+                float L = bstar == 0.0f ? (float)INFINITY : tdiv(fus * fus, fkappa * bstar);
                 float pmh, psh;
-                psi_pair_default<float>(__fdividef(dh, L), pmh, psh);
-                float chiu = __fdividef(fkappa, __logf(__fdividef(dh, lu)) - pmh + psi_m_default<float>(__fdividef(lu, L)));
-                float chiq = __fdividef(fkappa, __logf(__fdividef(dh, lq)) - psh + psi_s_default<float>(__fdividef(lq, L)));
+                psi_pair_default<float>(tdiv(dh, L), pmh, psh);
+                float chiu = tdiv(fkappa, tlog(tdiv(dh, lu)) - pmh + psi_m_default<float>(tdiv(lu, L)));
+                float chiq = tdiv(fkappa, tlog(tdiv(dh, lq)) - psh + psi_s_default<float>(tdiv(lq, L)));
                 float nus = chiu * U, nts = chiq * fdtheta, nqs = chiq * fdq;
                 float drift = fabsf(nus - fus) + fabsf(nts - fts) + fabsf(nqs - fqs);
                 fus = nus; fts = nts; fqs = nqs;
@@ -974,7 +990,10 @@ __device__ __forceinline__ Neighbours neighbours(double f, int n, int topo) {
     int i0 = (int)fl;
     r.w = f - fl;
     int i1 = i0 + 1;
-    if (topo == NE_PERIODIC) { i0 = wrapi(i0, n); i1 = wrapi(i1, n); }
+    if (topo == NE_PERIODIC) {
+        i0 = wrap_near(i0, n);
+        i1 = i0 == n - 1 ? 0 : i0 + 1;
+    }
     else {
         i0 = i0 < 0 ? 0 : (i0 > n - 1 ? n - 1 : i0);
         i1 = i1 < 0 ? 0 : (i1 > n - 1 ? n - 1 : i1);

@@ -252,8 +252,8 @@ __device__ __forceinline__ void ocean_surface(const GridDev& g, const View3<cons
     int kN = g.Nz - 1;
     To = (double)T(i, j, kN);
     So = (double)S(i, j, kN);
-    int ip = g.tx == NE_PERIODIC ? wrapi(i + 1, g.Nx) : i + 1;
-    int jp = g.ty == NE_PERIODIC ? wrapi(j + 1, g.Ny) : j + 1;
+    int ip = g.tx == NE_PERIODIC ? wrap_near(i + 1, g.Nx) : i + 1;
+    int jp = g.ty == NE_PERIODIC ? wrap_near(j + 1, g.Ny) : j + 1;
     uo = ((double)u(i, j, kN) + (double)u(ip, j, kN)) / 2.0;
     vo = ((double)v(i, j, kN) + (double)v(i, jp, kN)) / 2.0;
 }

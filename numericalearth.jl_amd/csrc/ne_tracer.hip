@@ -613,8 +613,8 @@ __global__ void __launch_bounds__(256) w_continuity_kernel(GridDev g, View3<cons
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int j = blockIdx.y * 4 + threadIdx.y;
     if (i >= g.Nx || j >= g.Ny) return;
-    const int ip = g.tx == NE_PERIODIC ? wrapi(i + 1, g.Nx) : i + 1;
-    const int jp = g.ty == NE_PERIODIC ? wrapi(j + 1, g.Ny) : j + 1;
+    const int ip = g.tx == NE_PERIODIC ? wrap_near(i + 1, g.Nx) : i + 1;
+    const int jp = g.ty == NE_PERIODIC ? wrap_near(j + 1, g.Ny) : j + 1;
     const double dyx = g.dy_xface[j], dxs = g.dx_yface[j], dxn = g.dx_yface[jp], Az = g.Az[j];
     double wk = 0.0;
     w(i, j, 0) = (Real)0;

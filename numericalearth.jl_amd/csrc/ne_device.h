@@ -2,7 +2,9 @@
 //
 // Formulas follow the reference expression they restate (cited per function), and the library
 // is compiled with -ffp-contract=off. The general FP64 flux instance keeps the reference's
-// evaluation order with the math library's elementary functions. The instance for the default
+// evaluation order with the math library's elementary functions, apart from the saturation
+// vapour pressure (one exponential of a sum, with ne_log, for a power times an exponential,
+// within a few ulp). The instance for the default
 // Edson constants ("Fast", and "Consts" with the default surface constants as literals) departs
 // from it at the ulp level: its own logarithm ne_log, arctangent ne_atan_ge1 and cube root
 // ne_cbrt_ge1 (each within 2 ulp), A·exp(−b·log R★) for A/R★^b, x·√x for (1 + x)^{3/2}, shared subexpressions of ψ_m and
@@ -161,14 +163,48 @@ __device__ __forceinline__ double latent_heat_vapor(const ne_params& p, double T
 __device__ __forceinline__ double virtual_temperature(const ne_params& p, double T, double q) {
     return gas_constant_air(p, q) / R_d(p) * T;
 }
-// saturation_vapor_pressure, liquid (interface_states.jl:63)
+// Natural logarithm of a positive, finite, normal double in about 30 vector instructions (the
+// library's log takes about 100): x = 2^e·m with m in [√½, √2), log x = e·ln 2 + 2·atanh(s),
+// s = (m − 1)/(m + 1), |s| ≤ 0.1716 — the argument reduction of Cody & Waite (Software Manual
+// for the Elementary Functions, 1980) — with atanh summed as its Taylor series to s¹⁹, whose
+// remainder is below 2⁻⁵⁵ of the result. m − 1 is exact; the division and m + 1 round, and
+// e·ln 2 is added in one fused operation with ln 2 carried in two parts, so the result is
+// within 2 ulp. Zero, negative, infinite and NaN arguments are not handled: the flux solve
+// passes none.
+__device__ __forceinline__ double ne_log(double x) {
+    double m = __builtin_amdgcn_frexp_mant(x);   // [0.5, 1)
+    int e = __builtin_amdgcn_frexp_exp(x);
+    if (m < 0.7071067811865476) {
+        m += m;
+        e -= 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double z = s * s;
+    double q = 2.0 / 19.0;
+    q = fma(q, z, 2.0 / 17.0);
+    q = fma(q, z, 2.0 / 15.0);
+    q = fma(q, z, 2.0 / 13.0);
+    q = fma(q, z, 2.0 / 11.0);
+    q = fma(q, z, 2.0 / 9.0);
+    q = fma(q, z, 2.0 / 7.0);
+    q = fma(q, z, 2.0 / 5.0);
+    q = fma(q, z, 2.0 / 3.0);
+    const double lm = fma(s * z, q, s + s);      // log m = 2s + 2s³/3 + … + 2s¹⁹/19
+    const double de = (double)e;
+    return fma(de, 0.6931471805599453, fma(de, 2.3190468138462996e-17, lm));
+}
+
+// saturation_vapor_pressure, liquid (interface_states.jl:63): pₜᵣ (T/Tₜᵣ)^(Δc/Rᵥ) e^b, evaluated
+// as pₜᵣ e^((Δc/Rᵥ)·log(T/Tₜᵣ) + b) — one logarithm (ne_log) and one exponential in place of the
+// library's pow (≈ 215 instructions) and a second exponential; both exponents are of size
+// below 3, so the result agrees to a few ulp. T is a positive, finite temperature.
 __device__ __forceinline__ double saturation_vapor_pressure(const ne_params& p, double T) {
     double dcp = p.water_vapor_heat_capacity - p.liquid_water_heat_capacity;
     double Rv = R_v(p);
     double Ttr = p.triple_point_temperature;
-    return p.triple_point_pressure * pow(T / Ttr, dcp / Rv) *
-           exp((p.reference_vaporization_enthalpy - dcp * p.reference_temperature) / Rv *
-               (1.0 / Ttr - 1.0 / T));
+    const double a = dcp / Rv * ne_log(T / Ttr);
+    const double b = (p.reference_vaporization_enthalpy - dcp * p.reference_temperature) / Rv * (1.0 / Ttr - 1.0 / T);
+    return p.triple_point_pressure * exp(a + b);
 }
 // surface_specific_humidity(::ImpureSaturationSpecificHumidity) (interface_states.jl:55-74)
 __device__ __forceinline__ double surface_specific_humidity(const ne_params& p, double pa, double Ts) {
@@ -240,38 +276,8 @@ __device__ __forceinline__ double psi_scalar(const ne_params& p, double z) {
 #define NE_PSI_SERIES_RADIUS 5e-4
 
 // Elementary functions by precision: FP64 uses the math library (the Fast flux instance its
-// own ne_log, ne_atan_ge1 and ne_cbrt_ge1 below); the FP32 pre-iteration of the mixed form uses the
-// hardware log₂/exp₂-based approximations.
-// Natural logarithm of a positive, finite, normal double in about 30 vector instructions (the
-// library's log takes about 100): x = 2^e·m with m in [√½, √2), log x = e·ln 2 + 2·atanh(s),
-// s = (m − 1)/(m + 1), |s| ≤ 0.1716 — the argument reduction of Cody & Waite (Software Manual
-// for the Elementary Functions, 1980) — with atanh summed as its Taylor series to s¹⁹, whose
-// remainder is below 2⁻⁵⁵ of the result. m − 1 is exact; the division and m + 1 round, and
-// e·ln 2 is added in one fused operation with ln 2 carried in two parts, so the result is
-// within 2 ulp. Zero, negative, infinite and NaN arguments are not handled: the flux solve
-// passes none.
-__device__ __forceinline__ double ne_log(double x) {
-    double m = __builtin_amdgcn_frexp_mant(x);   // [0.5, 1)
-    int e = __builtin_amdgcn_frexp_exp(x);
-    if (m < 0.7071067811865476) {
-        m += m;
-        e -= 1;
-    }
-    const double s = (m - 1.0) / (m + 1.0);
-    const double z = s * s;
-    double q = 2.0 / 19.0;
-    q = fma(q, z, 2.0 / 17.0);
-    q = fma(q, z, 2.0 / 15.0);
-    q = fma(q, z, 2.0 / 13.0);
-    q = fma(q, z, 2.0 / 11.0);
-    q = fma(q, z, 2.0 / 9.0);
-    q = fma(q, z, 2.0 / 7.0);
-    q = fma(q, z, 2.0 / 5.0);
-    q = fma(q, z, 2.0 / 3.0);
-    const double lm = fma(s * z, q, s + s);      // log m = 2s + 2s³/3 + … + 2s¹⁹/19
-    const double de = (double)e;
-    return fma(de, 0.6931471805599453, fma(de, 2.3190468138462996e-17, lm));
-}
+// own ne_log above, ne_atan_ge1 and ne_cbrt_ge1 below); the FP32 pre-iteration of the mixed form
+// uses the hardware log₂/exp₂-based approximations.
 
 // Arctangent of x ≥ 1 in about 35 vector instructions with one division (the library's atan
 // takes about 80 with two). The stability functions' arctangents have such arguments:

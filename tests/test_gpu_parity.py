@@ -221,6 +221,42 @@ def test_flux_solve_column_order_changes_nothing(require_gpu):
     assert torch.equal(gm.iterations, iters)   # the scrambled counts were overwritten by the real ones
 
 
+def test_flux_solve_across_the_stability_range(require_gpu):
+    """The solve over winds of 0.5–30 m/s and air 25 K colder to 4 K warmer than the sea, with
+    relative humidities of 30–100 % (the same exchange state given to both): ζ = h/L then runs
+    from strongly unstable (free convection, where the stability functions' arctangents, cube
+    roots and logarithms take arguments far from 1) to stable, and every column must match the
+    oracle as at the workload's own conditions."""
+    case, om, gm = _pair(seed=9)
+    ph = Phases(gm)
+    om.interp()
+    ph.interp()
+    torch.cuda.synchronize()
+    Nx, Ny, _ = case.shape
+    U = 10.0 ** np.linspace(np.log10(0.5), np.log10(30.0), Nx)[None, :] * np.ones((Ny, 1))
+    dT = np.linspace(-25.0, 4.0, Ny)[:, None] * np.ones((1, Nx))
+    rh = 0.3 + 0.7 * ((np.arange(Nx)[None, :] * 7 + np.arange(Ny)[:, None] * 3) % 11) / 10.0
+    sst_k = om.T[-1] + 273.15
+    Ta = sst_k + dT
+    es = 611.2 * np.exp(17.67 * (Ta - 273.15) / (Ta - 29.65))        # Bolton's saturation pressure
+    qa = rh * 0.622 * es / om.exchange_arrays["p"]
+    fields = {"u": U * 0.8, "v": U * 0.6, "T": Ta, "q": qa}
+    for k, v in fields.items():
+        om.exchange_arrays[k][...] = v
+        ph.exchange[k].copy_(torch.from_numpy(np.ascontiguousarray(v)))
+    om.flux_solve()
+    ph.flux_solve()
+    torch.cuda.synchronize()
+    us, ts, qs = (om.flux_arrays[k] for k in ("friction_velocity", "temperature_scale", "water_vapor_scale"))
+    wet = om.iterations > 0
+    with np.errstate(divide="ignore", invalid="ignore"):   # land columns: u★ = 0
+        zeta = 10.0 * 0.4 * 9.80665 * (ts + 0.608 * Ta * qs) / (Ta * us ** 2)
+    print(f"[flux] ζ from {zeta[wet].min():.2f} to {zeta[wet].max():.2f}; iterations up to {om.iterations.max()}",
+          flush=True)
+    assert zeta[wet].min() < -5.0 and zeta[wet].max() > 0.1
+    _compare_fluxes(gm, om, max_iteration_mismatch_frac=0.01)
+
+
 def test_flux_solve_fixed_iterations(require_gpu):
     p = to_params(SimilarityTheoryFluxes(solver_stop_criteria=FixedIterations(12)))
     case, om, gm = _pair(params=p, seed=3)

@@ -15,6 +15,13 @@ namespace {
 
 constexpr int BX = 64;
 constexpr int BY = 4;
+// Rows of the deferring K1's blocks (the mixed form's FP32 iteration). Its columns are cheap, and
+// blocks of 8 rows halve what each column pays of the block's fixed work (the column sort, the
+// counters' reduction, the launch): 158 → 125 µs at 1440×560×10 against 4 rows, and 263 µs
+// with 2 (same-box A/B; the FP64 K1 is 2 % slower with 8). A power of two.
+#ifndef NE_FLUX_MIXED_ROWS
+#define NE_FLUX_MIXED_ROWS 8
+#endif
 
 // Minimum wavefronts per SIMD requested for the flux-solve kernels (bounds their VGPRs).
 #ifndef NE_FLUX_WAVES_PER_EU
@@ -33,9 +40,10 @@ inline dim3 grid2d(int Nx, int Ny) { return dim3((Nx + BX - 1) / BX, (Ny + BY - 
 // block-local column (0 … BX·BY−1) that work-item t computes, the columns taken in key order.
 // Ties are ordered by the LDS atomics, which is immaterial (columns are independent).
 constexpr int SORT_BINS = 64;
+template <int N = BX * BY>
 __device__ __forceinline__ int sorted_block_column(int key, int t) {
     __shared__ int s_hist[SORT_BINS];
-    __shared__ int s_col[BX * BY];   // blocks of at most BX·BY work-items
+    __shared__ int s_col[N];   // blocks of at most N work-items
     if (t < SORT_BINS) s_hist[t] = 0;
     __syncthreads();
     const int rank = atomicAdd(&s_hist[key], 1);
@@ -65,9 +73,11 @@ __device__ __forceinline__ int solve_order_key(const FluxOut& out, int i, int j)
     return stable ? SORT_BINS / 2 - 1 + it : it;
 }
 
-// Block-wide reduction of the solver counters, then one atomic per block.
+// Block-wide reduction of the solver counters, then one atomic per block (blocks of a
+// power-of-two number of work-items, at most N).
+template <int N = BX * BY>
 __device__ void accumulate_counters(ne_counters* c, bool solved, int it, bool nonconv, bool nan) {
-    __shared__ unsigned s_it[BX * BY], s_solved[BX * BY], s_nc[BX * BY], s_nan[BX * BY], s_max[BX * BY];
+    __shared__ unsigned s_it[N], s_solved[N], s_nc[N], s_nan[N], s_max[N];
     int t = threadIdx.x + threadIdx.y * blockDim.x;
     int n = blockDim.x * blockDim.y;
     s_it[t] = solved ? (unsigned)it : 0u;
@@ -560,15 +570,15 @@ __device__ __forceinline__ void surface_finalize_column(const GridDev& g, const 
 // Defer (FP32 storage with parameters that take the FP32 iteration): a column whose FP32
 // iteration does not finish it is appended to `deferred` (count, then flattened columns) and left
 // to surface_deferred_kernel.
-template <typename Real, bool Fast, bool Consts, bool Defer = false>
-__global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(NE_FLUX_WAVES_PER_EU)))
+template <typename Real, bool Fast, bool Consts, bool Defer = false, int KBY = BY>
+__global__ void __launch_bounds__(BX* KBY) __attribute__((amdgpu_waves_per_eu(NE_FLUX_WAVES_PER_EU)))
 surface_update_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __restrict__ fi,
                       const double* __restrict__ fj, RadDev<Real> rd, const double* __restrict__ rfi,
                       const double* __restrict__ rfj, View3<Real> T, View3<const Real> S, View3<const Real> u,
                       View3<const Real> v, FluxOut out, NetOut n, RadOut ro, View3<double> frazil, double dt_frazil,
                       const uint8_t* __restrict__ freeze_flags, ne_counters* counters, int* deferred) {
     int i = blockIdx.x * BX + threadIdx.x;
-    int j = blockIdx.y * BY + threadIdx.y;
+    int j = blockIdx.y * KBY + threadIdx.y;
     if (out.iters.p) {
         // Columns of the block in order of the iteration count the previous solve took, the
         // stable ones last (solve_order_key): the fixed point runs until the slowest column of a
@@ -578,9 +588,9 @@ surface_update_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __
         const int t = threadIdx.x + threadIdx.y * BX;
         const bool own = i < g.Nx && j < g.Ny;
         const int key = own ? solve_order_key(out, i, j) : SORT_BINS - 1;
-        const int c = sorted_block_column(key, t);
+        const int c = sorted_block_column<BX * KBY>(key, t);
         i = blockIdx.x * BX + c % BX;
-        j = blockIdx.y * BY + c / BX;
+        j = blockIdx.y * KBY + c / BX;
     }
     bool in = i < g.Nx && j < g.Ny;
     ColumnFluxes r;
@@ -596,7 +606,7 @@ surface_update_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __
                                                  freeze_flags, r.Qt, r.Qv, r.Jv, r.Ts_C, i, j);
     }
     bool solved = in && !inactive && !(Defer && r.deferred);
-    accumulate_counters(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
+    accumulate_counters<BX * KBY>(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
                         solved && !finite3(r.us, r.ts, r.qs));
 }
 
@@ -892,11 +902,14 @@ hipError_t launch_surface_update(const GridDev& g, const ne_params& p, const Atm
     const int path = flux_path(p);
     const dim3 grid = grid2d(g.Nx, g.Ny);
     if (sizeof(Real) == 4 && path >= 1 && deferred && mixed_iteration_applies(p)) {
-        // the FP32 iteration alone at 6 wavefronts per SIMD, then the deferred columns
-        auto k1 = path == 2 ? surface_update_kernel<Real, true, true, true> : surface_update_kernel<Real, true, false, true>;
+        // the FP32 iteration alone at 6 wavefronts per SIMD in blocks of 8 rows, then the deferred
+        // columns
+        constexpr int KBY = NE_FLUX_MIXED_ROWS;
+        auto k1 = path == 2 ? surface_update_kernel<Real, true, true, true, KBY>
+                            : surface_update_kernel<Real, true, false, true, KBY>;
         auto k2 = path == 2 ? surface_deferred_kernel<Real, true, true> : surface_deferred_kernel<Real, true, false>;
-        hipLaunchKernelGGL(k1, grid, dim3(BX, BY), 0, s, g, p, a, fi, fj, r, rfi, rfj, T, S, u, v, out, n, ro, frazil,
-                           dt_frazil, freeze_flags, c, deferred);
+        hipLaunchKernelGGL(k1, dim3((g.Nx + BX - 1) / BX, (g.Ny + KBY - 1) / KBY), dim3(BX, KBY), 0, s, g, p, a, fi, fj,
+                           r, rfi, rfj, T, S, u, v, out, n, ro, frazil, dt_frazil, freeze_flags, c, deferred);
         const unsigned blocks = std::min<unsigned>(grid.x * grid.y, 256u);
         hipLaunchKernelGGL(k2, dim3(blocks), dim3(BX, BY), 0, s, g, p, a, fi, fj, r, rfi, rfj, T, S, u, v, out, n, ro,
                            frazil, dt_frazil, freeze_flags, c, deferred);

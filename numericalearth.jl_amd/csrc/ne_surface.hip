@@ -105,6 +105,51 @@ __device__ void accumulate_counters(ne_counters* c, bool solved, int it, bool no
     }
 }
 
+// accumulate_counters for a kernel that calls it once, at its end: each wavefront reduces its
+// lanes (ballots, cross-lane sums), then one barrier, and the block's first work-item adds the
+// wavefronts' partials to the global counters — one barrier where the tree over the block's
+// work-items takes one per level. Blocks of N work-items, a multiple of 64.
+template <int N>
+__device__ void accumulate_counters_once(ne_counters* c, bool solved, int it, bool nonconv, bool nan) {
+    constexpr int NW = N / 64;
+    __shared__ unsigned s_part[5][NW];
+    const unsigned long long bs = __ballot(solved);
+    const unsigned long long bnc = __ballot(solved && nonconv);
+    const unsigned long long bnan = __ballot(solved && nan);
+    unsigned sum = solved ? (unsigned)it : 0u, mx = sum;
+    for (int d = 32; d >= 1; d >>= 1) {
+        sum += (unsigned)__shfl_xor((int)sum, d, 64);
+        const unsigned o = (unsigned)__shfl_xor((int)mx, d, 64);
+        mx = o > mx ? o : mx;
+    }
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    const int w = t >> 6;
+    if ((t & 63) == 0) {
+        s_part[0][w] = sum;
+        s_part[1][w] = (unsigned)__popcll(bs);
+        s_part[2][w] = (unsigned)__popcll(bnc);
+        s_part[3][w] = (unsigned)__popcll(bnan);
+        s_part[4][w] = mx;
+    }
+    __syncthreads();
+    if (t != 0 || c == nullptr) return;
+    unsigned long long tot = 0;
+    unsigned n = 0, nc = 0, nn = 0, m = 0;
+    for (int k = 0; k < NW; k++) {
+        tot += s_part[0][k];
+        n += s_part[1][k];
+        nc += s_part[2][k];
+        nn += s_part[3][k];
+        m = s_part[4][k] > m ? s_part[4][k] : m;
+    }
+    if (n == 0) return;
+    atomicAdd((unsigned long long*)&c->total_iterations, tot);
+    atomicAdd(&c->solved_columns, n);
+    if (nc) atomicAdd(&c->nonconverged_columns, nc);
+    if (nn) atomicAdd(&c->nan_columns, nn);
+    atomicMax(&c->max_iterations, m);
+}
+
 __device__ __forceinline__ void store_fluxes(const FluxOut& o, int i, int j, const ColumnFluxes& r) {
     o.Qv.at2(i, j) = r.Qv;
     o.Qt.at2(i, j) = r.Qt;
@@ -606,7 +651,7 @@ surface_update_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __
                                                  freeze_flags, r.Qt, r.Qv, r.Jv, r.Ts_C, i, j);
     }
     bool solved = in && !inactive && !(Defer && r.deferred);
-    accumulate_counters<BX * KBY>(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
+    accumulate_counters_once<BX * KBY>(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
                         solved && !finite3(r.us, r.ts, r.qs));
 }
 

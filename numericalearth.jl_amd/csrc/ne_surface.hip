@@ -105,37 +105,39 @@ __device__ void accumulate_counters(ne_counters* c, bool solved, int it, bool no
     }
 }
 
-// accumulate_counters for a kernel that calls it once, at its end: each wavefront reduces its
-// lanes (ballots, cross-lane sums), then one barrier, and the block's first work-item adds the
-// wavefronts' partials to the global counters — one barrier where the tree over the block's
-// work-items takes one per level. Blocks of N work-items, a multiple of 64.
+// The solver counters of a block, added once at the kernel's end from per-work-item counts (a
+// persistent kernel sums them over its items first): each wavefront reduces its lanes
+// (cross-lane), then one barrier, and the block's first work-item adds the wavefronts' partials
+// to the global counters — one barrier where the tree over the block's work-items takes one per
+// level. Blocks of N work-items, a multiple of 64; called by every work-item.
 template <int N>
-__device__ void accumulate_counters_once(ne_counters* c, bool solved, int it, bool nonconv, bool nan) {
+__device__ void add_block_counts(ne_counters* c, unsigned iters, unsigned solved, unsigned nonconv, unsigned nan,
+                                 unsigned maxit) {
     constexpr int NW = N / 64;
     __shared__ unsigned s_part[5][NW];
-    const unsigned long long bs = __ballot(solved);
-    const unsigned long long bnc = __ballot(solved && nonconv);
-    const unsigned long long bnan = __ballot(solved && nan);
-    unsigned sum = solved ? (unsigned)it : 0u, mx = sum;
     for (int d = 32; d >= 1; d >>= 1) {
-        sum += (unsigned)__shfl_xor((int)sum, d, 64);
-        const unsigned o = (unsigned)__shfl_xor((int)mx, d, 64);
-        mx = o > mx ? o : mx;
+        iters += (unsigned)__shfl_xor((int)iters, d, 64);
+        solved += (unsigned)__shfl_xor((int)solved, d, 64);
+        nonconv += (unsigned)__shfl_xor((int)nonconv, d, 64);
+        nan += (unsigned)__shfl_xor((int)nan, d, 64);
+        const unsigned o = (unsigned)__shfl_xor((int)maxit, d, 64);
+        maxit = o > maxit ? o : maxit;
     }
     const int t = threadIdx.x + threadIdx.y * blockDim.x;
     const int w = t >> 6;
     if ((t & 63) == 0) {
-        s_part[0][w] = sum;
-        s_part[1][w] = (unsigned)__popcll(bs);
-        s_part[2][w] = (unsigned)__popcll(bnc);
-        s_part[3][w] = (unsigned)__popcll(bnan);
-        s_part[4][w] = mx;
+        s_part[0][w] = iters;
+        s_part[1][w] = solved;
+        s_part[2][w] = nonconv;
+        s_part[3][w] = nan;
+        s_part[4][w] = maxit;
     }
     __syncthreads();
     if (t != 0 || c == nullptr) return;
     unsigned long long tot = 0;
     unsigned n = 0, nc = 0, nn = 0, m = 0;
-    for (int k = 0; k < NW; k++) {
+    const int nw = ((int)(blockDim.x * blockDim.y) + 63) / 64;
+    for (int k = 0; k < nw; k++) {
         tot += s_part[0][k];
         n += s_part[1][k];
         nc += s_part[2][k];
@@ -148,6 +150,13 @@ __device__ void accumulate_counters_once(ne_counters* c, bool solved, int it, bo
     if (nc) atomicAdd(&c->nonconverged_columns, nc);
     if (nn) atomicAdd(&c->nan_columns, nn);
     atomicMax(&c->max_iterations, m);
+}
+
+// add_block_counts for a kernel with one column per work-item.
+template <int N>
+__device__ void accumulate_counters_once(ne_counters* c, bool solved, int it, bool nonconv, bool nan) {
+    add_block_counts<N>(c, solved ? (unsigned)it : 0u, solved ? 1u : 0u, (solved && nonconv) ? 1u : 0u,
+                        (solved && nan) ? 1u : 0u, solved ? (unsigned)it : 0u);
 }
 
 __device__ __forceinline__ void store_fluxes(const FluxOut& o, int i, int j, const ColumnFluxes& r) {
@@ -346,8 +355,8 @@ flux_solve_exchange_kernel(GridDev g, ne_params p, View3<const double> ua, View3
         store_fluxes(out, i, j, r);
     }
     bool solved = in && !inactive;
-    accumulate_counters(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
-                        solved && !finite3(r.us, r.ts, r.qs));
+    accumulate_counters_once<BX * BY>(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
+                                      solved && !finite3(r.us, r.ts, r.qs));
 }
 
 // The flux solve on the ring: the exchange state is read at the ring position (the
@@ -712,6 +721,8 @@ template <typename Real, bool Fast, bool Consts, bool FUSED>
 __global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(NE_FLUX_WAVES_PER_EU)))
 batch_surface_solve_kernel(const BatchRec<Real>* __restrict__ recs, const WorkItem* __restrict__ work, int nwork,
                            double dt_frazil, ne_counters* counters) {
+    // this work-item's counts over its items, added to the counters once at the end
+    unsigned n_it = 0u, n_solved = 0u, n_nc = 0u, n_nan = 0u, n_max = 0u;
     for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
         const WorkItem it = work[wi];
         const BatchRec<Real>& rc = recs[it.member];
@@ -735,10 +746,15 @@ batch_surface_solve_kernel(const BatchRec<Real>* __restrict__ recs, const WorkIt
                                                      cview(rc.S), rc.f, rc.n, rc.ro, rc.frazil, dt_frazil,
                                                      rc.freeze_flags, r.Qt, r.Qv, r.Jv, r.Ts_C, i, j);
         }
-        bool solved = in && !inactive;
-        accumulate_counters(counters, solved, solved ? r.iterations : 0, solved && !r.converged,
-                            solved && !finite3(r.us, r.ts, r.qs));
+        if (in && !inactive) {
+            n_it += (unsigned)r.iterations;
+            n_solved += 1u;
+            n_nc += r.converged ? 0u : 1u;
+            n_nan += finite3(r.us, r.ts, r.qs) ? 0u : 1u;
+            n_max = (unsigned)r.iterations > n_max ? (unsigned)r.iterations : n_max;
+        }
     }
+    add_block_counts<BX * BY>(counters, n_it, n_solved, n_nc, n_nan, n_max);
 }
 
 // Ensemble form of K2 over the same work list as the ensemble K1: τ of every column.

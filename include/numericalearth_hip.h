@@ -417,6 +417,18 @@ int ne_tracer_tendency_weno5_f64(ne_ctx* ctx, const ne_grid* grid, const ne_para
                                  const ne_ocean_state* ocean, const ne_net_ocean_fluxes* net,
                                  ne_view G_T, ne_view G_S, void* stream);
 
+/* One Quasi-Adams-Bashforth 2 step (the reference's timestepper = :QuasiAdamsBashforth2,
+ * ocean_simulation.jl:323; Oceananigans' QuasiAdamsBashforth2TimeStepper, whose χ is 0.1 there,
+ * NE_AB2_CHI) from G, the tendency of the state T, S (ne_tracer_tendency_weno5_f64), and G_prev,
+ * the last step's: on active cells c ← c + Δt·((3/2 + χ)·G − (1/2 + χ)·G_prev) for T and S,
+ * inactive cells left untouched. euler ≠ 0 (the first step, or a Δt other than the last
+ * step's) takes χ = −1/2 and does not read G_prev (its views may then have NULL data). The
+ * caller keeps G as the next step's G_prev. All views FP64 with T's layout. */
+#define NE_AB2_CHI 0.1
+int ne_tracer_ab2_update_f64(ne_ctx* ctx, const ne_grid* grid, ne_view T, ne_view S, ne_view G_T, ne_view G_S,
+                             ne_view G_prev_T, ne_view G_prev_S, double dt, double chi, int32_t euler,
+                             void* stream);
+
 /* One split Runge-Kutta stage, tendency fused: c_out = c0 + beta_dt·G(c_in) for T and S
  * on active cells (inactive cells are left untouched). c_out may alias c0 but not c_in. */
 int ne_tracer_rk_stage_f64(ne_ctx* ctx, const ne_grid* grid, const ne_params* params,
@@ -535,7 +547,10 @@ typedef struct ne_model_desc {
                                              slot n mod N, uploaded by the caller
                                              (ne_model_snapshots_at, ne_model_snapshot_resident) */
     int32_t radiation_slots;              /* the same for the radiation views */
-    int32_t reserved2;
+    int32_t timestepper;                  /* NE_TIMESTEPPER_SPLIT_RK3 (the reference's default) or
+                                             NE_TIMESTEPPER_QUASI_ADAMS_BASHFORTH2 (FP64 models only;
+                                             the step's tendency and the last one's are kept in the
+                                             stage buffers) */
     double  time_period;                  /* NE_TIME_CYCLICAL: the period (s), ≥ last − first time */
 } ne_model_desc;
 
@@ -545,6 +560,11 @@ typedef struct ne_model_desc {
  * first snapshot time, and between the last snapshot and the first one a period later the
  * two are interpolated across the wrap. */
 enum { NE_TIME_CLAMPED = 0, NE_TIME_CYCLICAL = 1 };
+
+/* Time stepping of T and S (ocean_simulation's timestepper, ocean_simulation.jl:323, :347):
+ * three split Runge-Kutta stages per step, or one Quasi-Adams-Bashforth 2 update (χ = NE_AB2_CHI;
+ * forward Euler on the first step and on a step whose Δt differs from the last step's). */
+enum { NE_TIMESTEPPER_SPLIT_RK3 = 0, NE_TIMESTEPPER_QUASI_ADAMS_BASHFORTH2 = 1 };
 
 /* Bytes of device workspace a model of this descriptor needs: four RK stage buffers of T, S,
  * one byte per column of freeze flags (the step's last stage marks the columns holding a
@@ -669,7 +689,9 @@ typedef struct ne_batch_tuning {
 } ne_batch_tuning;
 
 /* A batch of `precision` (NE_FP64 or NE_MIXED_FP32) holding the n_models given models
- * (n_models may be 0). All calls on one batch must be ordered on one stream. */
+ * (n_models may be 0; members step with split RK3: a Quasi-Adams-Bashforth 2 model is refused
+ * with NE_ERR_UNSUPPORTED, as is one with windowed snapshot views). All calls on one batch must
+ * be ordered on one stream. */
 int ne_batch_create(ne_ctx* ctx, int32_t precision, int n_models, ne_model* const* models, ne_batch** batch,
                     void* stream);
 /* Waits for the device, then releases the batch's device memory; the members are released. */

@@ -517,6 +517,21 @@ int ne_tracer_tendency_weno5_f64(ne_ctx* c, const ne_grid* g, const ne_params* p
                                    view<double>(GS), as_stream(stream)));
 }
 
+int ne_tracer_ab2_update_f64(ne_ctx* c, const ne_grid* g, ne_view T, ne_view S, ne_view GT, ne_view GS,
+                             ne_view GmT, ne_view GmS, double dt, double chi, int32_t euler, void* stream) {
+    if (!c) return NE_ERR_NULL;
+    int r;
+    if ((r = check_grid(g))) return r;
+    const ne_view* vs[] = {&T, &S, &GT, &GS};
+    for (const ne_view* x : vs)
+        if ((r = check_view3(*x))) return r;
+    if (!euler && ((r = check_view3(GmT)) || (r = check_view3(GmS)))) return r;
+    if (!std::isfinite(dt) || !std::isfinite(chi)) return NE_ERR_SHAPE;
+    return dev_err(launch_ab2_update(grid_dev(*g), view<double>(T), view<double>(S), view<const double>(GT),
+                                     view<const double>(GS), view<const double>(GmT), view<const double>(GmS), dt, chi,
+                                     euler != 0, nullptr, 0.0, 0.0, as_stream(stream)));
+}
+
 int ne_tracer_rk_stage_f64(ne_ctx* c, const ne_grid* g, const ne_params* p, ne_view u, ne_view v, ne_view w,
                            ne_view T0, ne_view S0, ne_view Tin, ne_view Sin, ne_view Tout, ne_view Sout,
                            const ne_net_ocean_fluxes* net, double beta_dt, void* stream) {
@@ -694,6 +709,34 @@ static int model_update_state(ne_model* m, double dt_frazil, hipStream_t s, hipE
     return dev_err(e);
 }
 
+// One Quasi-Adams-Bashforth 2 step of an FP64 model: the tendency of the state into one pair of
+// stage buffers, then the update with the last step's tendency from the other pair (not read on
+// a forward-Euler step); the pairs swap roles every step. Events as the RK step's: ev[1] after
+// the tendency, ev[2] and ev[3] after the update.
+static int model_ab2_step(ne_model* m, double dt, hipStream_t s, hipEvent_t* ev) {
+    ne_model_desc& d = m->d;
+    GridDev g = grid_dev(d.grid);
+    const bool euler = !(dt == m->last_dt);
+    ne_view GT = m->ab2_parity ? m->vT2 : m->vT1, GS = m->ab2_parity ? m->vS2 : m->vS1;
+    ne_view GmT = m->ab2_parity ? m->vT1 : m->vT2, GmS = m->ab2_parity ? m->vS1 : m->vS2;
+    if (ev && hipEventRecord(ev[0], s) != hipSuccess) return NE_ERR_DEVICE;
+    TracerForcing F = tracer_forcing(&d.params, d.net);
+    hipError_t e = launch_tendency<double>(g, view<const double>(d.ocean.u), view<const double>(d.ocean.v),
+                                           view<const double>(d.ocean.w), view<const double>(d.ocean.T),
+                                           view<const double>(d.ocean.S), view<const double>(d.net.J_T),
+                                           view<const double>(d.net.J_S), view<const double>(d.net.J_w), F,
+                                           view<double>(GT), view<double>(GS), s);
+    if (e != hipSuccess) return NE_ERR_DEVICE;
+    if (ev && hipEventRecord(ev[1], s) != hipSuccess) return NE_ERR_DEVICE;
+    e = launch_ab2_update(g, view<double>(d.ocean.T), view<double>(d.ocean.S), view<const double>(GT),
+                          view<const double>(GS), view<const double>(GmT), view<const double>(GmS), dt, NE_AB2_CHI,
+                          euler, m->freeze_flags, d.params.freshwater_melting_temperature, d.params.liquidus_slope, s);
+    if (e != hipSuccess) return NE_ERR_DEVICE;
+    if (ev && (hipEventRecord(ev[2], s) != hipSuccess || hipEventRecord(ev[3], s) != hipSuccess)) return NE_ERR_DEVICE;
+    m->ab2_parity ^= 1;
+    return NE_OK;
+}
+
 template <typename Real>
 static int model_ocean_step(ne_model* m, double dt, hipStream_t s, hipEvent_t* ev = nullptr) {
     ne_model_desc& d = m->d;
@@ -744,6 +787,10 @@ int ne_model_create(ne_ctx* c, const ne_model_desc* desc, ne_model** out, void* 
     if (desc->params.radiation && (r = check_rad(&desc->radiation))) return r;
     if (desc->precision != NE_FP64 && desc->precision != NE_MIXED_FP32) return NE_ERR_UNSUPPORTED;
     if (desc->fuse_freeze_clamp != 0) return NE_ERR_UNSUPPORTED;
+    if (desc->timestepper != NE_TIMESTEPPER_SPLIT_RK3 && desc->timestepper != NE_TIMESTEPPER_QUASI_ADAMS_BASHFORTH2)
+        return NE_ERR_UNSUPPORTED;
+    if (desc->timestepper == NE_TIMESTEPPER_QUASI_ADAMS_BASHFORTH2 && desc->precision != NE_FP64)
+        return NE_ERR_UNSUPPORTED;   // the mixed form's conservative FP32 stages are the RK step's
     if (!desc->atmos_fi || !desc->atmos_fj) return NE_ERR_NULL;
     if (desc->params.radiation && (!desc->radiation_fi || !desc->radiation_fj)) return NE_ERR_NULL;
     if (desc->n_atmosphere_times < 0 || desc->n_radiation_times < 0) return NE_ERR_SHAPE;
@@ -874,7 +921,10 @@ int ne_model_time_step(ne_model* m, double dt, void* stream) {
     if (r) return r;
     hipEvent_t* ev = nullptr;
     if (m->timing && m->recorded < m->max_steps) ev = &m->events[(size_t)m->recorded * 6];
-    r = m->d.precision == NE_FP64 ? model_ocean_step<double>(m, dt, s, ev) : model_ocean_step<float>(m, dt, s, ev);
+    if (m->d.timestepper == NE_TIMESTEPPER_QUASI_ADAMS_BASHFORTH2)
+        r = model_ab2_step(m, dt, s, ev);
+    else
+        r = m->d.precision == NE_FP64 ? model_ocean_step<double>(m, dt, s, ev) : model_ocean_step<float>(m, dt, s, ev);
     if (r) return r;
     m->time += dt;
     m->iteration += 1;

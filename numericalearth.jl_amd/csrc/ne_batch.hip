@@ -422,6 +422,8 @@ int check_member(const ne_batch* b, const ne_model* m) {
     if (m->d.precision != b->precision) return NE_ERR_UNSUPPORTED;
     // the device clock indexes whole series: windowed snapshot views stay with single models
     if (m->d.atmosphere_slots > 0 || m->d.radiation_slots > 0) return NE_ERR_UNSUPPORTED;
+    // the batched step is the split RK3 one
+    if (m->d.timestepper != NE_TIMESTEPPER_SPLIT_RK3) return NE_ERR_UNSUPPORTED;
     return NE_OK;
 }
 

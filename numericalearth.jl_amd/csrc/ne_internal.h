@@ -35,6 +35,7 @@ struct ne_model {
     double* beer = nullptr;
     double time = 0.0;
     double last_dt = 0.0;
+    int ab2_parity = 0;                 // Quasi-Adams-Bashforth 2: which stage-buffer pair takes G
     int64_t iteration = 0;
     bool timing = false;
     int max_steps = 0;
@@ -185,6 +186,12 @@ hipError_t launch_tendency(const GridDev& g, View3<const Real> u, View3<const Re
                            View3<const Real> T, View3<const Real> S, View3<const double> JT,
                            View3<const double> JS, View3<const double> Jw, TracerForcing forcing, View3<Real> GT,
                            View3<Real> GS, hipStream_t s);
+// Quasi-Adams–Bashforth 2 update of an FP64 state from the step's tendency G and the last
+// step's G⁻ (not read when euler), with the freeze flags of the step (ne_tracer.hip).
+hipError_t launch_ab2_update(const GridDev& g, View3<double> T, View3<double> S, View3<const double> GT,
+                             View3<const double> GS, View3<const double> GmT, View3<const double> GmS, double dt,
+                             double chi, bool euler, uint8_t* freeze_flags, double melt0, double slope,
+                             hipStream_t s);
 // Vertical velocity from continuity (all Nz+1 faces of w).
 template <typename Real>
 hipError_t launch_w_continuity(const GridDev& g, View3<const Real> u, View3<const Real> v, View3<Real> w,

@@ -873,6 +873,43 @@ template hipError_t launch_rk_stage<float>(const GridDev&, View3<const float>, V
                                            View3<const double>, View3<const double>, View3<const double>,
                                            TracerForcing, double, bool, double, hipStream_t);
 
+// Quasi-Adams–Bashforth 2 update (the reference's timestepper = :QuasiAdamsBashforth2,
+// ocean_simulation.jl:323, with Oceananigans' default χ = 0.1): in every active cell
+// c ← c + Δt·((3/2 + χ)·Gⁿ − (1/2 + χ)·G⁻), one work-item per column taking the levels in turn
+// (i fastest across a wavefront). A step taken as forward Euler (the first step, or a Δt other
+// than the last step's) has χ = −1/2 and reads no G⁻ (Oceananigans zeroes it). The column's flag
+// is set when a stored cell is below its freezing point, the test the RK step's last stage makes.
+__global__ void __launch_bounds__(256) ab2_update_kernel(GridDev g, View3<double> T, View3<double> S,
+                                                        View3<const double> GT, View3<const double> GS,
+                                                        View3<const double> GmT, View3<const double> GmS,
+                                                        double dt, double a, double b, int euler,
+                                                        uint8_t* freeze_flags, double melt0, double slope) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    if (i >= g.Nx || j >= g.Ny) return;
+    const int64_t col = (int64_t)i + (int64_t)j * g.Nx;
+    bool frz = false;
+    for (int k = g.kmin[col]; k < g.Nz; k++) {
+        const double gmT = euler ? 0.0 : GmT(i, j, k), gmS = euler ? 0.0 : GmS(i, j, k);
+        const double Tn = T(i, j, k) + dt * (a * GT(i, j, k) - b * gmT);
+        const double Sn = S(i, j, k) + dt * (a * GS(i, j, k) - b * gmS);
+        T(i, j, k) = Tn;
+        S(i, j, k) = Sn;
+        frz = frz || Tn < melt0 - slope * Sn;
+    }
+    if (freeze_flags) freeze_flags[col] = frz ? 1 : 0;
+}
+
+hipError_t launch_ab2_update(const GridDev& g, View3<double> T, View3<double> S, View3<const double> GT,
+                             View3<const double> GS, View3<const double> GmT, View3<const double> GmS, double dt,
+                             double chi, bool euler, uint8_t* freeze_flags, double melt0, double slope,
+                             hipStream_t s) {
+    const double c = euler ? -0.5 : chi;
+    hipLaunchKernelGGL(ab2_update_kernel, dim3((g.Nx + 63) / 64, (g.Ny + 3) / 4), dim3(64, 4), 0, s, g, T, S, GT, GS,
+                       GmT, GmS, dt, 1.5 + c, 0.5 + c, euler ? 1 : 0, freeze_flags, melt0, slope);
+    return hipGetLastError();
+}
+
 template <typename Real>
 hipError_t launch_tendency(const GridDev& g, View3<const Real> u, View3<const Real> v, View3<const Real> w,
                            View3<const Real> T, View3<const Real> S, View3<const double> JT,

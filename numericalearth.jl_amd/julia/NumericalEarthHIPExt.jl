@@ -24,7 +24,9 @@
 #   update_net_fluxes!(model, ocean)                                   → ne_assemble_net_ocean_fluxes_f64
 #   apply_air_sea_radiative_fluxes!(model)                             → ne_apply_air_sea_radiation_f64
 #   time_step!(ocean, Δt): T and S by three split-RK3 stages           → ne_tracer_rk_stage_{f64,f32}
-#                          (ocean_simulation.jl:347), then fill_halo_regions! of T and S
+#                          (ocean_simulation.jl:347), then fill_halo_regions! of T and S;
+#                          with timestepper = :QuasiAdamsBashforth2 (:323) → ne_tracer_tendency_weno5_f64
+#                          into the time stepper's Gⁿ, then               → ne_tracer_ab2_update_f64
 #
 # The ocean step is the tracer step only: the velocities are those of the wrapped model
 # (prescribed, or updated by the caller between steps; ne_compute_w_from_continuity_f64 derives
@@ -47,7 +49,7 @@ using Oceananigans.ImmersedBoundaries: ImmersedBoundaryGrid, GridFittedBottom
 using Oceananigans.Operators: Δzᶜᶜᶜ, Azᶜᶜᶜ, Δyᶠᶜᶜ, Δxᶜᶠᶜ
 using Oceananigans.BoundaryConditions: fill_halo_regions!
 using Oceananigans.OutputReaders: Cyclical
-using Oceananigans.TimeSteppers: tick!
+using Oceananigans.TimeSteppers: tick!, QuasiAdamsBashforth2TimeStepper
 using NumericalEarth
 using NumericalEarth.EarthSystemModels: EarthSystemModels, EarthSystemModel, NoSeaIceInterface
 using NumericalEarth.EarthSystemModels.InterfaceComputations
@@ -831,28 +833,37 @@ function EarthSystemModels.apply_air_sea_radiative_fluxes!(cm::HIPRadiatedModel)
     return nothing
 end
 
-"""
-    time_step!(ocean::HIPOcean, Δt)
-
-T and S advanced by the split Runge-Kutta 3 stages c⁽ᵐ⁾ = cⁿ + βₘ Δt G(c⁽ᵐ⁻¹⁾), β = (1/3, 1/2, 1)
-(ocean_simulation.jl:347), G the WENO5 advection tendency with the top flux boundary condition
-(ne_tracer_rk_stage_f64 / _f32 by T's element type), the last stage writing T and S in place;
-then their halos are filled and the ocean clock ticks, as Oceananigans' step leaves them.
-"""
-function Oceananigans.TimeSteppers.time_step!(ocean::HIPOcean, Δt)
-    model = ocean.simulation.model
-    T, S = model.tracers.T, model.tracers.S
-    u, v, w = model.velocities.u, model.velocities.v, model.velocities.w
-    net = net_ocean_fluxes(InterfaceComputations.net_fluxes(ocean.simulation), ocean)
-    g = ne_grid(model.grid)
+# The tracer step's parameters: the surface constants do not enter it, only the radiative forcing
+# (TwoColorRadiation adds its shortwave divergence to Gᵀ).
+function tracer_params(ocean::HIPOcean)
     p = with(default_params(); radiation_properties(nothing, ocean)...)
-    forcing = get_radiative_forcing(model)
+    forcing = get_radiative_forcing(ocean.simulation.model)
     if forcing isa TwoColorRadiation
         p = with(p; radiative_forcing = NE_RADIATIVE_FORCING_TWO_COLOR,
                  first_color_fraction = forcing.first_color_fraction,
                  first_absorption_coefficient = forcing.first_absorption_coefficient,
                  second_absorption_coefficient = forcing.second_absorption_coefficient)
     end
+    return p
+end
+
+"""
+    time_step!(ocean::HIPOcean, Δt)
+
+T and S advanced by the split Runge-Kutta 3 stages c⁽ᵐ⁾ = cⁿ + βₘ Δt G(c⁽ᵐ⁻¹⁾), β = (1/3, 1/2, 1)
+(ocean_simulation.jl:347), G the WENO5 advection tendency with the top flux boundary condition
+(ne_tracer_rk_stage_f64 / _f32 by T's element type), the last stage writing T and S in place;
+then their halos are filled and the ocean clock ticks, as Oceananigans' step leaves them. A model
+built with timestepper = :QuasiAdamsBashforth2 (ocean_simulation.jl:323) takes ab2_time_step!.
+"""
+function Oceananigans.TimeSteppers.time_step!(ocean::HIPOcean, Δt)
+    model = ocean.simulation.model
+    model.timestepper isa QuasiAdamsBashforth2TimeStepper && return ab2_time_step!(ocean, Δt)
+    T, S = model.tracers.T, model.tracers.S
+    u, v, w = model.velocities.u, model.velocities.v, model.velocities.w
+    net = net_ocean_fluxes(InterfaceComputations.net_fluxes(ocean.simulation), ocean)
+    g = ne_grid(model.grid)
+    p = tracer_params(ocean)
     f = Symbol(:ne_tracer_rk_stage_, suffix(eltype(parent(T))))
     fn = Libc.Libdl.dlsym(Libc.Libdl.dlopen(libne), f)
     T0, S0 = ne_view(T), ne_view(S)
@@ -868,6 +879,43 @@ function Oceananigans.TimeSteppers.time_step!(ocean::HIPOcean, Δt)
     end
     fill_halo_regions!((T, S))
     tick!(model.clock, Δt)
+    return nothing
+end
+
+"""
+    ab2_time_step!(ocean::HIPOcean, Δt)
+
+Oceananigans' Quasi-Adams-Bashforth 2 step of T and S for timestepper = :QuasiAdamsBashforth2: the
+tendency of the state into the time stepper's Gⁿ (ne_tracer_tendency_weno5_f64), then
+c ← c + Δt((3/2 + χ)Gⁿ − (1/2 + χ)G⁻) (ne_tracer_ab2_update_f64, χ the time stepper's), forward
+Euler when Δt differs from the clock's last Δt (the first step included); Gⁿ is then kept as G⁻,
+the halos filled and the clock ticked with its last Δt set. FP64 fields only.
+"""
+function ab2_time_step!(ocean::HIPOcean, Δt)
+    model = ocean.simulation.model
+    ts = model.timestepper
+    T, S = model.tracers.T, model.tracers.S
+    eltype(parent(T)) == Float64 || throw(ArgumentError("the Quasi-Adams-Bashforth 2 step is FP64"))
+    net = net_ocean_fluxes(InterfaceComputations.net_fluxes(ocean.simulation), ocean)
+    g = ne_grid(model.grid)
+    status = ccall((:ne_tracer_tendency_weno5_f64, libne), Cint,
+                   (Ptr{Cvoid}, Ref{NEGrid}, Ref{NEParams}, Ref{NEOceanState}, Ref{NENetOceanFluxes}, NEView, NEView,
+                    Ptr{Cvoid}),
+                   context(), g, tracer_params(ocean), ocean_state(ocean), net, ne_view(ts.Gⁿ.T), ne_view(ts.Gⁿ.S),
+                   stream_handle())
+    check(status, "ne_tracer_tendency_weno5_f64")
+    euler = Δt != model.clock.last_Δt
+    status = ccall((:ne_tracer_ab2_update_f64, libne), Cint,
+                   (Ptr{Cvoid}, Ref{NEGrid}, NEView, NEView, NEView, NEView, NEView, NEView, Float64, Float64, Int32,
+                    Ptr{Cvoid}),
+                   context(), g, ne_view(T), ne_view(S), ne_view(ts.Gⁿ.T), ne_view(ts.Gⁿ.S), ne_view(ts.G⁻.T),
+                   ne_view(ts.G⁻.S), Float64(Δt), Float64(ts.χ), Int32(euler), stream_handle())
+    check(status, "ne_tracer_ab2_update_f64")
+    parent(ts.G⁻.T) .= parent(ts.Gⁿ.T)
+    parent(ts.G⁻.S) .= parent(ts.Gⁿ.S)
+    fill_halo_regions!((T, S))
+    tick!(model.clock, Δt)
+    model.clock.last_Δt = Δt
     return nothing
 end
 

@@ -41,6 +41,8 @@ NE_STOP_CONVERGENCE, NE_STOP_FIXED_ITERATIONS = 0, 1
 NE_HUMIDITY_IMPURE_SATURATION, NE_HUMIDITY_FIXED = 0, 1
 NE_VELOCITY_RELATIVE, NE_VELOCITY_WIND = 0, 1
 NE_TIME_CLAMPED, NE_TIME_CYCLICAL = 0, 1
+NE_TIMESTEPPER_SPLIT_RK3, NE_TIMESTEPPER_QUASI_ADAMS_BASHFORTH2 = 0, 1
+NE_AB2_CHI = 0.1
 NE_BUDGET_NOALLOC_MAX_BLOCKS = 65536   # blocks of 64×4 columns the allocation-free budget holds partials for
 NO_POSITION = -(2 ** 63)   # ne_model_snapshots_at: a series with no snapshots
 
@@ -241,7 +243,7 @@ class ModelDesc(C.Structure):
         ("time_indexing", _i32),
         ("atmosphere_slots", _i32),
         ("radiation_slots", _i32),
-        ("reserved2", _i32),
+        ("timestepper", _i32),
         ("time_period", C.c_double),
     ]
 

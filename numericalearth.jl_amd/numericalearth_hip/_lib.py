@@ -56,6 +56,7 @@ SIGNATURES = {
                                        P(abi.NetOceanFluxes), P(abi.RadiativeFluxes), vp],
     "ne_tracer_tendency_weno5_f64": [vp, P(abi.Grid), P(abi.Params), P(abi.OceanState), P(abi.NetOceanFluxes),
                                      abi.View, abi.View, vp],
+    "ne_tracer_ab2_update_f64": [vp, P(abi.Grid)] + [abi.View] * 6 + [C.c_double, C.c_double, C.c_int32, vp],
     "ne_tracer_rk_stage_f64": [vp, P(abi.Grid), P(abi.Params)] + [abi.View] * 9 + [P(abi.NetOceanFluxes),
                                                                                  C.c_double, vp],
     "ne_budget_f64": [vp, P(abi.Grid), abi.View, C.c_double, vp, vp],

@@ -135,13 +135,23 @@ class OceanOnlyModel:
     def __init__(self, case: SurfaceCase, *, params: abi.Params | None = None, precision: str = "fp64",
                  radiation: bool = True, device: int = 0, time_indexing=None, atmosphere_window: int | None = None,
                  radiation_window: int | None = None, atmosphere_source=None, radiation_source=None,
-                 lookahead: int | None = None, copy_stream: bool = False):
+                 lookahead: int | None = None, copy_stream: bool = False,
+                 timestepper: str = "SplitRungeKutta3"):
         """time_indexing: Clamp() (default) or Cyclical(period) of both snapshot series.
+        timestepper: "SplitRungeKutta3" (default) or "QuasiAdamsBashforth2" (FP64), as ocean_simulation's
+        `timestepper` keyword (ocean_simulation.jl:323).
         atmosphere_window / radiation_window: keep only that many snapshots of the series on the device
         (a ring of slots, time_indices_in_memory) and stream the others from `atmosphere_source` /
         `radiation_source` (an ArraySnapshots or NpySnapshotStore; the case's arrays by default)."""
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
+        steppers = {"SplitRungeKutta3": abi.NE_TIMESTEPPER_SPLIT_RK3,
+                    "QuasiAdamsBashforth2": abi.NE_TIMESTEPPER_QUASI_ADAMS_BASHFORTH2}
+        if timestepper not in steppers:
+            raise ValueError(f"timestepper must be one of {sorted(steppers)}")
+        if timestepper == "QuasiAdamsBashforth2" and precision != "fp64":
+            raise ValueError("the Quasi-Adams-Bashforth 2 step is FP64 (the mixed form's stages are the RK ones)")
+        self.timestepper = timestepper
         self.case = case
         self.device = torch.device("cuda", device)
         self.ctx = Context.get(device)
@@ -244,6 +254,7 @@ class OceanOnlyModel:
         self._workspace = torch.empty(nbytes.value, dtype=torch.uint8, device=dev)
         d.workspace, d.workspace_bytes = self._workspace.data_ptr(), nbytes.value
         d.time_indexing, d.time_period = self.time_indexing_kind, self.time_period
+        d.timestepper = steppers[timestepper]
         d.atmosphere_slots = int(atmosphere_window) if "atmosphere" in resident else 0
         d.radiation_slots = int(radiation_window) if "radiation" in resident else 0
         self.desc = d

@@ -202,7 +202,11 @@ class OracleModel:
     """CPU OceanOnlyModel on a SurfaceCase (copies of its arrays)."""
 
     def __init__(self, case, params: abi.Params | None = None, radiation: bool = True,
-                 time_indexing: int = abi.NE_TIME_CLAMPED, time_period: float | None = None):
+                 time_indexing: int = abi.NE_TIME_CLAMPED, time_period: float | None = None,
+                 timestepper: str = "SplitRungeKutta3"):
+        if timestepper not in ("SplitRungeKutta3", "QuasiAdamsBashforth2"):
+            raise ValueError(timestepper)
+        self.timestepper = timestepper
         self.case = case
         self.time_indexing, self.time_period = time_indexing, time_period
         g = case.grid
@@ -286,8 +290,32 @@ class OracleModel:
         self._set_time_indices()
         lib().neo_model_update_state(C.byref(self._model), dt_frazil)
 
+    def ab2_step(self, dt: float):
+        """One Quasi-Adams-Bashforth 2 step of T and S (Oceananigans' QuasiAdamsBashforth2TimeStepper
+        with χ = 0.1, the reference's timestepper = :QuasiAdamsBashforth2, ocean_simulation.jl:323):
+        c ← c + Δt((3/2 + χ)Gⁿ − (1/2 + χ)G⁻) on active cells with Gⁿ the tendency of the state under
+        the fluxes the last update_state computed; forward Euler (χ = −1/2, G⁻ = 0) on the first step
+        and on a step whose Δt differs from the last one (ab2_step_field! and time_step!'s euler rule).
+        Gⁿ is kept as the next step's G⁻."""
+        GT, GS = self.tendency()
+        euler = self.iteration == 0 or dt != self.last_dt
+        chi = -0.5 if euler else abi.NE_AB2_CHI
+        if euler:
+            self.GmT, self.GmS = np.zeros_like(GT), np.zeros_like(GS)
+        Nz = self.case.shape[2]
+        active = np.arange(Nz)[:, None, None] >= self.kmin[None]
+        a, b = 1.5 + chi, 0.5 + chi
+        Tn = self.T + dt * (a * GT - b * self.GmT)
+        Sn = self.S + dt * (a * GS - b * self.GmS)
+        self.T[active] = Tn[active]
+        self.S[active] = Sn[active]
+        self.GmT, self.GmS = GT, GS
+
     def time_step(self, dt: float):
-        lib().neo_model_ocean_step(C.byref(self._model), dt)
+        if self.timestepper == "QuasiAdamsBashforth2":
+            self.ab2_step(dt)
+        else:
+            lib().neo_model_ocean_step(C.byref(self._model), dt)
         self.time += dt
         self.iteration += 1
         self.last_dt = dt

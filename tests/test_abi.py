@@ -77,6 +77,19 @@ def test_null_arguments_are_rejected_without_a_device():
     assert L.ne_model_rounding_residual(None, None, None) == abi.NE_ERR_NULL
     assert L.ne_stream_probe_f64(None, None, 5, None, 2, 10, None) == abi.NE_ERR_NULL
     assert L.ne_fill_halo_f64(None, None, abi.View(), 1, 1, 1, None) == abi.NE_ERR_NULL
+    v = abi.View()
+    assert L.ne_tracer_ab2_update_f64(None, None, v, v, v, v, v, v, 1.0, abi.NE_AB2_CHI, 0, None) == abi.NE_ERR_NULL
+
+
+def test_quasi_adams_bashforth_options_are_checked_on_the_host():
+    """The timestepper names of ocean_simulation (SplitRungeKutta3, QuasiAdamsBashforth2); the
+    Quasi-Adams-Bashforth 2 step is FP64 only, refused for the mixed form before any device work."""
+    from numericalearth_hip.model import OceanOnlyModel
+    with pytest.raises(ValueError):
+        OceanOnlyModel(None, precision="mixed", timestepper="QuasiAdamsBashforth2")
+    with pytest.raises(ValueError):
+        OceanOnlyModel(None, timestepper="RungeKutta4")
+    assert (abi.NE_TIMESTEPPER_SPLIT_RK3, abi.NE_TIMESTEPPER_QUASI_ADAMS_BASHFORTH2) == (0, 1)
 
 
 def test_julia_mirror_has_the_header_params_size():

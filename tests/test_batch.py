@@ -221,6 +221,23 @@ def test_members_are_locked_while_batched(require_gpu):
     models[0].time_step(DT)
 
 
+def test_quasi_adams_bashforth_members_are_refused(require_gpu):
+    """The batched step is the split RK3 one: a model with the Quasi-Adams-Bashforth 2 timestepper
+    is refused (NE_ERR_UNSUPPORTED) and still steps on its own."""
+    from numericalearth_hip.generators import make_case
+    case = make_case(24, 12, 4, seed=3)
+    m = OceanOnlyModel(case, timestepper="QuasiAdamsBashforth2")
+    ens = Ensemble([], precision="fp64")
+    with pytest.raises(NEError) as e:
+        ens.add(m)
+    assert e.value.status == abi.NE_ERR_UNSUPPORTED
+    ens.close()
+    m.time_step(1200.0)
+    torch.cuda.synchronize()
+    assert m.clock == (1200.0, 1)
+    m.close()
+
+
 def test_mixed_precisions_are_refused(require_gpu):
     specs = _specs(2, seed=13)
     a = _build(specs[:1], "fp64")[0]

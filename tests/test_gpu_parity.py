@@ -503,6 +503,34 @@ def test_model_steps_match_oracle(require_gpu, precision):
     assert gm.clock == (6000.0, 5)
 
 
+def test_model_ab2_steps_match_oracle(require_gpu):
+    """The reference's alternative timestepper, timestepper = :QuasiAdamsBashforth2
+    (ocean_simulation.jl:323): one tendency and one update per step, forward Euler on the first
+    step and on the step whose Δt changes (1200 s, then 900 s from the eighth step). T and S and
+    the fluxes after each step match the oracle's QAB2 model to 1e-12, and differ from the RK3
+    step's (the option is not ignored)."""
+    case = make_case(*SMALL, seed=23)
+    print(f"[case] shape={SMALL} seed=23 QAB2", flush=True)
+    om = O.OracleModel(case, timestepper="QuasiAdamsBashforth2").reconcile()
+    gm = OceanOnlyModel(case, timestepper="QuasiAdamsBashforth2")
+    rk = OceanOnlyModel(case)
+    torch.cuda.synchronize()
+    for step in range(12):
+        dt = 1200.0 if step < 7 else 900.0
+        om.time_step(dt)
+        gm.time_step(dt)
+        rk.time_step(dt)
+        torch.cuda.synchronize()
+        print(f"[ab2] step {step + 1} dt {dt}", flush=True)
+        assert_rel(_np(gm.T), om.T, RTOL, "T")
+        assert_rel(_np(gm.S), om.S, RTOL, "S")
+        _compare_fluxes_on_gpu_state(gm, om)
+    assert gm.clock == (7 * 1200.0 + 5 * 900.0, 12)
+    assert rel_err(_np(gm.T), _np(rk.T))[1] > 1e-9
+    for m in (gm, rk):
+        m.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_freezing_steps_clamp_every_flagged_column(require_gpu, precision):
     """Cold, windy air over water 2 mK above its freezing point on 10 m levels: each step cools

@@ -140,6 +140,13 @@ MUTANTS = [
     ("cycle counted from the end of the first period", "py",
      "cycle = round((x - x % period) / period)", "cycle = round((x - x % period) / period) + (1 if x < 0 else 0)",
      TSER_PINS),
+    ("QAB2 with chi = 0 (plain AB2)", "py",
+     "chi = -0.5 if euler else abi.NE_AB2_CHI", "chi = -0.5 if euler else 0.0", [NET, "-k", "ab2"]),
+    ("QAB2 without forward Euler on a changed step", "py",
+     "euler = self.iteration == 0 or dt != self.last_dt", "euler = self.iteration == 0",
+     [NET, "-k", "ab2"]),
+    ("QAB2 weights that do not sum to one", "py",
+     "a, b = 1.5 + chi, 0.5 + chi", "a, b = 1.5 + chi, 0.5 - chi", [NET, "-k", "ab2"]),
 ]
 
 

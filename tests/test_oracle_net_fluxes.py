@@ -234,3 +234,60 @@ def test_rk3_advances_a_constant_forcing_by_one_full_step():
     _ocean_step(m, dt)
     np.testing.assert_allclose(m.T[-1][wet], (T0[-1] - dt * JT / dz)[wet], rtol=1e-15)
     assert np.array_equal(m.T[:-1], T0[:-1])
+
+# ---------------------------------------------------------------------------
+# Quasi-Adams-Bashforth 2 step (the reference's alternative timestepper)
+# ---------------------------------------------------------------------------
+
+def _salinity_decay(m, dt, Js=2.5e-4, z=-0.35):
+    """Fluxes for the linear surface-salinity equation of rate λ = z/Δt and fixed point S* = −Jˢ/Jʷ."""
+    wet = m.case.grid.kmin < m.case.grid.spec.Nz
+    dz = m.case.grid.dz[-1]
+    Jw = -z * dz / dt
+    m.net_arrays["J_w"][...] = np.where(wet, Jw, 0.0)
+    m.net_arrays["J_S"][...] = np.where(wet, Js, 0.0)
+    return -Js / Jw
+
+
+def test_ab2_first_step_and_a_changed_step_are_forward_euler():
+    """The first QAB2 step, and a step whose Δt differs from the last one, are forward Euler: the
+    departure from S* is multiplied by 1 + z (z = λΔt), not by the AB2 factor; a step with the last
+    Δt is not (its factor involves the previous tendency)."""
+    m, wet = _still_model()
+    m.timestepper = "QuasiAdamsBashforth2"
+    dt = 1200.0
+    Sstar = _salinity_decay(m, dt, z=-0.35)
+    S0 = m.S[-1].copy()
+    m.ab2_step(dt)
+    m.iteration, m.last_dt = 1, dt
+    np.testing.assert_allclose(m.S[-1][wet], (Sstar + (S0 - Sstar) * (1 - 0.35))[wet], rtol=1e-15)
+    S1 = m.S[-1].copy()
+    m.ab2_step(dt)                                   # same Δt: an AB2 step
+    m.iteration, m.last_dt = 2, dt
+    assert not np.allclose(m.S[-1][wet], (Sstar + (S1 - Sstar) * (1 - 0.35))[wet], rtol=1e-6)
+    S2 = m.S[-1].copy()
+    dt2 = 600.0
+    Sstar2 = _salinity_decay(m, dt2, z=-0.2)
+    m.ab2_step(dt2)                                  # Δt changed: Euler again
+    np.testing.assert_allclose(m.S[-1][wet], (Sstar2 + (S2 - Sstar2) * (1 - 0.2))[wet], rtol=1e-15)
+
+
+@pytest.mark.parametrize("z", [-1e-3, -2e-3])
+def test_ab2_local_error_is_chi_z_squared(z):
+    """From the exact history of y' = λy (y = S − S*: the state yₙ and the previous tendency
+    λyₙe^{−z}), one QAB2 step gives yₙ(1 + z((3/2 + χ) − (1/2 + χ)e^{−z})), whose departure from the
+    exact yₙe^{z} is χz²yₙ + O(z³): the scheme is consistent (its two weights sum to one) with
+    Oceananigans' χ = 0.1. Coefficients that do not sum to one leave an O(z) error; χ = 0 an O(z³)
+    one."""
+    m, wet = _still_model()
+    m.timestepper = "QuasiAdamsBashforth2"
+    dt = 1200.0
+    Sstar = _salinity_decay(m, dt, z=z)
+    y = (m.S[-1] - Sstar)
+    Sprev = m.S.copy()
+    Sprev[-1] = Sstar + y * np.exp(-z)
+    m.GmT, m.GmS = m.tendency(m.T, Sprev)
+    m.iteration, m.last_dt = 5, dt
+    m.ab2_step(dt)
+    ratio = ((m.S[-1] - Sstar) - y * np.exp(z))[wet] / (y[wet] * z * z)
+    assert np.all(np.abs(ratio - 0.1) < 1e-3), ratio

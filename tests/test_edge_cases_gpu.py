@@ -1,6 +1,7 @@
 """Edge shapes of the coupled step against the oracle on the GPU: a single column, a single
 level, two levels, ragged sizes that fill no tile, bounded longitude, and a domain that is
-land everywhere. Same FP64 bar as the other model tests (1e-12 relative)."""
+land everywhere, with the split RK3 and the Quasi-Adams-Bashforth 2 timesteppers. Same FP64 bar
+as the other model tests (1e-12 relative)."""
 import numpy as np
 import pytest
 import torch
@@ -25,10 +26,10 @@ EDGE_CASES = [
 ]
 
 
-def _step_and_compare(case, steps=3, dt=1200.0):
+def _step_and_compare(case, steps=3, dt=1200.0, timestepper="SplitRungeKutta3"):
     p = to_params()
-    om = O.OracleModel(case, params=abi.Params.from_buffer_copy(p))
-    gm = OceanOnlyModel(case, params=abi.Params.from_buffer_copy(p))
+    om = O.OracleModel(case, params=abi.Params.from_buffer_copy(p), timestepper=timestepper)
+    gm = OceanOnlyModel(case, params=abi.Params.from_buffer_copy(p), timestepper=timestepper)
     om.reconcile()
     torch.cuda.synchronize()
     _compare_fluxes(gm, om)
@@ -46,11 +47,12 @@ def _step_and_compare(case, steps=3, dt=1200.0):
     return gm, om
 
 
+@pytest.mark.parametrize("timestepper", ["SplitRungeKutta3", "QuasiAdamsBashforth2"])
 @pytest.mark.parametrize("name,shape,topology_x,longitude", EDGE_CASES, ids=[c[0] for c in EDGE_CASES])
-def test_edge_shapes_step_like_oracle(require_gpu, name, shape, topology_x, longitude):
+def test_edge_shapes_step_like_oracle(require_gpu, name, shape, topology_x, longitude, timestepper):
     case = make_case(*shape, seed=23, topology_x=topology_x, longitude=longitude)
-    print(f"[edge] {name}: shape={shape} topology_x={topology_x} longitude={longitude}", flush=True)
-    _step_and_compare(case)
+    print(f"[edge] {name}: shape={shape} topology_x={topology_x} longitude={longitude} {timestepper}", flush=True)
+    _step_and_compare(case, timestepper=timestepper)
 
 
 def test_all_land_domain(require_gpu):

@@ -1079,7 +1079,8 @@ __host__ __device__ inline ne_time_interp time_indices(const double* times, int 
 // (13/3)s_r² + e_r² with 4ε. The FP64 weights are normalised with one division:
 // multiplying every α_r = d_r (1 + (τ/(β_r+ε))²) by Π_s (β_s+ε)² leaves α_r/Σα unchanged;
 // the division is a reciprocal refined by two Newton steps (within 2 ulp). In FP32 the
-// Π(β+ε)² products could underflow, so FP32 keeps per-stencil reciprocals instead.
+// Π(β+ε)² products could underflow, so FP32 forms the three 1/(β_r+ε) from one reciprocal of
+// Π(β+ε) (at least ε³) and the other two factors, and normalises with a second.
 // Fused multiply-adds are allowed here (contract(fast)).
 // ---------------------------------------------------------------------------
 
@@ -1119,13 +1120,17 @@ __device__ __forceinline__ float weno5z_core(float d0, float d1, float d2, float
     const float eps = (float)NE_WENO_EPS;
     const float s0 = d1 - d0, s1 = d2 - d1, s2 = d3 - d2;
     const float e0 = fmaf(3.0f, d1, -d0), e1 = d1 + d2, e2 = fmaf(3.0f, d2, -d3);
-    const float b0 = (13.0f / 12.0f) * s0 * s0 + 0.25f * e0 * e0;
-    const float b1 = (13.0f / 12.0f) * s1 * s1 + 0.25f * e1 * e1;
-    const float b2 = (13.0f / 12.0f) * s2 * s2 + 0.25f * e2 * e2;
+    const float b0 = (13.0f / 12.0f) * s0 * s0 + 0.25f * e0 * e0 + eps;
+    const float b1 = (13.0f / 12.0f) * s1 * s1 + 0.25f * e1 * e1 + eps;
+    const float b2 = (13.0f / 12.0f) * s2 * s2 + 0.25f * e2 * e2 + eps;
+    // 1/(βᵣ+ε) from one reciprocal of the product (≥ ε³, no underflow) and the other two factors
+    const float b12 = b1 * b2;
+    const float q = __builtin_amdgcn_rcpf(b0 * b12);
     const float tau = fabsf(b0 - b2);
-    const float g0 = tau * __builtin_amdgcn_rcpf(b0 + eps);
-    const float g1 = tau * __builtin_amdgcn_rcpf(b1 + eps);
-    const float g2 = tau * __builtin_amdgcn_rcpf(b2 + eps);
+    const float tq = tau * q;
+    const float g0 = tq * b12;
+    const float g1 = tq * (b0 * b2);
+    const float g2 = tq * (b0 * b1);
     const float a0 = 0.1f * (1.0f + g0 * g0);
     const float a1 = 0.6f * (1.0f + g1 * g1);
     const float a2 = 0.3f * (1.0f + g2 * g2);

@@ -243,14 +243,15 @@ def serving_report(args):
     print(json.dumps(serving_measure(args, [float(x) for x in args.serve.split(",")])), flush=True)
 
 
-def form_measure(workload: str, precision: str, seed: int, steps: int, warmup: int, dt: float) -> dict:
-    """ms per step of one precision form on the workload (inputs resident in HBM)."""
+def form_measure(workload: str, precision: str, seed: int, steps: int, warmup: int, dt: float,
+                 timestepper: str = "SplitRungeKutta3") -> dict:
+    """ms per step of one precision form (and timestepper) on the workload (inputs resident in HBM)."""
     import torch
     from numericalearth_hip.generators import make_case
     from numericalearth_hip.model import OceanOnlyModel
 
     Nx, Ny, Nz = WORKLOADS[workload]
-    m = OceanOnlyModel(make_case(Nx, Ny, Nz, seed=seed), precision=precision)
+    m = OceanOnlyModel(make_case(Nx, Ny, Nz, seed=seed), precision=precision, timestepper=timestepper)
     for _ in range(warmup):
         m.time_step(dt)
     m.timing_begin(steps)
@@ -262,7 +263,7 @@ def form_measure(workload: str, precision: str, seed: int, steps: int, warmup: i
     el = time.perf_counter() - t0
     kernel_ms, nrec = m.timing_end()
     m.close()
-    return {"precision": precision, "steps": steps, "ms_per_step": 1e3 * el / steps,
+    return {"precision": precision, "timestepper": timestepper, "steps": steps, "ms_per_step": 1e3 * el / steps,
             "cell_updates_per_s": Nx * Ny * Nz * steps / el,
             "kernels_ms_per_step": {k: v / nrec for k, v in kernel_ms.items()}}
 
@@ -517,6 +518,10 @@ def main():
         form["speedup_mixed_over_fp64"] = (ms_per_step / form["ms_per_step"] if other == "mixed"
                                            else form["ms_per_step"] / ms_per_step)
         out["forms"] = {other: form}
+        if args.precision == "fp64":   # the reference's alternative timestepper, FP64
+            ab2 = form_measure(args.workload, "fp64", args.seed, 100, 10, args.dt, "QuasiAdamsBashforth2")
+            ab2["speedup_over_split_rk3"] = ms_per_step / ab2["ms_per_step"]
+            out["forms"]["fp64_quasi_adams_bashforth2"] = ab2
         ens = ensemble_measure(64, args.seed, "fp64", 20, 3, args.dt)
         out["ensemble_64"] = {k: ens[k] for k in ("members", "cells", "smallest", "largest", "sequential_ms_per_step",
                                                   "batched_graph_ms_per_step", "batched_graph_cell_updates_per_s",

@@ -9,7 +9,8 @@
 // from it at the ulp level: its own logarithm ne_log, arctangent ne_atan_ge1 and cube root
 // ne_cbrt_ge1 (each within 2 ulp), A·exp(−b·log R★) for A/R★^b, x·√x for (1 + x)^{3/2}, shared subexpressions of ψ_m and
 // ψ_s, and a degree-7 Taylor polynomial of ψ at |ζ| ≤ 5·10⁻⁴; with the default surface constants
-// also logarithms of quotients as differences of logarithms and ζ as products with 1/L.
+// also logarithms of quotients as differences of logarithms, ζ as products with 1/L, and its
+// remaining quotients as products with a reciprocal refined by two Newton steps (quotient).
 // Those differences stay far inside the 1e-12 comparison with
 // the oracle, but near a stop threshold they can move a column's fixed point by one iteration
 // (its fluxes then differ by that iteration's drift, below the stop tolerance). The WENO
@@ -163,6 +164,17 @@ __device__ __forceinline__ double latent_heat_vapor(const ne_params& p, double T
 __device__ __forceinline__ double virtual_temperature(const ne_params& p, double T, double q) {
     return gas_constant_air(p, q) / R_d(p) * T;
 }
+// a/b for b > 0 normal: hardware reciprocal, two Newton steps, one product.
+__device__ __forceinline__ double quotient(double a, double b) {
+#pragma clang fp contract(fast)
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    return a * r;
+}
+
 // Natural logarithm of a positive, finite, normal double in about 30 vector instructions (the
 // library's log takes about 100): x = 2^e·m with m in [√½, √2), log x = e·ln 2 + 2·atanh(s),
 // s = (m − 1)/(m + 1), |s| ≤ 0.1716 — the argument reduction of Cody & Waite (Software Manual
@@ -335,7 +347,7 @@ __device__ __forceinline__ float tcbrt_ge1(float x) { return fcbrt(x); }
 __device__ __forceinline__ double tsqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float tsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 template <typename T>
-__device__ __forceinline__ T tdiv(T a, T b) { return a / b; }
+__device__ __forceinline__ T tdiv(T a, T b) { return quotient(a, b); }
 template <>
 __device__ __forceinline__ float tdiv<float>(float a, float b) { return a * frcp(b); }
 
@@ -867,7 +879,13 @@ __device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p_in, doubl
             double bstar = goTv * (ts * onedq + dTv * qs);
             double Usg2 = gustiness_squared(p, us, bstar, hbl);
             double U = sqrt(du2dv2 + Usg2);
-            double lu = momentum_roughness(p, us);
+            // LogForm: 1/u★ once (hardware reciprocal and two Newton steps), for ℓᵤ's smooth-wall
+            // term and for 1/L; the Charnock term with Cg/g folded
+            const double rus = LogForm ? quotient(1.0, us) : 0.0;
+            double lu = LogForm ? jmin((p.charnock_parameter / p.roughness_gravitational_acceleration) * (us * us) +
+                                           (p.smooth_wall_parameter * p.air_kinematic_viscosity) * rus,
+                                       p.maximum_momentum_roughness)
+                                : momentum_roughness(p, us);
             double lq, lt, lnlu = 0.0, lnlq = 0.0;
             if constexpr (LogForm) {
                 lnlu = ne_log(lu);
@@ -886,12 +904,12 @@ __device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p_in, doubl
             double dh = jmax(za - p.zero_plane_displacement, 2.0 * lu);
             double chiu, chiq, chit;
             if constexpr (LogForm) {
-                const double invL = kappa * bstar / (us * us);
+                const double invL = (kappa * bstar) * (rus * rus);
                 const double lndh = 2.0 * lu > zd ? ln2 + lnlu : ln_zd;
                 double pmh, psh;
                 psi_pair_default(dh * invL, pmh, psh);
-                chiu = kappa / ((lndh - lnlu) - pmh + psi_m_default(lu * invL));
-                chiq = kappa / ((lndh - lnlq) - psh + psi_s_default(lq * invL));
+                chiu = quotient(kappa, (lndh - lnlu) - pmh + psi_m_default(lu * invL));
+                chiq = quotient(kappa, (lndh - lnlq) - psh + psi_s_default(lq * invL));
                 chit = chiq;
             } else if (Fast) {
                 double L = bstar == 0.0 ? (double)INFINITY : us * us / (kappa * bstar);
@@ -1084,16 +1102,6 @@ __host__ __device__ inline ne_time_interp time_indices(const double* times, int 
 // Fused multiply-adds are allowed here (contract(fast)).
 // ---------------------------------------------------------------------------
 
-// a/b for b > 0 normal: hardware reciprocal, two Newton steps, one product.
-__device__ __forceinline__ double quotient(double a, double b) {
-#pragma clang fp contract(fast)
-    double r = __builtin_amdgcn_rcp(b);
-    double e = fma(-b, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-b, r, 1.0);
-    r = fma(r, e, r);
-    return a * r;
-}
 
 // Σ_r ω_r (q_r − c₂) of the 5-point WENO-Z reconstruction from the four differences.
 __device__ __forceinline__ double weno5z_core(double d0, double d1, double d2, double d3) {

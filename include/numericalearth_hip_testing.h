@@ -33,6 +33,12 @@ int ne_atan_ge1_f64(ne_ctx* ctx, const double* x, double* y, int64_t n, void* st
  * the math library's) so its accuracy can be checked against a correctly rounded one. */
 int ne_cbrt_ge1_f64(ne_ctx* ctx, const double* x, double* y, int64_t n, void* stream);
 
+/* The flux solve's FP64 square root applied elementwise to x[n] (DEVICE arrays): y[i] = √x[i]
+ * within 1 ulp for positive, finite, normal x (other arguments are not defined). It exposes the
+ * library's own square root (the hardware's reciprocal square root refined by Goldschmidt steps,
+ * in place of the math library's correctly rounded one) so its accuracy can be checked. */
+int ne_sqrt_f64(ne_ctx* ctx, const double* x, double* y, int64_t n, void* stream);
+
 /* A kernel of known HBM traffic, to read hardware memory counters against: out_j[i] =
  * Σ_k in_k[i] + j for i < n, coalesced and i fastest, so one launch reads n_in·n·8 bytes and
  * writes n_out·n·8 bytes. `in` and `out` are HOST arrays of n_in ≤ NE_STREAM_PROBE_MAX_IN and

@@ -6,8 +6,8 @@
 // vapour pressure (one exponential of a sum, with ne_log, for a power times an exponential,
 // within a few ulp). The instance for the default
 // Edson constants ("Fast", and "Consts" with the default surface constants as literals) departs
-// from it at the ulp level: its own logarithm ne_log, arctangent ne_atan_ge1 and cube root
-// ne_cbrt_ge1 (each within 2 ulp), A·exp(−b·log R★) for A/R★^b, x·√x for (1 + x)^{3/2}, shared subexpressions of ψ_m and
+// from it at the ulp level: its own logarithm ne_log, arctangent ne_atan_ge1, cube root
+// ne_cbrt_ge1 and square root ne_sqrt (each within 2 ulp), A·exp(−b·log R★) for A/R★^b, x·√x for (1 + x)^{3/2}, shared subexpressions of ψ_m and
 // ψ_s, and a degree-7 Taylor polynomial of ψ at |ζ| ≤ 5·10⁻⁴; with the default surface constants
 // also logarithms of quotients as differences of logarithms, ζ as products with 1/L, and its
 // remaining quotients as products with a reciprocal refined by two Newton steps (quotient).
@@ -328,6 +328,23 @@ __device__ __forceinline__ double ne_cbrt_ge1(double x) {
     return fma(-(y0 * t), r, y0);
 }
 
+// Square root of a positive, finite, normal double in 12 vector instructions (the library's
+// correctly rounded sqrt takes about 21): the hardware's reciprocal square root (about 2⁻²⁶),
+// two Goldschmidt steps on g ≈ √x and h ≈ 1/(2√x), then g + h(x − g²) with the residual x − g²
+// formed in one fused operation; within 1 ulp (test_elementary_gpu.py: against numpy's sqrt).
+// Zero, negative, infinite and NaN arguments are not handled: the flux solve passes none.
+__device__ __forceinline__ double ne_sqrt(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    return fma(fma(-g, g, x), h, g);
+}
+
 // FP32 functions of the mixed form's flux iteration: the hardware's approximations (log₂, exp₂,
 // square root and reciprocal, each within about 1 FP32 ulp for normal arguments) without the
 // library's handling of denormal and special arguments, which takes 10–17 instructions per call
@@ -344,7 +361,7 @@ __device__ __forceinline__ double tatan_ge1(double x) { return ne_atan_ge1(x); }
 __device__ __forceinline__ float tatan_ge1(float x) { return atanf(x); }
 __device__ __forceinline__ double tcbrt_ge1(double x) { return ne_cbrt_ge1(x); }
 __device__ __forceinline__ float tcbrt_ge1(float x) { return fcbrt(x); }
-__device__ __forceinline__ double tsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ double tsqrt(double x) { return ne_sqrt(x); }
 __device__ __forceinline__ float tsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 template <typename T>
 __device__ __forceinline__ T tdiv(T a, T b) { return quotient(a, b); }
@@ -878,7 +895,7 @@ __device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p_in, doubl
             // (iteration 0 always iterates; ConvergenceStopCriteria / FixedIterations tests below)
             double bstar = goTv * (ts * onedq + dTv * qs);
             double Usg2 = gustiness_squared(p, us, bstar, hbl);
-            double U = sqrt(du2dv2 + Usg2);
+            double U = LogForm ? ne_sqrt(du2dv2 + Usg2) : sqrt(du2dv2 + Usg2);
             // LogForm: 1/u★ once (hardware reciprocal and two Newton steps), for ℓᵤ's smooth-wall
             // term and for 1/L; the Charnock term with Cg/g folded
             const double rus = LogForm ? quotient(1.0, us) : 0.0;

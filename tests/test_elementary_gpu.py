@@ -47,6 +47,15 @@ def _device_cbrt_ge1(x):
     return yd.cpu().numpy()
 
 
+def _device_sqrt(x):
+    ctx = Context.get(0)
+    xd = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).cuda()
+    yd = torch.empty_like(xd)
+    check(ctx.lib.ne_sqrt_f64(ctx.handle, xd.data_ptr(), yd.data_ptr(), xd.numel(), stream_handle()), "ne_sqrt_f64")
+    torch.cuda.synchronize()
+    return yd.cpu().numpy()
+
+
 def _ulps(y, ref):
     return np.abs(y - ref) / np.spacing(np.abs(ref))
 
@@ -117,6 +126,24 @@ def test_cbrt_ge1_within_one_ulp(require_gpu):
         assert worst <= 1.0, name
     y = _device_cbrt_ge1(np.array([1.0, 8.0, 27.0, 2.0 ** 120]))
     assert y[0] == 1.0 and y[1] == 2.0 and y[2] == 3.0 and y[3] == 2.0 ** 40
+
+
+def test_sqrt_within_one_ulp(require_gpu):
+    rng = np.random.default_rng(53)
+    samples = {
+        "[1, 4)": rng.uniform(1.0, 4.0, 1 << 20),
+        "wind speeds squared": 10.0 ** rng.uniform(-4.0, 3.5, 1 << 18),
+        "stability arguments 1 − 15ζ and their roots": np.concatenate(
+            [1.0 + 15.0 * 10.0 ** rng.uniform(-4, 4, 1 << 18), np.sqrt(1.0 + 15.0 * 10.0 ** rng.uniform(-4, 4, 1 << 18))]),
+        "normal range": 10.0 ** rng.uniform(-300.0, 300.0, 1 << 20),
+    }
+    for name, x in samples.items():
+        y = _device_sqrt(x)
+        worst = float(np.max(_ulps(y, np.sqrt(x))))
+        print(f"[sqrt] {name}: worst {worst:.2f} ulp over {x.size} arguments", flush=True)
+        assert worst <= 1.0, name
+    sq = np.arange(1.0, 1 << 16) ** 2
+    assert np.array_equal(_device_sqrt(sq), np.sqrt(sq))          # exact at perfect squares
 
 
 @pytest.mark.parametrize("n_in,n_out,n", [(5, 2, 1000003), (0, 1, 77), (8, 4, 4096)])

@@ -179,9 +179,9 @@ __device__ __forceinline__ double quotient(double a, double b) {
 // library's log takes about 100): x = 2^e·m with m in [√½, √2), log x = e·ln 2 + 2·atanh(s),
 // s = (m − 1)/(m + 1), |s| ≤ 0.1716 — the argument reduction of Cody & Waite (Software Manual
 // for the Elementary Functions, 1980) — with atanh summed as its Taylor series to s¹⁹, whose
-// remainder is below 2⁻⁵⁵ of the result. m − 1 is exact; the division and m + 1 round, and
-// e·ln 2 is added in one fused operation with ln 2 carried in two parts, so the result is
-// within 2 ulp. Zero, negative, infinite and NaN arguments are not handled: the flux solve
+// remainder is below 2⁻⁵⁵ of the result. m − 1 is exact; m + 1 and the quotient (a reciprocal
+// refined by two Newton steps, 6 instructions in place of a division's 11) round, and e·ln 2 is
+// added in one fused operation with ln 2 carried in two parts, so the result is within 2 ulp. Zero, negative, infinite and NaN arguments are not handled: the flux solve
 // passes none.
 __device__ __forceinline__ double ne_log(double x) {
     double m = __builtin_amdgcn_frexp_mant(x);   // [0.5, 1)
@@ -190,7 +190,7 @@ __device__ __forceinline__ double ne_log(double x) {
         m += m;
         e -= 1;
     }
-    const double s = (m - 1.0) / (m + 1.0);
+    const double s = quotient(m - 1.0, m + 1.0);
     const double z = s * s;
     double q = 2.0 / 19.0;
     q = fma(q, z, 2.0 / 17.0);

@@ -39,6 +39,12 @@ int ne_cbrt_ge1_f64(ne_ctx* ctx, const double* x, double* y, int64_t n, void* st
  * in place of the math library's correctly rounded one) so its accuracy can be checked. */
 int ne_sqrt_f64(ne_ctx* ctx, const double* x, double* y, int64_t n, void* stream);
 
+/* The flux solve's FP64 exponential applied elementwise to x[n] (DEVICE arrays): y[i] = e^x[i]
+ * within 2 ulp for |x| ≤ 700 (other arguments are not defined). It exposes the library's own
+ * exponential (reduction by ln 2 and a degree-13 Taylor polynomial, in place of the math
+ * library's) so its accuracy can be checked. */
+int ne_exp_f64(ne_ctx* ctx, const double* x, double* y, int64_t n, void* stream);
+
 /* A kernel of known HBM traffic, to read hardware memory counters against: out_j[i] =
  * Σ_k in_k[i] + j for i < n, coalesced and i fastest, so one launch reads n_in·n·8 bytes and
  * writes n_out·n·8 bytes. `in` and `out` are HOST arrays of n_in ≤ NE_STREAM_PROBE_MAX_IN and

@@ -405,6 +405,13 @@ int ne_atan_ge1_f64(ne_ctx* c, const double* x, double* y, int64_t n, void* stre
     return dev_err(launch_atan_ge1(x, y, n, as_stream(stream)));
 }
 
+int ne_exp_f64(ne_ctx* c, const double* x, double* y, int64_t n, void* stream) {
+    if (!c || !x || !y) return NE_ERR_NULL;
+    if (n < 0 || n > ((int64_t)1 << 38)) return NE_ERR_SHAPE;
+    if (n == 0) return NE_OK;
+    return dev_err(launch_exp(x, y, n, as_stream(stream)));
+}
+
 int ne_sqrt_f64(ne_ctx* c, const double* x, double* y, int64_t n, void* stream) {
     if (!c || !x || !y) return NE_ERR_NULL;
     if (n < 0 || n > ((int64_t)1 << 38)) return NE_ERR_SHAPE;

@@ -122,6 +122,7 @@ hipError_t launch_log(const double* x, double* y, int64_t n, hipStream_t s);
 hipError_t launch_atan_ge1(const double* x, double* y, int64_t n, hipStream_t s);
 hipError_t launch_cbrt_ge1(const double* x, double* y, int64_t n, hipStream_t s);
 hipError_t launch_sqrt(const double* x, double* y, int64_t n, hipStream_t s);
+hipError_t launch_exp(const double* x, double* y, int64_t n, hipStream_t s);
 hipError_t launch_fractional_indices_nodes(const GridDev& g, const double* xn, int nx, int tx, const double* yn,
                                           int ny, int ty, double* fi, double* fj, hipStream_t s);
 hipError_t launch_fractional_indices(const GridDev& g, const ne_source_grid& src, double* fi, double* fj,

@@ -206,6 +206,11 @@ __global__ void atan_ge1_kernel(const double* x, double* y, int64_t n) {
     if (i < n) y[i] = ne_atan_ge1(x[i]);
 }
 
+__global__ void exp_kernel(const double* x, double* y, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) y[i] = ne_exp(x[i]);
+}
+
 __global__ void sqrt_kernel(const double* x, double* y, int64_t n) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i < n) y[i] = ne_sqrt(x[i]);
@@ -841,6 +846,11 @@ RadOut rad_out(const ne_radiative_fluxes& r) {
 
 hipError_t launch_immersed_levels(const GridDev& g, View3<const double> bottom, int32_t* kmin, hipStream_t s) {
     hipLaunchKernelGGL(immersed_levels_kernel, grid2d(g.Nx, g.Ny), dim3(BX, BY), 0, s, g, bottom, kmin);
+    return hipGetLastError();
+}
+
+hipError_t launch_exp(const double* x, double* y, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(exp_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, n);
     return hipGetLastError();
 }
 

@@ -41,6 +41,7 @@ SIGNATURES = {
     "ne_atan_ge1_f64": [vp, vp, vp, C.c_int64, vp],
     "ne_cbrt_ge1_f64": [vp, vp, vp, C.c_int64, vp],
     "ne_sqrt_f64": [vp, vp, vp, C.c_int64, vp],
+    "ne_exp_f64": [vp, vp, vp, C.c_int64, vp],
     "ne_stream_probe_f64": [vp, vp, C.c_int32, vp, C.c_int32, C.c_int64, vp],
     "ne_fill_halo_f64": [vp, vp, abi.View, C.c_int32, C.c_int32, C.c_int32, vp],
     "ne_fill_halo_f32": [vp, vp, abi.View, C.c_int32, C.c_int32, C.c_int32, vp],

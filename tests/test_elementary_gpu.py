@@ -56,6 +56,15 @@ def _device_sqrt(x):
     return yd.cpu().numpy()
 
 
+def _device_exp(x):
+    ctx = Context.get(0)
+    xd = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).cuda()
+    yd = torch.empty_like(xd)
+    check(ctx.lib.ne_exp_f64(ctx.handle, xd.data_ptr(), yd.data_ptr(), xd.numel(), stream_handle()), "ne_exp_f64")
+    torch.cuda.synchronize()
+    return yd.cpu().numpy()
+
+
 def _ulps(y, ref):
     return np.abs(y - ref) / np.spacing(np.abs(ref))
 
@@ -144,6 +153,21 @@ def test_sqrt_within_one_ulp(require_gpu):
         assert worst <= 1.0, name
     sq = np.arange(1.0, 1 << 16) ** 2
     assert np.array_equal(_device_sqrt(sq), np.sqrt(sq))          # exact at perfect squares
+
+
+def test_exp_within_two_ulp(require_gpu):
+    rng = np.random.default_rng(59)
+    samples = {
+        "[-ln 2/2, ln 2/2]": rng.uniform(-0.35, 0.35, 1 << 20),
+        "the solve's arguments": rng.uniform(-60.0, 60.0, 1 << 20),
+        "[-700, 700]": rng.uniform(-700.0, 700.0, 1 << 20),
+    }
+    for name, x in samples.items():
+        y = _device_exp(x)
+        worst = float(np.max(_ulps(y, np.exp(x))))
+        print(f"[exp] {name}: worst {worst:.2f} ulp over {x.size} arguments", flush=True)
+        assert worst <= 2.0, name
+    assert _device_exp(np.array([0.0]))[0] == 1.0
 
 
 @pytest.mark.parametrize("n_in,n_out,n", [(5, 2, 1000003), (0, 1, 77), (8, 4, 4096)])

@@ -920,7 +920,17 @@ __device__ __forceinline__ ColumnFluxes flux_column(const ne_params& p_in, doubl
         while (!done32) {
             // (iteration 0 always iterates; ConvergenceStopCriteria / FixedIterations tests below)
             double bstar = goTv * (ts * onedq + dTv * qs);
-            double Usg2 = gustiness_squared(p, us, bstar, hbl);
+            double Usg2;
+            if constexpr (LogForm) {
+                // the default convective gustiness with ne_cbrt_ge1's first value and Halley step,
+                // valid for any x in FP32's normal range: below 10⁻³⁰, β∛x is far under the
+                // minimum gustiness the maximum keeps, so x is taken as 0 there
+                const double x = jmax(0.0, -us * bstar) * hbl;
+                const double Ug = jmax(p.minimum_gustiness, p.gustiness_parameter * (x < 1e-30 ? 0.0 : ne_cbrt_ge1(x)));
+                Usg2 = Ug * Ug;
+            } else {
+                Usg2 = gustiness_squared(p, us, bstar, hbl);
+            }
             double U = LogForm ? ne_sqrt(du2dv2 + Usg2) : sqrt(du2dv2 + Usg2);
             // LogForm: 1/u★ once (hardware reciprocal and two Newton steps), for ℓᵤ's smooth-wall
             // term and for 1/L; the Charnock term with Cg/g folded

@@ -445,10 +445,14 @@ def test_tendency_two_color_matches_oracle(require_gpu):
     print(f"[two-colour] tendency G_T {eT:.2e} G_S {eS:.2e}", flush=True)
 
 
-def test_model_two_color_steps_match_oracle(require_gpu):
+@pytest.mark.parametrize("timestepper", ["SplitRungeKutta3", "QuasiAdamsBashforth2"])
+def test_model_two_color_steps_match_oracle(require_gpu, timestepper):
     from numericalearth_hip.params import TwoColorRadiation
     p = to_params(radiative_forcing=TwoColorRadiation())
-    case, om, gm = _pair(SMALL, seed=15, params=p, z=(-200.0, 0.0))
+    case = make_case(*SMALL, seed=15, z=(-200.0, 0.0))
+    print(f"[case] shape={SMALL} seed=15 two-colour {timestepper}", flush=True)
+    om = O.OracleModel(case, params=abi.Params.from_buffer_copy(p), timestepper=timestepper)
+    gm = OceanOnlyModel(case, params=abi.Params.from_buffer_copy(p), timestepper=timestepper)
     om.reconcile()
     torch.cuda.synchronize()
     assert_rel(_np(gm.shortwave_surface_flux), om.shortwave_surface_flux, RTOL, "J0")

@@ -423,7 +423,8 @@ int ne_tracer_tendency_weno5_f64(ne_ctx* ctx, const ne_grid* grid, const ne_para
  * the last step's: on active cells c ← c + Δt·((3/2 + χ)·G − (1/2 + χ)·G_prev) for T and S,
  * inactive cells left untouched. euler ≠ 0 (the first step, or a Δt other than the last
  * step's) takes χ = −1/2 and does not read G_prev (its views may then have NULL data). The
- * caller keeps G as the next step's G_prev. All views FP64 with T's layout. */
+ * caller keeps G as the next step's G_prev. All views FP64; T and S are updated in place and
+ * may share storage with none of the others (NE_ERR_SHAPE). */
 #define NE_AB2_CHI 0.1
 int ne_tracer_ab2_update_f64(ne_ctx* ctx, const ne_grid* grid, ne_view T, ne_view S, ne_view G_T, ne_view G_S,
                              ne_view G_prev_T, ne_view G_prev_S, double dt, double chi, int32_t euler,

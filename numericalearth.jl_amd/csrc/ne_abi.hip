@@ -541,6 +541,10 @@ int ne_tracer_ab2_update_f64(ne_ctx* c, const ne_grid* g, ne_view T, ne_view S, 
         if ((r = check_view3(*x))) return r;
     if (!euler && ((r = check_view3(GmT)) || (r = check_view3(GmS)))) return r;
     if (!std::isfinite(dt) || !std::isfinite(chi)) return NE_ERR_SHAPE;
+    // T and S are updated in place: neither may share storage with a tendency
+    for (const void* q : {GT.data, GS.data, GmT.data, GmS.data})
+        if (q && (q == T.data || q == S.data)) return NE_ERR_SHAPE;
+    if (T.data == S.data) return NE_ERR_SHAPE;
     return dev_err(launch_ab2_update(grid_dev(*g), view<double>(T), view<double>(S), view<const double>(GT),
                                      view<const double>(GS), view<const double>(GmT), view<const double>(GmS), dt, chi,
                                      euler != 0, nullptr, 0.0, 0.0, as_stream(stream)));

@@ -716,3 +716,38 @@ def test_mixed_precision_model_tracks_fp64(require_gpu):
     err = np.max(np.abs(sst32 - sst64)[wet] / np.abs(sst64[wet]))
     print(f"[mixed] SST relative error after 10 steps: {err:.2e}", flush=True)
     assert err < 1e-5
+
+
+def test_ab2_update_entry_point_bit_exact(require_gpu):
+    """ne_tracer_ab2_update_f64, the per-phase Quasi-Adams-Bashforth 2 update the Julia binding
+    calls: on active cells T + Δt((3/2 + χ)G − (1/2 + χ)G⁻) evaluated as the oracle's step does
+    (bit for bit: the library is built without contraction), inactive cells untouched; forward
+    Euler (χ = −1/2) does not read G⁻; a tendency view that shares T's storage is refused."""
+    import ctypes as C
+    from numericalearth_hip._lib import check
+    from numericalearth_hip.model import stream_handle, torch_view
+    case = make_case(37, 19, 5, seed=31)
+    gm = OceanOnlyModel(case)
+    Nz, Ny, Nx = gm.T.shape
+    g = torch.Generator(device="cuda").manual_seed(7)
+    G = [torch.randn((Nz, Ny, Nx), dtype=torch.float64, device="cuda", generator=g) * 1e-6 for _ in range(4)]
+    active = np.arange(Nz)[:, None, None] >= case.grid.kmin[None]
+    dt, chi = 1200.0, abi.NE_AB2_CHI
+    for euler in (0, 1):
+        T0, S0 = _np(gm.T).copy(), _np(gm.S).copy()
+        gmT = G[2] if not euler else torch.full_like(G[2], float("nan"))   # never read on an Euler step
+        check(gm.lib.ne_tracer_ab2_update_f64(gm.ctx.handle, C.byref(gm.grid.struct), torch_view(gm.T),
+                                              torch_view(gm.S), torch_view(G[0]), torch_view(G[1]), torch_view(gmT),
+                                              torch_view(G[3]), dt, chi, euler, stream_handle()),
+              "ne_tracer_ab2_update_f64")
+        torch.cuda.synchronize()
+        c = -0.5 if euler else chi
+        a, b = 1.5 + c, 0.5 + c
+        Gm = [np.zeros_like(T0), np.zeros_like(S0)] if euler else [_np(G[2]), _np(G[3])]
+        Tref = np.where(active, T0 + dt * (a * _np(G[0]) - b * Gm[0]), T0)
+        Sref = np.where(active, S0 + dt * (a * _np(G[1]) - b * Gm[1]), S0)
+        assert np.array_equal(_np(gm.T), Tref) and np.array_equal(_np(gm.S), Sref), f"euler={euler}"
+    assert gm.lib.ne_tracer_ab2_update_f64(gm.ctx.handle, C.byref(gm.grid.struct), torch_view(gm.T), torch_view(gm.S),
+                                           torch_view(gm.T), torch_view(G[1]), torch_view(G[2]), torch_view(G[3]),
+                                           dt, chi, 0, stream_handle()) == abi.NE_ERR_SHAPE
+    gm.close()

@@ -472,10 +472,12 @@ def cell_updates(case: SurfaceCase) -> int:
     return Nx * Ny * Nz
 
 
-def bytes_per_cell_update(precision: str, Nz: int, n_stages: int = 3) -> float:
-    """Algorithmic HBM bytes per cell-update: B = B₃ + B₂/Nz with B₃ = (7 + 9(n_st − 1))·s₃ and
-    B₂ = 7.25·s₂ᵢ + 14·s₂ₒ."""
+def bytes_per_cell_update(precision: str, Nz: int, n_stages: int = 3,
+                          timestepper: str = "SplitRungeKutta3") -> float:
+    """Algorithmic HBM bytes per cell-update: B = B₃ + B₂/Nz with B₃ = (7 + 9(n_st − 1))·s₃ for the
+    split RK3 step (15·s₃ for the Quasi-Adams-Bashforth 2 step: the tendency reads u, v, w, T, S
+    and writes G_T, G_S; the update reads T, S, G, G⁻ and writes T, S) and B₂ = 7.25·s₂ᵢ + 14·s₂ₒ."""
     s3, s2i, s2o = (8, 8, 8) if precision == "fp64" else (4, 4, 8)
-    B3 = (7 + 9 * (n_stages - 1)) * s3
+    B3 = (15 if timestepper == "QuasiAdamsBashforth2" else 7 + 9 * (n_stages - 1)) * s3
     B2 = 7.25 * s2i + 14 * s2o
     return B3 + B2 / Nz

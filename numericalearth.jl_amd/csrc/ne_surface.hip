@@ -643,7 +643,9 @@ surface_update_kernel(GridDev g, ne_params p, AtmosDev<Real> a, const double* __
                       const uint8_t* __restrict__ freeze_flags, ne_counters* counters, int* deferred) {
     int i = blockIdx.x * BX + threadIdx.x;
     int j = blockIdx.y * KBY + threadIdx.y;
-    if (out.iters.p) {
+    if (out.iters.p && !Defer) {
+        // (Not for the mixed form's FP32 iteration, whose counts vary less: without the sort and
+        // its barriers its K1 took 124 → 117 µs at 1440×560×10.)
         // Columns of the block in order of the iteration count the previous solve took, the
         // stable ones last (solve_order_key): the fixed point runs until the slowest column of a
         // wavefront converges, so wavefronts of columns with like counts leave fewer lanes idle,
@@ -738,8 +740,9 @@ batch_surface_solve_kernel(const BatchRec<Real>* __restrict__ recs, const WorkIt
         const BatchRec<Real>& rc = recs[it.member];
         const GridDev g = rc.g;
         int col = it.index + (int)threadIdx.x;
-        if (rc.f.iters.p && blockDim.x > 64) {
-            // the item's columns in the single model's order (previous iteration count, stability)
+        if (rc.f.iters.p && blockDim.x > 64 && !(sizeof(Real) == 4 && Fast)) {
+            // the item's columns in the single model's order (previous iteration count, stability;
+            // the mixed form's FP32 iteration, as there, takes them unsorted)
             const int key = col < g.Nx * g.Ny ? solve_order_key(rc.f, col % g.Nx, col / g.Nx) : SORT_BINS - 1;
             col = it.index + sorted_block_column(key, (int)threadIdx.x);
         }

@@ -48,6 +48,17 @@ def test_library_exports_every_declared_entry_point():
     assert sorted(_lib.SIGNATURES) == names
 
 
+def test_integration_notes_index_every_declared_entry_point():
+    """The maintainer's notes list each exported symbol (a `name_{f64,f32}` entry names both)."""
+    text = (REPO / "INTEGRATION.md").read_text()
+    documented = set()
+    for name in re.findall(r"`(ne_[\w{},]+)`", text):
+        m = re.fullmatch(r"(ne_\w*?)\{([\w,]+)\}", name)
+        documented |= {m.group(1) + s for s in m.group(2).split(",")} if m else {name}
+    undocumented = [n for n in declared_functions() if n not in documented]
+    assert not undocumented, f"not in INTEGRATION.md: {undocumented}"
+
+
 def test_version_and_defaults_match_python_mirror():
     L = _lib.load()
     assert L.ne_version() == 1
